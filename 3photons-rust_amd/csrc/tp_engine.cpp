@@ -1,0 +1,723 @@
+// tp_engine.cpp -- C ABI of the engine (include/tp.h): contexts, stream
+// tables, and the orchestration of the gfx950 kernels for one batch
+// (tp_simulate_batch) or one whole run (tp_run / tp_run_device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tp.h"
+#include "tp_kernels.h"
+#include "tp_rng_host.h"
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct Tables { // device copies of GF(2) jump tables for one (seeding, batch size, grid)
+    DevBuf bt, stride;
+    int kb = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    DevBuf mt;       // M^(2^k)
+    int km = 0;
+    std::map<std::string, std::unique_ptr<Tables>> tables;
+    std::map<std::string, std::vector<uint32_t>> ranf_polys; // host copies keyed by R
+    DevBuf polys, draws, origin, p0, per_batch, merged;
+    tp_stats last{};
+    bool timed = false;
+    // Host staging of the RANF origin: must outlive the asynchronous copies.
+    int32_t origin0[56];
+    int32_t p00 = 0;
+};
+
+const int KM = 24;              // intra-call offsets < 2^24 draws
+const int RANF_R = 128;         // RANF rounds per expansion segment
+const uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
+
+int fail(Ctx *c, int code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((ctx), TP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+int ensure(Ctx *c, DevBuf &b, size_t bytes)
+{
+    if (b.n >= bytes && b.p)
+        return TP_OK;
+    if (b.p) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess)
+        return fail(c, TP_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    b.n = bytes;
+    return TP_OK;
+}
+
+int upload_mats(Ctx *c, DevBuf &b, const std::vector<tp::Gf2Mat> &mats)
+{
+    size_t bytes = std::max<size_t>(1, mats.size()) * sizeof(tp::Gf2Mat);
+    int rc = ensure(c, b, bytes);
+    if (rc)
+        return rc;
+    if (!mats.empty())
+        HIPCHK(c, hipMemcpy(b.p, mats.data(), mats.size() * sizeof(tp::Gf2Mat), hipMemcpyHostToDevice));
+    return TP_OK;
+}
+
+int ensure_mt(Ctx *c)
+{
+    if (c->km)
+        return TP_OK;
+    const std::vector<tp::Gf2Mat> &t = tp::xoshiro_step_pow2();
+    std::vector<tp::Gf2Mat> v(t.begin(), t.begin() + KM);
+    int rc = upload_mats(c, c->mt, v);
+    if (rc)
+        return rc;
+    c->km = KM;
+    return TP_OK;
+}
+
+int bits_for(uint64_t n)
+{
+    int k = 0;
+    while (k < 64 && (n >> k))
+        ++k;
+    return k;
+}
+
+// Batch-advance matrix B (skip-ahead: M^(12*batch_size); jump: the 2^128
+// jump) as pow2 table up to nb, plus B^grid.
+int ensure_tables(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int grid, Tables **out)
+{
+    int kb = std::max(1, bits_for(nb));
+    std::string key = std::to_string(seeding) + ":" + std::to_string(batch_size) + ":" + std::to_string(grid);
+    auto it = c->tables.find(key);
+    if (it != c->tables.end() && it->second->kb >= kb) {
+        *out = it->second.get();
+        return TP_OK;
+    }
+    tp::Gf2Mat B;
+    if (seeding == TP_SEED_JUMP)
+        tp::gf2_jump128(B);
+    else
+        tp::gf2_pow_from_table(tp::xoshiro_step_pow2(), 12ull * batch_size, B);
+    std::vector<tp::Gf2Mat> bt = tp::gf2_pow2_table(B, kb);
+    tp::Gf2Mat S;
+    tp::gf2_pow_from_table(bt, static_cast<uint64_t>(grid), S);
+    if (bits_for(static_cast<uint64_t>(grid)) > kb) {
+        std::vector<tp::Gf2Mat> bt2 = tp::gf2_pow2_table(B, bits_for(static_cast<uint64_t>(grid)));
+        tp::gf2_pow_from_table(bt2, static_cast<uint64_t>(grid), S);
+    }
+    std::unique_ptr<Tables> t(new Tables);
+    int rc = upload_mats(c, t->bt, bt);
+    if (rc)
+        return rc;
+    rc = upload_mats(c, t->stride, std::vector<tp::Gf2Mat>(1, S));
+    if (rc)
+        return rc;
+    t->kb = kb;
+    *out = t.get();
+    c->tables[key] = std::move(t);
+    return TP_OK;
+}
+
+// x^(55*R*2^k) mod (x^55 + x^31 - 1) for k < K, uploaded to c->polys.
+int ensure_polys(Ctx *c, int R, int K)
+{
+    std::string key = std::to_string(R) + ":" + std::to_string(K);
+    auto it = c->ranf_polys.find(key);
+    if (it == c->ranf_polys.end()) {
+        std::vector<uint32_t> flat;
+        tp::RanfPoly q = tp::ranf_poly_xpow(55ull * static_cast<uint64_t>(R));
+        for (int k = 0; k < K; ++k) {
+            flat.insert(flat.end(), q.begin(), q.end());
+            q = tp::ranf_poly_mul(q, q);
+        }
+        it = c->ranf_polys.emplace(key, std::move(flat)).first;
+    }
+    int rc = ensure(c, c->polys, it->second.size() * sizeof(uint32_t));
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->polys.p, it->second.data(), it->second.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, c->stream));
+    return TP_OK;
+}
+
+tpd::Params make_params(const tp_params *p)
+{
+    tpd::Params P;
+    std::memset(&P, 0, sizeof P);
+    P.e_total = p->e_total;
+    P.cut_beam = p->cut_beam;
+    P.cut_pp = p->cut_pp;
+    P.e_min = p->e_min;
+    P.cut_plane = p->cut_plane;
+    P.g_a = p->g_a;
+    P.g_bp = p->g_beta_p;
+    P.g_bm = p->g_beta_m;
+    for (int i = 0; i < 5; ++i)
+        P.sc[i] = p->sigma_contribs[i];
+    // Incoming momenta (evgen.rs:66-69) and their spinor pieces (spinor.rs:37-44).
+    const double e = p->e_total;
+    P.el_x = -e / 2.;
+    P.el_e = e / 2.;
+    const double pos_x = e / 2., pos_e = e / 2.;
+    P.xx0 = std::sqrt(P.el_e + 0.);
+    P.f0re = P.el_x / P.xx0;
+    P.f0im = 0. / P.xx0;
+    P.xx1 = std::sqrt(pos_e + 0.);
+    P.f1re = pos_x / P.xx1;
+    P.f1im = 0. / P.xx1;
+    const double s01re = P.f0re * P.xx1 - P.f1re * P.xx0;
+    const double s01im = P.f0im * P.xx1 - P.f1im * P.xx0;
+    const double N = -(2. * 1.41421356237309504880); // -RAC8 (spinor.rs:19)
+    P.ns01re = N * s01re;
+    P.ns01im = N * s01im;
+    P.nt01re = N * -s01re; // t = -conj(s)
+    P.nt01im = N * s01im;
+    P.sort = p->sort_photons ? 1 : 0;
+    return P;
+}
+
+int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *state)
+{
+    if (!params || !state)
+        return fail(c, TP_ERR_ARG, "null params or rng state");
+    if (rng_kind != TP_RNG_RANF && rng_kind != TP_RNG_XOSHIRO)
+        return fail(c, TP_ERR_ARG, "unknown rng kind " + std::to_string(rng_kind));
+    if (params->precision != TP_PREC_F64)
+        return fail(c, TP_ERR_UNSUPPORTED, "precision " + std::to_string(params->precision) + " not available");
+    if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
+        return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
+    return TP_OK;
+}
+
+int grid_for(uint64_t nb)
+{
+    // 256 CUs x 8 workgroups of 4 wavefronts: 8 wavefronts per SIMD.
+    const uint64_t g = 2048;
+    return static_cast<int>(std::min<uint64_t>(nb, g));
+}
+
+// Enqueue a whole run (or one closure-sized batch when batch_size == n).
+int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, const void *state0,
+                uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
+{
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    tpk::FusedArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.P = make_params(params);
+    a.num_events = num_events;
+    a.batch_size = batch_size;
+    a.precision = params->precision;
+    int rc = ensure_mt(c);
+    if (rc)
+        return rc;
+    a.mt = static_cast<const uint64_t *>(c->mt.p);
+    a.km = c->km;
+    if (rng_kind == TP_RNG_XOSHIRO) {
+        const tp_xoshiro_state *x = static_cast<const tp_xoshiro_state *>(state0);
+        std::memcpy(a.s0, x->s, sizeof a.s0);
+        const int grid = grid_for(nb);
+        Tables *t = nullptr;
+        rc = ensure_tables(c, seeding, batch_size, nb, grid, &t);
+        if (rc)
+            return rc;
+        a.bt = static_cast<const uint64_t *>(t->bt.p);
+        a.stride = static_cast<const uint64_t *>(t->stride.p);
+        a.kb = t->kb;
+        a.nb = nb;
+        a.b_first = 0;
+        a.out = per_batch_dev;
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        tpk::launch_fused(a, 0, grid, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        return TP_OK;
+    }
+    // RANF: expand the stream of a chunk of batches into HBM, then run the
+    // fused kernel on it.
+    const tp_ranf_state *rs = static_cast<const tp_ranf_state *>(state0);
+    if (rs->index > 55)
+        return fail(c, TP_ERR_ARG, "RANF state index out of range");
+    const uint64_t chunk = std::min<uint64_t>(nb, RANF_CHUNK_BATCHES);
+    const size_t per_batch_draws = 12ull * batch_size;
+    rc = ensure(c, c->draws, chunk * per_batch_draws * sizeof(int32_t));
+    if (rc)
+        return rc;
+    rc = ensure(c, c->origin, chunk * 56 * sizeof(int32_t));
+    if (rc)
+        return rc;
+    rc = ensure(c, c->p0, chunk * sizeof(int32_t));
+    if (rc)
+        return rc;
+    // Segments needed to cover one chunk: every segment spans R rounds.
+    const uint64_t total_draws = 12ull * num_events;
+    const uint64_t p0_master = 55u - rs->index;
+    uint64_t max_rounds = (seeding == TP_SEED_JUMP) ? (per_batch_draws + 54) / 55 + 1
+                                                    : (p0_master + total_draws + 54) / 55 + 1;
+    int K = std::max(1, bits_for(max_rounds / RANF_R + 1));
+    rc = ensure_polys(c, RANF_R, K);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // previous users of the staging copy are done
+    int32_t *origin0 = c->origin0;
+    std::memcpy(origin0, rs->numbers, sizeof c->origin0);
+    c->p00 = static_cast<int32_t>(p0_master);
+    int32_t &p00 = c->p00;
+    bool first_kernel = true;
+    for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        const uint64_t b1 = std::min(nb, b0 + chunk);
+        const uint64_t cnb = b1 - b0;
+        tpk::RanfArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.origin = static_cast<const int32_t *>(c->origin.p);
+        r.p0 = static_cast<const int32_t *>(c->p0.p);
+        r.polys = static_cast<const uint32_t *>(c->polys.p);
+        r.K = K;
+        r.R = RANF_R;
+        r.out = static_cast<int32_t *>(c->draws.p);
+        uint64_t npieces;
+        if (seeding == TP_SEED_JUMP) {
+            tpk::launch_ranf_seed(rs->seed, b0, cnb, static_cast<int32_t *>(c->origin.p),
+                                  static_cast<int32_t *>(c->p0.p), c->stream);
+            HIPCHK(c, hipGetLastError());
+            if (b0 == 0) { // batch 0 runs on the caller's state itself
+                HIPCHK(c, hipMemcpyAsync(c->origin.p, origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(c->p0.p, &p00, sizeof p00, hipMemcpyHostToDevice, c->stream));
+            }
+            npieces = cnb;
+            r.d_begin = 0;
+            r.d_end = per_batch_draws;
+            r.seg_first = 0;
+            r.piece_stride = per_batch_draws;
+            r.nsegs = ((uint64_t)(max_rounds) + RANF_R - 1) / RANF_R + 1;
+        } else {
+            if (b0 == 0) {
+                HIPCHK(c, hipMemcpyAsync(c->origin.p, origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(c->p0.p, &p00, sizeof p00, hipMemcpyHostToDevice, c->stream));
+            }
+            npieces = 1;
+            r.d_begin = b0 * per_batch_draws;
+            r.d_end = std::min<uint64_t>(b1 * per_batch_draws, total_draws);
+            const uint64_t q_begin = p0_master + r.d_begin, q_end = p0_master + r.d_end;
+            r.seg_first = q_begin / 55 / RANF_R;
+            const uint64_t seg_last = (q_end - 1) / 55 / RANF_R;
+            r.nsegs = seg_last - r.seg_first + 1;
+            r.piece_stride = 0;
+        }
+        if (first_kernel) {
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        }
+        tpk::launch_ranf_expand(r, npieces, c->stream);
+        HIPCHK(c, hipGetLastError());
+        a.nb = cnb;
+        a.b_first = b0;
+        a.draws = static_cast<const int32_t *>(c->draws.p);
+        a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+        if (first_kernel) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            first_kernel = false;
+        }
+        tpk::launch_fused(a, 1, grid_for(cnb), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    return TP_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess)
+        return 0.f;
+    return ms;
+}
+
+} // namespace
+
+extern "C" int tp_create(int device, void **ctx_out)
+{
+    if (!ctx_out)
+        return TP_ERR_ARG;
+    *ctx_out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return TP_ERR_HIP;
+    if (device < 0 || device >= count)
+        return TP_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess)
+        return TP_ERR_HIP;
+    Ctx *c = new Ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return TP_ERR_HIP;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return TP_ERR_HIP;
+        }
+    *ctx_out = c;
+    return TP_OK;
+}
+
+extern "C" int tp_destroy(void *ctx)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged})
+        if (b->p)
+            (void)hipFree(b->p);
+    for (auto &kv : c->tables) {
+        if (kv.second->bt.p)
+            (void)hipFree(kv.second->bt.p);
+        if (kv.second->stride.p)
+            (void)hipFree(kv.second->stride.p);
+    }
+    for (auto &e : c->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TP_OK;
+}
+
+extern "C" const char *tp_last_error(const void *ctx)
+{
+    const Ctx *c = static_cast<const Ctx *>(ctx);
+    return c ? c->err.c_str() : "null context";
+}
+
+extern "C" int tp_rng_seed(int rng_kind, uint64_t seed, void *state)
+{
+    if (!state)
+        return TP_ERR_ARG;
+    if (rng_kind == TP_RNG_RANF) {
+        tp::RanfState g;
+        tp::ranf_seeded(g, static_cast<int32_t>(seed));
+        tp_ranf_state *o = static_cast<tp_ranf_state *>(state);
+        std::memcpy(o->numbers, g.numbers, sizeof o->numbers);
+        o->index = g.index;
+        o->seed = g.seed;
+        return TP_OK;
+    }
+    if (rng_kind == TP_RNG_XOSHIRO) {
+        tp::xoshiro_seed(seed, static_cast<tp_xoshiro_state *>(state)->s);
+        return TP_OK;
+    }
+    return TP_ERR_ARG;
+}
+
+extern "C" int tp_rng_init(int rng_kind, void *state)
+{
+    return tp_rng_seed(rng_kind, rng_kind == TP_RNG_RANF ? static_cast<uint64_t>(tp::RANF_SEED) : 12345u, state);
+}
+
+extern "C" int tp_rng_skip(int rng_kind, void *state, uint64_t n)
+{
+    if (!state)
+        return TP_ERR_ARG;
+    if (rng_kind == TP_RNG_XOSHIRO) {
+        tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(state)->s, n);
+        return TP_OK;
+    }
+    if (rng_kind != TP_RNG_RANF)
+        return TP_ERR_ARG;
+    tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
+    if (s->index > 55)
+        return TP_ERR_ARG;
+    tp::RanfState g;
+    std::memcpy(g.numbers, s->numbers, sizeof g.numbers);
+    g.index = s->index;
+    g.seed = s->seed;
+    tp::ranf_advance(g, n);
+    std::memcpy(s->numbers, g.numbers, sizeof g.numbers);
+    s->index = g.index;
+    return TP_OK;
+}
+
+extern "C" int tp_rng_jump(int rng_kind, void *state)
+{
+    if (!state)
+        return TP_ERR_ARG;
+    if (rng_kind == TP_RNG_XOSHIRO) {
+        static tp::Gf2Mat J;
+        static std::once_flag once;
+        std::call_once(once, [] { tp::gf2_jump128(J); });
+        uint64_t *s = static_cast<tp_xoshiro_state *>(state)->s;
+        tp::gf2_apply(J, s, s);
+        return TP_OK;
+    }
+    if (rng_kind != TP_RNG_RANF)
+        return TP_ERR_ARG;
+    tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
+    const int32_t seed = static_cast<int32_t>(static_cast<uint32_t>(s->seed) + 123456u);
+    return tp_rng_seed(TP_RNG_RANF, static_cast<uint64_t>(static_cast<uint32_t>(seed)), state);
+}
+
+extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const void *rng_state, uint32_t n,
+                         double *u, double *p, int32_t *keep, double *m)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    int rc = check_common(c, params, rng_kind, rng_state);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return TP_OK;
+    if (n > (1u << 20))
+        return fail(c, TP_ERR_ARG, "tp_events: n exceeds 2^20");
+    HIPCHK(c, hipSetDevice(c->device));
+    tpk::FusedArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.P = make_params(params);
+    rc = ensure_mt(c);
+    if (rc)
+        return rc;
+    a.mt = static_cast<const uint64_t *>(c->mt.p);
+    a.km = c->km;
+    int source = 0;
+    if (rng_kind == TP_RNG_XOSHIRO) {
+        std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
+    } else {
+        // Expand 12 n draws from the given state: one piece, skip-ahead.
+        const tp_ranf_state *rs = static_cast<const tp_ranf_state *>(rng_state);
+        if (rs->index > 55)
+            return fail(c, TP_ERR_ARG, "RANF state index out of range");
+        const uint64_t D = 12ull * n;
+        rc = ensure(c, c->draws, D * sizeof(int32_t));
+        if (!rc)
+            rc = ensure(c, c->origin, 56 * sizeof(int32_t));
+        if (!rc)
+            rc = ensure(c, c->p0, sizeof(int32_t));
+        if (rc)
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
+        c->p00 = static_cast<int32_t>(55u - rs->index);
+        const uint64_t max_rounds = (c->p00 + D + 54) / 55 + 1;
+        const int K = std::max(1, bits_for(max_rounds / RANF_R + 1));
+        rc = ensure_polys(c, RANF_R, K);
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
+        tpk::RanfArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.origin = static_cast<const int32_t *>(c->origin.p);
+        r.p0 = static_cast<const int32_t *>(c->p0.p);
+        r.polys = static_cast<const uint32_t *>(c->polys.p);
+        r.K = K;
+        r.R = RANF_R;
+        r.out = static_cast<int32_t *>(c->draws.p);
+        r.d_begin = 0;
+        r.d_end = D;
+        r.seg_first = 0;
+        r.nsegs = (c->p00 + D - 1) / 55 / RANF_R + 1;
+        tpk::launch_ranf_expand(r, 1, c->stream);
+        HIPCHK(c, hipGetLastError());
+        a.draws = static_cast<const int32_t *>(c->draws.p);
+        source = 1;
+    }
+    DevBuf bu, bp, bk, bm;
+    tpk::EventOut o{nullptr, nullptr, nullptr, nullptr};
+    struct Item {
+        void *host;
+        DevBuf *buf;
+        size_t bytes;
+    } items[4] = {{u, &bu, 12ull * n * sizeof(double)},
+                  {p, &bp, 12ull * n * sizeof(double)},
+                  {keep, &bk, n * sizeof(int32_t)},
+                  {m, &bm, 5ull * n * sizeof(double)}};
+    for (Item &it : items)
+        if (it.host && (rc = ensure(c, *it.buf, it.bytes)) != TP_OK)
+            break;
+    if (rc == TP_OK) {
+        o.u = static_cast<double *>(bu.p);
+        o.p = static_cast<double *>(bp.p);
+        o.keep = static_cast<int32_t *>(bk.p);
+        o.m = static_cast<double *>(bm.p);
+        tpk::launch_events(a, source, n, o, c->stream);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            rc = fail(c, TP_ERR_HIP, std::string("events kernel: ") + hipGetErrorString(e));
+        for (Item &it : items)
+            if (rc == TP_OK && it.host) {
+                e = hipMemcpyAsync(it.host, it.buf->p, it.bytes, hipMemcpyDeviceToHost, c->stream);
+                if (e != hipSuccess)
+                    rc = fail(c, TP_ERR_HIP, std::string("events copy: ") + hipGetErrorString(e));
+            }
+        e = hipStreamSynchronize(c->stream);
+        if (rc == TP_OK && e != hipSuccess)
+            rc = fail(c, TP_ERR_HIP, std::string("events sync: ") + hipGetErrorString(e));
+    }
+    for (Item &it : items)
+        if (it.buf->p)
+            (void)hipFree(it.buf->p);
+    return rc;
+}
+
+extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kind, void *rng_state, uint32_t n,
+                                 tp_acc *out)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    int rc = check_common(c, params, rng_kind, rng_state);
+    if (rc)
+        return rc;
+    if (!out)
+        return fail(c, TP_ERR_ARG, "null output");
+    if (n > 64u * TP_BATCH_SIZE)
+        return fail(c, TP_ERR_ARG, "n exceeds 640000 events per call");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) { // the reference's closure returns an empty accumulator
+        std::memset(out, 0, sizeof *out);
+        return TP_OK;
+    }
+    rc = ensure(c, c->per_batch, sizeof(tp_acc));
+    if (rc)
+        return rc;
+    rc = enqueue_run(c, params, rng_kind, TP_SEED_SKIPAHEAD, rng_state, n, n, c->per_batch.p);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->per_batch.p, sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // `&mut rng` side effect: 12 draws per event (evgen.rs:182-187).
+    if (rng_kind == TP_RNG_XOSHIRO) {
+        tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, 12ull * n);
+    } else {
+        tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
+        tp::RanfState g;
+        std::memcpy(g.numbers, s->numbers, sizeof g.numbers);
+        g.index = s->index;
+        g.seed = s->seed;
+        tp::ranf_advance(g, 12ull * n);
+        std::memcpy(s->numbers, g.numbers, sizeof g.numbers);
+        s->index = g.index;
+    }
+    return TP_OK;
+}
+
+extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, int seeding, const void *rng_state0,
+                             uint64_t num_events, tp_acc *per_batch_dev, tp_acc *merged_dev, tp_stats *stats)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    int rc = check_common(c, params, rng_kind, rng_state0);
+    if (rc)
+        return rc;
+    if (seeding != TP_SEED_SKIPAHEAD && seeding != TP_SEED_JUMP)
+        return fail(c, TP_ERR_ARG, "unknown seeding " + std::to_string(seeding));
+    if (num_events == 0)
+        return fail(c, TP_ERR_ARG, "Must simulate at least one event");
+    if (!per_batch_dev)
+        return fail(c, TP_ERR_ARG, "null per-batch output");
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
+    // Event 0 and 1 brackets the first kernel; 2 follows the event loop.
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    rc = enqueue_run(c, params, rng_kind, seeding, rng_state0, num_events, TP_BATCH_SIZE, per_batch_dev);
+    if (rc)
+        return rc;
+    if (merged_dev) {
+        tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    c->last.events = num_events;
+    c->last.batches = nb;
+    c->timed = true;
+    if (stats)
+        *stats = c->last;
+    return TP_OK;
+}
+
+extern "C" int tp_sync(void *ctx, tp_stats *stats)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->timed) {
+        c->last.device_ms = elapsed(c->ev[0], c->ev[3]);
+        c->last.kernel_ms = elapsed(c->ev[1], c->ev[2]);
+    }
+    if (stats)
+        *stats = c->last;
+    return TP_OK;
+}
+
+extern "C" int tp_run(void *ctx, const tp_params *params, int rng_kind, int seeding, const void *rng_state0,
+                      uint64_t num_events, tp_acc *per_batch_out, tp_acc *merged_out, tp_stats *stats)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    if (!per_batch_out)
+        return fail(c, TP_ERR_ARG, "null per-batch output");
+    if (num_events == 0)
+        return fail(c, TP_ERR_ARG, "Must simulate at least one event");
+    const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
+    int rc = ensure(c, c->per_batch, nb * sizeof(tp_acc));
+    if (rc)
+        return rc;
+    rc = tp_run_device(ctx, params, rng_kind, seeding, rng_state0, num_events, static_cast<tp_acc *>(c->per_batch.p),
+                       nullptr, nullptr);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(per_batch_out, c->per_batch.p, nb * sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
+    rc = tp_sync(ctx, stats);
+    if (rc)
+        return rc;
+    if (merged_out) { // ReproducibleAccumulator::get_merged_result (multi_threading.rs:107-126)
+        tp_acc m = per_batch_out[0];
+        for (uint64_t b = 1; b < nb; ++b) {
+            m.selected_events += per_batch_out[b].selected_events;
+            for (int i = 0; i < 5; ++i)
+                m.spm2[i] += per_batch_out[b].spm2[i];
+            for (int i = 0; i < 5; ++i)
+                m.vars[i] += per_batch_out[b].vars[i];
+            m.sigma += per_batch_out[b].sigma;
+            m.variance += per_batch_out[b].variance;
+        }
+        *merged_out = m;
+    }
+    return TP_OK;
+}

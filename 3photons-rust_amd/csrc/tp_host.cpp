@@ -1,0 +1,700 @@
+// tp_host.cpp -- host side of the 3photons event loop: configuration,
+// constants, finalisation and the reference's output formats.
+//
+// Restates src/config.rs, src/coupling.rs, src/evgen.rs:40-77,
+// src/resacc.rs:59-223, src/resfin.rs and src/output.rs of the reference
+// (HadrienG2/3photons-rust).  Built with -ffp-contract=off: every expression
+// keeps Rust's left-to-right evaluation and IEEE rounding; powi(k) with a
+// run-time base is binary powering (x*x, (x*x)*(x*x), x*(x*x)) and powi on a
+// compile-time constant base is the host pow(), which is what LLVM folds it to.
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+
+#include "tp_host.h"
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+const double SQRT_2 = 1.41421356237309504880;
+
+// ---- Rust-compatible number formatting -----------------------------------
+
+// Display for f64 (`{}`): shortest digits that round-trip, never an
+// exponent.  The reference's shipped outputs print negative zero as "0".
+std::string rust_display(double x)
+{
+    if (std::isnan(x))
+        return "NaN";
+    if (std::isinf(x))
+        return x > 0 ? "inf" : "-inf";
+    if (x == 0)
+        return "0";
+    char buf[64];
+    int p;
+    for (p = 1; p <= 17; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*e", p - 1, x);
+        if (std::strtod(buf, nullptr) == x)
+            break;
+    }
+    // buf = [-]d[.ddd]e[+-]XX
+    std::string s(buf);
+    bool neg = s[0] == '-';
+    if (neg)
+        s.erase(0, 1);
+    size_t epos = s.find('e');
+    int exp10 = std::atoi(s.c_str() + epos + 1);
+    std::string digits;
+    for (size_t i = 0; i < epos; ++i)
+        if (std::isdigit(static_cast<unsigned char>(s[i])))
+            digits += s[i];
+    while (digits.size() > 1 && digits.back() == '0')
+        digits.pop_back();
+    std::string out;
+    if (exp10 >= 0) {
+        if (static_cast<int>(digits.size()) <= exp10 + 1) {
+            out = digits + std::string(static_cast<size_t>(exp10 + 1) - digits.size(), '0');
+        } else {
+            out = digits.substr(0, static_cast<size_t>(exp10 + 1)) + "." +
+                  digits.substr(static_cast<size_t>(exp10 + 1));
+        }
+    } else {
+        out = "0." + std::string(static_cast<size_t>(-exp10 - 1), '0') + digits;
+    }
+    return neg ? "-" + out : out;
+}
+
+// `{:.Ne}` for f64: exact decimal rounding like printf, Rust's exponent
+// spelling (no '+', no padding), zero printed unsigned as in the shipped
+// outputs.
+std::string rust_exp(double x, int prec)
+{
+    if (std::isnan(x))
+        return "NaN";
+    if (std::isinf(x))
+        return x > 0 ? "inf" : "-inf";
+    if (x == 0)
+        x = 0.0;
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.*e", prec, x);
+    std::string s(buf);
+    size_t epos = s.find('e');
+    int e = std::atoi(s.c_str() + epos + 1);
+    return s.substr(0, epos) + "e" + std::to_string(e);
+}
+
+// `{:.N}` for f64.
+std::string rust_fixed(double x, int prec)
+{
+    if (std::isnan(x))
+        return "NaN";
+    if (std::isinf(x))
+        return x > 0 ? "inf" : "-inf";
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%.*f", prec, x);
+    return buf;
+}
+
+std::string pad_left(const std::string &s, size_t w)
+{
+    return s.size() >= w ? s : std::string(w - s.size(), ' ') + s;
+}
+std::string pad_right(const std::string &s, size_t w)
+{
+    return s.size() >= w ? s : s + std::string(w - s.size(), ' ');
+}
+
+// write_engineering (output.rs:226-269), %g-like with SIG_DIGITS = 14.
+std::string write_engineering(double x, int sig_digits)
+{
+    int precision = sig_digits - 1;
+    if (x == 0)
+        return "0";
+    double log_x = std::log10(std::fabs(x));
+    if (log_x >= -3. && log_x <= static_cast<double>(sig_digits)) {
+        long p = static_cast<long>(precision) - static_cast<long>(std::trunc(log_x));
+        if (log_x < 0.)
+            p += 1;
+        if (p < 0)
+            p = 0;
+        std::string s = rust_fixed(x, static_cast<int>(p));
+        if (s.find('.') != std::string::npos) {
+            while (!s.empty() && s.back() == '0')
+                s.pop_back();
+            if (!s.empty() && s.back() == '.')
+                s.pop_back();
+        }
+        return s;
+    }
+    return rust_exp(x, precision);
+}
+
+const int SIG_DIGITS = 15 - 1; // f64::DIGITS - 1 (output.rs:217-222)
+
+struct Out {
+    std::string s;
+    void line(const std::string &l) { s += l + "\n"; }
+    // writeln_3p with a (key, value) pair (output.rs:376-416)
+    void kv(const char *key, const std::string &v) { s += " " + pad_right(key, 31) + ": " + v + "\n"; }
+};
+
+int copy_out(const std::string &s, char *buf, int len)
+{
+    if (buf && len > 0) {
+        size_t n = s.size() < static_cast<size_t>(len - 1) ? s.size() : static_cast<size_t>(len - 1);
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return static_cast<int>(s.size()) + 1;
+}
+
+// Rust's f64::from_str accepts decimal floats and inf/nan; strtod also takes
+// hex floats, which Rust rejects.
+bool parse_f64(const std::string &t, double *out)
+{
+    if (t.empty())
+        return false;
+    if (t.find('x') != std::string::npos || t.find('X') != std::string::npos)
+        return false;
+    errno = 0;
+    char *end = nullptr;
+    double v = std::strtod(t.c_str(), &end);
+    if (*end != 0)
+        return false;
+    *out = v;
+    return true;
+}
+
+bool parse_u64(const std::string &t, uint64_t *out)
+{
+    size_t i = (t.size() > 1 && t[0] == '+') ? 1 : 0;
+    if (i >= t.size())
+        return false;
+    uint64_t v = 0;
+    for (; i < t.size(); ++i) {
+        if (!std::isdigit(static_cast<unsigned char>(t[i])))
+            return false;
+        uint64_t d = static_cast<uint64_t>(t[i] - '0');
+        if (v > (UINT64_MAX - d) / 10)
+            return false;
+        v = v * 10 + d;
+    }
+    *out = v;
+    return true;
+}
+
+bool parse_i32(const std::string &t, int32_t *out)
+{
+    size_t i = 0;
+    bool neg = false;
+    if (!t.empty() && (t[0] == '+' || t[0] == '-')) {
+        neg = t[0] == '-';
+        i = 1;
+    }
+    if (i >= t.size())
+        return false;
+    long long v = 0;
+    for (; i < t.size(); ++i) {
+        if (!std::isdigit(static_cast<unsigned char>(t[i])))
+            return false;
+        v = v * 10 + (t[i] - '0');
+        if (v > 2147483648LL)
+            return false;
+    }
+    if (neg)
+        v = -v;
+    if (v > 2147483647LL || v < -2147483648LL)
+        return false;
+    *out = static_cast<int32_t>(v);
+    return true;
+}
+
+// ConfigItem::parse_bool (config.rs:187-195): Fortran booleans, else Rust's.
+bool parse_bool(const std::string &t, int32_t *out)
+{
+    std::string low;
+    for (char c : t)
+        low += static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    if (low == ".true." || t == "true") {
+        *out = 1;
+        return true;
+    }
+    if (low == ".false." || t == "false") {
+        *out = 0;
+        return true;
+    }
+    return false;
+}
+
+int cfg_error(char *err, int len, const std::string &msg)
+{
+    copy_out(msg, err, len);
+    return TP_ERR_CONFIG;
+}
+
+} // namespace
+
+extern "C" int tp_config_load(const char *path, tp_config *cfg, char *err, int err_len)
+{
+    static const char *names[18] = {"num_events", "e_total", "beam_photons_cut", "photon_photon_cut",
+                                    "e_min", "beam_photon_plane_cut", "alpha", "alpha_z",
+                                    "gev2_to_picobarn", "m_z0", "g_z0", "sin2_weinberg",
+                                    "branching_ep_em", "beta_plus", "beta_moins", "num_bins",
+                                    "impr", "plot"};
+    if (!path || !cfg)
+        return cfg_error(err, err_len, "null argument");
+    std::memset(cfg, 0, sizeof *cfg);
+    FILE *f = std::fopen(path, "rb");
+    if (!f)
+        return cfg_error(err, err_len, std::string("failed to load the configuration: cannot open ") + path);
+    std::string text;
+    char chunk[4096];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0)
+        text.append(chunk, got);
+    std::fclose(f);
+    // str::lines() then split_whitespace().next() per line (config.rs:69-71).
+    std::string toks[18];
+    int n = 0;
+    size_t pos = 0;
+    while (n < 18 && pos <= text.size()) {
+        size_t eol = text.find('\n', pos);
+        std::string line = text.substr(pos, eol == std::string::npos ? std::string::npos : eol - pos);
+        pos = (eol == std::string::npos) ? text.size() + 1 : eol + 1;
+        size_t a = 0;
+        while (a < line.size() && std::isspace(static_cast<unsigned char>(line[a])))
+            ++a;
+        if (a == line.size())
+            continue;
+        size_t b = a;
+        while (b < line.size() && !std::isspace(static_cast<unsigned char>(line[b])))
+            ++b;
+        toks[n++] = line.substr(a, b - a);
+    }
+    if (n < 18)
+        return cfg_error(err, err_len, std::string("missing configuration of ") + names[n]);
+    tp_config c;
+    std::memset(&c, 0, sizeof c);
+    double *dst[14] = {&c.e_total, &c.cut_beam, &c.cut_pp, &c.e_min, &c.cut_plane, &c.alpha,
+                       &c.alpha_z, &c.gev2_to_picobarn, &c.m_z0, &c.g_z0, &c.sin2_weinberg,
+                       &c.branching_ep_em, &c.beta_plus, &c.beta_minus};
+    if (!parse_u64(toks[0], &c.num_events))
+        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[0]);
+    for (int i = 0; i < 14; ++i)
+        if (!parse_f64(toks[1 + i], dst[i]))
+            return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[1 + i]);
+    if (!parse_i32(toks[15], &c.num_bins))
+        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[15]);
+    if (!parse_bool(toks[16], &c.impr))
+        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[16]);
+    if (!parse_bool(toks[17], &c.plot))
+        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[17]);
+    c.parsed = 1;
+    *cfg = c;
+    // The ensure! checks of config.rs:110-124 come after the echo.
+    if (c.num_events == 0)
+        return cfg_error(err, err_len, "Please simulate at least one event");
+    if (c.plot)
+        return cfg_error(err, err_len, "Plotting is not supported by this version");
+    if (c.impr)
+        return cfg_error(err, err_len,
+                         "Individual result printing is not supported. This debugging feature has a "
+                         "run-time performance cost even when unused. It should be implemented at "
+                         "compile-time instead.");
+    if (err && err_len > 0)
+        err[0] = 0;
+    return TP_OK;
+}
+
+extern "C" int tp_config_format(const tp_config *c, char *buf, int len)
+{
+    Out o;
+    o.line("ITOT           : " + std::to_string(c->num_events));
+    o.line("ETOT           : " + rust_display(c->e_total));
+    o.line("oCutpar.ACUT   : " + rust_display(c->cut_beam));
+    o.line("oCutpar.BCUT   : " + rust_display(c->cut_pp));
+    o.line("oCutpar.EMIN   : " + rust_display(c->e_min));
+    o.line("oCutpar.SINCUT : " + rust_display(c->cut_plane));
+    o.line("ALPHA          : " + rust_display(c->alpha));
+    o.line("ALPHAZ         : " + rust_display(c->alpha_z));
+    o.line("CONVERS        : " + rust_display(c->gev2_to_picobarn));
+    o.line("oParam.MZ0     : " + rust_display(c->m_z0));
+    o.line("oParam.GZ0     : " + rust_display(c->g_z0));
+    o.line("SIN2W          : " + rust_display(c->sin2_weinberg));
+    o.line("BREPEM         : " + rust_display(c->branching_ep_em));
+    o.line("BETAPLUS       : " + rust_display(c->beta_plus));
+    o.line("BETAMOINS      : " + rust_display(c->beta_minus));
+    o.line("NBIN           : " + std::to_string(c->num_bins));
+    o.line(std::string("oParam.IMPR    : ") + (c->impr ? "true" : "false"));
+    o.line(std::string("PLOT           : ") + (c->plot ? "true" : "false"));
+    return copy_out(o.s, buf, len);
+}
+
+namespace {
+
+// Constants of ResultsAccumulator::new (resacc.rs:59-100) and
+// EventGenerator::new (evgen.rs:40-77).
+struct AccConsts {
+    double ev_weight, fact_com, norm_weight, propagator, delta, relat_width;
+    double sc[5];
+};
+
+AccConsts acc_consts(const tp_config *c)
+{
+    AccConsts k;
+    // evgen.rs:49-62, NUM_OUTGOING = 3.
+    double z_n = 2. * std::log(PI / 2.);
+    z_n -= 2. * std::log(1.);
+    z_n = z_n - std::log(2.);
+    double ln_weight = (2. * 3. - 4.) * std::log(c->e_total) + z_n;
+    k.ev_weight = std::exp(ln_weight);
+    // resacc.rs:64-100.
+    double fact_com = 1. / 6. * c->gev2_to_picobarn;
+    double relat_width = c->g_z0 / c->m_z0;
+    double p_aa = 2.;
+    double p_ab = 1. - 4. * c->sin2_weinberg;
+    double p_bb = p_ab + 8. * (c->sin2_weinberg * c->sin2_weinberg);
+    double m2 = c->m_z0 * c->m_z0;
+    double m4 = m2 * m2;
+    double c_aa = fact_com * p_aa;
+    double c_ab = fact_com * p_ab / m2;
+    double c_bb = fact_com * p_bb / m4;
+    double r = c->e_total / c->m_z0;
+    double dzeta = r * r;
+    double delta = (dzeta - 1.) / relat_width;
+    double propagator = 1. / (1. + delta * delta);
+    double n_ev = static_cast<double>(c->num_events);
+    double norm = std::pow(2. * PI, -5.) / n_ev;
+    double norm_weight = k.ev_weight * norm;
+    double com = norm_weight / 4.;
+    double aa = com * c_aa;
+    double bb = com * c_bb * propagator / (relat_width * relat_width);
+    double ab = com * c_ab * 2. * c->beta_plus * propagator / relat_width;
+    k.fact_com = fact_com;
+    k.norm_weight = norm_weight;
+    k.propagator = propagator;
+    k.delta = delta;
+    k.relat_width = relat_width;
+    k.sc[0] = aa;
+    k.sc[1] = bb * (c->beta_plus * c->beta_plus);
+    k.sc[2] = bb * (c->beta_minus * c->beta_minus);
+    k.sc[3] = ab * delta;
+    k.sc[4] = -ab;
+    return k;
+}
+
+} // namespace
+
+extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
+{
+    if (!c || !p)
+        return TP_ERR_ARG;
+    std::memset(p, 0, sizeof *p);
+    AccConsts k = acc_consts(c);
+    // Couplings::new (coupling.rs:23-33).
+    double e2 = 4. * PI * c->alpha;
+    double e2_z = 4. * PI * c->alpha_z;
+    double cos2w = 1. - c->sin2_weinberg;
+    double m2 = c->m_z0 * c->m_z0;
+    double m4 = m2 * m2;
+    double g_beta = -std::sqrt(e2_z / (4. * cos2w * c->sin2_weinberg)) / m4;
+    double x = std::sqrt(e2);
+    p->e_total = c->e_total;
+    p->cut_beam = c->cut_beam;
+    p->cut_pp = c->cut_pp;
+    p->e_min = c->e_min;
+    p->cut_plane = c->cut_plane;
+    p->g_a = -(x * (x * x));
+    p->g_beta_p = g_beta;
+    p->g_beta_m = g_beta;
+    for (int i = 0; i < 5; ++i)
+        p->sigma_contribs[i] = k.sc[i];
+    p->sort_photons = 1;
+    p->precision = TP_PREC_F64;
+    return TP_OK;
+}
+
+extern "C" void tp_acc_merge(tp_acc *into, const tp_acc *o)
+{
+    into->selected_events += o->selected_events;
+    for (int i = 0; i < 5; ++i)
+        into->spm2[i] += o->spm2[i];
+    for (int i = 0; i < 5; ++i)
+        into->vars[i] += o->vars[i];
+    into->sigma += o->sigma;
+    into->variance += o->variance;
+}
+
+extern "C" int tp_finalize(const tp_config *cfg, const tp_acc *acc, tp_final *out)
+{
+    if (!cfg || !acc || !out)
+        return TP_ERR_ARG;
+    AccConsts k = acc_consts(cfg);
+    double spm2v[5], varsv[5];
+    double n_ev = static_cast<double>(cfg->num_events);
+    for (int c = 0; c < 5; ++c) {
+        double v = (acc->vars[c] - acc->spm2[c] * acc->spm2[c] / n_ev) / (n_ev - 1.);
+        varsv[c] = std::sqrt(v / n_ev) / std::fabs(acc->spm2[c] / n_ev);
+        spm2v[c] = acc->spm2[c];
+    }
+    double spm2[2][5], vars[2][5];
+    for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 5; ++c) {
+            spm2[s][c] = spm2v[c];
+            vars[s][c] = varsv[c];
+        }
+    double polar_p = -2. * cfg->sin2_weinberg;
+    double polar_m = 1. + polar_p;
+    const double polars[2] = {polar_m, polar_p};
+    for (int c = 1; c < 5; ++c)
+        for (int s = 0; s < 2; ++s)
+            spm2[s][c] *= polars[s];
+    for (int c = 1; c < 3; ++c)
+        for (int s = 0; s < 2; ++s)
+            spm2[s][c] *= polars[s];
+    double incident_flux = 1. / (2. * (cfg->e_total * cfg->e_total));
+    double scale = k.fact_com * incident_flux * k.norm_weight;
+    for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 5; ++c)
+            spm2[s][c] *= scale;
+    double gm_z0 = cfg->g_z0 * cfg->m_z0;
+    for (int c = 1; c < 5; ++c)
+        for (int s = 0; s < 2; ++s)
+            spm2[s][c] *= k.propagator / gm_z0;
+    for (int c = 1; c < 3; ++c)
+        for (int s = 0; s < 2; ++s)
+            spm2[s][c] /= gm_z0;
+    for (int s = 0; s < 2; ++s)
+        spm2[s][3] *= k.delta;
+    auto colsum = [&](int col) { return (0. + spm2[0][col]) + spm2[1][col]; };
+    auto inc_num = [&](int col) {
+        double a = spm2[0][col] * vars[0][col], b = spm2[1][col] * vars[1][col];
+        return std::sqrt(0. + (a * a + b * b));
+    };
+    double beta_min = std::sqrt(colsum(0) / colsum(1));
+    double ss_denom = colsum(0);
+    double ss_norm = 1. / (2. * std::sqrt(ss_denom));
+    double ss_p = colsum(1) * ss_norm;
+    double ss_m = colsum(2) * ss_norm;
+    double inc_common = inc_num(0) / (2. * std::fabs(ss_denom));
+    double inc_ss_p = inc_num(1) / std::fabs(colsum(1)) + inc_common;
+    double inc_ss_m = inc_num(2) / std::fabs(colsum(2)) + inc_common;
+    double variance = (acc->variance - acc->sigma * acc->sigma / n_ev) / (n_ev - 1.);
+    double prec = std::sqrt(variance / n_ev) / std::fabs(acc->sigma / n_ev);
+    double sigma = acc->sigma * incident_flux;
+    out->selected_events = acc->selected_events;
+    std::memcpy(out->spm2, spm2, sizeof spm2);
+    std::memcpy(out->vars, vars, sizeof vars);
+    out->sigma = sigma;
+    out->prec = prec;
+    out->variance = variance;
+    out->beta_min = beta_min;
+    out->ss_p = ss_p;
+    out->inc_ss_p = inc_ss_p;
+    out->ss_m = ss_m;
+    out->inc_ss_m = inc_ss_m;
+    return TP_OK;
+}
+
+extern "C" int tp_format_res_data(const tp_config *cfg, const tp_final *res, char *buf, int len)
+{
+    Out o;
+    auto F = [](double x) { return write_engineering(x, SIG_DIGITS); };
+    o.kv("Nombre d'evenements", std::to_string(cfg->num_events));
+    o.kv("... apres coupure", std::to_string(res->selected_events));
+    o.kv("energie dans le CdM      (GeV)", F(cfg->e_total));
+    o.kv("coupure / cos(photon,faisceau)", F(cfg->cut_beam));
+    o.kv("coupure / cos(photon,photon)", F(cfg->cut_pp));
+    o.kv("coupure / sin(normale,faisceau)", F(cfg->cut_plane));
+    o.kv("coupure sur l'energie    (GeV)", F(cfg->e_min));
+    o.kv("1/(constante de structure fine)", F(1. / cfg->alpha));
+    o.kv("1/(structure fine au pic)", F(1. / cfg->alpha_z));
+    o.kv("facteur de conversion GeV-2/pb", F(cfg->gev2_to_picobarn));
+    o.kv("Masse du Z0              (GeV)", F(cfg->m_z0));
+    o.kv("Largeur du Z0            (GeV)", F(cfg->g_z0));
+    o.kv("Sinus^2 Theta Weinberg", F(cfg->sin2_weinberg));
+    o.kv("Taux de branchement Z--->e+e-", F(cfg->branching_ep_em));
+    o.kv("Beta plus", F(cfg->beta_plus));
+    o.kv("Beta moins", F(cfg->beta_minus));
+    o.line(" ---------------------------------------------");
+    o.kv("Section Efficace          (pb)", F(res->sigma));
+    o.kv("Ecart-Type                (pb)", F(res->sigma * res->prec));
+    o.kv("Precision Relative", F(res->prec));
+    o.line(" ---------------------------------------------");
+    o.kv("Beta minimum", F(res->beta_min));
+    o.kv("Stat. Significance  B+(pb-1/2)", F(res->ss_p));
+    o.kv("Incert. Stat. Sign. B+(pb-1/2)", F(res->ss_p * res->inc_ss_p));
+    o.kv("Stat. Significance  B-(pb-1/2)", F(res->ss_m));
+    o.kv("Incert. Stat. Sign. B-(pb-1/2)", F(res->ss_m * res->inc_ss_m));
+    o.s += "\n";
+    const int decs = 7;      // clamp_max(SIG_DIGITS - 1, 7)
+    const size_t width = 15; // decimals + 8
+    for (int sp = 0; sp < 2; ++sp) {
+        for (int k = 0; k < 5; ++k) {
+            double v = res->spm2[sp][k];
+            o.s += pad_left(std::to_string(sp + 1), 3) + pad_left(std::to_string(k + 1), 3) +
+                   pad_left(rust_exp(v, decs), width) +
+                   pad_left(rust_exp(std::fabs(v) * res->vars[sp][k], decs), width) +
+                   pad_left(rust_exp(res->vars[sp][k], decs), width) + "\n";
+        }
+        o.s += "\n";
+    }
+    for (int k = 0; k < 5; ++k) {
+        double tmp1 = (0. + res->spm2[0][k]) + res->spm2[1][k];
+        double a = res->spm2[0][k] * res->vars[0][k], b = res->spm2[1][k] * res->vars[1][k];
+        double tmp2 = std::sqrt(0. + (a * a + b * b));
+        o.s += "   " + pad_left(std::to_string(k + 1), 3) + pad_left(rust_exp(tmp1 / 4., decs), width) +
+               pad_left(rust_exp(tmp2 / 4., decs), width) +
+               pad_left(rust_exp(tmp2 / std::fabs(tmp1), decs), width) + "\n";
+    }
+    return copy_out(o.s, buf, len);
+}
+
+extern "C" int tp_format_report(const tp_config *cfg, const tp_final *res, char *buf, int len)
+{
+    Out o;
+    const double(&spm2)[2][5] = res->spm2;
+    const double(&vars)[2][5] = res->vars;
+    // FinalResults::eric (resfin.rs:66-97).
+    double mu_th = cfg->branching_ep_em * cfg->gev2_to_picobarn /
+                   (8. * 9. * 5. * std::pow(PI, 2.) * cfg->m_z0 * cfg->g_z0);
+    double sigma0[2], alpha0[2], beta0[2], lambda0[2], mu0[2];
+    for (int s = 0; s < 2; ++s) {
+        sigma0[s] = spm2[s][0] / 2.;
+        alpha0[s] = spm2[s][4] / 2.;
+        beta0[s] = -spm2[s][3] / 2.;
+        lambda0[s] = (spm2[s][2] - spm2[s][1]) / 2.;
+        mu0[s] = (spm2[s][2] + spm2[s][1]) / 2.;
+    }
+    double mu_num = ((((0. + spm2[0][1]) + spm2[1][1]) + spm2[0][2]) + spm2[1][2]) / 4.;
+    o.s += "\n";
+    o.line("       :        -          +");
+    o.line("sigma0  : " + rust_fixed(sigma0[0], 6) + " | " + rust_fixed(sigma0[1], 6));
+    o.line("alpha0  : " + rust_exp(alpha0[0], 5) + " | " + rust_exp(alpha0[1], 4));
+    o.line("beta0   : " + rust_display(beta0[0]) + " | " + rust_display(beta0[1]));
+    o.line("lambda0 : " + rust_fixed(lambda0[0], 4) + " | " + rust_fixed(lambda0[1], 4));
+    o.line("mu0     : " + rust_fixed(mu0[0], 4) + " | " + rust_fixed(mu0[1], 5));
+    o.line("mu/lamb : " + rust_fixed(mu0[0] / lambda0[0], 5) + " | " + rust_fixed(mu0[1] / lambda0[1], 5));
+    o.line("mu (num): " + rust_fixed(mu_num, 4));
+    o.line("rapport : " + rust_fixed(mu_num / mu_th, 6));
+    o.line("mu (th) : " + rust_fixed(mu_th, 4));
+    // FinalResults::fawzi (resfin.rs:101-194).
+    double e = cfg->e_total;
+    double mre = cfg->m_z0 / e;
+    double gre = cfg->g_z0 * cfg->m_z0 / (e * e);
+    double x = 1. - mre * mre;
+    double den = x * x + gre * gre;
+    double sdz_re = x / den, sdz_im = -gre / den;
+    double del = (1. - cfg->cut_pp) / 2.;
+    double eps = 2. * cfg->e_min / e;
+    double bra = cfg->m_z0 / (3. * 6. * std::pow(PI, 3.) * 16. * 120.);
+    double r = e / cfg->m_z0;
+    double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    double sig = 12. * PI / (cfg->m_z0 * cfg->m_z0) * cfg->branching_ep_em * cfg->g_z0 * bra / (e * e) * r8 *
+                 (sdz_re * sdz_re + sdz_im * sdz_im) * cfg->gev2_to_picobarn;
+    double eps_2 = eps * eps;
+    double eps_4 = eps_2 * eps_2;
+    double del_2 = del * del;
+    double del_3 = del * del_2;
+    double f1 = 1. - 15. * eps_4 - 9. / 7. * (1. - 70. * eps_4) * del_2 + 6. / 7. * (1. + 70. * eps_4) * del_3;
+    double g1 = 1. - 30. * eps_4 - 9. / 7. * (1. - 70. * eps_4) * del - 90. * eps_4 * del_2 -
+                1. / 7. * (1. - 420. * eps_4) * del_3;
+    double g2 = 1. - 25. * eps_4 - 6. / 7. * (1. - 70. * eps_4) * del - 3. / 7. * (1. + 210. * eps_4) * del_2 -
+                8. / 21. * (1. - 52.5 * eps_4) * del_3;
+    double g3 = 1. - 195. / 11. * eps_4 - 18. / 77. * (1. - 7. * eps_4) * del -
+                9. / 11. * (9. / 7. - 70. * eps_4) * del_2 - 8. / 11. * (1. - 105. / 11. * eps_4) * del_3;
+    double c = cfg->cut_plane;
+    double c3 = c * (c * c);
+    double ff = f1 * (1. - c3);
+    double gg = g1 - 27. / 16. * g2 * c + 11. / 16. * g3 * c3;
+    double sig_p = sig * (ff + 2. * gg);
+    double sig_m = sig_p + 2. * sig * gg;
+    double mc_p = ((0. + spm2[0][1]) + spm2[1][1]) / 4.;
+    double mc_m = ((0. + spm2[0][2]) + spm2[1][2]) / 4.;
+    auto incr = [&](int col) {
+        double a = spm2[0][col] * vars[0][col], b = spm2[1][col] * vars[1][col];
+        return std::sqrt(0. + (a * a + b * b)) / std::fabs((0. + spm2[0][col]) + spm2[1][col]);
+    };
+    double incr_p = incr(1), incr_m = incr(2);
+    o.s += "\n";
+    o.line("s (pb) :   Sig_cut_Th    Sig_Th      Rapport");
+    o.line("       :   Sig_Num");
+    o.line("       :   Ecart_relatif  Incertitude");
+    o.s += "\n";
+    o.line("s+(pb) : " + rust_fixed(sig_p, 5) + " | " + rust_fixed(sig * 3., 5) + " | " +
+           rust_fixed(sig_p / (3. * sig), 6));
+    o.line("       : " + rust_fixed(mc_p, 5));
+    o.line("       : " + rust_fixed(mc_p / sig_p - 1., 6) + " | " + rust_fixed(incr_p, 8) + " | " +
+           rust_fixed((mc_p / sig_p - 1.) / incr_p, 2));
+    o.s += "\n";
+    o.line("s-(pb) : " + rust_fixed(sig_m, 5) + " | " + rust_fixed(sig * 5., 4) + " | " +
+           rust_fixed(sig_m / (5. * sig), 6));
+    o.line("       : " + rust_fixed(mc_m, 5));
+    o.line("       : " + rust_fixed(mc_m / sig_m - 1., 6) + " | " + rust_fixed(incr_m, 9) + " | " +
+           rust_fixed((mc_m / sig_m - 1.) / incr_m, 2));
+    o.s += "\n";
+    return copy_out(o.s, buf, len);
+}
+
+extern "C" int tp_dump_results(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
+{
+    if (!cfg || !res)
+        return TP_ERR_ARG;
+    std::string report(static_cast<size_t>(tp_format_report(cfg, res, nullptr, 0)), '\0');
+    tp_format_report(cfg, res, &report[0], static_cast<int>(report.size()));
+    std::fputs(report.c_str(), stdout);
+    std::fflush(stdout);
+    // "[day]-[month repr:short]-[year repr:last_two] [hour]:[minute]:[second]" (output.rs:232-236), UTC.
+    std::time_t now = std::time(nullptr);
+    std::tm tmv;
+    gmtime_r(&now, &tmv);
+    char stamp[64];
+    std::strftime(stamp, sizeof stamp, "%d-%b-%y %H:%M:%S", &tmv);
+    std::string base = dir ? std::string(dir) + "/" : std::string();
+    {
+        FILE *f = std::fopen((base + "res.times").c_str(), "w");
+        if (!f)
+            return TP_ERR_IO;
+        Out o;
+        o.s += std::string(" ") + stamp + "\n";
+        o.line(" ---------------------------------------------");
+        o.kv("Temps ecoule", "???");
+        o.kv("Temps ecoule utilisateur", write_engineering(elapsed_s, SIG_DIGITS));
+        o.kv("Temps ecoule systeme", "???");
+        o.kv("Temps ecoule par evenement",
+             write_engineering(elapsed_s / static_cast<double>(cfg->num_events), SIG_DIGITS));
+        std::fputs(o.s.c_str(), f);
+        std::fclose(f);
+    }
+    {
+        std::string text(static_cast<size_t>(tp_format_res_data(cfg, res, nullptr, 0)), '\0');
+        tp_format_res_data(cfg, res, &text[0], static_cast<int>(text.size()));
+        FILE *f = std::fopen((base + "res.data").c_str(), "w");
+        if (!f)
+            return TP_ERR_IO;
+        std::fputs(text.c_str(), f);
+        std::fclose(f);
+    }
+    {
+        FILE *f = std::fopen((base + "pil.mc").c_str(), "a");
+        if (!f)
+            return TP_ERR_IO;
+        auto colsum = [&](int col) { return (0. + res->spm2[0][col]) + res->spm2[1][col]; };
+        double res1 = colsum(0);
+        double res2 = colsum(1) * (cfg->beta_plus * cfg->beta_plus);
+        double res3 = colsum(2) * (cfg->beta_minus * cfg->beta_minus);
+        double res4 = colsum(3) * cfg->beta_plus;
+        std::string l = rust_display(cfg->e_total) + " " + rust_display(res1 / 4.) + " " +
+                        rust_display(res2 / 4.) + " " + rust_display(res3 / 4.) + " " +
+                        rust_display(res4 / 4.) + " " + rust_display((res1 + res2 + res3 + res4) / 4.) +
+                        " " + rust_display(res->sigma);
+        std::fprintf(f, "%s\n%s\n", stamp, l.c_str());
+        std::fclose(f);
+    }
+    return TP_OK;
+}

@@ -1,0 +1,71 @@
+// tp_kernels.h -- host-side launch interface of the gfx950 kernels
+// (implemented in tp_kernels.hip).  Plain structs of device pointers; no
+// ownership here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tp_physics.h"
+
+namespace tpk {
+
+// Threads per batch workgroup of the fused kernel.
+constexpr int TPB = 256;
+
+// Arguments of the fused event-loop kernel.
+struct FusedArgs {
+    tpd::Params P;
+    uint64_t nb;           // batches in this launch
+    uint64_t b_first;      // global index of the first batch of this launch
+    uint64_t num_events;   // events of the whole run (for the last, partial batch)
+    uint32_t batch_size;   // events per full batch
+    int32_t precision;     // tp_precision
+    // xoshiro source: state of batch b, thread chunk start e0 is
+    //   M^(12*e0) * B^b * s0  with B = M^(12*batch_size) or the 2^128 jump.
+    uint64_t s0[4];
+    const uint64_t *mt;    // M^(2^k), k < km, each 256x4 words
+    const uint64_t *bt;    // B^(2^k), k < kb
+    const uint64_t *stride;// B^(grid size)
+    int32_t km, kb;
+    // buffer source: RANF values (int32 in [0, 1e9)) of this launch's batches,
+    // batch (b - b_first) at offset 12*batch_size*(b - b_first).
+    const int32_t *draws;
+    // output: one accumulator per batch, index b - b_first
+    void *out;             // tp_acc*
+};
+
+// Arguments of the RANF expansion kernel (one wavefront per segment of R
+// rounds).  A piece is one stream origin: the reference's master stream in
+// skip-ahead mode, or one reseeded batch stream in jump mode.
+struct RanfArgs {
+    const int32_t *origin; // [npieces][56] round X[1..55] at [1..55]
+    const int32_t *p0;     // [npieces] values of the origin round already consumed
+    uint64_t d_begin, d_end; // window of draws (piece-relative) to write
+    uint64_t seg_first;    // first segment index of the window
+    uint64_t nsegs;        // segments per piece in this launch
+    uint64_t piece_stride; // output offset between pieces
+    const uint32_t *polys; // [K][55] x^(55*R*2^k) mod (x^55 + x^31 - 1)
+    int32_t K;
+    int32_t R;             // rounds per segment
+    int32_t *out;
+};
+
+void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer*/, int grid, hipStream_t st);
+// Per-event diagnostics: one thread per event, same device code as the loop.
+// Event e draws from offset 12*e (xoshiro: a.s0 with a.mt; buffer: a.draws).
+struct EventOut {
+    double *u;     // [n][12]
+    double *p;     // [n][3][4]
+    int32_t *keep; // [n]
+    double *m;     // [n][5]
+};
+void launch_events(const FusedArgs &a, int source, uint32_t n, const EventOut &o, hipStream_t st);
+void launch_ranf_expand(const RanfArgs &a, uint64_t npieces, hipStream_t st);
+// Per-batch RANF origins of jump mode: seeded_new(seed0 + 123456*b).
+void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0,
+                      hipStream_t st);
+// In-order left fold of nb accumulators (resacc.rs:133-139) into *merged.
+void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st);
+
+} // namespace tpk

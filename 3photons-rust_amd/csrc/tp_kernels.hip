@@ -1,0 +1,483 @@
+// tp_kernels.hip -- gfx950 kernels of the 3photons event loop.
+//
+//  fused_kernel      one workgroup (4 wavefronts) per 10 000-event batch,
+//                    grid-strided over batches; each thread owns a contiguous
+//                    chunk of the batch, draws its uniforms (xoshiro256+ in
+//                    registers, or RANF values from the expanded stream),
+//                    runs RAMBO -> cuts -> spinors -> amplitudes -> weight in
+//                    registers and keeps 12 running sums; wavefront butterfly
+//                    and LDS reductions give one accumulator per batch.
+//  ranf_expand       RANF stream into HBM, one wavefront per segment of R
+//                    rounds, 55 lanes = the 55 values of a round.
+//  ranf_seed         jump-mode origins, seeded_new(seed + 123456 b) per batch.
+//  merge_kernel      in-order left fold of the batch accumulators.
+//
+// Compiled with -ffp-contract=off: the f64 path keeps the reference's
+// operation order (HadrienG2/3photons-rust src/evgen.rs, evcut.rs, spinor.rs,
+// matelems.rs, resacc.rs).
+#include "tp_kernels.h"
+
+namespace tpk {
+
+using namespace tpd;
+
+struct DevAcc { // layout of tp_acc
+    uint64_t selected;
+    double spm2[5], vars[5], sigma, variance;
+};
+
+// ---- xoshiro256+ -----------------------------------------------------------
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+__device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
+{
+    uint64_t r = s[0] + s[3];
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return r;
+}
+
+// rand's Standard f64: (u64 >> 11) * 2^-53 (exact conversion, exact scaling).
+__device__ __forceinline__ double xo_uniform(uint64_t s[4]) { return (double)(xo_next(s) >> 11) * 0x1.0p-53; }
+
+// s <- M s for a 256x256 GF(2) matrix stored as 256 columns of 4 words.  The
+// matrix address is uniform across the wavefront.
+__device__ __forceinline__ void gf2_apply(const uint64_t *__restrict__ m, uint64_t s[4])
+{
+    uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint64_t x = s[w];
+#pragma unroll 16
+        for (int b = 0; b < 64; ++b) {
+            uint64_t mask = 0ull - ((x >> b) & 1ull);
+            const uint64_t *c = m + 4 * (64 * w + b);
+            r0 ^= c[0] & mask;
+            r1 ^= c[1] & mask;
+            r2 ^= c[2] & mask;
+            r3 ^= c[3] & mask;
+        }
+    }
+    s[0] = r0;
+    s[1] = r1;
+    s[2] = r2;
+    s[3] = r3;
+}
+
+// State of (batch b, draw offset off inside the batch): M^off B^b s0.
+__device__ __forceinline__ void xo_state_at(const FusedArgs &a, uint64_t b, uint64_t off, uint64_t s[4])
+{
+    s[0] = a.s0[0];
+    s[1] = a.s0[1];
+    s[2] = a.s0[2];
+    s[3] = a.s0[3];
+    for (int k = 0; k < a.kb; ++k)
+        if ((b >> k) & 1ull) // uniform across the workgroup
+            gf2_apply(a.bt + 1024 * k, s);
+    for (int k = 0; k < a.km; ++k)
+        if ((off >> k) & 1ull) // per lane
+            gf2_apply(a.mt + 1024 * k, s);
+}
+
+// ---- per-thread accumulation (resacc.rs:121-129) ---------------------------
+struct Acc {
+    double spm2[5], vars[5], sigma, var;
+    uint32_t n;
+};
+
+__device__ __forceinline__ void acc_zero(Acc &A)
+{
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        A.spm2[c] = 0.;
+        A.vars[c] = 0.;
+    }
+    A.sigma = 0.;
+    A.var = 0.;
+    A.n = 0;
+}
+
+__device__ __forceinline__ void acc_add(Acc &A, const Params &P, const double m[5])
+{
+    A.n += 1;
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        A.spm2[c] = A.spm2[c] + m[c];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        A.vars[c] = A.vars[c] + m[c] * m[c];
+    double w = weight_f64(P, m);
+    A.sigma = A.sigma + w;
+    A.var = A.var + w * w;
+}
+
+// Kahan-compensated accumulation for the mixed-precision path.
+struct AccK {
+    double s[12], c[12];
+    uint32_t n;
+};
+
+__device__ __forceinline__ void kahan(double &s, double &c, double x)
+{
+    double y = x - c;
+    double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Fixed-shape wavefront butterfly then in-order wave combine; deterministic.
+template <int NW>
+__device__ __forceinline__ void block_reduce_store(double v[12], uint32_t n, DevAcc *out, double (*red)[13])
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            v[j] += __shfl_xor(v[j], off);
+        n += __shfl_xor(n, off);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            red[wid][j] = v[j];
+        red[wid][12] = (double)n;
+    }
+    __syncthreads();
+    if (threadIdx.x < 13) {
+        const int j = threadIdx.x;
+        double t = red[0][j];
+#pragma unroll
+        for (int w = 1; w < NW; ++w)
+            t = t + red[w][j];
+        if (j < 5)
+            out->spm2[j] = t;
+        else if (j < 10)
+            out->vars[j - 5] = t;
+        else if (j == 10)
+            out->sigma = t;
+        else if (j == 11)
+            out->variance = t;
+        else
+            out->selected = (uint64_t)t;
+    }
+    __syncthreads();
+}
+
+// ---- the fused event loop ------------------------------------------------
+template <int SOURCE>
+__global__ __launch_bounds__(TPB) void fused_f64_kernel(FusedArgs a)
+{
+    __shared__ double red[TPB / 64][13];
+    const uint32_t tid = threadIdx.x;
+    const Params &P = a.P;
+    uint64_t st[4] = {0, 0, 0, 0}; // xoshiro state at this thread's chunk start
+    bool have = false;
+    DevAcc *out = reinterpret_cast<DevAcc *>(a.out);
+    for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
+        const uint64_t b = a.b_first + lb;
+        const uint64_t start = b * a.batch_size;
+        const uint64_t left = a.num_events - start;
+        const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
+        const uint32_t q = n / TPB, r = n % TPB;
+        const uint32_t len = q + (tid < r ? 1u : 0u);
+        const uint32_t e0 = tid * q + (tid < r ? tid : r);
+        uint64_t s[4];
+        if (SOURCE == 0) {
+            if (!have || n != a.batch_size)
+                xo_state_at(a, b, 12ull * e0, st);
+            else
+                gf2_apply(a.stride, st);
+            have = true;
+            s[0] = st[0];
+            s[1] = st[1];
+            s[2] = st[2];
+            s[3] = st[3];
+        }
+        const int32_t *src = nullptr;
+        if (SOURCE == 1)
+            src = a.draws + (size_t)lb * 12 * a.batch_size + 12 * (size_t)e0;
+        Acc A;
+        acc_zero(A);
+        for (uint32_t k = 0; k < len; ++k) {
+            double u[12];
+            if (SOURCE == 0) {
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    u[j] = xo_uniform(s);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    u[j] = (double)src[12 * k + j] * 1e-9;
+            }
+            double p[3][4], m[5];
+            if (event_f64(P, u, p, m))
+                acc_add(A, P, m);
+        }
+        double v[12];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            v[c] = A.spm2[c];
+            v[5 + c] = A.vars[c];
+        }
+        v[10] = A.sigma;
+        v[11] = A.var;
+        block_reduce_store<TPB / 64>(v, A.n, out + lb, red);
+    }
+}
+
+void launch_fused(const FusedArgs &a, int source, int grid, hipStream_t st)
+{
+    if (source == 0)
+        hipLaunchKernelGGL(fused_f64_kernel<0>, dim3(grid), dim3(TPB), 0, st, a);
+    else
+        hipLaunchKernelGGL(fused_f64_kernel<1>, dim3(grid), dim3(TPB), 0, st, a);
+}
+
+template <int SOURCE> __global__ __launch_bounds__(256) void events_kernel(FusedArgs a, uint32_t n, EventOut o)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    double u[12];
+    if (SOURCE == 0) {
+        uint64_t s[4];
+        s[0] = a.s0[0];
+        s[1] = a.s0[1];
+        s[2] = a.s0[2];
+        s[3] = a.s0[3];
+        const uint64_t off = 12ull * e;
+        for (int k = 0; k < a.km; ++k)
+            if ((off >> k) & 1ull)
+                gf2_apply(a.mt + 1024 * k, s);
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            u[j] = xo_uniform(s);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            u[j] = (double)a.draws[12ull * e + j] * 1e-9;
+    }
+    double p[3][4], m[5] = {0., 0., 0., 0., 0.};
+    const bool k = event_f64(a.P, u, p, m);
+    if (!k) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            m[c] = 0.;
+    }
+    if (o.u)
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            o.u[12ull * e + j] = u[j];
+    if (o.p)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                o.p[12ull * e + 4 * i + j] = p[i][j];
+    if (o.keep)
+        o.keep[e] = k ? 1 : 0;
+    if (o.m)
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            o.m[5ull * e + c] = m[c];
+}
+
+void launch_events(const FusedArgs &a, int source, uint32_t n, const EventOut &o, hipStream_t st)
+{
+    const unsigned blocks = (n + 255) / 256;
+    if (source == 0)
+        hipLaunchKernelGGL(events_kernel<0>, dim3(blocks), dim3(256), 0, st, a, n, o);
+    else
+        hipLaunchKernelGGL(events_kernel<1>, dim3(blocks), dim3(256), 0, st, a, n, o);
+}
+
+// ---- RANF expansion ------------------------------------------------------
+constexpr int32_t MODULO = 1000000000;
+
+__device__ __forceinline__ int32_t norm_mod(int32_t v)
+{
+    v += (v < 0) ? MODULO : 0;
+    v += (v < 0) ? MODULO : 0;
+    v -= (v >= MODULO) ? MODULO : 0;
+    v -= (v >= MODULO) ? MODULO : 0;
+    return v;
+}
+
+// One IRN55 round (ranf.rs:120-135) with lane L holding X[L+1].  The three
+// dependency phases of the round are folded into closed forms in the old
+// values, so all 55 lanes update at once:
+//   i <= 24:       X[i] - X[i+31]
+//   25 <= i <= 48: X[i] - X[i-24] + X[i+7]
+//   i >= 49:       X[i] - X[i-24] + X[i-48] - X[i-17]        (all mod 1e9)
+__device__ __forceinline__ int32_t ranf_next_round(int32_t x, int32_t *win)
+{
+    const int lane = threadIdx.x;
+    if (lane < 55)
+        win[lane] = x;
+    __syncthreads();
+    int32_t v = x;
+    const int i = lane + 1;
+    if (lane < 55) {
+        if (i <= 24)
+            v = win[i - 1] - win[i + 30];
+        else if (i <= 48)
+            v = norm_mod(win[i - 1] - win[i - 25]) + win[i + 6];
+        else
+            v = norm_mod(win[i - 1] - win[i - 25]) + norm_mod(win[i - 49] - win[i - 18]);
+        v = norm_mod(v);
+    }
+    __syncthreads();
+    return v;
+}
+
+// X_{r+m} from X_r with c = x^(55 m) mod P: X_{r+m}[i] = sum_k c_k Y[i-1+k]
+// over the window Y = (X_r, X_{r+1}).
+__device__ __forceinline__ int32_t ranf_jump(int32_t x, const uint32_t *__restrict__ c, int32_t *win)
+{
+    const int lane = threadIdx.x;
+    int32_t nx = ranf_next_round(x, win);
+    if (lane < 55) {
+        win[lane] = x;
+        win[55 + lane] = nx;
+    }
+    __syncthreads();
+    uint64_t acc = 0;
+    if (lane < 55) {
+#pragma unroll 5
+        for (int k = 0; k < 55; ++k) {
+            acc += (uint64_t)c[k] * (uint64_t)(uint32_t)win[lane + k];
+            if ((k & 15) == 15)
+                acc %= (uint64_t)MODULO;
+        }
+        acc %= (uint64_t)MODULO;
+    }
+    __syncthreads();
+    return lane < 55 ? (int32_t)acc : 0;
+}
+
+__global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
+{
+    __shared__ int32_t win[112];
+    const int lane = threadIdx.x;
+    const uint64_t piece = blockIdx.x / a.nsegs;
+    const uint64_t seg = a.seg_first + blockIdx.x % a.nsegs;
+    const uint64_t p0 = (uint64_t)a.p0[piece];
+    const uint64_t q_begin = p0 + a.d_begin, q_end = p0 + a.d_end;
+    const uint64_t r_begin = seg * (uint64_t)a.R;
+    if (r_begin * 55 >= q_end)
+        return; // whole workgroup
+    int32_t x = lane < 55 ? a.origin[56 * piece + lane + 1] : 0;
+    for (int k = 0; k < a.K; ++k)
+        if ((seg >> k) & 1ull)
+            x = ranf_jump(x, a.polys + 55 * k, win);
+    int32_t *out = a.out + piece * a.piece_stride;
+    for (int j = 0; j < a.R; ++j) {
+        const uint64_t r = r_begin + j;
+        if (r * 55 >= q_end)
+            break;
+        if (lane < 55) {
+            const uint64_t qq = 55 * r + (uint64_t)(54 - lane); // X[lane+1] is emitted at 55 - (lane+1)
+            if (qq >= q_begin && qq < q_end)
+                out[qq - q_begin] = x;
+        }
+        if ((r + 1) * 55 < q_end)
+            x = ranf_next_round(x, win);
+    }
+}
+
+void launch_ranf_expand(const RanfArgs &a, uint64_t npieces, hipStream_t st)
+{
+    const uint64_t blocks = npieces * a.nsegs;
+    hipLaunchKernelGGL(ranf_expand_kernel, dim3((unsigned)blocks), dim3(64), 0, st, a);
+}
+
+// seeded_new (ranf.rs:50-82) for jump mode, one thread per batch.
+__global__ void ranf_seed_kernel(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb)
+        return;
+    const uint64_t b = b_first + i;
+    int32_t x[56];
+    const int32_t seed = (int32_t)((uint32_t)seed0 + (uint32_t)(123456ull * b));
+#pragma unroll
+    for (int k = 0; k < 56; ++k)
+        x[k] = 0;
+    x[55] = seed;
+    int32_t jj = seed, kk = 1;
+    for (int t = 1; t < 55; ++t) {
+        const int ii = (21 * t) % 55;
+        // dynamic index into a private array: stays in scratch-free registers
+        // only if unrolled; the loop is short and runs once per batch.
+        x[ii] = kk;
+        kk = (int32_t)((uint32_t)jj - (uint32_t)kk);
+        if (kk < 0)
+            kk = (int32_t)((uint32_t)kk + (uint32_t)MODULO);
+        jj = x[ii];
+    }
+    for (int rr = 0; rr < 10; ++rr) {
+        for (int t = 1; t < 25; ++t) {
+            x[t] -= x[t + 31];
+            if (x[t] < 0)
+                x[t] += MODULO;
+        }
+        for (int t = 25; t < 56; ++t) {
+            x[t] -= x[t - 24];
+            if (x[t] < 0)
+                x[t] += MODULO;
+        }
+    }
+    for (int k = 0; k < 56; ++k)
+        origin[56 * i + k] = x[k];
+    p0[i] = 0;
+}
+
+void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((nb + 63) / 64);
+    hipLaunchKernelGGL(ranf_seed_kernel, dim3(blocks), dim3(64), 0, st, seed0, b_first, nb, origin, p0);
+}
+
+// ---- in-order merge --------------------------------------------------------
+__global__ __launch_bounds__(64) void merge_kernel(const DevAcc *in, uint64_t nb, DevAcc *out)
+{
+    const int j = threadIdx.x;
+    if (j >= 13)
+        return;
+    if (j == 12) {
+        uint64_t t = in[0].selected;
+        for (uint64_t b = 1; b < nb; ++b)
+            t += in[b].selected;
+        out->selected = t;
+        return;
+    }
+    const double *base = reinterpret_cast<const double *>(in) + 1 + j; // 13 words per accumulator
+    double t = base[0];
+    uint64_t b = 1;
+    for (; b + 8 <= nb; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v[u] = base[13 * (b + u)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            t = t + v[u];
+    }
+    for (; b < nb; ++b)
+        t = t + base[13 * b];
+    reinterpret_cast<double *>(out)[1 + j] = t;
+}
+
+void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st)
+{
+    hipLaunchKernelGGL(merge_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<const DevAcc *>(per_batch), nb,
+                       reinterpret_cast<DevAcc *>(merged));
+}
+
+} // namespace tpk

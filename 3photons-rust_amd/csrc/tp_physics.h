@@ -1,0 +1,329 @@
+// tp_physics.h -- per-event path of the 3photons event loop on gfx950:
+// RAMBO (evgen), cuts (evcut), spinor products (spinor), helicity amplitudes
+// and matrix elements (matelems), per-event weight (resacc integrate).
+//
+// Everything lives in registers; there is no per-event memory traffic.  The
+// f64 path keeps the reference's operation order exactly (the translation
+// unit is compiled with -ffp-contract=off); it differs from the reference only
+// where the reference computes known zeros or exact negations:
+//   * s(j,i) is taken as -s(i,j) (exactly equal but for the sign of a zero);
+//   * the helicity sums skip the zero amplitudes (adding +-0 to a non-zero sum
+//     is exact, so the sums are unchanged);
+//   * gemv terms multiplied by the zero Y/Z beam components are dropped (they
+//     add +-0);
+//   * the beam/photon-plane cut is skipped when cut_plane <= 0 (it can never
+//     reject then).
+// Citations are to HadrienG2/3photons-rust src/ files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpd {
+
+// Per-run constants, computed on the host with the same IEEE operations.
+struct Params {
+    double e_total;
+    double cut_beam, cut_pp, e_min, cut_plane;
+    double g_a, g_bp, g_bm;
+    double sc[5];
+    double el_x, el_e;            // incoming electron X and E (evgen.rs:66-69)
+    double xx0, f0re, f0im;       // electron spinor: sqrt(E+Z), (X, Y)/sqrt(E+Z)
+    double xx1, f1re, f1im;       // positron spinor
+    double ns01re, ns01im;        // -RAC8 * s(e-, e+)
+    double nt01re, nt01im;        // -RAC8 * t(e-, e+)
+    int sort;
+    int pad;
+};
+
+template <class T> struct C {
+    T re, im;
+};
+template <class T> __device__ __forceinline__ C<T> mk(T re, T im) { return C<T>{re, im}; }
+template <class T> __device__ __forceinline__ C<T> cmul(C<T> a, C<T> b)
+{
+    return C<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// num-complex naive division (spinor.rs divides with it).
+template <class T> __device__ __forceinline__ C<T> cdiv(C<T> a, C<T> b)
+{
+    T n = b.re * b.re + b.im * b.im;
+    T re = a.re * b.re + a.im * b.im;
+    T im = a.im * b.re - a.re * b.im;
+    return C<T>{re / n, im / n};
+}
+template <class T> __device__ __forceinline__ C<T> cadd(C<T> a, C<T> b) { return C<T>{a.re + b.re, a.im + b.im}; }
+template <class T> __device__ __forceinline__ C<T> csq(C<T> z) { return cmul(z, z); }
+template <class T> __device__ __forceinline__ C<T> neg(C<T> z) { return C<T>{-z.re, -z.im}; }
+// t = -conj(s) (spinor.rs:63-66)
+template <class T> __device__ __forceinline__ C<T> tof(C<T> s) { return C<T>{-s.re, s.im}; }
+
+// ---- RAMBO: evgen.rs:174-207 (generate_raw) and 89-118 (generate) ---------
+
+// Massless momentum of one photon from (cos theta, phi, e1, e2) uniforms.
+__device__ __forceinline__ void raw_photon_f64(double uc, double uphi, double ue1, double ue2, double q[4])
+{
+    const double two_pi = 2. * 3.14159265358979323846;
+    double cos_t = 2. * uc - 1.;
+    double phi = two_pi * uphi;
+    double em = ue1 * ue2;
+    double sphi, cphi;
+    sincos(phi, &sphi, &cphi);
+    double st = sqrt(1. - cos_t * cos_t);
+    double en = -log(em + 2.2250738585072014e-308);
+    q[0] = en * (st * sphi);
+    q[1] = en * (st * cphi);
+    q[2] = en * cos_t;
+    q[3] = en;
+}
+
+// Conformal map of three massless q onto the sqrt(s) rest frame, then the
+// energy sort.  p[i] = photon i (X, Y, Z, E).
+__device__ __forceinline__ void conformal_f64(const Params &P, const double q[3][4], double p[3][4])
+{
+    double r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r[k] = (q[0][k] + q[1][k]) + q[2][k];
+    double rn2 = r[3] * r[3] - ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
+    double alpha = P.e_total / rn2;
+    double rn = sqrt(rn2);
+    double beta = 1. / (rn + r[3]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double rq = q[i][0] * r[0];
+        rq = q[i][1] * r[1] + rq;
+        rq = q[i][2] * r[2] + rq;
+        p[i][3] = alpha * (r[3] * q[i][3] - rq);
+        double b = beta * rq - q[i][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            p[i][k] = alpha * (rn * q[i][k] + b * r[k]);
+    }
+}
+
+template <class T> __device__ __forceinline__ void swap4(T a[4], T b[4])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T t = a[k];
+        a[k] = b[k];
+        b[k] = t;
+    }
+}
+
+// Sort by decreasing energy with strict '>' swaps over (0,1),(0,2),(1,2)
+// (evgen.rs:109-118).
+template <class T> __device__ __forceinline__ void sort3(T p[3][4])
+{
+    if (p[1][3] > p[0][3])
+        swap4(p[0], p[1]);
+    if (p[2][3] > p[0][3])
+        swap4(p[0], p[2]);
+    if (p[2][3] > p[1][3])
+        swap4(p[1], p[2]);
+}
+
+// ---- Cuts: evcut.rs:42-96 -------------------------------------------------
+__device__ __forceinline__ bool keep_f64(const Params &P, const double p[3][4])
+{
+    double emin;
+    if (P.sort) {
+        emin = p[2][3];
+    } else {
+        emin = p[0][3];
+        emin = (emin < p[1][3]) ? emin : p[1][3];
+        emin = (emin < p[2][3]) ? emin : p[2][3];
+    }
+    bool ok = !(emin < P.e_min);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double num = p[i][0] * P.el_x;
+        double den = p[i][3] * P.el_e;
+        ok = ok && !(fabs(num) > P.cut_beam * den);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 3; ++j) {
+            double num = (p[i][0] * p[j][0] + p[i][1] * p[j][1]) + p[i][2] * p[j][2];
+            double den = p[i][3] * p[j][3];
+            ok = ok && !(num > P.cut_pp * den);
+        }
+    if (P.cut_plane > 0.) {
+        double n0 = p[0][1] * p[1][2] - p[0][2] * p[1][1];
+        double n1 = p[0][2] * p[1][0] - p[0][0] * p[1][2];
+        double n2 = p[0][0] * p[1][1] - p[0][1] * p[1][0];
+        double num = P.el_x * n0;
+        double nn = sqrt((n0 * n0 + n1 * n1) + n2 * n2);
+        ok = ok && !(fabs(num) < P.cut_plane * (P.el_e * nn));
+    }
+    return ok;
+}
+
+// ---- Spinor products and amplitudes: spinor.rs:31-162 ----------------------
+//
+// Particle indices follow the reference: 0 = e-, 1 = e+, 2..4 = photons.
+template <class T> struct Spin {
+    C<T> s[5][5]; // only off-diagonal entries are set
+};
+
+template <class T>
+__device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T f1re, T f1im, const T p[3][4],
+                                                Spin<T> &S)
+{
+    T xx[3];
+    C<T> fx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xx[k] = sqrt(p[k][3] + p[k][2]);
+        // f32::MIN_POSITIVE / f64::MIN_POSITIVE
+        const T minpos = sizeof(T) == 8 ? (T)2.2250738585072014e-308 : (T)1.17549435e-38f;
+        if (xx[k] > minpos)
+            fx[k] = mk<T>(p[k][0] / xx[k], p[k][1] / xx[k]);
+        else
+            fx[k] = mk<T>(sqrt((T)2. * p[k][3]), (T)0.);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        C<T> s0k = mk<T>(f0re * xx[k] - fx[k].re * xx0, f0im * xx[k] - fx[k].im * xx0);
+        C<T> s1k = mk<T>(f1re * xx[k] - fx[k].re * xx1, f1im * xx[k] - fx[k].im * xx1);
+        S.s[0][2 + k] = s0k;
+        S.s[2 + k][0] = neg(s0k);
+        S.s[1][2 + k] = s1k;
+        S.s[2 + k][1] = neg(s1k);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = k + 1; l < 3; ++l) {
+            C<T> skl = mk<T>(fx[k].re * xx[l] - fx[l].re * xx[k], fx[k].im * xx[l] - fx[l].im * xx[k]);
+            S.s[2 + k][2 + l] = skl;
+            S.s[2 + l][2 + k] = neg(skl);
+        }
+}
+
+// -RAC8*x*y/z is ((-RAC8*x)*y)/z; the (-RAC8)*s(e-,e+) and (-RAC8)*t(e-,e+)
+// factors are per-run constants (ns01, nt01).
+template <class T>
+__device__ __forceinline__ C<T> a_ppm(const Spin<T> &S, C<T> ns01, int k1, int k2, int k3)
+{
+    C<T> num = cmul(ns01, csq(S.s[0][k3]));
+    C<T> den = cmul(cmul(cmul(S.s[0][k1], S.s[0][k2]), S.s[1][k1]), S.s[1][k2]);
+    return cdiv(num, den);
+}
+template <class T>
+__device__ __forceinline__ C<T> a_pmm(const Spin<T> &S, C<T> nt01, int k1, int k2, int k3)
+{
+    C<T> num = cmul(nt01, csq(tof(S.s[1][k1])));
+    C<T> den = cmul(cmul(cmul(tof(S.s[1][k2]), tof(S.s[1][k3])), tof(S.s[0][k2])), tof(S.s[0][k3]));
+    return cdiv(num, den);
+}
+template <class T>
+__device__ __forceinline__ C<T> bp_ppm(const Spin<T> &S, C<T> nt01, int k1, int k2, int k3)
+{
+    return cmul(nt01, csq(cmul(tof(S.s[k1][k2]), S.s[k3][0])));
+}
+template <class T>
+__device__ __forceinline__ C<T> bp_pmm(const Spin<T> &S, C<T> ns01, int k1, int k2, int k3)
+{
+    return cmul(ns01, csq(cmul(tof(S.s[k1][1]), S.s[k2][k3])));
+}
+template <class T> __device__ __forceinline__ C<T> bm_ppp(const Spin<T> &S, C<T> ns01)
+{
+    const int k1 = 2, k2 = 3, k3 = 4;
+    C<T> s = cadd(cadd(csq(cmul(tof(S.s[k1][k2]), tof(S.s[k3][1]))), csq(cmul(tof(S.s[k1][k3]), tof(S.s[k2][1])))),
+                  csq(cmul(tof(S.s[k2][k3]), tof(S.s[k1][1]))));
+    return cmul(ns01, s);
+}
+template <class T> __device__ __forceinline__ C<T> bm_mmm(const Spin<T> &S, C<T> nt01)
+{
+    const int k1 = 2, k2 = 3, k3 = 4;
+    C<T> s = cadd(cadd(csq(cmul(S.s[k1][0], S.s[k2][k3])), csq(cmul(S.s[k2][0], S.s[k1][k3]))),
+                  csq(cmul(S.s[k3][0], S.s[k1][k2])));
+    return cmul(nt01, s);
+}
+
+// Helicity sums of the five matrix-element contributions (matelems.rs:53-86):
+// m[0] = sum|A g_a|^2, m[1] = sum|B+ g_b+|^2, m[2] = sum|B- g_b-|^2,
+// m[3] + i m[4] = sum (2 A g_a) conj(B+ g_b+).  Helicities MMP..PPM carry A and
+// B+ (spinor.rs:72-98), MMM and PPP carry B- only (spinor.rs:100-113).
+template <class T>
+__device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> nt01, T g_a, T g_bp, T g_bm,
+                                              T m[5])
+{
+    // (A, B+) for h = MMP, MPM, MPP, PMM, PMP, PPM.
+    C<T> a[6], b[6];
+    a[0] = a_pmm(S, nt01, 4, 2, 3);
+    b[0] = bp_pmm(S, ns01, 4, 2, 3);
+    a[1] = a_pmm(S, nt01, 3, 2, 4);
+    b[1] = bp_pmm(S, ns01, 3, 2, 4);
+    a[2] = a_ppm(S, ns01, 3, 4, 2);
+    b[2] = bp_ppm(S, nt01, 3, 4, 2);
+    a[3] = a_pmm(S, nt01, 2, 3, 4);
+    b[3] = bp_pmm(S, ns01, 2, 3, 4);
+    a[4] = a_ppm(S, ns01, 2, 4, 3);
+    b[4] = bp_ppm(S, nt01, 2, 4, 3);
+    a[5] = a_ppm(S, ns01, 2, 3, 4);
+    b[5] = bp_ppm(S, nt01, 2, 3, 4);
+    T sA = 0, sP = 0, sR = 0, sI = 0;
+#pragma unroll
+    for (int h = 0; h < 6; ++h) {
+        C<T> A = mk<T>(a[h].re * g_a, a[h].im * g_a);
+        C<T> Pp = mk<T>(b[h].re * g_bp, b[h].im * g_bp);
+        C<T> X = cmul(mk<T>((T)2. * A.re, (T)2. * A.im), mk<T>(Pp.re, -Pp.im));
+        T nA = A.re * A.re + A.im * A.im;
+        T nP = Pp.re * Pp.re + Pp.im * Pp.im;
+        if (h == 0) {
+            sA = nA;
+            sP = nP;
+            sR = X.re;
+            sI = X.im;
+        } else {
+            sA = sA + nA;
+            sP = sP + nP;
+            sR = sR + X.re;
+            sI = sI + X.im;
+        }
+    }
+    C<T> bmm = bm_mmm(S, nt01);
+    C<T> bpp = bm_ppp(S, ns01);
+    C<T> Bm = mk<T>(bmm.re * g_bm, bmm.im * g_bm);
+    C<T> Bp = mk<T>(bpp.re * g_bm, bpp.im * g_bm);
+    m[0] = sA;
+    m[1] = sP;
+    m[2] = (Bm.re * Bm.re + Bm.im * Bm.im) + (Bp.re * Bp.re + Bp.im * Bp.im);
+    m[3] = sR;
+    m[4] = sI;
+}
+
+// Per-event weight: dot(m, sigma_contribs) as a left fold (resacc.rs:126).
+__device__ __forceinline__ double weight_f64(const Params &P, const double m[5])
+{
+    double w = m[0] * P.sc[0];
+    w = w + m[1] * P.sc[1];
+    w = w + m[2] * P.sc[2];
+    w = w + m[3] * P.sc[3];
+    w = w + m[4] * P.sc[4];
+    return w;
+}
+
+// Whole f64 event from 12 uniforms: returns false if cut; else m[5].
+__device__ __forceinline__ bool event_f64(const Params &P, const double u[12], double p[3][4], double m[5])
+{
+    double q[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        raw_photon_f64(u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3], q[i]);
+    conformal_f64(P, q, p);
+    if (P.sort)
+        sort3(p);
+    if (!keep_f64(P, p))
+        return false;
+    Spin<double> S;
+    spinor_products<double>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
+    helicity_sums<double>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+    return true;
+}
+
+} // namespace tpd

@@ -1,0 +1,258 @@
+// tp_rng_host.cpp -- host-side stream algebra (see tp_rng_host.h).
+#include "tp_rng_host.h"
+
+#include <cstring>
+#include <mutex>
+
+namespace tp {
+
+// ---- RANF ------------------------------------------------------------------
+
+void ranf_round(int32_t *x)
+{
+    for (int i = 1; i < 25; ++i) {
+        x[i] -= x[i + 31];
+        if (x[i] < 0)
+            x[i] += RANF_MODULO;
+    }
+    for (int i = 25; i < 56; ++i) {
+        x[i] -= x[i - 24];
+        if (x[i] < 0)
+            x[i] += RANF_MODULO;
+    }
+}
+
+void ranf_seeded(RanfState &g, int32_t seed)
+{
+    std::memset(&g, 0, sizeof g);
+    g.index = 55;
+    g.seed = seed;
+    g.numbers[55] = seed;
+    int32_t j = seed, k = 1;
+    for (int i = 1; i < 55; ++i) {
+        int ii = (21 * i) % 55;
+        g.numbers[ii] = k;
+        // i32 arithmetic wraps in the reference's release build.
+        k = static_cast<int32_t>(static_cast<uint32_t>(j) - static_cast<uint32_t>(k));
+        if (k < 0)
+            k = static_cast<int32_t>(static_cast<uint32_t>(k) + static_cast<uint32_t>(RANF_MODULO));
+        j = g.numbers[ii];
+    }
+    for (int r = 0; r < 10; ++r)
+        ranf_round(g.numbers);
+}
+
+static inline uint64_t mulmod(uint64_t a, uint64_t b) { return (a * b) % RANF_MODULO; }
+
+RanfPoly ranf_poly_mul(const RanfPoly &a, const RanfPoly &b)
+{
+    // Product of degree <= 108, then reduce with x^55 = 1 - x^31.
+    uint64_t prod[109] = {0};
+    for (int i = 0; i < 55; ++i) {
+        if (!a[i])
+            continue;
+        for (int j = 0; j < 55; ++j)
+            prod[i + j] = (prod[i + j] + mulmod(a[i], b[j])) % RANF_MODULO;
+    }
+    for (int d = 108; d >= 55; --d) {
+        uint64_t c = prod[d];
+        if (!c)
+            continue;
+        prod[d] = 0;
+        prod[d - 55] = (prod[d - 55] + c) % RANF_MODULO;
+        prod[d - 24] = (prod[d - 24] + RANF_MODULO - c) % RANF_MODULO;
+    }
+    RanfPoly r(55);
+    for (int i = 0; i < 55; ++i)
+        r[i] = static_cast<uint32_t>(prod[i]);
+    return r;
+}
+
+RanfPoly ranf_poly_xpow(uint64_t n)
+{
+    RanfPoly result(55, 0), base(55, 0);
+    result[0] = 1;
+    // base = x (or reduced higher power once n is large)
+    base[1] = 1;
+    while (n) {
+        if (n & 1)
+            result = ranf_poly_mul(result, base);
+        n >>= 1;
+        if (n)
+            base = ranf_poly_mul(base, base);
+    }
+    return result;
+}
+
+void ranf_apply_round_jump(const RanfPoly &c, const int32_t *x_in, int32_t *x_out)
+{
+    // Window Y[0..109] = X_r[1..55], X_{r+1}[1..55]; X_{r+m}[i] = sum_k c_k Y[i-1+k].
+    int32_t next[56];
+    std::memcpy(next, x_in, sizeof next);
+    ranf_round(next);
+    uint64_t y[110];
+    for (int i = 0; i < 55; ++i) {
+        y[i] = static_cast<uint64_t>(x_in[i + 1]);
+        y[55 + i] = static_cast<uint64_t>(next[i + 1]);
+    }
+    x_out[0] = 0;
+    for (int i = 0; i < 55; ++i) {
+        uint64_t acc = 0;
+        for (int k = 0; k < 55; ++k)
+            acc = (acc + mulmod(c[k], y[i + k])) % RANF_MODULO;
+        x_out[i + 1] = static_cast<int32_t>(acc);
+    }
+}
+
+void ranf_advance(RanfState &g, uint64_t n)
+{
+    if (n <= g.index) {
+        g.index -= static_cast<uint32_t>(n);
+        return;
+    }
+    uint64_t rest = n - g.index;
+    uint64_t rounds = (rest + 54) / 55;
+    RanfPoly c = ranf_poly_xpow(55 * rounds);
+    int32_t out[56];
+    ranf_apply_round_jump(c, g.numbers, out);
+    std::memcpy(g.numbers, out, sizeof out);
+    g.index = static_cast<uint32_t>(55 * rounds - rest);
+}
+
+// ---- xoshiro256+ -------------------------------------------------------------
+
+static uint64_t splitmix64(uint64_t &x)
+{
+    x += 0x9e3779b97f4a7c15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+void xoshiro_seed(uint64_t seed, uint64_t s[4])
+{
+    uint64_t x = seed;
+    for (int i = 0; i < 4; ++i)
+        s[i] = splitmix64(x);
+}
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+static inline void xo_update(uint64_t *s)
+{
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+}
+
+uint64_t xoshiro_next(uint64_t s[4])
+{
+    uint64_t r = s[0] + s[3];
+    xo_update(s);
+    return r;
+}
+
+void gf2_identity(Gf2Mat &m)
+{
+    std::memset(&m, 0, sizeof m);
+    for (int i = 0; i < 256; ++i)
+        m.col[i][i / 64] = 1ull << (i % 64);
+}
+
+void gf2_step(Gf2Mat &m)
+{
+    gf2_identity(m);
+    for (int i = 0; i < 256; ++i)
+        xo_update(m.col[i]);
+}
+
+void gf2_jump128(Gf2Mat &m)
+{
+    static const uint64_t J[4] = {0x180ec6d33cfd0abaull, 0xd5a61266f0c9392cull, 0xa9582618e03fc9aaull,
+                                  0x39abdc4529b1661cull};
+    for (int i = 0; i < 256; ++i) {
+        uint64_t s[4] = {0, 0, 0, 0};
+        s[i / 64] = 1ull << (i % 64);
+        uint64_t a[4] = {0, 0, 0, 0};
+        for (int w = 0; w < 4; ++w)
+            for (int b = 0; b < 64; ++b) {
+                if (J[w] & (1ull << b))
+                    for (int q = 0; q < 4; ++q)
+                        a[q] ^= s[q];
+                xo_update(s);
+            }
+        std::memcpy(m.col[i], a, sizeof a);
+    }
+}
+
+void gf2_apply(const Gf2Mat &m, const uint64_t in[4], uint64_t out[4])
+{
+    uint64_t r[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 4; ++w) {
+        uint64_t x = in[w];
+        while (x) {
+            int b = __builtin_ctzll(x);
+            x &= x - 1;
+            const uint64_t *c = m.col[w * 64 + b];
+            r[0] ^= c[0];
+            r[1] ^= c[1];
+            r[2] ^= c[2];
+            r[3] ^= c[3];
+        }
+    }
+    std::memcpy(out, r, sizeof r);
+}
+
+void gf2_mul(const Gf2Mat &a, const Gf2Mat &b, Gf2Mat &out)
+{
+    Gf2Mat tmp;
+    for (int i = 0; i < 256; ++i)
+        gf2_apply(a, b.col[i], tmp.col[i]);
+    out = tmp;
+}
+
+std::vector<Gf2Mat> gf2_pow2_table(const Gf2Mat &base, int count)
+{
+    std::vector<Gf2Mat> t(static_cast<size_t>(count));
+    if (count == 0)
+        return t;
+    t[0] = base;
+    for (int k = 1; k < count; ++k)
+        gf2_mul(t[k - 1], t[k - 1], t[k]);
+    return t;
+}
+
+void gf2_pow_from_table(const std::vector<Gf2Mat> &table, uint64_t n, Gf2Mat &out)
+{
+    gf2_identity(out);
+    for (size_t k = 0; n && k < table.size(); ++k, n >>= 1)
+        if (n & 1)
+            gf2_mul(table[k], out, out);
+}
+
+const std::vector<Gf2Mat> &xoshiro_step_pow2()
+{
+    static std::once_flag once;
+    static std::vector<Gf2Mat> table;
+    std::call_once(once, [] {
+        Gf2Mat m;
+        gf2_step(m);
+        table = gf2_pow2_table(m, 64);
+    });
+    return table;
+}
+
+void xoshiro_advance(uint64_t s[4], uint64_t n)
+{
+    const std::vector<Gf2Mat> &t = xoshiro_step_pow2();
+    for (int k = 0; n; ++k, n >>= 1)
+        if (n & 1)
+            gf2_apply(t[static_cast<size_t>(k)], s, s);
+}
+
+} // namespace tp

@@ -1,0 +1,64 @@
+// tp_rng_host.h -- host-side stream algebra for the engine: generator
+// initialisation, skip-ahead and jump tables.
+//
+// RANF (src/random/ranf.rs) is Knuth's subtractive lagged-Fibonacci
+// generator, X_g = X_{g-55} - X_{g-24} mod 1e9 in generation order; each
+// IRN55 round computes 55 new values in index order and the stream emits them
+// from index 55 down to 1.  The recurrence is linear over Z/1e9 with
+// characteristic polynomial x^55 + x^31 - 1, so jumping n values ahead is
+// multiplying the 55-value window by x^n mod that polynomial.
+//
+// xoshiro256+ (src/random/standard.rs via rand_xoshiro 0.6.0) has a state
+// update that is linear over GF(2); skip-ahead by n steps is the 256x256 bit
+// matrix M^n, built here by squaring and kept as 256 columns of 4 words.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace tp {
+
+// ---- RANF ------------------------------------------------------------------
+constexpr int32_t RANF_MODULO = 1000000000;
+constexpr int32_t RANF_SEED = 234612947;
+
+struct RanfState {
+    int32_t numbers[56];
+    uint32_t index;
+    int32_t seed;
+};
+
+void ranf_seeded(RanfState &g, int32_t seed);     // IN55 + 10 warm-up rounds (ranf.rs:50-82)
+void ranf_round(int32_t *numbers /*[56]*/);       // IRN55 (ranf.rs:120-135)
+
+// Polynomial modulo x^55 + x^31 - 1 with coefficients mod 1e9.
+using RanfPoly = std::vector<uint32_t>; // 55 coefficients, c[k] of x^k
+RanfPoly ranf_poly_xpow(uint64_t n);       // x^n mod P
+RanfPoly ranf_poly_mul(const RanfPoly &a, const RanfPoly &b);
+// Apply a jump polynomial (x^(55 m)) to the round X[1..55]: returns X_{r+m}.
+void ranf_apply_round_jump(const RanfPoly &c, const int32_t *x_in /*[56]*/, int32_t *x_out /*[56]*/);
+// Advance a generator state by n single draws (same effect as n random()).
+void ranf_advance(RanfState &g, uint64_t n);
+
+// ---- xoshiro256+ -------------------------------------------------------------
+void xoshiro_seed(uint64_t seed, uint64_t s[4]); // seed_from_u64 (SplitMix64 expansion)
+uint64_t xoshiro_next(uint64_t s[4]);
+
+struct Gf2Mat {
+    uint64_t col[256][4]; // column i = image of unit vector e_i
+};
+void gf2_identity(Gf2Mat &m);
+void gf2_step(Gf2Mat &m);                    // one xoshiro256+ state update
+void gf2_jump128(Gf2Mat &m);                 // the published 2^128 jump()
+void gf2_mul(const Gf2Mat &a, const Gf2Mat &b, Gf2Mat &out); // out = a * b
+void gf2_apply(const Gf2Mat &m, const uint64_t in[4], uint64_t out[4]);
+// Tables T[k] = base^(2^k), k = 0..count-1.
+std::vector<Gf2Mat> gf2_pow2_table(const Gf2Mat &base, int count);
+// base^n from a pow2 table of base.
+void gf2_pow_from_table(const std::vector<Gf2Mat> &table, uint64_t n, Gf2Mat &out);
+// M^(2^k) for the single-step matrix M, k = 0..63 (built once, cached).
+const std::vector<Gf2Mat> &xoshiro_step_pow2();
+// Advance a state by n steps with the cached tables.
+void xoshiro_advance(uint64_t s[4], uint64_t n);
+
+} // namespace tp

@@ -1,0 +1,300 @@
+"""Python binding of the MI355X engine for the 3photons event loop.
+
+Thin ctypes layer over ``build/libtp.so`` (C ABI in ``include/tp.h`` and
+``include/tp_host.h``).  It mirrors the host side of the reference
+(HadrienG2/3photons-rust): ``Configuration.load`` (src/config.rs:57-128),
+``params_from_config`` (src/coupling.rs:23-33, src/resacc.rs:59-100),
+``run_simulation`` (src/scheduling/mod.rs:31-59) and ``finalize``
+(src/resacc.rs:142-223), with the event loop itself on the GPU.
+
+There is no CPU fallback: if the HIP library is missing, importing this module
+raises; if no GPU is usable, ``Engine()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libtp.so")
+
+BATCH_SIZE = 10000
+RNG_RANF, RNG_XOSHIRO = 0, 1
+SEED_SKIPAHEAD, SEED_JUMP = 0, 1
+PREC_F64, PREC_MIXED = 0, 1
+TP_OK = 0
+
+RNG_KINDS = {"ranf": RNG_RANF, "xoshiro": RNG_XOSHIRO}
+SEEDINGS = {"skipahead": SEED_SKIPAHEAD, "jump": SEED_JUMP}
+
+
+class TpParams(C.Structure):
+    _fields_ = [("e_total", C.c_double), ("cut_beam", C.c_double), ("cut_pp", C.c_double),
+                ("e_min", C.c_double), ("cut_plane", C.c_double), ("g_a", C.c_double),
+                ("g_beta_p", C.c_double), ("g_beta_m", C.c_double), ("sigma_contribs", C.c_double * 5),
+                ("sort_photons", C.c_int32), ("precision", C.c_int32)]
+
+
+class TpAcc(C.Structure):
+    _fields_ = [("selected_events", C.c_uint64), ("spm2", C.c_double * 5), ("vars", C.c_double * 5),
+                ("sigma", C.c_double), ("variance", C.c_double)]
+
+    def as_array(self) -> np.ndarray:
+        """[selected, spm2*5, vars*5, sigma, variance] as float64."""
+        return np.array([self.selected_events, *self.spm2, *self.vars, self.sigma, self.variance],
+                        dtype=np.float64)
+
+
+ACC_DTYPE = np.dtype([("selected_events", np.uint64), ("spm2", np.float64, 5), ("vars", np.float64, 5),
+                      ("sigma", np.float64), ("variance", np.float64)])
+assert ACC_DTYPE.itemsize == C.sizeof(TpAcc) == 104
+
+
+class RanfState(C.Structure):
+    _fields_ = [("numbers", C.c_int32 * 56), ("index", C.c_uint32), ("seed", C.c_int32)]
+
+
+class XoshiroState(C.Structure):
+    _fields_ = [("s", C.c_uint64 * 4)]
+
+
+class TpStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("kernel_ms", C.c_double), ("events", C.c_uint64),
+                ("batches", C.c_uint64)]
+
+
+class TpConfig(C.Structure):
+    _fields_ = [("num_events", C.c_uint64)] + [
+        (n, C.c_double) for n in ("e_total", "cut_beam", "cut_pp", "e_min", "cut_plane", "alpha", "alpha_z",
+                                  "gev2_to_picobarn", "m_z0", "g_z0", "sin2_weinberg", "branching_ep_em",
+                                  "beta_plus", "beta_minus")] + [
+        (n, C.c_int32) for n in ("num_bins", "impr", "plot", "parsed")]
+
+
+class TpFinal(C.Structure):
+    _fields_ = [("selected_events", C.c_uint64), ("spm2", (C.c_double * 5) * 2), ("vars", (C.c_double * 5) * 2)] + [
+        (n, C.c_double) for n in ("sigma", "prec", "variance", "beta_min", "ss_p", "inc_ss_p", "ss_m", "inc_ss_m")]
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libtp.so not built at {LIB_PATH}; run __graft_entry__.build() or make -C 3photons-rust_amd")
+    lib = C.CDLL(LIB_PATH)
+    vp, P = C.c_void_p, C.POINTER
+    sig = {
+        "tp_create": (C.c_int, [C.c_int, P(vp)]),
+        "tp_destroy": (C.c_int, [vp]),
+        "tp_last_error": (C.c_char_p, [vp]),
+        "tp_rng_init": (C.c_int, [C.c_int, vp]),
+        "tp_rng_seed": (C.c_int, [C.c_int, C.c_uint64, vp]),
+        "tp_rng_skip": (C.c_int, [C.c_int, vp, C.c_uint64]),
+        "tp_rng_jump": (C.c_int, [C.c_int, vp]),
+        "tp_simulate_batch": (C.c_int, [vp, P(TpParams), C.c_int, vp, C.c_uint32, P(TpAcc)]),
+        "tp_run": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, P(TpAcc), P(TpStats)]),
+        "tp_run_device": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, vp, P(TpStats)]),
+        "tp_events": (C.c_int, [vp, P(TpParams), C.c_int, vp, C.c_uint32, vp, vp, vp, vp]),
+        "tp_sync": (C.c_int, [vp, P(TpStats)]),
+        "tp_config_load": (C.c_int, [C.c_char_p, P(TpConfig), C.c_char_p, C.c_int]),
+        "tp_config_format": (C.c_int, [P(TpConfig), C.c_char_p, C.c_int]),
+        "tp_params_from_config": (C.c_int, [P(TpConfig), P(TpParams)]),
+        "tp_acc_merge": (None, [P(TpAcc), P(TpAcc)]),
+        "tp_finalize": (C.c_int, [P(TpConfig), P(TpAcc), P(TpFinal)]),
+        "tp_format_res_data": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
+        "tp_format_report": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
+        "tp_dump_results": (C.c_int, [P(TpConfig), P(TpFinal), C.c_double, C.c_char_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class TpError(RuntimeError):
+    pass
+
+
+# ---- host side --------------------------------------------------------------
+
+class Configuration:
+    """The 18 items of `valeurs` (config.rs:8-53)."""
+
+    def __init__(self, raw: TpConfig):
+        self.raw = raw
+
+    @classmethod
+    def load(cls, path: str) -> "Configuration":
+        raw = TpConfig()
+        err = C.create_string_buffer(1024)
+        rc = lib.tp_config_load(path.encode(), C.byref(raw), err, len(err))
+        if rc != TP_OK:
+            raise TpError(err.value.decode())
+        return cls(raw)
+
+    def __getattr__(self, name):
+        return getattr(self.raw, name)
+
+    def with_events(self, n: int) -> "Configuration":
+        raw = TpConfig.from_buffer_copy(self.raw)
+        raw.num_events = n
+        return Configuration(raw)
+
+    def display(self) -> str:
+        return _text(lib.tp_config_format, C.byref(self.raw))
+
+
+def _text(fn, *args) -> str:
+    n = fn(*args, None, 0)
+    buf = C.create_string_buffer(n)
+    fn(*args, buf, n)
+    return buf.value.decode()
+
+
+def params_from_config(cfg: Configuration, sort: bool = True, precision: int = PREC_F64) -> TpParams:
+    p = TpParams()
+    rc = lib.tp_params_from_config(C.byref(cfg.raw), C.byref(p))
+    if rc != TP_OK:
+        raise TpError(f"tp_params_from_config: {rc}")
+    p.sort_photons = 1 if sort else 0
+    p.precision = precision
+    return p
+
+
+def rng_state(kind: int, seed: int | None = None):
+    st = RanfState() if kind == RNG_RANF else XoshiroState()
+    rc = lib.tp_rng_init(kind, C.byref(st)) if seed is None else lib.tp_rng_seed(kind, seed, C.byref(st))
+    if rc != TP_OK:
+        raise TpError(f"rng init: {rc}")
+    return st
+
+
+def rng_skip(kind: int, st, n: int) -> None:
+    if lib.tp_rng_skip(kind, C.byref(st), n) != TP_OK:
+        raise TpError("tp_rng_skip")
+
+
+def rng_jump(kind: int, st) -> None:
+    if lib.tp_rng_jump(kind, C.byref(st)) != TP_OK:
+        raise TpError("tp_rng_jump")
+
+
+def merge(accs) -> TpAcc:
+    """Left fold in batch order (resacc.rs:133-139)."""
+    out = TpAcc.from_buffer_copy(accs[0])
+    for a in accs[1:]:
+        lib.tp_acc_merge(C.byref(out), C.byref(a))
+    return out
+
+
+def finalize(cfg: Configuration, acc: TpAcc) -> TpFinal:
+    f = TpFinal()
+    if lib.tp_finalize(C.byref(cfg.raw), C.byref(acc), C.byref(f)) != TP_OK:
+        raise TpError("tp_finalize")
+    return f
+
+
+def format_res_data(cfg: Configuration, res: TpFinal) -> str:
+    return _text(lib.tp_format_res_data, C.byref(cfg.raw), C.byref(res))
+
+
+def format_report(cfg: Configuration, res: TpFinal) -> str:
+    return _text(lib.tp_format_report, C.byref(cfg.raw), C.byref(res))
+
+
+# ---- engine -------------------------------------------------------------------
+
+class Engine:
+    """One tp context: one GPU, one stream (tp_create / tp_destroy)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib.tp_create(device, C.byref(h))
+        if rc != TP_OK:
+            raise TpError(f"tp_create(device={device}) failed with status {rc}: no usable GPU")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.tp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != TP_OK:
+            raise TpError(f"{what}: status {rc}: {lib.tp_last_error(self.h).decode()}")
+
+    def simulate_batch(self, params: TpParams, kind: int, st, n: int) -> TpAcc:
+        """One call of the reference's per-batch closure; advances st."""
+        out = TpAcc()
+        self._check(lib.tp_simulate_batch(self.h, C.byref(params), kind, C.byref(st), n, C.byref(out)),
+                    "tp_simulate_batch")
+        return out
+
+    def run(self, params: TpParams, kind: int, seeding: int, st, num_events: int):
+        """Whole run: (per-batch accumulators as a structured array, merged TpAcc, TpStats)."""
+        nb = (num_events + BATCH_SIZE - 1) // BATCH_SIZE
+        per = np.zeros(nb, dtype=ACC_DTYPE)
+        merged, stats = TpAcc(), TpStats()
+        self._check(lib.tp_run(self.h, C.byref(params), kind, seeding, C.byref(st), num_events,
+                               per.ctypes.data_as(C.c_void_p), C.byref(merged), C.byref(stats)), "tp_run")
+        return per, merged, stats
+
+    def run_device(self, params: TpParams, kind: int, seeding: int, st, num_events: int,
+                   per_batch_ptr: int, merged_ptr: int | None) -> None:
+        self._check(lib.tp_run_device(self.h, C.byref(params), kind, seeding, C.byref(st), num_events,
+                                      C.c_void_p(per_batch_ptr), C.c_void_p(merged_ptr or 0), None),
+                    "tp_run_device")
+
+    def sync(self) -> TpStats:
+        s = TpStats()
+        self._check(lib.tp_sync(self.h, C.byref(s)), "tp_sync")
+        return s
+
+    def events(self, params: TpParams, kind: int, st, n: int):
+        """Per-event diagnostics: (u[n,12], p[n,3,4], keep[n], m[n,5])."""
+        u = np.zeros((n, 12)); p = np.zeros((n, 3, 4)); k = np.zeros(n, dtype=np.int32); m = np.zeros((n, 5))
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(lib.tp_events(self.h, C.byref(params), kind, C.byref(st), n, ptr(u), ptr(p), ptr(k), ptr(m)),
+                    "tp_events")
+        return u, p, k, m
+
+
+@dataclass
+class RunResult:
+    per_batch: np.ndarray
+    merged: TpAcc
+    final: TpFinal
+    stats: TpStats
+
+
+def run_simulation(cfg: Configuration, rng: str = "ranf", seeding: str = "skipahead", sort: bool = True,
+                   precision: int = PREC_F64, engine: Engine | None = None) -> RunResult:
+    """scheduling::run_simulation (scheduling/mod.rs:31-59) on the GPU."""
+    kind = RNG_KINDS[rng]
+    params = params_from_config(cfg, sort=sort, precision=precision)
+    st = rng_state(kind)
+    own = engine is None
+    eng = engine or Engine()
+    try:
+        per, merged, stats = eng.run(params, kind, SEEDINGS[seeding], st, cfg.num_events)
+    finally:
+        if own:
+            eng.close()
+    return RunResult(per, merged, finalize(cfg, merged), stats)

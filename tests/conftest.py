@@ -1,0 +1,68 @@
+"""Shared test setup: markers, import paths, build-on-demand, cached runs."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "3photons-rust_amd")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+for p in (PKG, os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+def _ensure_built():
+    """Build the oracle and the engine if nothing is built yet."""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "build", "liboracle.so")):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
+    if not os.path.exists(os.path.join(PKG, "build", "libtp.so")):
+        subprocess.run(["make", "-C", PKG, "-j8"], check=True, capture_output=True)
+
+
+_ensure_built()
+
+
+def golden_path(name: str) -> str:
+    return os.path.join(GOLDEN, name)
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle_ffi import Oracle
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def oracle32():
+    from oracle_ffi import Oracle
+    return Oracle(f32=True)
+
+
+@pytest.fixture(scope="session")
+def valeurs():
+    return golden_path("valeurs")
+
+
+_RUN_CACHE: dict = {}
+
+
+@pytest.fixture(scope="session")
+def oracle_run(oracle, valeurs):
+    """Cached full oracle runs at the shipped configuration (N = 1e7)."""
+    def run(rng="ranf", faster_threading=False, sort=True, faster_evgen=False, n=None, per_batch=False):
+        key = (rng, faster_threading, sort, faster_evgen, n, per_batch)
+        if key not in _RUN_CACHE:
+            cfg = oracle.config(valeurs, n)
+            opts = oracle.opts(rng, faster_threading, sort, threads=os.cpu_count() or 8, faster_evgen=faster_evgen)
+            merged, per = oracle.run(cfg, opts, per_batch)
+            _RUN_CACHE[key] = (cfg, merged, per, oracle.finalize(cfg, merged))
+        return _RUN_CACHE[key]
+    return run

@@ -1,0 +1,160 @@
+"""GPU parity: the HIP event loop (through the C ABI) against the oracle on
+the same streams, and against the reference's shipped outputs at N = 1e7."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import trois_photons as tp
+from conftest import PKG, golden_path
+from numdiff import compare_texts
+from test_host import to_tp_acc
+
+pytestmark = pytest.mark.gpu
+
+RNG = {"ranf": tp.RNG_RANF, "xoshiro": tp.RNG_XOSHIRO}
+
+
+@pytest.fixture(scope="module")
+def engine():
+    print("creating engine on cuda:0", flush=True)
+    with tp.Engine(0) as e:
+        yield e
+
+
+@pytest.fixture(scope="module")
+def cfg(valeurs):
+    return tp.Configuration.load(valeurs)
+
+
+def ulps(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.spacing(np.maximum(np.abs(a), np.abs(b)))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(a == b, 0.0, np.abs(a - b) / scale)
+
+
+@pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
+@pytest.mark.parametrize("sort", [True, False])
+def test_events_match_oracle(engine, cfg, oracle, valeurs, rng, sort):
+    n = 20000
+    print(f"events rng={rng} sort={sort} n={n}", flush=True)
+    params = tp.params_from_config(cfg, sort=sort)
+    st = tp.rng_state(RNG[rng])
+    tp.rng_skip(RNG[rng], st, 12 * 777)  # start mid-stream
+    u, p, keep, m = engine.events(params, RNG[rng], st, n)
+    g = oracle.rng(rng)
+    oracle.skip(g, 12 * 777)
+    ou = oracle.uniforms(g, 12 * n).reshape(n, 12)
+    assert np.array_equal(u, ou), "uniform stream differs"
+    op, okeep, _, om, _ = oracle.events(oracle.config(valeurs), oracle.opts(rng, sort=sort), ou)
+    assert np.array_equal(keep.astype(bool), okeep), f"{np.sum(keep.astype(bool) != okeep)} cut decisions differ"
+    # momenta: ulps measured against the event's energy scale (components can be ~0)
+    ph = op[:, 2:, :]
+    escale = np.abs(ph[:, :, 3]).max(axis=1)[:, None, None]
+    dp = np.abs(p - ph) / np.spacing(escale)
+    print(f"momenta ulp(E): max {dp.max():.1f} p99 {np.percentile(dp, 99):.1f} exact {np.mean(dp == 0):.3f}")
+    assert dp.max() <= 64
+    k = okeep
+    mk, omk = m[k], om[k]
+    rel = np.abs(mk[:, :3] - omk[:, :3]) / np.abs(omk[:, :3])
+    scale_x = 2 * np.sqrt(omk[:, 0] * omk[:, 1])
+    relx = np.abs(mk[:, 3:] - omk[:, 3:]) / scale_x[:, None]
+    print(f"m rel: max {rel.max():.2e} mixed {relx.max():.2e}; exact fraction {np.mean(mk == omk):.3f}")
+    assert rel.max() < 1e-11 and relx.max() < 1e-11
+    assert np.all(m[~k] == 0)
+
+
+def acc_close(a, b, n_events):
+    """selected exact; sums within accumulated-rounding bounds."""
+    a, b = np.asarray(a), np.asarray(b)
+    assert a[0] == b[0], f"selected {a[0]} != {b[0]}"
+    vars_ = b[6:11]
+    # |sum| <= sqrt(n * sum x^2): bound rounding differences by that scale
+    scale_m = np.sqrt(max(n_events, 1) * np.abs(vars_)) + 1e-300
+    assert np.all(np.abs(a[1:6] - b[1:6]) <= 1e-12 * scale_m), (a[1:6], b[1:6])
+    assert np.all(np.abs(a[6:11] - b[6:11]) <= 1e-12 * np.abs(b[6:11]) + 1e-300)
+    ss = np.sqrt(max(n_events, 1) * abs(b[12])) + 1e-300
+    assert abs(a[11] - b[11]) <= 1e-12 * ss
+    assert abs(a[12] - b[12]) <= 1e-12 * abs(b[12]) + 1e-300
+
+
+@pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
+@pytest.mark.parametrize("jump", [False, True])
+@pytest.mark.parametrize("n_events", [1, 9999, 35017])
+def test_run_per_batch_matches_oracle(engine, cfg, oracle, valeurs, rng, jump, n_events):
+    print(f"run rng={rng} jump={jump} N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c)
+    st = tp.rng_state(RNG[rng])
+    per, merged, _ = engine.run(params, RNG[rng], tp.SEED_JUMP if jump else tp.SEED_SKIPAHEAD, st, n_events)
+    ocfg = oracle.config(valeurs, n_events)
+    om, oper = oracle.run(ocfg, oracle.opts(rng, faster_threading=jump, threads=4), per_batch=True)
+    nb = (n_events + 9999) // 10000
+    for b in range(nb):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]], np.float64)
+        acc_close(a, oper[b].as_array(), min(10000, n_events - 10000 * b))
+    acc_close(merged.as_array(), om.as_array(), n_events)
+
+
+@pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
+def test_simulate_batch_is_the_closure(engine, cfg, oracle, valeurs, rng):
+    """Successive calls consume the stream like main.rs:103-128 with &mut rng."""
+    params = tp.params_from_config(cfg)
+    st = tp.rng_state(RNG[rng])
+    g = oracle.rng(rng)
+    ocfg = oracle.config(valeurs)
+    opts = oracle.opts(rng)
+    for n in (10000, 7, 0, 1, 23456):
+        print(f"simulate_batch rng={rng} n={n}", flush=True)
+        acc = engine.simulate_batch(params, RNG[rng], st, n)
+        oacc = oracle.simulate_batch(ocfg, opts, g, n)
+        acc_close(acc.as_array(), oacc.as_array(), n)
+        if rng == "xoshiro":
+            assert list(st.s) == list(g.xs)
+        else:
+            assert st.index == g.index and list(st.numbers)[1:] == list(g.numbers)[1:]
+
+
+GOLDEN = [("features_", "ranf", "skipahead", True), ("features_standard-random", "xoshiro", "skipahead", True),
+          ("features_multi-threading,faster-threading", "ranf", "jump", True),
+          ("features_standard-random,multi-threading,faster-threading", "xoshiro", "jump", True),
+          ("features_no-photon-sorting", "ranf", "skipahead", False)]
+
+
+@pytest.mark.parametrize("suffix,rng,seeding,sort", GOLDEN)
+def test_full_run_reproduces_golden(engine, cfg, oracle_run, suffix, rng, seeding, sort):
+    """N = 1e7 with the shipped valeurs: res.data numerically equal to the
+    reference's file at its printed precision, finals within 1e-12 of the
+    oracle's."""
+    print(f"golden {suffix}", flush=True)
+    res = tp.run_simulation(cfg, rng=rng, seeding=seeding, sort=sort, engine=engine)
+    ocfg, omerged, _, ofin = oracle_run(rng, seeding == "jump", sort)
+    assert res.merged.selected_events == omerged.selected
+    for f in ("sigma", "prec", "variance", "beta_min", "ss_p", "inc_ss_p", "ss_m", "inc_ss_m"):
+        a, b = getattr(res.final, f), getattr(ofin, f)
+        assert abs(a - b) <= 1e-12 * abs(b), (f, a, b)
+    sp, osp = np.array(res.final.spm2), np.array(ofin.spm2)
+    ok = np.abs(sp - osp) <= 1e-12 * np.abs(osp)
+    ok[:, 4] = np.abs(sp[:, 4] - osp[:, 4]) <= 1e-9 * np.abs(osp[:, 4])  # I_MX cancels (printed rel. error ~3)
+    assert ok.all(), (sp, osp)
+    text = tp.format_res_data(cfg, res.final)
+    golden = open(golden_path("res.data-" + suffix)).read()
+    errs = compare_texts(golden, text, digits_slack=1.0)
+    assert errs == [], errs
+
+
+def test_cli_end_to_end(tmp_path, valeurs):
+    exe = os.path.join(PKG, "build", "trois_photons")
+    shutil.copy(valeurs, tmp_path / "valeurs")
+    print("cli run", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    golden = open(golden_path("res.data-features_")).read()
+    assert compare_texts(golden, (tmp_path / "res.data").read_text(), digits_slack=1.0) == []
+    assert compare_texts(open(golden_path("stdout.log-features_")).read(), r.stdout, digits_slack=1.0) == []
+    assert (tmp_path / "res.times").exists() and (tmp_path / "pil.mc").exists()
