@@ -31,6 +31,8 @@ struct Tables { // device copies of GF(2) jump tables for one (seeding, batch si
 
 struct Ctx {
     int device = 0;
+    int cus = 1;                    // compute units of the device
+    int blocks_per_cu[2] = {1, 1};  // resident fused workgroups per CU, per source
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
@@ -220,11 +222,12 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
     return TP_OK;
 }
 
-int grid_for(uint64_t nb)
+// Persistent grid: as many batch workgroups as can be resident at once (CUs x
+// occupancy), each striding over batches; never more than there are batches.
+int grid_for(const Ctx *c, int source, uint64_t nb)
 {
-    // 256 CUs x 8 workgroups of 4 wavefronts: 8 wavefronts per SIMD.
-    const uint64_t g = 2048;
-    return static_cast<int>(std::min<uint64_t>(nb, g));
+    const uint64_t g = static_cast<uint64_t>(c->cus) * static_cast<uint64_t>(c->blocks_per_cu[source]);
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(nb, g)));
 }
 
 // Enqueue a whole run (or one closure-sized batch when batch_size == n).
@@ -246,7 +249,7 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
     if (rng_kind == TP_RNG_XOSHIRO) {
         const tp_xoshiro_state *x = static_cast<const tp_xoshiro_state *>(state0);
         std::memcpy(a.s0, x->s, sizeof a.s0);
-        const int grid = grid_for(nb);
+        const int grid = grid_for(c, 0, nb);
         Tables *t = nullptr;
         rc = ensure_tables(c, seeding, batch_size, nb, grid, &t);
         if (rc)
@@ -347,7 +350,7 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
             HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
             first_kernel = false;
         }
-        tpk::launch_fused(a, 1, grid_for(cnb), c->stream);
+        tpk::launch_fused(a, 1, grid_for(c, 1, cnb), c->stream);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -387,6 +390,11 @@ extern "C" int tp_create(int device, void **ctx_out)
             delete c;
             return TP_ERR_HIP;
         }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cus = prop.multiProcessorCount;
+    c->blocks_per_cu[0] = tpk::fused_blocks_per_cu(0);
+    c->blocks_per_cu[1] = tpk::fused_blocks_per_cu(1);
     *ctx_out = c;
     return TP_OK;
 }

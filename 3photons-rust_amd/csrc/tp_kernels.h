@@ -10,8 +10,6 @@
 
 namespace tpk {
 
-// Threads per batch workgroup of the fused kernel.
-constexpr int TPB = 256;
 
 // Arguments of the fused event-loop kernel.
 struct FusedArgs {
@@ -52,6 +50,7 @@ struct RanfArgs {
 };
 
 void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer*/, int grid, hipStream_t st);
+int fused_blocks_per_cu(int source); // resident workgroups per CU (occupancy)
 // Per-event diagnostics: one thread per event, same device code as the loop.
 // Event e draws from offset 12*e (xoshiro: a.s0 with a.mt; buffer: a.draws).
 struct EventOut {

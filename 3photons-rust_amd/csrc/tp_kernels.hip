@@ -1,12 +1,12 @@
 // tp_kernels.hip -- gfx950 kernels of the 3photons event loop.
 //
-//  fused_kernel      one workgroup (4 wavefronts) per 10 000-event batch,
-//                    grid-strided over batches; each thread owns a contiguous
-//                    chunk of the batch, draws its uniforms (xoshiro256+ in
-//                    registers, or RANF values from the expanded stream),
-//                    runs RAMBO -> cuts -> spinors -> amplitudes -> weight in
-//                    registers and keeps 12 running sums; wavefront butterfly
-//                    and LDS reductions give one accumulator per batch.
+//  fused_kernel      one wavefront per 10 000-event batch, grid-strided over
+//                    batches; uniforms from xoshiro256+ in registers or from
+//                    the expanded RANF stream, RAMBO -> cuts in registers,
+//                    passing events compacted through an LDS queue, spinors
+//                    -> amplitudes -> weight on full wavefronts, 12 running
+//                    sums per lane, butterfly reduction to one accumulator
+//                    per batch.
 //  ranf_expand       RANF stream into HBM, one wavefront per segment of R
 //                    rounds, 55 lanes = the 55 values of a round.
 //  ranf_seed         jump-mode origins, seeded_new(seed + 123456 b) per batch.
@@ -43,20 +43,32 @@ __device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
 }
 
 // rand's Standard f64: (u64 >> 11) * 2^-53 (exact conversion, exact scaling).
-__device__ __forceinline__ double xo_uniform(uint64_t s[4]) { return (double)(xo_next(s) >> 11) * 0x1.0p-53; }
+// Evaluated as hi * 2^-32 + (lo >> 11) * 2^-53: both terms and their sum are
+// exact, so the fma returns the same double as the reference's conversion.
+__device__ __forceinline__ double xo_uniform(uint64_t s[4])
+{
+    const uint64_t x = xo_next(s);
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    return fma((double)hi, 0x1.0p-32, (double)(lo >> 11) * 0x1.0p-53);
+}
 
 // s <- M s for a 256x256 GF(2) matrix stored as 256 columns of 4 words.  The
 // matrix address is uniform across the wavefront.
-__device__ __forceinline__ void gf2_apply(const uint64_t *__restrict__ m, uint64_t s[4])
+// Read through the constant address space so that the uniform columns come in
+// with scalar loads into SGPRs instead of 64 identical vector loads.
+typedef const __attribute__((address_space(4))) uint64_t cu64;
+
+__device__ __forceinline__ void gf2_apply(const uint64_t *__restrict__ mg, uint64_t s[4])
 {
+    cu64 *m = (cu64 *)mg;
     uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
         uint64_t x = s[w];
-#pragma unroll 16
+#pragma unroll 4
         for (int b = 0; b < 64; ++b) {
             uint64_t mask = 0ull - ((x >> b) & 1ull);
-            const uint64_t *c = m + 4 * (64 * w + b);
+            cu64 *c = m + 4 * (64 * w + b);
             r0 ^= c[0] & mask;
             r1 ^= c[1] & mask;
             r2 ^= c[2] & mask;
@@ -170,13 +182,30 @@ __device__ __forceinline__ void block_reduce_store(double v[12], uint32_t n, Dev
 }
 
 // ---- the fused event loop ------------------------------------------------
+//
+// One wavefront per batch.  Each iteration every lane generates one event
+// (uniforms -> RAMBO -> sort -> cuts); the photon momenta of the events that
+// pass are appended to a queue in LDS (ballot + prefix count), and whenever 64
+// are queued the whole wavefront takes them and evaluates spinors, amplitudes
+// and weights on full lanes.  Cut events therefore cost no amplitude work, and
+// the amplitude code never runs with masked-off lanes except for the final
+// partial flush of a batch.  The accumulation order is the queue order, a
+// function of the data only, so results are reproducible run to run.
+//
+// Event assignment inside the batch: with the in-register xoshiro stream each
+// lane owns a contiguous chunk (one skip-ahead per chunk); with a stream read
+// from HBM lane l takes events l, l+64, ... (coalesced loads).
+constexpr int QCAP = 128;
+
 template <int SOURCE>
-__global__ __launch_bounds__(TPB) void fused_f64_kernel(FusedArgs a)
+__global__ __launch_bounds__(64) void fused_f64_kernel(FusedArgs a)
 {
-    __shared__ double red[TPB / 64][13];
-    const uint32_t tid = threadIdx.x;
+    __shared__ double queue[12][QCAP]; // photon momenta, structure of arrays
+    __shared__ double red[1][13];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t lanemask_lt = (1ull << lane) - 1ull;
     const Params &P = a.P;
-    uint64_t st[4] = {0, 0, 0, 0}; // xoshiro state at this thread's chunk start
+    uint64_t st[4] = {0, 0, 0, 0}; // xoshiro state at this lane's chunk start
     bool have = false;
     DevAcc *out = reinterpret_cast<DevAcc *>(a.out);
     for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
@@ -184,10 +213,17 @@ __global__ __launch_bounds__(TPB) void fused_f64_kernel(FusedArgs a)
         const uint64_t start = b * a.batch_size;
         const uint64_t left = a.num_events - start;
         const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
-        const uint32_t q = n / TPB, r = n % TPB;
-        const uint32_t len = q + (tid < r ? 1u : 0u);
-        const uint32_t e0 = tid * q + (tid < r ? tid : r);
-        uint64_t s[4];
+        const uint32_t q = n / 64, r = n % 64;
+        const uint32_t iters = q + (r ? 1u : 0u);
+        uint32_t len, e0;
+        if (SOURCE == 0) {
+            len = q + (lane < r ? 1u : 0u);
+            e0 = lane * q + (lane < r ? lane : r);
+        } else {
+            len = q + (lane < r ? 1u : 0u); // events lane, lane+64, ...
+            e0 = lane;
+        }
+        uint64_t s[4] = {0, 0, 0, 0};
         if (SOURCE == 0) {
             if (!have || n != a.batch_size)
                 xo_state_at(a, b, 12ull * e0, st);
@@ -199,26 +235,66 @@ __global__ __launch_bounds__(TPB) void fused_f64_kernel(FusedArgs a)
             s[2] = st[2];
             s[3] = st[3];
         }
-        const int32_t *src = nullptr;
-        if (SOURCE == 1)
-            src = a.draws + (size_t)lb * 12 * a.batch_size + 12 * (size_t)e0;
+        const int32_t *src = (SOURCE == 1) ? a.draws + (size_t)lb * 12 * a.batch_size : nullptr;
         Acc A;
         acc_zero(A);
-        for (uint32_t k = 0; k < len; ++k) {
-            double u[12];
-            if (SOURCE == 0) {
+        uint32_t qn = 0; // queued events (uniform)
+        for (uint32_t k = 0; k < iters; ++k) {
+            bool pass = false;
+            double p[3][4];
+            if (k < len) {
+                double u[12];
+                if (SOURCE == 0) {
 #pragma unroll
-                for (int j = 0; j < 12; ++j)
-                    u[j] = xo_uniform(s);
-            } else {
+                    for (int j = 0; j < 12; ++j)
+                        u[j] = xo_uniform(s);
+                } else {
+                    const int32_t *e = src + 12 * (size_t)(k * 64 + lane);
 #pragma unroll
-                for (int j = 0; j < 12; ++j)
-                    u[j] = (double)src[12 * k + j] * 1e-9;
+                    for (int j = 0; j < 12; ++j)
+                        u[j] = (double)e[j] * 1e-9;
+                }
+                pass = generate_keep_f64(P, u, p);
             }
-            double p[3][4], m[5];
-            if (event_f64(P, u, p, m))
+            const uint64_t mask = __ballot(pass);
+            if (pass) {
+                const uint32_t slot = qn + (uint32_t)__popcll(mask & lanemask_lt);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        queue[4 * i + j][slot] = p[i][j];
+            }
+            qn += (uint32_t)__popcll(mask);
+            if (qn >= 64) {
+                __syncthreads();
+                const uint32_t slot = qn - 64 + lane;
+                double pq[3][4], m[5];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        pq[i][j] = queue[4 * i + j][slot];
+                __syncthreads();
+                qn -= 64;
+                matrix_elements_f64(P, pq, m);
                 acc_add(A, P, m);
+            }
         }
+        if (qn > 0) { // partial flush
+            __syncthreads();
+            if (lane < qn) {
+                double pq[3][4], m[5];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        pq[i][j] = queue[4 * i + j][lane];
+                matrix_elements_f64(P, pq, m);
+                acc_add(A, P, m);
+            }
+        }
+        __syncthreads();
         double v[12];
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
@@ -227,16 +303,24 @@ __global__ __launch_bounds__(TPB) void fused_f64_kernel(FusedArgs a)
         }
         v[10] = A.sigma;
         v[11] = A.var;
-        block_reduce_store<TPB / 64>(v, A.n, out + lb, red);
+        block_reduce_store<1>(v, A.n, out + lb, red);
     }
+}
+
+int fused_blocks_per_cu(int source)
+{
+    int n = 0;
+    hipError_t e = source == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64_kernel<0>, 64, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64_kernel<1>, 64, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 void launch_fused(const FusedArgs &a, int source, int grid, hipStream_t st)
 {
     if (source == 0)
-        hipLaunchKernelGGL(fused_f64_kernel<0>, dim3(grid), dim3(TPB), 0, st, a);
+        hipLaunchKernelGGL(fused_f64_kernel<0>, dim3(grid), dim3(64), 0, st, a);
     else
-        hipLaunchKernelGGL(fused_f64_kernel<1>, dim3(grid), dim3(TPB), 0, st, a);
+        hipLaunchKernelGGL(fused_f64_kernel<1>, dim3(grid), dim3(64), 0, st, a);
 }
 
 template <int SOURCE> __global__ __launch_bounds__(256) void events_kernel(FusedArgs a, uint32_t n, EventOut o)
@@ -445,39 +529,69 @@ void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *ori
 }
 
 // ---- in-order merge --------------------------------------------------------
-__global__ __launch_bounds__(64) void merge_kernel(const DevAcc *in, uint64_t nb, DevAcc *out)
+// The fold must stay a left fold in batch order (its rounding is the
+// reference's), so it is one dependent chain of adds per field.  Tiles of
+// accumulators are staged through LDS with coalesced loads by the whole
+// workgroup; 12 lanes then run the 12 chains out of LDS, so the chain pays
+// the add latency, not the HBM latency.  The event count is an integer sum
+// and is reduced in any order.
+constexpr int MERGE_TILE = 512;
+constexpr int MERGE_THREADS = 256;
+
+__global__ __launch_bounds__(MERGE_THREADS) void merge_kernel(const DevAcc *in, uint64_t nb, DevAcc *out)
 {
-    const int j = threadIdx.x;
-    if (j >= 13)
-        return;
-    if (j == 12) {
-        uint64_t t = in[0].selected;
-        for (uint64_t b = 1; b < nb; ++b)
-            t += in[b].selected;
-        out->selected = t;
-        return;
-    }
-    const double *base = reinterpret_cast<const double *>(in) + 1 + j; // 13 words per accumulator
-    double t = base[0];
-    uint64_t b = 1;
-    for (; b + 8 <= nb; b += 8) {
-        double v[8];
+    __shared__ double tile[12][MERGE_TILE + 1]; // +1: the 12 chains hit distinct banks
+    __shared__ unsigned long long cnt;
+    const int tid = threadIdx.x;
+    if (tid == 0)
+        cnt = 0;
+    const uint64_t *words = reinterpret_cast<const uint64_t *>(in); // 13 words per accumulator
+    double t = 0.;
+    unsigned long long my_cnt = 0;
+    for (uint64_t base = 0; base < nb; base += MERGE_TILE) {
+        const uint64_t m = (nb - base) < (uint64_t)MERGE_TILE ? (nb - base) : (uint64_t)MERGE_TILE;
+        for (uint64_t f = tid; f < 13 * m; f += MERGE_THREADS) {
+            const uint64_t e = f / 13, w = f % 13;
+            const uint64_t v = words[13 * base + f];
+            if (w == 0)
+                my_cnt += v;
+            else
+                tile[w - 1][e] = __longlong_as_double((long long)v);
+        }
+        __syncthreads();
+        if (tid < 12) {
+            uint64_t e = 0;
+            if (base == 0) {
+                t = tile[tid][0];
+                e = 1;
+            }
+            // 16 independent LDS reads in flight, then 16 dependent adds.
+            for (; e + 16 <= m; e += 16) {
+                double v[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            v[u] = base[13 * (b + u)];
+                for (int u = 0; u < 16; ++u)
+                    v[u] = tile[tid][e + u];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            t = t + v[u];
+                for (int u = 0; u < 16; ++u)
+                    t = t + v[u];
+            }
+            for (; e < m; ++e)
+                t = t + tile[tid][e];
+        }
+        __syncthreads();
     }
-    for (; b < nb; ++b)
-        t = t + base[13 * b];
-    reinterpret_cast<double *>(out)[1 + j] = t;
+    atomicAdd(&cnt, my_cnt);
+    __syncthreads();
+    if (tid < 12)
+        reinterpret_cast<double *>(out)[1 + tid] = t;
+    if (tid == 0)
+        out->selected = cnt;
 }
 
 void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st)
 {
-    hipLaunchKernelGGL(merge_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<const DevAcc *>(per_batch), nb,
-                       reinterpret_cast<DevAcc *>(merged));
+    hipLaunchKernelGGL(merge_kernel, dim3(1), dim3(MERGE_THREADS), 0, st,
+                       reinterpret_cast<const DevAcc *>(per_batch), nb, reinterpret_cast<DevAcc *>(merged));
 }
 
 } // namespace tpk

@@ -308,8 +308,8 @@ __device__ __forceinline__ double weight_f64(const Params &P, const double m[5])
     return w;
 }
 
-// Whole f64 event from 12 uniforms: returns false if cut; else m[5].
-__device__ __forceinline__ bool event_f64(const Params &P, const double u[12], double p[3][4], double m[5])
+// RAMBO event from 12 uniforms (photons sorted) and its cut decision.
+__device__ __forceinline__ bool generate_keep_f64(const Params &P, const double u[12], double p[3][4])
 {
     double q[3][4];
 #pragma unroll
@@ -318,11 +318,23 @@ __device__ __forceinline__ bool event_f64(const Params &P, const double u[12], d
     conformal_f64(P, q, p);
     if (P.sort)
         sort3(p);
-    if (!keep_f64(P, p))
-        return false;
+    return keep_f64(P, p);
+}
+
+// Helicity sums m[5] of an event that passed the cuts.
+__device__ __forceinline__ void matrix_elements_f64(const Params &P, const double p[3][4], double m[5])
+{
     Spin<double> S;
     spinor_products<double>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
     helicity_sums<double>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+}
+
+// Whole f64 event from 12 uniforms: returns false if cut; else m[5].
+__device__ __forceinline__ bool event_f64(const Params &P, const double u[12], double p[3][4], double m[5])
+{
+    if (!generate_keep_f64(P, u, p))
+        return false;
+    matrix_elements_f64(P, p, m);
     return true;
 }
 
