@@ -32,7 +32,7 @@ struct Tables { // device copies of GF(2) jump tables for one (seeding, batch si
 struct Ctx {
     int device = 0;
     int cus = 1;                    // compute units of the device
-    int blocks_per_cu[2] = {1, 1};  // resident fused workgroups per CU, per source
+    int blocks_per_cu[2][2] = {{1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
@@ -215,7 +215,7 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
         return fail(c, TP_ERR_ARG, "null params or rng state");
     if (rng_kind != TP_RNG_RANF && rng_kind != TP_RNG_XOSHIRO)
         return fail(c, TP_ERR_ARG, "unknown rng kind " + std::to_string(rng_kind));
-    if (params->precision != TP_PREC_F64)
+    if (params->precision != TP_PREC_F64 && params->precision != TP_PREC_MIXED)
         return fail(c, TP_ERR_UNSUPPORTED, "precision " + std::to_string(params->precision) + " not available");
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
         return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
@@ -224,9 +224,9 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
 
 // Persistent grid: as many batch workgroups as can be resident at once (CUs x
 // occupancy), each striding over batches; never more than there are batches.
-int grid_for(const Ctx *c, int source, uint64_t nb)
+int grid_for(const Ctx *c, int source, int precision, uint64_t nb)
 {
-    const uint64_t g = static_cast<uint64_t>(c->cus) * static_cast<uint64_t>(c->blocks_per_cu[source]);
+    const uint64_t g = static_cast<uint64_t>(c->cus) * static_cast<uint64_t>(c->blocks_per_cu[source][precision]);
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(nb, g)));
 }
 
@@ -249,7 +249,7 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
     if (rng_kind == TP_RNG_XOSHIRO) {
         const tp_xoshiro_state *x = static_cast<const tp_xoshiro_state *>(state0);
         std::memcpy(a.s0, x->s, sizeof a.s0);
-        const int grid = grid_for(c, 0, nb);
+        const int grid = grid_for(c, 0, a.precision, nb);
         Tables *t = nullptr;
         rc = ensure_tables(c, seeding, batch_size, nb, grid, &t);
         if (rc)
@@ -350,7 +350,7 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
             HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
             first_kernel = false;
         }
-        tpk::launch_fused(a, 1, grid_for(c, 1, cnb), c->stream);
+        tpk::launch_fused(a, 1, grid_for(c, 1, a.precision, cnb), c->stream);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -393,8 +393,9 @@ extern "C" int tp_create(int device, void **ctx_out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cus = prop.multiProcessorCount;
-    c->blocks_per_cu[0] = tpk::fused_blocks_per_cu(0);
-    c->blocks_per_cu[1] = tpk::fused_blocks_per_cu(1);
+    for (int src = 0; src < 2; ++src)
+        for (int prec = 0; prec < 2; ++prec)
+            c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
     *ctx_out = c;
     return TP_OK;
 }
@@ -513,6 +514,7 @@ extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const
     tpk::FusedArgs a;
     std::memset(&a, 0, sizeof a);
     a.P = make_params(params);
+    a.precision = params->precision;
     rc = ensure_mt(c);
     if (rc)
         return rc;

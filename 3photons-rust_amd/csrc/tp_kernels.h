@@ -50,7 +50,7 @@ struct RanfArgs {
 };
 
 void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer*/, int grid, hipStream_t st);
-int fused_blocks_per_cu(int source); // resident workgroups per CU (occupancy)
+int fused_blocks_per_cu(int source, int precision); // resident workgroups per CU (occupancy)
 // Per-event diagnostics: one thread per event, same device code as the loop.
 // Event e draws from offset 12*e (xoshiro: a.s0 with a.mt; buffer: a.draws).
 struct EventOut {

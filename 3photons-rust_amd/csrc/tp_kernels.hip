@@ -97,42 +97,24 @@ __device__ __forceinline__ void xo_state_at(const FusedArgs &a, uint64_t b, uint
 }
 
 // ---- per-thread accumulation (resacc.rs:121-129) ---------------------------
-struct Acc {
-    double spm2[5], vars[5], sigma, var;
+// Sums s = (spm2[5], vars[5], sigma, variance); the mixed-precision path
+// (PREC 1) keeps Kahan compensations c alongside.
+template <int PREC> struct Acc {
+    double s[12];
+    double c[PREC ? 12 : 1];
     uint32_t n;
 };
 
-__device__ __forceinline__ void acc_zero(Acc &A)
+template <int PREC> __device__ __forceinline__ void acc_zero(Acc<PREC> &A)
 {
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        A.spm2[c] = 0.;
-        A.vars[c] = 0.;
-    }
-    A.sigma = 0.;
-    A.var = 0.;
+    for (int j = 0; j < 12; ++j)
+        A.s[j] = 0.;
+#pragma unroll
+    for (int j = 0; j < (PREC ? 12 : 1); ++j)
+        A.c[j] = 0.;
     A.n = 0;
 }
-
-__device__ __forceinline__ void acc_add(Acc &A, const Params &P, const double m[5])
-{
-    A.n += 1;
-#pragma unroll
-    for (int c = 0; c < 5; ++c)
-        A.spm2[c] = A.spm2[c] + m[c];
-#pragma unroll
-    for (int c = 0; c < 5; ++c)
-        A.vars[c] = A.vars[c] + m[c] * m[c];
-    double w = weight_f64(P, m);
-    A.sigma = A.sigma + w;
-    A.var = A.var + w * w;
-}
-
-// Kahan-compensated accumulation for the mixed-precision path.
-struct AccK {
-    double s[12], c[12];
-    uint32_t n;
-};
 
 __device__ __forceinline__ void kahan(double &s, double &c, double x)
 {
@@ -140,6 +122,34 @@ __device__ __forceinline__ void kahan(double &s, double &c, double x)
     double t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+template <int PREC> __device__ __forceinline__ void acc_add(Acc<PREC> &A, const Params &P, const double m[5])
+{
+    double x[12];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        x[k] = m[k];
+        x[5 + k] = m[k] * m[k];
+    }
+    const double w = weight_f64(P, m);
+    x[10] = w;
+    x[11] = w * w;
+    A.n += 1;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        if (PREC == 0)
+            A.s[j] = A.s[j] + x[j];
+        else
+            kahan(A.s[j], A.c[PREC ? j : 0], x[j]);
+    }
+}
+
+template <int PREC> __device__ __forceinline__ void acc_values(const Acc<PREC> &A, double v[12])
+{
+#pragma unroll
+    for (int j = 0; j < 12; ++j)
+        v[j] = PREC ? A.s[j] - A.c[PREC ? j : 0] : A.s[j];
 }
 
 // Fixed-shape wavefront butterfly then in-order wave combine; deterministic.
@@ -197,8 +207,8 @@ __device__ __forceinline__ void block_reduce_store(double v[12], uint32_t n, Dev
 // from HBM lane l takes events l, l+64, ... (coalesced loads).
 constexpr int QCAP = 128;
 
-template <int SOURCE>
-__global__ __launch_bounds__(64) void fused_f64_kernel(FusedArgs a)
+template <int SOURCE, int PREC>
+__global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
 {
     __shared__ double queue[12][QCAP]; // photon momenta, structure of arrays
     __shared__ double red[1][13];
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(64) void fused_f64_kernel(FusedArgs a)
             s[3] = st[3];
         }
         const int32_t *src = (SOURCE == 1) ? a.draws + (size_t)lb * 12 * a.batch_size : nullptr;
-        Acc A;
+        Acc<PREC> A;
         acc_zero(A);
         uint32_t qn = 0; // queued events (uniform)
         for (uint32_t k = 0; k < iters; ++k) {
@@ -254,7 +264,7 @@ __global__ __launch_bounds__(64) void fused_f64_kernel(FusedArgs a)
                     for (int j = 0; j < 12; ++j)
                         u[j] = (double)e[j] * 1e-9;
                 }
-                pass = generate_keep_f64(P, u, p);
+                pass = generate_keep<PREC>(P, u, p);
             }
             const uint64_t mask = __ballot(pass);
             if (pass) {
@@ -277,7 +287,7 @@ __global__ __launch_bounds__(64) void fused_f64_kernel(FusedArgs a)
                         pq[i][j] = queue[4 * i + j][slot];
                 __syncthreads();
                 qn -= 64;
-                matrix_elements_f64(P, pq, m);
+                matrix_elements<PREC>(P, pq, m);
                 acc_add(A, P, m);
             }
         }
@@ -290,40 +300,49 @@ __global__ __launch_bounds__(64) void fused_f64_kernel(FusedArgs a)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         pq[i][j] = queue[4 * i + j][lane];
-                matrix_elements_f64(P, pq, m);
+                matrix_elements<PREC>(P, pq, m);
                 acc_add(A, P, m);
             }
         }
         __syncthreads();
         double v[12];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            v[c] = A.spm2[c];
-            v[5 + c] = A.vars[c];
-        }
-        v[10] = A.sigma;
-        v[11] = A.var;
+        acc_values(A, v);
         block_reduce_store<1>(v, A.n, out + lb, red);
     }
 }
 
-int fused_blocks_per_cu(int source)
+template <int SOURCE, int PREC> int blocks_per_cu()
 {
     int n = 0;
-    hipError_t e = source == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64_kernel<0>, 64, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_f64_kernel<1>, 64, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_kernel<SOURCE, PREC>, 64, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int fused_blocks_per_cu(int source, int precision)
+{
+    if (source == 0)
+        return precision == 0 ? blocks_per_cu<0, 0>() : blocks_per_cu<0, 1>();
+    return precision == 0 ? blocks_per_cu<1, 0>() : blocks_per_cu<1, 1>();
 }
 
 void launch_fused(const FusedArgs &a, int source, int grid, hipStream_t st)
 {
-    if (source == 0)
-        hipLaunchKernelGGL(fused_f64_kernel<0>, dim3(grid), dim3(64), 0, st, a);
-    else
-        hipLaunchKernelGGL(fused_f64_kernel<1>, dim3(grid), dim3(64), 0, st, a);
+    const dim3 g(grid), b(64);
+    if (source == 0) {
+        if (a.precision == 0)
+            hipLaunchKernelGGL((fused_kernel<0, 0>), g, b, 0, st, a);
+        else
+            hipLaunchKernelGGL((fused_kernel<0, 1>), g, b, 0, st, a);
+    } else {
+        if (a.precision == 0)
+            hipLaunchKernelGGL((fused_kernel<1, 0>), g, b, 0, st, a);
+        else
+            hipLaunchKernelGGL((fused_kernel<1, 1>), g, b, 0, st, a);
+    }
 }
 
-template <int SOURCE> __global__ __launch_bounds__(256) void events_kernel(FusedArgs a, uint32_t n, EventOut o)
+template <int SOURCE, int PREC>
+__global__ __launch_bounds__(256) void events_kernel(FusedArgs a, uint32_t n, EventOut o)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n)
@@ -348,7 +367,9 @@ template <int SOURCE> __global__ __launch_bounds__(256) void events_kernel(Fused
             u[j] = (double)a.draws[12ull * e + j] * 1e-9;
     }
     double p[3][4], m[5] = {0., 0., 0., 0., 0.};
-    const bool k = event_f64(a.P, u, p, m);
+    const bool k = generate_keep<PREC>(a.P, u, p);
+    if (k)
+        matrix_elements<PREC>(a.P, p, m);
     if (!k) {
 #pragma unroll
         for (int c = 0; c < 5; ++c)
@@ -374,11 +395,18 @@ template <int SOURCE> __global__ __launch_bounds__(256) void events_kernel(Fused
 
 void launch_events(const FusedArgs &a, int source, uint32_t n, const EventOut &o, hipStream_t st)
 {
-    const unsigned blocks = (n + 255) / 256;
-    if (source == 0)
-        hipLaunchKernelGGL(events_kernel<0>, dim3(blocks), dim3(256), 0, st, a, n, o);
-    else
-        hipLaunchKernelGGL(events_kernel<1>, dim3(blocks), dim3(256), 0, st, a, n, o);
+    const dim3 g((n + 255) / 256), b(256);
+    if (source == 0) {
+        if (a.precision == 0)
+            hipLaunchKernelGGL((events_kernel<0, 0>), g, b, 0, st, a, n, o);
+        else
+            hipLaunchKernelGGL((events_kernel<0, 1>), g, b, 0, st, a, n, o);
+    } else {
+        if (a.precision == 0)
+            hipLaunchKernelGGL((events_kernel<1, 0>), g, b, 0, st, a, n, o);
+        else
+            hipLaunchKernelGGL((events_kernel<1, 1>), g, b, 0, st, a, n, o);
+    }
 }
 
 // ---- RANF expansion ------------------------------------------------------

@@ -297,6 +297,63 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> 
     m[4] = sI;
 }
 
+// ---- mixed precision: f32 spinor/amplitude algebra ------------------------
+//
+// The f32 instantiations contract multiply-adds into fma and divide by a
+// hardware reciprocal (1 ulp): this path trades the reference's rounding for
+// speed and is judged by its error against the f64 path on the same stream.
+// After the cuts every spinor product of the shipped configuration has
+// |s|^2 = 2 p_i.p_j >= 2.5 GeV^2, and every amplitude denominator stays far
+// inside the f32 range.
+template <> __device__ __forceinline__ C<float> cmul(C<float> a, C<float> b)
+{
+    return C<float>{fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re)};
+}
+template <> __device__ __forceinline__ C<float> cdiv(C<float> a, C<float> b)
+{
+    const float inv = __builtin_amdgcn_rcpf(fmaf(b.re, b.re, b.im * b.im));
+    return C<float>{fmaf(a.re, b.re, a.im * b.im) * inv, fmaf(a.im, b.re, -(a.re * b.im)) * inv};
+}
+
+// ez[k] = E_k + Z_k is formed in f64 by the caller: for a photon close to -Z
+// the f32 difference of two rounded values could go negative.
+__device__ __forceinline__ void spinor_products_f32(const Params &P, const float p[3][4], const float ez[3],
+                                                    Spin<float> &S)
+{
+    const float xx0 = (float)P.xx0, f0re = (float)P.f0re, f0im = (float)P.f0im;
+    const float xx1 = (float)P.xx1, f1re = (float)P.f1re, f1im = (float)P.f1im;
+    float xx[3];
+    C<float> fx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xx[k] = __builtin_amdgcn_sqrtf(ez[k]);
+        if (xx[k] > 1.17549435e-38f) {
+            const float inv = __builtin_amdgcn_rcpf(xx[k]);
+            fx[k] = mk<float>(p[k][0] * inv, p[k][1] * inv);
+        } else {
+            fx[k] = mk<float>(__builtin_amdgcn_sqrtf(2.f * p[k][3]), 0.f);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        C<float> s0k = mk<float>(fmaf(f0re, xx[k], -(fx[k].re * xx0)), fmaf(f0im, xx[k], -(fx[k].im * xx0)));
+        C<float> s1k = mk<float>(fmaf(f1re, xx[k], -(fx[k].re * xx1)), fmaf(f1im, xx[k], -(fx[k].im * xx1)));
+        S.s[0][2 + k] = s0k;
+        S.s[2 + k][0] = neg(s0k);
+        S.s[1][2 + k] = s1k;
+        S.s[2 + k][1] = neg(s1k);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = k + 1; l < 3; ++l) {
+            C<float> skl = mk<float>(fmaf(fx[k].re, xx[l], -(fx[l].re * xx[k])),
+                                     fmaf(fx[k].im, xx[l], -(fx[l].im * xx[k])));
+            S.s[2 + k][2 + l] = skl;
+            S.s[2 + l][2 + k] = neg(skl);
+        }
+}
+
 // Per-event weight: dot(m, sigma_contribs) as a left fold (resacc.rs:126).
 __device__ __forceinline__ double weight_f64(const Params &P, const double m[5])
 {
@@ -327,6 +384,46 @@ __device__ __forceinline__ void matrix_elements_f64(const Params &P, const doubl
     Spin<double> S;
     spinor_products<double>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
     helicity_sums<double>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+}
+
+// Mixed precision keeps RAMBO (massless q, conformal map), the sort and the
+// cuts in f64: the conformal map divides by M^2 = R_E^2 - |R|^2, which
+// cancels for nearly collinear q and would turn f32 rounding of q into
+// percent-level momentum errors on rare events.  So the mixed path sees the
+// same momenta and cut decisions as the f64 path, and only the spinor and
+// amplitude algebra runs in f32.
+__device__ __forceinline__ void matrix_elements_mixed(const Params &P, const double p[3][4], double m[5])
+{
+    float pf[3][4], ez[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            pf[i][j] = (float)p[i][j];
+        ez[i] = (float)fmax(p[i][3] + p[i][2], 0.);
+    }
+    Spin<float> S;
+    spinor_products_f32(P, pf, ez, S);
+    float mf[5];
+    helicity_sums<float>(S, mk((float)P.ns01re, (float)P.ns01im), mk((float)P.nt01re, (float)P.nt01im),
+                         (float)P.g_a, (float)P.g_bp, (float)P.g_bm, mf);
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        m[c] = (double)mf[c];
+}
+
+template <int PREC>
+__device__ __forceinline__ bool generate_keep(const Params &P, const double u[12], double p[3][4])
+{
+    return generate_keep_f64(P, u, p);
+}
+
+template <int PREC> __device__ __forceinline__ void matrix_elements(const Params &P, const double p[3][4], double m[5])
+{
+    if (PREC == 0)
+        matrix_elements_f64(P, p, m);
+    else
+        matrix_elements_mixed(P, p, m);
 }
 
 // Whole f64 event from 12 uniforms: returns false if cut; else m[5].

@@ -1,16 +1,19 @@
 """Benchmark of the 3photons event loop on one MI355X.
 
 Workload (one step): e+e- -> 3 photons at the shipped valeurs, xoshiro256+
-stream (seed_from_u64(12345)), f64, 1e9 events = 1e5 batches of 10 000, with
-batch accumulators and their in-order fold left in HBM (tp_run_device).
-Prints ONE JSON line (metric/value/... plus roofline and cpu_baseline).
+stream (seed_from_u64(12345)), 1e9 events = 1e5 batches of 10 000, batch
+accumulators and their in-order fold left in HBM (tp_run_device).  The f64
+path gives `value`; the mixed-precision path (f32 spinor/amplitude algebra)
+is timed on the same stream and reported beside it with its per-quantity
+relative error against the f64 run.  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--events N]
-                  [--precision f64|mixed] [--cpu-sample N] [--no-cpu]
+                  [--no-mixed] [--cpu-sample N] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -28,7 +31,6 @@ METRIC = "events/s on 1 MI355X (FP64, FP32); GPU-s to equal σ error: flow vs RA
 F_ALG = 852.3
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
-HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline(sample_events: int) -> dict:
@@ -60,13 +62,34 @@ def traffic_from_profiles(kernel: str):
         return None
 
 
+def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged):
+    kind = tp.RNG_XOSHIRO
+
+    def step():
+        st = tp.rng_state(kind)
+        eng.run_device(params, kind, tp.SEED_SKIPAHEAD, st, events, per_batch.data_ptr(), merged.data_ptr())
+        return eng.sync()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kernel_ms.append(step().kernel_ms)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    acc = tp.TpAcc.from_buffer_copy(bytes(merged.cpu().numpy().tobytes()))
+    return steps * events / dt, dt, sum(kernel_ms) / len(kernel_ms), acc
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--events", type=int, default=1_000_000_000)
-    ap.add_argument("--precision", choices=["f64", "mixed"], default="f64")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -81,36 +104,15 @@ def main() -> int:
     import trois_photons as tp
 
     cfg = tp.Configuration.load(VALEURS).with_events(args.events)
-    prec = tp.PREC_F64 if args.precision == "f64" else tp.PREC_MIXED
-    params = tp.params_from_config(cfg, precision=prec)
-    kind = tp.RNG_XOSHIRO
     nb = (args.events + tp.BATCH_SIZE - 1) // tp.BATCH_SIZE
     dev = torch.device("cuda:0")
     per_batch = torch.empty(nb * 104, dtype=torch.uint8, device=dev)
     merged = torch.empty(104, dtype=torch.uint8, device=dev)
     eng = tp.Engine(0)
 
-    def step():
-        st = tp.rng_state(kind)
-        eng.run_device(params, kind, tp.SEED_SKIPAHEAD, st, args.events, per_batch.data_ptr(), merged.data_ptr())
-        return eng.sync()
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s = step()
-        kernel_ms.append(s.kernel_ms)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-
-    value = args.steps * args.events / dt
-    k_ms = sum(kernel_ms) / len(kernel_ms)
+    p64 = tp.params_from_config(cfg, precision=tp.PREC_F64)
+    value, dt, k_ms, acc64 = measure(tp, torch, eng, p64, args.events, args.steps, args.warmup, per_batch, merged)
     achieved = F_ALG * args.events / (k_ms * 1e-3) / 1e12
-    peak = FP64_PEAK_TFLOPS if prec == tp.PREC_F64 else FP32_PEAK_TFLOPS
-    kname = "fused_f64_xoshiro" if prec == tp.PREC_F64 else "fused_mixed_xoshiro"
     out = {
         "metric": METRIC,
         "value": value,
@@ -122,15 +124,34 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == tp.PREC_F64 else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic (xoshiro256+ stream, shipped valeurs)",
         "config": {"workload": "e+e- -> 3 photons, RAMBO + cuts + helicity amplitudes, xoshiro256+ seed 12345, "
-                               f"{args.events:.3g} events in {nb} batches of 10000, {args.precision}",
-                   "events": args.events, "batches": nb, "rng": "xoshiro256+", "precision": args.precision},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic_from_profiles(kname),
-                     "kernel": kname, "kernel_ms": k_ms, "flops_per_event": F_ALG},
+                               f"{args.events:.3g} events in {nb} batches of 10000, f64",
+                   "events": args.events, "batches": nb, "rng": "xoshiro256+", "precision": "f64"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_from_profiles("fused_f64_xoshiro"),
+                     "kernel": "fused_kernel<0,0> (f64, xoshiro)", "kernel_ms": k_ms, "flops_per_event": F_ALG},
     }
+    if not args.no_mixed:
+        pm = tp.params_from_config(cfg, precision=tp.PREC_MIXED)
+        v32, dt32, k32, acc32 = measure(tp, torch, eng, pm, args.events, args.steps, args.warmup, per_batch, merged)
+        f64, f32 = tp.finalize(cfg, acc64), tp.finalize(cfg, acc32)
+        rel = {q: abs(getattr(f32, q) - getattr(f64, q)) / abs(getattr(f64, q))
+               for q in ("sigma", "prec", "variance", "beta_min", "ss_p", "ss_m")}
+        names = ["A", "B+", "B-", "R_MX", "I_MX"]
+        for k in range(5):
+            rel[f"spm2[{names[k]}]"] = abs(acc32.spm2[k] - acc64.spm2[k]) / abs(acc64.spm2[k])
+            rel[f"vars[{names[k]}]"] = abs(acc32.vars[k] - acc64.vars[k]) / abs(acc64.vars[k])
+        a32 = F_ALG * args.events / (k32 * 1e-3) / 1e12
+        out["fp32_mixed"] = {
+            "value": v32, "unit": "events/s", "ms_per_step": dt32 * 1e3 / args.steps, "dtype": "f32+f64",
+            "selected_events": int(acc32.selected_events), "selected_events_f64": int(acc64.selected_events),
+            "rel_err_vs_f64": rel,
+            "roofline": {"bound": "valu", "achieved": a32, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": a32 / FP32_PEAK_TFLOPS, "kernel": "fused_kernel<0,1> (mixed, xoshiro)",
+                         "kernel_ms": k32, "flops_per_event": F_ALG},
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out))
