@@ -48,7 +48,7 @@ struct Ctx {
     int32_t p00 = 0;
 };
 
-const int KM = 24;              // intra-call offsets < 2^24 draws
+const int KM = 32;              // intra-call offsets < 2^32 draws
 const int RANF_R = 128;         // RANF rounds per expansion segment
 const uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
 
@@ -598,6 +598,92 @@ extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const
     for (Item &it : items)
         if (it.buf->p)
             (void)hipFree(it.buf->p);
+    return rc;
+}
+
+extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, const void *rng_state, const void *u_in,
+                          uint64_t n, int precision, int on_device, void *F, void *dF_du, void *dF_dp)
+{
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c)
+        return TP_ERR_ARG;
+    if (!params)
+        return fail(c, TP_ERR_ARG, "null params");
+    if (precision != TP_PREC_F64 && precision != TP_PREC_F32)
+        return fail(c, TP_ERR_ARG, "tp_adjoint: precision must be TP_PREC_F64 or TP_PREC_F32");
+    if (!u_in && rng_kind != TP_RNG_XOSHIRO)
+        return fail(c, TP_ERR_UNSUPPORTED, "tp_adjoint draws its points from a xoshiro stream or u_in");
+    if (!u_in && !rng_state)
+        return fail(c, TP_ERR_ARG, "null rng state");
+    if (n == 0)
+        return TP_OK;
+    if (!u_in && 12 * n >= (1ull << KM))
+        return fail(c, TP_ERR_ARG, "tp_adjoint: n exceeds the stream offset range");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_mt(c);
+    if (rc)
+        return rc;
+    tpk::AdjArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.P = make_params(params);
+    a.w = params->ev_weight;
+    a.n = n;
+    a.per_thread = n >= (1u << 20) ? 16 : 1;
+    a.mt = static_cast<const uint64_t *>(c->mt.p);
+    a.km = c->km;
+    if (!u_in)
+        std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
+    const size_t es = precision == TP_PREC_F32 ? sizeof(float) : sizeof(double);
+    const size_t bytes[3] = {5 * n * es, 60 * n * es, 60 * n * es};
+    void *outs[3] = {F, dF_du, dF_dp};
+    DevBuf tmp[4];
+    if (on_device) {
+        a.u_in = static_cast<const double *>(u_in);
+        a.F = F;
+        a.dF_du = dF_du;
+        a.dF_dp = dF_dp;
+    } else {
+        if (u_in) {
+            rc = ensure(c, tmp[3], 12 * n * sizeof(double));
+            if (rc)
+                return rc;
+            HIPCHK(c, hipMemcpyAsync(tmp[3].p, u_in, 12 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            a.u_in = static_cast<const double *>(tmp[3].p);
+        }
+        for (int k = 0; k < 3 && rc == TP_OK; ++k)
+            if (outs[k])
+                rc = ensure(c, tmp[k], bytes[k]);
+        a.F = F ? tmp[0].p : nullptr;
+        a.dF_du = dF_du ? tmp[1].p : nullptr;
+        a.dF_dp = dF_dp ? tmp[2].p : nullptr;
+    }
+    if (rc == TP_OK) {
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        tpk::launch_adjoint(a, precision, c->stream);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            rc = fail(c, TP_ERR_HIP, std::string("adjoint kernel: ") + hipGetErrorString(e));
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    }
+    if (rc == TP_OK && !on_device)
+        for (int k = 0; k < 3 && rc == TP_OK; ++k)
+            if (outs[k]) {
+                hipError_t e = hipMemcpyAsync(outs[k], tmp[k].p, bytes[k], hipMemcpyDeviceToHost, c->stream);
+                if (e != hipSuccess)
+                    rc = fail(c, TP_ERR_HIP, std::string("adjoint copy: ") + hipGetErrorString(e));
+            }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == TP_OK && e != hipSuccess)
+        rc = fail(c, TP_ERR_HIP, std::string("adjoint sync: ") + hipGetErrorString(e));
+    if (rc == TP_OK) {
+        c->last.kernel_ms = elapsed(c->ev[1], c->ev[2]);
+        c->last.device_ms = c->last.kernel_ms;
+        c->last.events = n;
+        c->last.batches = 0;
+    }
+    for (DevBuf &b : tmp)
+        if (b.p)
+            (void)hipFree(b.p);
     return rc;
 }
 

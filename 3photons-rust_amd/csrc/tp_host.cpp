@@ -414,6 +414,7 @@ extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
     p->g_beta_m = g_beta;
     for (int i = 0; i < 5; ++i)
         p->sigma_contribs[i] = k.sc[i];
+    p->ev_weight = k.ev_weight;
     p->sort_photons = 1;
     p->precision = TP_PREC_F64;
     return TP_OK;

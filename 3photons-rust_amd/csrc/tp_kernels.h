@@ -68,3 +68,23 @@ void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *ori
 void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st);
 
 } // namespace tpk
+
+namespace tpk {
+
+// Arguments of the adjoint kernel (tp_adjoint).
+struct AdjArgs {
+    tpd::Params P;
+    double w;              // RAMBO event weight
+    uint64_t n;            // points
+    uint32_t per_thread;   // consecutive points per thread
+    // uniforms: u_in (n x 12 doubles) if non-null, else xoshiro from s0 via mt
+    const double *u_in;
+    uint64_t s0[4];
+    const uint64_t *mt;
+    int32_t km;
+    void *F, *dF_du, *dF_dp; // [n][5], [n][5][12], [n][5][12] of double or float
+};
+
+void launch_adjoint(const AdjArgs &a, int precision /*TP_PREC_F64 or TP_PREC_F32*/, hipStream_t st);
+
+} // namespace tpk

@@ -623,3 +623,60 @@ void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t 
 }
 
 } // namespace tpk
+
+// ---- adjoint ---------------------------------------------------------------
+#include "tp_adjoint.h"
+
+namespace tpk {
+
+template <class T> __global__ __launch_bounds__(128) void adjoint_kernel(AdjArgs a)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t first = t * a.per_thread;
+    if (first >= a.n)
+        return;
+    const AdjParams<T> P = adj_params<T>(a.P, a.w);
+    uint64_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
+    if (!a.u_in) {
+        const uint64_t off = 12ull * first;
+        for (int k = 0; k < a.km; ++k)
+            if ((off >> k) & 1ull)
+                gf2_apply(a.mt + 1024 * k, s);
+    }
+    T *F = reinterpret_cast<T *>(a.F), *Gu = reinterpret_cast<T *>(a.dF_du), *Gp = reinterpret_cast<T *>(a.dF_dp);
+    const uint64_t last = first + a.per_thread < a.n ? first + a.per_thread : a.n;
+    for (uint64_t i = first; i < last; ++i) {
+        T u[12];
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            u[j] = a.u_in ? (T)a.u_in[12 * i + j] : (T)xo_uniform(s);
+        for (int c = 0; c < 5; ++c) {
+            T lam[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};
+            lam[c] = (T)1;
+            T m[5], du[12], dp[12];
+            event_adjoint<T>(P, u, lam, m, du, dp);
+            if (F)
+                F[5 * i + c] = P.w * m[c];
+            if (Gu)
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    Gu[60 * i + 12 * c + j] = du[j];
+            if (Gp)
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    Gp[60 * i + 12 * c + j] = dp[j];
+        }
+    }
+}
+
+void launch_adjoint(const AdjArgs &a, int precision, hipStream_t st)
+{
+    const uint64_t threads = (a.n + a.per_thread - 1) / a.per_thread;
+    const dim3 g((unsigned)((threads + 127) / 128)), b(128);
+    if (precision == 2)
+        hipLaunchKernelGGL(adjoint_kernel<float>, g, b, 0, st, a);
+    else
+        hipLaunchKernelGGL(adjoint_kernel<double>, g, b, 0, st, a);
+}
+
+} // namespace tpk

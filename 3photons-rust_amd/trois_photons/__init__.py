@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libtp.so")
 BATCH_SIZE = 10000
 RNG_RANF, RNG_XOSHIRO = 0, 1
 SEED_SKIPAHEAD, SEED_JUMP = 0, 1
-PREC_F64, PREC_MIXED = 0, 1
+PREC_F64, PREC_MIXED, PREC_F32 = 0, 1, 2
 TP_OK = 0
 
 RNG_KINDS = {"ranf": RNG_RANF, "xoshiro": RNG_XOSHIRO}
@@ -35,7 +35,7 @@ class TpParams(C.Structure):
     _fields_ = [("e_total", C.c_double), ("cut_beam", C.c_double), ("cut_pp", C.c_double),
                 ("e_min", C.c_double), ("cut_plane", C.c_double), ("g_a", C.c_double),
                 ("g_beta_p", C.c_double), ("g_beta_m", C.c_double), ("sigma_contribs", C.c_double * 5),
-                ("sort_photons", C.c_int32), ("precision", C.c_int32)]
+                ("ev_weight", C.c_double), ("sort_photons", C.c_int32), ("precision", C.c_int32)]
 
 
 class TpAcc(C.Structure):
@@ -96,6 +96,7 @@ def _load() -> C.CDLL:
         "tp_run": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, P(TpAcc), P(TpStats)]),
         "tp_run_device": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, vp, P(TpStats)]),
         "tp_events": (C.c_int, [vp, P(TpParams), C.c_int, vp, C.c_uint32, vp, vp, vp, vp]),
+        "tp_adjoint": (C.c_int, [vp, P(TpParams), C.c_int, vp, vp, C.c_uint64, C.c_int, C.c_int, vp, vp, vp]),
         "tp_sync": (C.c_int, [vp, P(TpStats)]),
         "tp_config_load": (C.c_int, [C.c_char_p, P(TpConfig), C.c_char_p, C.c_int]),
         "tp_config_format": (C.c_int, [P(TpConfig), C.c_char_p, C.c_int]),
@@ -266,6 +267,27 @@ class Engine:
         s = TpStats()
         self._check(lib.tp_sync(self.h, C.byref(s)), "tp_sync")
         return s
+
+    def adjoint(self, params: TpParams, n: int, precision: int = PREC_F64, kind: int = RNG_XOSHIRO, st=None,
+                u: np.ndarray | None = None):
+        """F[n,5], dF/du[n,5,12], dF/dp[n,5,12] (host arrays) from tp_adjoint."""
+        dt = np.float32 if precision == PREC_F32 else np.float64
+        F = np.zeros((n, 5), dt); gu = np.zeros((n, 5, 12), dt); gp = np.zeros((n, 5, 12), dt)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        uin = None
+        if u is not None:
+            uin = np.ascontiguousarray(u, dtype=np.float64).reshape(n, 12)
+        self._check(lib.tp_adjoint(self.h, C.byref(params), kind, C.byref(st) if st is not None else None,
+                                   ptr(uin) if uin is not None else None, n, precision, 0, ptr(F), ptr(gu), ptr(gp)),
+                    "tp_adjoint")
+        return F, gu, gp
+
+    def adjoint_device(self, params: TpParams, n: int, precision: int, st, F_ptr: int, gu_ptr: int, gp_ptr: int):
+        self._check(lib.tp_adjoint(self.h, C.byref(params), RNG_XOSHIRO, C.byref(st), None, n, precision, 1,
+                                   C.c_void_p(F_ptr), C.c_void_p(gu_ptr), C.c_void_p(gp_ptr)), "tp_adjoint")
+
+    def last_stats(self) -> TpStats:
+        return self.sync()
 
     def events(self, params: TpParams, kind: int, st, n: int):
         """Per-event diagnostics: (u[n,12], p[n,3,4], keep[n], m[n,5])."""

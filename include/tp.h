@@ -58,10 +58,10 @@ typedef enum { TP_SEED_SKIPAHEAD = 0, TP_SEED_JUMP = 1 } tp_seeding;
 
 /* Arithmetic of the per-event path.
  *   TP_PREC_F64   -- IEEE double, no contraction, reference operation order.
- *   TP_PREC_MIXED -- f64 RAMBO conformal map, cuts and weight; f32 spinor
- *                    products and helicity amplitudes; Kahan-compensated f64
- *                    batch sums. */
-typedef enum { TP_PREC_F64 = 0, TP_PREC_MIXED = 1 } tp_precision;
+ *   TP_PREC_MIXED -- f64 RAMBO, cuts and weight; f32 spinor products and
+ *                    helicity amplitudes; Kahan-compensated f64 batch sums.
+ *   TP_PREC_F32   -- everything in float (tp_adjoint only). */
+typedef enum { TP_PREC_F64 = 0, TP_PREC_MIXED = 1, TP_PREC_F32 = 2 } tp_precision;
 
 /* Everything the per-event path reads, all computed on the host exactly as
  * the reference does (coupling.rs:23-33, resacc.rs:59-100, evgen.rs:40-77). */
@@ -70,6 +70,7 @@ typedef struct {
     double cut_beam, cut_pp, e_min, cut_plane; /* evcut.rs:11-23 */
     double g_a, g_beta_p, g_beta_m;       /* coupling.rs:10-20 */
     double sigma_contribs[5];             /* resacc.rs:94-100 */
+    double ev_weight;                     /* RAMBO event weight (evgen.rs:59-62) */
     int32_t sort_photons;                 /* 1 = sort by energy (default), 0 = no-photon-sorting */
     int32_t precision;                    /* tp_precision */
 } tp_params;
@@ -156,6 +157,22 @@ int tp_run_device(void *ctx, const tp_params *params, int rng_kind, int seeding,
  * m[n][5] (zeros for rejected events).  Host buffers; any may be null. */
 int tp_events(void *ctx, const tp_params *params, int rng_kind, const void *rng_state, uint32_t n,
               double *u, double *p, int32_t *keep, double *m);
+
+/* Adjoint of the event integrand.  No reference counterpart: the reference
+ * only evaluates F_c(u) = |M|^2_c(p(u)) * cut(p(u)) * w for its five
+ * contributions c (matelems.rs:53-86, evcut.rs:42-96, evgen.rs:89-207 with w
+ * the constant RAMBO weight); this entry point also returns, for each of n
+ * points, the hand-written reverse-mode derivatives
+ *   dF_c/du_j  (j < 12, the uniforms in the evgen draw order) and
+ *   dF_c/dp_k  (k < 12, the sorted photon momenta, photon-major X,Y,Z,E).
+ * Points take their uniforms from u_in (n x 12) when it is non-null, else
+ * from *rng_state (12 draws per point, not advanced).  precision is
+ * TP_PREC_F64 (double outputs) or TP_PREC_F32 (float outputs, all-f32
+ * arithmetic).  Outputs F[n][5], dF_du[n][5][12], dF_dp[n][5][12] and u_in
+ * are device pointers when on_device is 1, host pointers when 0; any output
+ * may be null.  Cut points have F = 0 and zero derivatives. */
+int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, const void *rng_state, const void *u_in,
+               uint64_t n, int precision, int on_device, void *F, void *dF_du, void *dF_dp);
 
 /* Wait for all work enqueued on the context stream; fills stats timings of the
  * last tp_run_device when stats is non-null. */

@@ -139,3 +139,39 @@ class Oracle:
         f = self.Final()
         self._fn("or_finalize")(C.byref(cfg), C.byref(acc), C.byref(f))
         return f
+
+
+class OracleGrad:
+    """Forward-mode (dual number) f64 gradients of the oracle's event
+    integrand (oracle/or_grad.cpp)."""
+
+    def __init__(self, oracle: Oracle, cfg: OrConfig, sort: bool = True):
+        path = os.path.join(ORACLE_DIR, "liborgrad.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"oracle gradient not built: {path}")
+        self.lib = C.CDLL(path)
+
+        class K(C.Structure):
+            _fields_ = [(n, C.c_double) for n in ("ev_weight", "g_a", "g_bp", "g_bm", "fact_com", "norm_weight",
+                                                   "propagator", "delta_z0")] + [("sc", C.c_double * 5)]
+        k = K()
+        oracle.lib.or_consts(C.byref(cfg), C.byref(k))
+        self.k = np.array([cfg.e_total, cfg.cut_beam, cfg.cut_pp, cfg.e_min, cfg.cut_plane, k.g_a, k.g_bp, k.g_bm,
+                           k.ev_weight], dtype=np.float64)
+        self.sort = int(sort)
+
+    def grad(self, u: np.ndarray):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 12)
+        n = u.shape[0]
+        F = np.zeros((n, 5)); gu = np.zeros((n, 5, 12)); gp = np.zeros((n, 5, 12))
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self.lib.or_grad(ptr(self.k), self.sort, C.c_uint64(n), ptr(u), ptr(F), ptr(gu), ptr(gp))
+        return F, gu, gp
+
+    def m2_at(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 5, 4)
+        n = p.shape[0]
+        F = np.zeros((n, 5))
+        self.lib.or_m2_at(self.k.ctypes.data_as(C.c_void_p), C.c_uint64(n), p.ctypes.data_as(C.c_void_p),
+                          F.ctypes.data_as(C.c_void_p))
+        return F
