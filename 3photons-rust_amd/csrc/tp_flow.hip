@@ -1,0 +1,649 @@
+// tp_flow.hip -- bf16-MFMA kernels of the RealNVP proposal (see tp_flow.h).
+//
+// Matrix products use v_mfma_f32_16x16x16_bf16.  Activations are kept
+// transposed (features x samples): for H^T = W X^T the weight tile is the A
+// operand and the 16-sample activation tile the B operand, and the lane
+// layout of the f32 result (lane l holds rows 4(l/16)..4(l/16)+3 of column
+// l%16) is exactly the B layout the next product needs, so a whole coupling
+// layer runs lane-locally in registers.  The coupling state of one sample is
+// spread over the four lanes l%16 + 16 g, lane group g holding features
+// 4g..4g+3 -- the same rows the s and t outputs land in.
+#include "tp_flow.h"
+#include "tp_adjoint.h"
+#include "tp_xoshiro.h"
+
+namespace tpf {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u); // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ s4 pack4(f4 v)
+{
+    s4 r;
+    r.x = (short)f2bf(v.x);
+    r.y = (short)f2bf(v.y);
+    r.z = (short)f2bf(v.z);
+    r.w = (short)f2bf(v.w);
+    return r;
+}
+__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
+__device__ __forceinline__ float silu(float x) { return x * sigm(x); }
+__device__ __forceinline__ float dsilu(float x)
+{
+    const float s = sigm(x);
+    return s * (1.f + x * (1.f - s));
+}
+__device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + log1pf(__expf(-fabsf(x))); }
+
+// ---- LDS image of one layer (bf16, padded row strides against bank conflicts)
+constexpr int S_W1 = 20, S_W2 = 68, S_W3 = 68, S_W1T = 68, S_W2T = 68, S_W3T = 36;
+constexpr int L_W1 = 0, L_W2 = L_W1 + H * S_W1, L_W3 = L_W2 + H * S_W2, L_FWD = L_W3 + O * S_W3;
+constexpr int L_W1T = L_FWD, L_W2T = L_W1T + DP * S_W1T, L_W3T = L_W2T + H * S_W2T, L_ALL = L_W3T + H * S_W3T;
+
+struct LdsLayer {
+    uint16_t w[L_ALL];
+    float b[H + H + O];
+};
+
+__device__ __forceinline__ void load_layer(LdsLayer &s, const uint16_t *wl, bool transposed)
+{
+    const float *unused = nullptr;
+    (void)unused;
+    for (int i = threadIdx.x; i < H * DP; i += blockDim.x) {
+        const int r = i / DP, c = i % DP;
+        s.w[L_W1 + r * S_W1 + c] = wl[OFF_W1 + i];
+        if (transposed)
+            s.w[L_W1T + c * S_W1T + r] = wl[OFF_W1 + i];
+    }
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
+        const int r = i / H, c = i % H;
+        s.w[L_W2 + r * S_W2 + c] = wl[OFF_W2 + i];
+        if (transposed)
+            s.w[L_W2T + c * S_W2T + r] = wl[OFF_W2 + i];
+    }
+    for (int i = threadIdx.x; i < O * H; i += blockDim.x) {
+        const int r = i / H, c = i % H;
+        s.w[L_W3 + r * S_W3 + c] = wl[OFF_W3 + i];
+        if (transposed)
+            s.w[L_W3T + c * S_W3T + r] = wl[OFF_W3 + i];
+    }
+    for (int i = threadIdx.x; i < H; i += blockDim.x) {
+        s.b[i] = bf2f(wl[OFF_B1 + i]);
+        s.b[H + i] = bf2f(wl[OFF_B2 + i]);
+    }
+    for (int i = threadIdx.x; i < O; i += blockDim.x)
+        s.b[2 * H + i] = bf2f(wl[OFF_B3 + i]);
+}
+
+// A operand of a 16x16 tile of a row-major LDS matrix: rows 16 mb .., cols 16 kb ..
+__device__ __forceinline__ s4 atile(const uint16_t *m, int stride, int mb, int kb)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = 16 * mb + (lane & 15), c = 16 * kb + 4 * (lane >> 4);
+    return *reinterpret_cast<const s4 *>(m + r * stride + c);
+}
+
+struct MlpFwd {
+    s4 a;          // bf16 input (features 4g..4g+3)
+    f4 h1p[4];     // pre-activations, 4 row blocks
+    s4 h1[4];
+    f4 h2p[4];
+    s4 h2[4];
+    f4 o0, o1;     // s raw (rows 0..15), t raw (rows 16..31)
+};
+
+__device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &r)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    r.a = pack4(a_in);
+    const f4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f4 c = mfma(atile(s.w + L_W1, S_W1, mb, 0), r.a, zero);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            c[i] += s.b[16 * mb + 4 * g + i];
+        r.h1p[mb] = c;
+        f4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            h[i] = silu(c[i]);
+        r.h1[mb] = pack4(h);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f4 c = zero;
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            c = mfma(atile(s.w + L_W2, S_W2, mb, kb), r.h1[kb], c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            c[i] += s.b[H + 16 * mb + 4 * g + i];
+        r.h2p[mb] = c;
+        f4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            h[i] = silu(c[i]);
+        r.h2[mb] = pack4(h);
+    }
+    f4 o0 = zero, o1 = zero;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        o0 = mfma(atile(s.w + L_W3, S_W3, 0, kb), r.h2[kb], o0);
+        o1 = mfma(atile(s.w + L_W3, S_W3, 1, kb), r.h2[kb], o1);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        o0[i] += s.b[2 * H + 4 * g + i];
+        o1[i] += s.b[2 * H + 16 + 4 * g + i];
+    }
+    r.o0 = o0;
+    r.o1 = o1;
+}
+
+__device__ __forceinline__ bool transformed(uint32_t cond, int j) { return j < D && !((cond >> j) & 1u); }
+__device__ __forceinline__ bool conditioning(uint32_t cond, int j) { return j < D && ((cond >> j) & 1u); }
+
+__device__ __forceinline__ f4 masked(f4 z, uint32_t cond)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    f4 a;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        a[i] = conditioning(cond, 4 * g + i) ? z[i] : 0.f;
+    return a;
+}
+
+// Sum over the four lane groups of one sample (lanes l, l^16, l^32, l^48).
+__device__ __forceinline__ float sample_sum(float x)
+{
+    x += __shfl_xor(x, 16);
+    x += __shfl_xor(x, 32);
+    return x;
+}
+
+// ---- draws ------------------------------------------------------------------
+// v for n samples from the xoshiro stream: sample i uses draws 12 (sample0 + i)
+// .. +11; one thread per run of 64 samples (one skip-ahead per run).  The f32
+// values are clamped to [2^-24, 1 - 2^-24] so that logit(v) is finite.
+__global__ __launch_bounds__(128) void flow_draw_kernel(FlowKernelArgs a, float *v)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t first = 64 * t;
+    if (first >= a.n)
+        return;
+    uint64_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
+    const uint64_t off = 12ull * (a.sample0 + first);
+    for (int k = 0; k < a.km; ++k)
+        if ((off >> k) & 1ull)
+            tpd::gf2_apply(a.mt + 1024 * k, s);
+    const uint64_t last = first + 64 < a.n ? first + 64 : a.n;
+    for (uint64_t i = first; i < last; ++i) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            v[16 * i + j] = fminf(fmaxf((float)tpd::xo_uniform(s), 0x1.0p-24f), 1.f - 0x1.0p-24f);
+#pragma unroll
+        for (int j = 12; j < 16; ++j)
+            v[16 * i + j] = 0.5f;
+    }
+}
+
+// ---- forward ----------------------------------------------------------------
+constexpr int TILES = 8;                 // 16-sample tiles per wavefront
+constexpr int CHUNK = 4 * TILES * 16;    // samples per 256-thread block pass
+
+__global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a, const float *v)
+{
+    __shared__ LdsLayer s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+    for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
+        f4 z[TILES];
+        float ld[TILES];
+        bool valid[TILES];
+        uint64_t idx[TILES];
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            idx[t] = base + (uint64_t)(wave * TILES + t) * 16 + (lane & 15);
+            valid[t] = idx[t] < a.n;
+            f4 vv = {0.5f, 0.5f, 0.5f, 0.5f};
+            if (valid[t])
+                vv = *reinterpret_cast<const f4 *>(v + 16 * idx[t] + 4 * g);
+            float l = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float x = vv[i];
+                z[t][i] = logf(x) - log1pf(-x);
+                if (4 * g + i < D)
+                    l += logf(x) + log1pf(-x); // log v (1 - v)
+            }
+            ld[t] = l;
+        }
+        for (int l = 0; l < L; ++l) {
+            __syncthreads();
+            load_layer(s, a.wbf + (size_t)l * LP, false);
+            __syncthreads();
+            const uint32_t cond = COND_MASK[l];
+#pragma unroll
+            for (int t = 0; t < TILES; ++t) {
+                if (a.zsave && valid[t])
+                    *reinterpret_cast<f4 *>(a.zsave + ((size_t)l * a.n + idx[t]) * 16 + 4 * g) = z[t];
+                MlpFwd r;
+                mlp_forward(s, masked(z[t], cond), r);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (transformed(cond, 4 * g + i)) {
+                        const float sc = FLOW_SCALE * tanhf(r.o0[i]);
+                        z[t][i] = z[t][i] * __expf(sc) + r.o1[i];
+                        ld[t] -= sc;
+                    }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            float l = ld[t];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * g + i < D) // - log u (1 - u) = softplus(-z) + softplus(z)
+                    l += softplus(-z[t][i]) + softplus(z[t][i]);
+            l = sample_sum(l);
+            if (!valid[t])
+                continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 4 * g + i;
+                if (j < D) {
+                    const double u = 1. / (1. + exp(-(double)z[t][i]));
+                    if (a.u_out)
+                        a.u_out[12 * idx[t] + j] = u;
+                    if (a.u32_out)
+                        a.u32_out[12 * idx[t] + j] = (float)u;
+                }
+            }
+            if (g == 0) {
+                if (a.logq_out)
+                    a.logq_out[idx[t]] = l;
+                if (a.invq_out)
+                    a.invq_out[idx[t]] = 1. / (a.alpha * exp((double)l) + (1. - a.alpha));
+            }
+        }
+    }
+}
+
+// ---- inverse: density of given points u -------------------------------------
+__global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
+{
+    __shared__ LdsLayer s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+    for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
+        f4 z[TILES];
+        float ld[TILES];
+        bool valid[TILES];
+        uint64_t idx[TILES];
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            idx[t] = base + (uint64_t)(wave * TILES + t) * 16 + (lane & 15);
+            valid[t] = idx[t] < a.n;
+            float l = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 4 * g + i;
+                double u = 0.5;
+                if (valid[t] && j < D)
+                    u = a.u_in[12 * idx[t] + j];
+                z[t][i] = (float)(log(u) - log1p(-u));
+                if (j < D)
+                    l -= (float)(log(u) + log1p(-u)); // - log u (1 - u)
+            }
+            ld[t] = l;
+        }
+        for (int l = L - 1; l >= 0; --l) {
+            __syncthreads();
+            load_layer(s, a.wbf + (size_t)l * LP, false);
+            __syncthreads();
+            const uint32_t cond = COND_MASK[l];
+#pragma unroll
+            for (int t = 0; t < TILES; ++t) {
+                MlpFwd r;
+                mlp_forward(s, masked(z[t], cond), r);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (transformed(cond, 4 * g + i)) {
+                        const float sc = FLOW_SCALE * tanhf(r.o0[i]);
+                        z[t][i] = (z[t][i] - r.o1[i]) * __expf(-sc);
+                        ld[t] -= sc;
+                    }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            float l = ld[t];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * g + i < D) // log v (1 - v) = -softplus(-z) - softplus(z)
+                    l -= softplus(-z[t][i]) + softplus(z[t][i]);
+            l = sample_sum(l);
+            if (valid[t] && g == 0) {
+                if (a.logq_out)
+                    a.logq_out[idx[t]] = l;
+                if (a.invq_out)
+                    a.invq_out[idx[t]] = 1. / (a.alpha * exp((double)l) + (1. - a.alpha));
+            }
+        }
+    }
+}
+
+// ---- backward (training) -----------------------------------------------------
+// loss per sample: log q(u) - log(f(u) + eps); gz = dloss/dz at the flow
+// output.  Walks the layers top-down, recomputing each MLP from the saved
+// layer input, and stores the bf16 activations and deltas (feature-major) for
+// the batch weight-gradient products.
+__global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
+{
+    __shared__ LdsLayer s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+    for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
+        f4 gz[TILES];
+        bool valid[TILES];
+        uint64_t idx[TILES];
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            idx[t] = base + (uint64_t)(wave * TILES + t) * 16 + (lane & 15);
+            valid[t] = idx[t] < a.n;
+            gz[t] = valid[t] ? *reinterpret_cast<const f4 *>(a.gz + 16 * idx[t] + 4 * g) : (f4){0.f, 0.f, 0.f, 0.f};
+        }
+        for (int l = L - 1; l >= 0; --l) {
+            __syncthreads();
+            load_layer(s, a.wbf + (size_t)l * LP, true);
+            __syncthreads();
+            const uint32_t cond = COND_MASK[l];
+            uint16_t *sv = a.save + (size_t)l * SAVE_F * a.n;
+#pragma unroll 1
+            for (int t = 0; t < TILES; ++t) {
+                f4 zin = valid[t] ? *reinterpret_cast<const f4 *>(a.zsave + ((size_t)l * a.n + idx[t]) * 16 + 4 * g)
+                                  : (f4){0.f, 0.f, 0.f, 0.f};
+                MlpFwd r;
+                mlp_forward(s, masked(zin, cond), r);
+                f4 ob0, ob1, zb;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int j = 4 * g + i;
+                    if (transformed(cond, j)) {
+                        const float th = tanhf(r.o0[i]);
+                        const float es = __expf(FLOW_SCALE * th);
+                        const float sbar = gz[t][i] * zin[i] * es - 1.f; // the -sum s term of log q
+                        ob0[i] = sbar * FLOW_SCALE * (1.f - th * th);
+                        ob1[i] = gz[t][i];
+                        zb[i] = gz[t][i] * es;
+                    } else {
+                        ob0[i] = 0.f;
+                        ob1[i] = 0.f;
+                        zb[i] = j < D ? gz[t][i] : 0.f;
+                    }
+                }
+                const s4 obb0 = pack4(ob0), obb1 = pack4(ob1);
+                const f4 zero = {0.f, 0.f, 0.f, 0.f};
+                s4 h2b[4], h1b[4];
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb) {
+                    f4 c = mfma(atile(s.w + L_W3T, S_W3T, mb, 0), obb0, zero);
+                    c = mfma(atile(s.w + L_W3T, S_W3T, mb, 1), obb1, c);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        c[i] *= dsilu(r.h2p[mb][i]);
+                    h2b[mb] = pack4(c);
+                }
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb) {
+                    f4 c = zero;
+#pragma unroll
+                    for (int kb = 0; kb < 4; ++kb)
+                        c = mfma(atile(s.w + L_W2T, S_W2T, mb, kb), h2b[kb], c);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        c[i] *= dsilu(r.h1p[mb][i]);
+                    h1b[mb] = pack4(c);
+                }
+                f4 ab = zero;
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb)
+                    ab = mfma(atile(s.w + L_W1T, S_W1T, 0, kb), h1b[kb], ab);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (conditioning(cond, 4 * g + i))
+                        zb[i] += ab[i];
+                gz[t] = zb;
+                if (valid[t]) {
+                    const uint64_t n = idx[t];
+                    const uint64_t N = a.n;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int row = 4 * g + i;
+                        sv[(SAVE_A + row) * N + n] = (uint16_t)r.a[i];
+                        sv[(SAVE_OB + row) * N + n] = (uint16_t)obb0[i];
+                        sv[(SAVE_OB + 16 + row) * N + n] = (uint16_t)obb1[i];
+#pragma unroll
+                        for (int mb = 0; mb < 4; ++mb) {
+                            sv[(SAVE_H1 + 16 * mb + row) * N + n] = (uint16_t)r.h1[mb][i];
+                            sv[(SAVE_H2 + 16 * mb + row) * N + n] = (uint16_t)r.h2[mb][i];
+                            sv[(SAVE_H2B + 16 * mb + row) * N + n] = (uint16_t)h2b[mb][i];
+                            sv[(SAVE_H1B + 16 * mb + row) * N + n] = (uint16_t)h1b[mb][i];
+                        }
+                    }
+                }
+            }
+        }
+        if (a.gv)
+#pragma unroll
+            for (int t = 0; t < TILES; ++t)
+                if (valid[t])
+                    *reinterpret_cast<f4 *>(a.gv + 16 * idx[t] + 4 * g) = gz[t];
+    }
+}
+
+// ---- batch weight gradients -------------------------------------------------
+// dW[m][k] = inv_n sum_n delta[m][n] act[k][n] on MFMA with the sample index
+// as the reduction dimension; one 256-thread block per 16x16 output tile,
+// each wavefront over a quarter of the samples, partial tiles combined in a
+// fixed order.  Tiles: per layer 4 (W1), 16 (W2), 8 (W3).
+struct WTile {
+    int delta_row, act_row, param_off, ld, mb, kb;
+};
+
+__device__ __forceinline__ WTile wtile(int t)
+{
+    WTile w;
+    if (t < 4) {
+        w = {SAVE_H1B, SAVE_A, OFF_W1, DP, t, 0};
+    } else if (t < 20) {
+        const int u = t - 4;
+        w = {SAVE_H2B, SAVE_H1, OFF_W2, H, u / 4, u % 4};
+    } else {
+        const int u = t - 20;
+        w = {SAVE_OB, SAVE_H2, OFF_W3, H, u / 4, u % 4};
+    }
+    return w;
+}
+
+__global__ __launch_bounds__(256) void flow_wgrad_kernel(const uint16_t *save, uint64_t n, float inv_n, float *grad)
+{
+    __shared__ f4 part[4][64];
+    const int layer = blockIdx.x / 28, tt = blockIdx.x % 28;
+    const WTile w = wtile(tt);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint16_t *sv = save + (size_t)layer * SAVE_F * n;
+    const uint16_t *drow = sv + (size_t)(w.delta_row + 16 * w.mb + (lane & 15)) * n;
+    const uint16_t *arow = sv + (size_t)(w.act_row + 16 * w.kb + (lane & 15)) * n;
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    const uint64_t per = ((n / 16 + 3) / 4) * 16;
+    const uint64_t n0 = wave * per, n1 = n0 + per < n ? n0 + per : n;
+    for (uint64_t s0 = n0; s0 + 16 <= n1; s0 += 16) {
+        const uint64_t col = s0 + 4 * (lane >> 4);
+        const s4 A = *reinterpret_cast<const s4 *>(drow + col);
+        const s4 B = *reinterpret_cast<const s4 *>(arow + col);
+        c = mfma(A, B, c);
+    }
+    part[wave][lane] = c;
+    __syncthreads();
+    if (wave == 0) {
+        f4 t = part[0][lane];
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            t += part[q][lane];
+        float *gl = grad + (size_t)layer * LP + w.param_off;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 16 * w.mb + 4 * (lane >> 4) + i, colk = 16 * w.kb + (lane & 15);
+            gl[row * w.ld + colk] = t[i] * inv_n;
+        }
+    }
+}
+
+// Bias gradients: sums over samples of one delta row, fixed-shape reduction.
+__global__ __launch_bounds__(256) void flow_bgrad_kernel(const uint16_t *save, uint64_t n, float inv_n, float *grad)
+{
+    __shared__ float red[256];
+    const int layer = blockIdx.x / (H + H + O), r = blockIdx.x % (H + H + O);
+    int row, off;
+    if (r < H) {
+        row = SAVE_H1B + r;
+        off = OFF_B1 + r;
+    } else if (r < 2 * H) {
+        row = SAVE_H2B + (r - H);
+        off = OFF_B2 + (r - H);
+    } else {
+        row = SAVE_OB + (r - 2 * H);
+        off = OFF_B3 + (r - 2 * H);
+    }
+    const uint16_t *d = save + (size_t)layer * SAVE_F * n + (size_t)row * n;
+    float acc = 0.f;
+    for (uint64_t i = threadIdx.x; i < n; i += 256)
+        acc += bf2f(d[i]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        grad[(size_t)layer * LP + off] = red[0] * inv_n;
+}
+
+__global__ void flow_adam_kernel(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr,
+                                 float b1, float b2, float eps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM)
+        return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.f - powf(b1, (float)step)), vh = vi / (1.f - powf(b2, (float)step));
+    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+    w[i] = wi;
+    wbf[i] = f2bf(wi);
+}
+
+__global__ void flow_to_bf16_kernel(const float *w, uint16_t *wbf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < NPARAM)
+        wbf[i] = f2bf(w[i]);
+}
+
+// ---- launchers ---------------------------------------------------------------
+static unsigned grid_for(uint64_t n) { return (unsigned)((n + CHUNK - 1) / CHUNK); }
+
+void launch_flow_draw(const FlowKernelArgs &a, float *v, hipStream_t st)
+{
+    const uint64_t threads = (a.n + 63) / 64;
+    hipLaunchKernelGGL(flow_draw_kernel, dim3((unsigned)((threads + 127) / 128)), dim3(128), 0, st, a, v);
+}
+
+void launch_flow_forward_v(const FlowKernelArgs &a, const float *v, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_forward_kernel, dim3(grid_for(a.n)), dim3(256), 0, st, a, v);
+}
+
+void launch_flow_inverse(const FlowKernelArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_inverse_kernel, dim3(grid_for(a.n)), dim3(256), 0, st, a);
+}
+
+void launch_flow_backward(const FlowBackArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_backward_kernel, dim3(grid_for(a.n)), dim3(256), 0, st, a);
+}
+
+void launch_flow_wgrad(const uint16_t *save, uint64_t n, float inv_n, float *grad, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_wgrad_kernel, dim3(L * 28), dim3(256), 0, st, save, n, inv_n, grad);
+    hipLaunchKernelGGL(flow_bgrad_kernel, dim3(L * (H + H + O)), dim3(256), 0, st, save, n, inv_n, grad);
+}
+
+void launch_flow_adam(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr, float beta1,
+                      float beta2, float eps, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, w, wbf, m, v, g, step, lr,
+                       beta1, beta2, eps);
+}
+
+void launch_flow_to_bf16(const float *w, uint16_t *wbf, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_to_bf16_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, w, wbf);
+}
+
+// ---- integrand and its gradient for training --------------------------------
+// f(u) = sum_c lam_c |M|^2_c(u) cut(u) from the hand-written adjoint, and
+// gz = dloss/dz at the flow output for loss = log q - log(f + eps):
+//   dloss/du = -(1 - 2u)/(u (1 - u)) - f'(u)/(f + eps),  dz = du * u (1 - u).
+__global__ __launch_bounds__(128) void flow_integrand_kernel(tpd::Params P, double lam0, double lam1, double lam2,
+                                                             double lam3, double lam4, uint64_t n, const double *u,
+                                                             double eps, float *gz, float *logf, double *f_out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const tpd::AdjParams<double> A = tpd::adj_params<double>(P, 1.0);
+    double uu[12], m[5], du[12], dp[12];
+    const double lam[5] = {lam0, lam1, lam2, lam3, lam4};
+#pragma unroll
+    for (int j = 0; j < 12; ++j)
+        uu[j] = u[12 * i + j];
+    const bool keep = tpd::event_adjoint<double>(A, uu, lam, m, du, dp);
+    double f = 0.;
+    if (keep)
+        for (int c = 0; c < 5; ++c)
+            f += lam[c] * m[c];
+    const double fe = f + eps;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        const double w = uu[j] * (1. - uu[j]);
+        gz[16 * i + j] = (float)(-(1. - 2. * uu[j]) - w * du[j] / fe);
+    }
+#pragma unroll
+    for (int j = 12; j < 16; ++j)
+        gz[16 * i + j] = 0.f;
+    logf[i] = (float)log(fe);
+    if (f_out)
+        f_out[i] = f;
+}
+
+void launch_flow_integrand(const tpd::Params &P, const double lam[5], uint64_t n, const double *u, double eps,
+                           float *gz, float *logf, double *f_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_integrand_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, P, lam[0], lam[1],
+                       lam[2], lam[3], lam[4], n, u, eps, gz, logf, f_out);
+}
+
+} // namespace tpf

@@ -16,6 +16,7 @@
 // operation order (HadrienG2/3photons-rust src/evgen.rs, evcut.rs, spinor.rs,
 // matelems.rs, resacc.rs).
 #include "tp_kernels.h"
+#include "tp_xoshiro.h"
 
 namespace tpk {
 
@@ -25,61 +26,6 @@ struct DevAcc { // layout of tp_acc
     uint64_t selected;
     double spm2[5], vars[5], sigma, variance;
 };
-
-// ---- xoshiro256+ -----------------------------------------------------------
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-
-__device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
-{
-    uint64_t r = s[0] + s[3];
-    uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl64(s[3], 45);
-    return r;
-}
-
-// rand's Standard f64: (u64 >> 11) * 2^-53 (exact conversion, exact scaling).
-// Evaluated as hi * 2^-32 + (lo >> 11) * 2^-53: both terms and their sum are
-// exact, so the fma returns the same double as the reference's conversion.
-__device__ __forceinline__ double xo_uniform(uint64_t s[4])
-{
-    const uint64_t x = xo_next(s);
-    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
-    return fma((double)hi, 0x1.0p-32, (double)(lo >> 11) * 0x1.0p-53);
-}
-
-// s <- M s for a 256x256 GF(2) matrix stored as 256 columns of 4 words.  The
-// matrix address is uniform across the wavefront.
-// Read through the constant address space so that the uniform columns come in
-// with scalar loads into SGPRs instead of 64 identical vector loads.
-typedef const __attribute__((address_space(4))) uint64_t cu64;
-
-__device__ __forceinline__ void gf2_apply(const uint64_t *__restrict__ mg, uint64_t s[4])
-{
-    cu64 *m = (cu64 *)mg;
-    uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        uint64_t x = s[w];
-#pragma unroll 4
-        for (int b = 0; b < 64; ++b) {
-            uint64_t mask = 0ull - ((x >> b) & 1ull);
-            cu64 *c = m + 4 * (64 * w + b);
-            r0 ^= c[0] & mask;
-            r1 ^= c[1] & mask;
-            r2 ^= c[2] & mask;
-            r3 ^= c[3] & mask;
-        }
-    }
-    s[0] = r0;
-    s[1] = r1;
-    s[2] = r2;
-    s[3] = r3;
-}
 
 // State of (batch b, draw offset off inside the batch): M^off B^b s0.
 __device__ __forceinline__ void xo_state_at(const FusedArgs &a, uint64_t b, uint64_t off, uint64_t s[4])

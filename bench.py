@@ -8,7 +8,9 @@ is timed on the same stream and reported beside it with its per-quantity
 relative error against the f64 run.  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--events N]
-                  [--no-mixed] [--cpu-sample N] [--no-cpu]
+                  [--no-mixed] [--no-adjoint] [--cpu-sample N] [--no-cpu]
+  python bench.py --copy-calibration   # 3 plain 1 GiB device copies, for
+                                       # scaling FETCH_SIZE / WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -51,15 +53,19 @@ def cpu_baseline(sample_events: int) -> dict:
                       f"({(sample_events + 9999) // 10000} batches), oracle/liboracle.so, {dt:.2f} s wall"}
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 --pmc
+    summary profiles/<name>.md (its 'HBM bytes per launch:' line), if any."""
+    path = os.path.join(ROOT, "profiles", f"{name}.md")
     if not os.path.exists(path):
         return None
-    try:
-        return json.load(open(path)).get("bytes_per_launch")
-    except Exception:
-        return None
+    for line in open(path):
+        if line.startswith("HBM bytes per launch:"):
+            try:
+                return float(line.split(":", 1)[1].split()[0])
+            except ValueError:
+                return None
+    return None
 
 
 def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged):
@@ -83,6 +89,44 @@ def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged):
     return steps * events / dt, dt, sum(kernel_ms) / len(kernel_ms), acc
 
 
+def measure_adjoint(tp, torch, eng, params, n: int) -> dict:
+    """Forward |M|^2*cut*w for the 5 contributions plus the hand-written
+    gradients w.r.t. the 12 uniforms and 12 photon momentum components, for
+    n points of the xoshiro256+ seed-42 stream, outputs left in HBM."""
+    dev = torch.device("cuda:0")
+    res = {"points": n, "stream": "xoshiro256+ seed_from_u64(42), 12 uniforms per point",
+           "outputs_per_point": "5 values + 5x12 du + 5x12 dp"}
+    F = torch.empty(n * 5 * 8, dtype=torch.uint8, device=dev)
+    gu = torch.empty(n * 60 * 8, dtype=torch.uint8, device=dev)
+    gp = torch.empty(n * 60 * 8, dtype=torch.uint8, device=dev)
+    for prec, name in ((tp.PREC_F64, "f64"), (tp.PREC_F32, "f32")):
+        st = tp.rng_state(tp.RNG_XOSHIRO, 42)
+        eng.adjoint_device(params, 4096, prec, st, F.data_ptr(), gu.data_ptr(), gp.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.adjoint_device(params, n, prec, st, F.data_ptr(), gu.data_ptr(), gp.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k_ms = eng.sync().kernel_ms
+        res[name] = {"points_per_s": n / dt, "wall_ms": dt * 1e3, "kernel_ms": k_ms}
+    return res
+
+
+def copy_calibration() -> int:
+    """Three device-to-device copies of 1 GiB: the reference point that turns
+    the FETCH_SIZE / WRITE_SIZE counters of a profiled run into bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"launches": 3, "bytes_read": n * 4, "bytes_written": n * 4}}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,12 +136,17 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-points", type=int, default=1 << 24)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the event loop shards by "
               "batch range across GPUs as described in DESIGN.md, but no multi-GPU run is part of it.",
               file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration()
 
     import torch
     sys.path.insert(0, PKG)
@@ -130,7 +179,7 @@ def main() -> int:
                                f"{args.events:.3g} events in {nb} batches of 10000, f64",
                    "events": args.events, "batches": nb, "rng": "xoshiro256+", "precision": "f64"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_from_profiles("fused_f64_xoshiro"),
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_from_profiles("fused_kernel_f64_xoshiro_1e9_traffic"),
                      "kernel": "fused_kernel<0,0> (f64, xoshiro)", "kernel_ms": k_ms, "flops_per_event": F_ALG},
     }
     if not args.no_mixed:
@@ -152,6 +201,8 @@ def main() -> int:
                          "frac": a32 / FP32_PEAK_TFLOPS, "kernel": "fused_kernel<0,1> (mixed, xoshiro)",
                          "kernel_ms": k32, "flops_per_event": F_ALG},
         }
+    if not args.no_adjoint:
+        out["adjoint"] = measure_adjoint(tp, torch, eng, p64, args.adjoint_points)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out))
