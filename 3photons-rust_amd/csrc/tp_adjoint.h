@@ -101,12 +101,29 @@ template <class T> __device__ __forceinline__ void add_t(SpinAdj<T> &D, const Sp
     D.tb[i][j] = caddv(D.tb[i][j], cscale(e, cmul(g, cconj(tinv(V, i, j)))));
 }
 
+template <class T> __device__ __forceinline__ T log_sigmoid(T x) { return -(fmax(-x, (T)0) + log1p(exp(-fabs(x)))); }
+template <class T> __device__ __forceinline__ T sigmoid(T x) { return (T)1 / ((T)1 + exp(-x)); }
+
 // Forward value and reverse sweep of one event.  Returns false (and zeros)
 // for a cut event.  u[12] uniforms, lam[5] channel weights; outputs m[5]
 // (|M|^2 per channel), du[12] = dL/du * w, dp[12] = dL/dp_sorted * w.
-template <class T>
+//
+// SOFT = 1 and 2 replace the energy, beam and photon-pair cuts
+// (evcut.rs:42-96) by the smooth factor S = prod_k sigmoid(margin_k / tau)
+// over the dimensionless margins (E_i - E_min) / E_min, cut_beam - |X_i| / E_i
+// and cut_pp - cos(theta_ij), and m[c] becomes |M|^2_c * S.  The plane cut
+// stays a hard gate.  These are training integrands of the learned proposal,
+// whose gradient must see the cut boundaries:
+//   SOFT 1: L = S sum_c lam_c |M|^2_c; events with a margin below
+//           max(-30 tau, -1/2) are dropped (S < 1e-13 for tau <= 1/60);
+//   SOFT 2: L = log(min(f, fcap)) + log S with f = sum_c lam_c |M|^2_c,
+//           whose gradient keeps pointing back into the cuts however far
+//           outside an event lies; the cap keeps the soft and collinear
+//           singularities just outside the cuts from dominating (events with
+//           a photon of zero energy or f <= 0 are dropped).
+template <class T, int SOFT = 0>
 __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[12], const T lam[5], T m[5], T du[12],
-                                              T dp[12])
+                                              T dp[12], T tau = (T)0, T fcap = (T)INFINITY)
 {
     const T one = (T)1, two = (T)2, two_pi = (T)(2. * 3.14159265358979323846);
     // ---- forward: massless photons from the uniforms (evgen.rs:174-207)
@@ -159,7 +176,41 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
             ps[j][k] = perm[j] == 0 ? p[0][k] : (perm[j] == 1 ? p[1][k] : p[2][k]);
     // ---- cuts (evcut.rs:42-96)
     bool keep = true;
-    {
+    T lsig = (T)0;                                     // log S (SOFT)
+    T gE[3] = {(T)0, (T)0, (T)0}, gB[3] = {(T)0, (T)0, (T)0}, gP[3] = {(T)0, (T)0, (T)0}; // dlogS/dmargin
+    if (SOFT) {
+        // SOFT 1 keeps E_i >= E_min / 2 > 0; SOFT 2 only needs E_i > 0
+        const T far = SOFT == 1 ? fmax((T)-30 * tau, (T)-0.5) : (T)-1 + (T)1e-12;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            keep = keep && (ps[i][3] - P.e_min) / P.e_min > far;
+        if (keep) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const T mE = (ps[i][3] - P.e_min) / P.e_min;
+                const T mB = P.cut_beam - fabs(ps[i][0]) / ps[i][3];
+                lsig = lsig + log_sigmoid(mE / tau) + log_sigmoid(mB / tau);
+                gE[i] = sigmoid(-mE / tau) / tau;
+                gB[i] = sigmoid(-mB / tau) / tau;
+                keep = keep && (SOFT == 2 || mB > far);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
+                const T cs = ((ps[i][0] * ps[j][0] + ps[i][1] * ps[j][1]) + ps[i][2] * ps[j][2]) / (ps[i][3] * ps[j][3]);
+                const T mP = P.cut_pp - cs;
+                lsig = lsig + log_sigmoid(mP / tau);
+                gP[k] = sigmoid(-mP / tau) / tau;
+                keep = keep && (SOFT == 2 || mP > far);
+            }
+        }
+        if (P.cut_plane > (T)0) {
+            const T n0 = ps[0][1] * ps[1][2] - ps[0][2] * ps[1][1];
+            const T n1 = ps[0][2] * ps[1][0] - ps[0][0] * ps[1][2];
+            const T n2 = ps[0][0] * ps[1][1] - ps[0][1] * ps[1][0];
+            keep = keep && !(fabs(P.el_x * n0) < P.cut_plane * (P.el_e * sqrt((n0 * n0 + n1 * n1) + n2 * n2)));
+        }
+    } else {
         T emin = ps[2][3];
         if (!P.sort)
             emin = fmin(fmin(ps[0][3], ps[1][3]), ps[2][3]);
@@ -267,6 +318,35 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
         m[3] = sR;
         m[4] = sI;
     }
+    // SOFT 1: L = S f, f = sum_c lam_c m_c: the amplitude sweep below runs with
+    // lam * S and f S dlogS/dp joins the momentum adjoint.  SOFT 2: L = log f +
+    // log S: the sweep runs with lam / f and dlogS/dp joins.
+    T lm[5];
+    T fS = (T)0;
+    {
+        const T S = SOFT ? exp(lsig) : (T)1;
+        T f = (T)0;
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            f = f + lam[c] * m[c];
+        if (SOFT == 2 && !(f > (T)0 && f < (T)INFINITY)) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                m[c] = (T)0;
+#pragma unroll
+            for (int j = 0; j < 12; ++j)
+                du[j] = dp[j] = (T)0;
+            return false;
+        }
+        const bool capped = SOFT == 2 && f > fcap;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            lm[c] = SOFT == 2 ? (capped ? (T)0 : lam[c] / f) : lam[c] * S;
+            if (SOFT)
+                m[c] = capped ? m[c] * (fcap / f) * S : m[c] * S;
+        }
+        fS = SOFT == 2 ? (T)1 : f * S;
+    }
     // ---- reverse: amplitudes -> spinor products
     SpinAdj<T> D;
 #pragma unroll
@@ -275,9 +355,9 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
         for (int j = 0; j < 5; ++j)
             D.sb[i][j] = D.tb[i][j] = C<T>{(T)0, (T)0};
     const T kappa = two * P.g_a * P.g_bp;
-    const C<T> Xbar = C<T>{lam[3], lam[4]};
-    const T cA = two * lam[0] * P.g_a * P.g_a, cP = two * lam[1] * P.g_bp * P.g_bp;
-    const T cM = two * lam[2] * P.g_bm * P.g_bm;
+    const C<T> Xbar = C<T>{lm[3], lm[4]};
+    const T cA = two * lm[0] * P.g_a * P.g_a, cP = two * lm[1] * P.g_bp * P.g_bp;
+    const T cM = two * lm[2] * P.g_bm * P.g_bm;
 #pragma unroll
     for (int h = 0; h < 6; ++h) {
         const int k1 = ak[h][0], k2 = ak[h][1], k3 = ak[h][2];
@@ -387,6 +467,30 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
         const T h = small[k] ? (T)0 : xxb[k] / (two * xx[k]);
         pb[k][3] = pb[k][3] + h;
         pb[k][2] = pb[k][2] + h;
+    }
+    if (SOFT) { // fS * dlogS/dp of the sorted momenta
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const T E = ps[i][3];
+            pb[i][3] = pb[i][3] + fS * gE[i] / P.e_min;
+            pb[i][0] = pb[i][0] - fS * gB[i] * ((ps[i][0] >= (T)0 ? one : -one) / E);
+            pb[i][3] = pb[i][3] + fS * gB[i] * (fabs(ps[i][0]) / (E * E));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
+            const T EE = ps[i][3] * ps[j][3];
+            const T cs = ((ps[i][0] * ps[j][0] + ps[i][1] * ps[j][1]) + ps[i][2] * ps[j][2]) / EE;
+            const T g = fS * gP[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T pic = ps[i][c], pjc = ps[j][c];
+                pb[i][c] = pb[i][c] - g * pjc / EE;
+                pb[j][c] = pb[j][c] - g * pic / EE;
+            }
+            pb[i][3] = pb[i][3] + g * cs / ps[i][3];
+            pb[j][3] = pb[j][3] + g * cs / ps[j][3];
+        }
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k)

@@ -1,0 +1,94 @@
+// tp_ctx.h -- the engine context shared by the event-loop and flow entry
+// points (internal; not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "tp.h"
+#include "tp_physics.h"
+
+namespace tpe {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct Tables { // device copies of GF(2) jump tables for one (seeding, batch size, grid)
+    DevBuf bt, stride;
+    int kb = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    int cus = 1;                    // compute units of the device
+    int blocks_per_cu[3][2] = {{1, 1}, {1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    DevBuf mt;       // M^(2^k)
+    int km = 0;
+    std::map<std::string, std::unique_ptr<Tables>> tables;
+    std::map<std::string, std::vector<uint32_t>> ranf_polys; // host copies keyed by R
+    DevBuf polys, draws, origin, p0, per_batch, merged;
+    tp_stats last{};
+    bool timed = false;
+    // Host staging of the RANF origin: must outlive the asynchronous copies.
+    int32_t origin0[56];
+    int32_t p00 = 0;
+};
+
+constexpr int KM = 32;              // intra-call offsets < 2^32 draws
+constexpr int RANF_R = 128;         // RANF rounds per expansion segment
+constexpr uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
+
+inline int fail(Ctx *c, int code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((ctx), TP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+inline int ensure(Ctx *c, DevBuf &b, size_t bytes)
+{
+    if (b.n >= bytes && b.p)
+        return TP_OK;
+    if (b.p) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess)
+        return fail(c, TP_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    b.n = bytes;
+    return TP_OK;
+}
+
+// Persistent grid: as many batch workgroups as can be resident at once (CUs x
+// occupancy), each striding over batches; never more than there are batches.
+inline int grid_for(const Ctx *c, int source, int precision, uint64_t nb)
+{
+    const uint64_t g = static_cast<uint64_t>(c->cus) * static_cast<uint64_t>(c->blocks_per_cu[source][precision]);
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(nb, g)));
+}
+
+// Device-side constants of a run, computed exactly as the reference does on
+// the host (coupling.rs, resacc.rs:59-100, evgen.rs:66-69, spinor.rs:37-44).
+tpd::Params make_params(const tp_params *p);
+
+} // namespace tpe

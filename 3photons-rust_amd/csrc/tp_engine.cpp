@@ -16,72 +16,51 @@
 #include "tp.h"
 #include "tp_kernels.h"
 #include "tp_rng_host.h"
+#include "tp_ctx.h"
+
+namespace tpe {
+tpd::Params make_params(const tp_params *p)
+{
+    tpd::Params P;
+    std::memset(&P, 0, sizeof P);
+    P.e_total = p->e_total;
+    P.cut_beam = p->cut_beam;
+    P.cut_pp = p->cut_pp;
+    P.e_min = p->e_min;
+    P.cut_plane = p->cut_plane;
+    P.g_a = p->g_a;
+    P.g_bp = p->g_beta_p;
+    P.g_bm = p->g_beta_m;
+    for (int i = 0; i < 5; ++i)
+        P.sc[i] = p->sigma_contribs[i];
+    // Incoming momenta (evgen.rs:66-69) and their spinor pieces (spinor.rs:37-44).
+    const double e = p->e_total;
+    P.el_x = -e / 2.;
+    P.el_e = e / 2.;
+    const double pos_x = e / 2., pos_e = e / 2.;
+    P.xx0 = std::sqrt(P.el_e + 0.);
+    P.f0re = P.el_x / P.xx0;
+    P.f0im = 0. / P.xx0;
+    P.xx1 = std::sqrt(pos_e + 0.);
+    P.f1re = pos_x / P.xx1;
+    P.f1im = 0. / P.xx1;
+    const double s01re = P.f0re * P.xx1 - P.f1re * P.xx0;
+    const double s01im = P.f0im * P.xx1 - P.f1im * P.xx0;
+    const double N = -(2. * 1.41421356237309504880); // -RAC8 (spinor.rs:19)
+    P.ns01re = N * s01re;
+    P.ns01im = N * s01im;
+    P.nt01re = N * -s01re; // t = -conj(s)
+    P.nt01im = N * s01im;
+    P.sort = p->sort_photons ? 1 : 0;
+    return P;
+}
+
+} // namespace tpe
 
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
+using namespace tpe;
 
-struct Tables { // device copies of GF(2) jump tables for one (seeding, batch size, grid)
-    DevBuf bt, stride;
-    int kb = 0;
-};
-
-struct Ctx {
-    int device = 0;
-    int cus = 1;                    // compute units of the device
-    int blocks_per_cu[2][2] = {{1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::string err;
-    DevBuf mt;       // M^(2^k)
-    int km = 0;
-    std::map<std::string, std::unique_ptr<Tables>> tables;
-    std::map<std::string, std::vector<uint32_t>> ranf_polys; // host copies keyed by R
-    DevBuf polys, draws, origin, p0, per_batch, merged;
-    tp_stats last{};
-    bool timed = false;
-    // Host staging of the RANF origin: must outlive the asynchronous copies.
-    int32_t origin0[56];
-    int32_t p00 = 0;
-};
-
-const int KM = 32;              // intra-call offsets < 2^32 draws
-const int RANF_R = 128;         // RANF rounds per expansion segment
-const uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
-
-int fail(Ctx *c, int code, const std::string &msg)
-{
-    if (c)
-        c->err = msg;
-    return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                              \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail((ctx), TP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
-int ensure(Ctx *c, DevBuf &b, size_t bytes)
-{
-    if (b.n >= bytes && b.p)
-        return TP_OK;
-    if (b.p) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipFree(b.p);
-        b.p = nullptr;
-        b.n = 0;
-    }
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess)
-        return fail(c, TP_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    b.n = bytes;
-    return TP_OK;
-}
 
 int upload_mats(Ctx *c, DevBuf &b, const std::vector<tp::Gf2Mat> &mats)
 {
@@ -173,42 +152,6 @@ int ensure_polys(Ctx *c, int R, int K)
     return TP_OK;
 }
 
-tpd::Params make_params(const tp_params *p)
-{
-    tpd::Params P;
-    std::memset(&P, 0, sizeof P);
-    P.e_total = p->e_total;
-    P.cut_beam = p->cut_beam;
-    P.cut_pp = p->cut_pp;
-    P.e_min = p->e_min;
-    P.cut_plane = p->cut_plane;
-    P.g_a = p->g_a;
-    P.g_bp = p->g_beta_p;
-    P.g_bm = p->g_beta_m;
-    for (int i = 0; i < 5; ++i)
-        P.sc[i] = p->sigma_contribs[i];
-    // Incoming momenta (evgen.rs:66-69) and their spinor pieces (spinor.rs:37-44).
-    const double e = p->e_total;
-    P.el_x = -e / 2.;
-    P.el_e = e / 2.;
-    const double pos_x = e / 2., pos_e = e / 2.;
-    P.xx0 = std::sqrt(P.el_e + 0.);
-    P.f0re = P.el_x / P.xx0;
-    P.f0im = 0. / P.xx0;
-    P.xx1 = std::sqrt(pos_e + 0.);
-    P.f1re = pos_x / P.xx1;
-    P.f1im = 0. / P.xx1;
-    const double s01re = P.f0re * P.xx1 - P.f1re * P.xx0;
-    const double s01im = P.f0im * P.xx1 - P.f1im * P.xx0;
-    const double N = -(2. * 1.41421356237309504880); // -RAC8 (spinor.rs:19)
-    P.ns01re = N * s01re;
-    P.ns01im = N * s01im;
-    P.nt01re = N * -s01re; // t = -conj(s)
-    P.nt01im = N * s01im;
-    P.sort = p->sort_photons ? 1 : 0;
-    return P;
-}
-
 int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *state)
 {
     if (!params || !state)
@@ -220,14 +163,6 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
         return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
     return TP_OK;
-}
-
-// Persistent grid: as many batch workgroups as can be resident at once (CUs x
-// occupancy), each striding over batches; never more than there are batches.
-int grid_for(const Ctx *c, int source, int precision, uint64_t nb)
-{
-    const uint64_t g = static_cast<uint64_t>(c->cus) * static_cast<uint64_t>(c->blocks_per_cu[source][precision]);
-    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(nb, g)));
 }
 
 // Enqueue a whole run (or one closure-sized batch when batch_size == n).
@@ -393,7 +328,7 @@ extern "C" int tp_create(int device, void **ctx_out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cus = prop.multiProcessorCount;
-    for (int src = 0; src < 2; ++src)
+    for (int src = 0; src < 3; ++src)
         for (int prec = 0; prec < 2; ++prec)
             c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
     *ctx_out = c;

@@ -9,6 +9,12 @@
 // The three matrix products of every MLP run on bf16 MFMA with f32
 // accumulation; everything else is f32.
 //
+// The flow's own randomness (v, and the uniform points of the mixture) is the
+// SplitMix64 sequence used as a counter-based generator: draw k of key is
+// mix64(key + (k + 1) * 0x9E3779B97F4A7C15) (Steele, Lea and Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014), so every lane
+// computes its own draws with no skip-ahead and no buffer.
+//
 // Parameter block of one layer (f32 master copy and bf16 copy, same layout):
 //   W1[64][16] (input features 12..15 are zero padding), b1[64],
 //   W2[64][64], b2[64], W3[32][64], b3[32] with rows 0..11 = s outputs and
@@ -35,23 +41,22 @@ constexpr int SAVE_A = 0, SAVE_H1 = 16, SAVE_H2 = 80, SAVE_OB = 144, SAVE_H2B = 
 struct FlowKernelArgs {
     const uint16_t *wbf; // bf16 parameters [L][LP]
     uint64_t n;          // samples
-    // v source: xoshiro stream (s0 + 12 draws per sample from sample offset),
-    // or explicit v (n x 12 f32, device) if v_in is set
+    // Input of the forward pass: explicit v (n x 16 f32, device) when v_in is
+    // set, else the counter stream: v_ij from draw 12 (sample0 + i) + j of key.
     const float *v_in;
-    uint64_t s0[4];
-    const uint64_t *mt;
-    int32_t km;
+    uint64_t key;
     uint64_t sample0;    // stream offset of sample 0, in samples
-    // outputs
-    double *u_out;       // [n][12] f64 (may be null)
-    float *u32_out;      // [n][12] f32 (may be null)
+    // outputs (each may be null)
+    double *u_out;       // [n][12] f64
+    float *z_out;        // [n][12] f32 flow output z = logit(u)
     float *logq_out;     // [n]
-    float *zsave;        // [L][n][16] layer inputs (training), may be null
-    // inverse direction: u_in [n][12] f64 (density of given points)
+    float *zsave;        // [L][n][16] layer inputs (training)
+    // Input of the inverse pass: u_in [n][12] f64 when set, else uniform u from
+    // the counter stream (written to u_out).
     const double *u_in;
     // mixture bookkeeping for the weighted event loop: q_mix = alpha q + (1 - alpha)
     double alpha;
-    double *invq_out;    // [n] 1 / q_mix (may be null)
+    double *invq_out;    // [n] 1 / q_mix
 };
 
 struct FlowBackArgs {
@@ -63,11 +68,9 @@ struct FlowBackArgs {
     float *gv;           // [n][16] dloss/dz at the flow input (may be null)
 };
 
-// v[n][16] f32 from the xoshiro stream (features 12..15 = 0.5).
-void launch_flow_draw(const FlowKernelArgs &a, float *v, hipStream_t st);
-// u = T(v), log q (and 1/q_mix, z per layer) from v[n][16].
-void launch_flow_forward_v(const FlowKernelArgs &a, const float *v, hipStream_t st);
-// log q (and 1/q_mix) of given points u_in.
+// u = T(v), log q (and 1/q_mix, z per layer).
+void launch_flow_forward(const FlowKernelArgs &a, hipStream_t st);
+// log q (and 1/q_mix) of given or uniformly drawn points.
 void launch_flow_inverse(const FlowKernelArgs &a, hipStream_t st);
 void launch_flow_backward(const FlowBackArgs &a, hipStream_t st);
 // Weight and bias gradients over the batch: grad[L][LP] = (1/scale) sum_n ...
@@ -82,7 +85,16 @@ namespace tpd {
 struct Params;
 }
 namespace tpf {
-// f(u) = sum_c lam_c |M|^2_c cut, and gz = dloss/dz for loss = log q - log(f + eps).
+// Training integrand and the gradient of the loss log q - log g at the flow
+// output z (gz, each component clipped to [-clip, clip]), with f = sum_c
+// lam_c |M|^2_c (RAMBO weight 1) and S the cuts smoothed over tau:
+//   target 0: g = f cut + eps;  target 1: g = f S + eps;
+//   target 2: g = min(f, fcap) S (log g = log min(f, fcap) + log S,
+//   differentiated directly).
+// f_out = f cut (0) or g (1, 2); logf = log g (-100 for events target 2 drops).
 void launch_flow_integrand(const tpd::Params &P, const double lam[5], uint64_t n, const double *u, double eps,
-                           float *gz, float *logf, double *f_out, hipStream_t st);
+                           double tau, double fcap, int target, float clip, float *gz, float *logf, double *f_out,
+                           hipStream_t st);
+// *out = mean(logq - logf) over n samples.
+void launch_flow_loss(const float *logq, const float *logf, uint64_t n, float *out, hipStream_t st);
 } // namespace tpf

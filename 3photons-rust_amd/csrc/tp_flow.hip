@@ -170,38 +170,43 @@ __device__ __forceinline__ float sample_sum(float x)
     return x;
 }
 
-// ---- draws ------------------------------------------------------------------
-// v for n samples from the xoshiro stream: sample i uses draws 12 (sample0 + i)
-// .. +11; one thread per run of 64 samples (one skip-ahead per run).  The f32
-// values are clamped to [2^-24, 1 - 2^-24] so that logit(v) is finite.
-__global__ __launch_bounds__(128) void flow_draw_kernel(FlowKernelArgs a, float *v)
+// ---- counter-stream draws ----------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t first = 64 * t;
-    if (first >= a.n)
-        return;
-    uint64_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
-    const uint64_t off = 12ull * (a.sample0 + first);
-    for (int k = 0; k < a.km; ++k)
-        if ((off >> k) & 1ull)
-            tpd::gf2_apply(a.mt + 1024 * k, s);
-    const uint64_t last = first + 64 < a.n ? first + 64 : a.n;
-    for (uint64_t i = first; i < last; ++i) {
-#pragma unroll
-        for (int j = 0; j < 12; ++j)
-            v[16 * i + j] = fminf(fmaxf((float)tpd::xo_uniform(s), 0x1.0p-24f), 1.f - 0x1.0p-24f);
-#pragma unroll
-        for (int j = 12; j < 16; ++j)
-            v[16 * i + j] = 0.5f;
-    }
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t stream_u64(uint64_t key, uint64_t k)
+{
+    return mix64(key + (k + 1) * 0x9E3779B97F4A7C15ull);
+}
+// (2 m + 1) 2^-24 with m the top 23 bits: strictly inside (0, 1), so logit(v)
+// is finite.
+__device__ __forceinline__ float stream_v(uint64_t key, uint64_t k)
+{
+    return (float)(uint32_t)(stream_u64(key, k) >> 41) * 0x1.0p-23f + 0x1.0p-24f;
+}
+// (2 m + 1) 2^-53 with m the top 52 bits: strictly inside (0, 1).
+__device__ __forceinline__ double stream_u(uint64_t key, uint64_t k)
+{
+    return (double)(stream_u64(key, k) >> 12) * 0x1.0p-52 + 0x1.0p-53;
+}
+// u = sigmoid(z) in f64, kept inside (0, 1) for z beyond +-37.
+__device__ __forceinline__ double sigmoid_u(float z)
+{
+    const double u = 1. / (1. + exp(-(double)z));
+    return fmin(fmax(u, 0x1.0p-60), 0x1.fffffffffffffp-1);
 }
 
 // ---- forward ----------------------------------------------------------------
-constexpr int TILES = 8;                 // 16-sample tiles per wavefront
-constexpr int CHUNK = 4 * TILES * 16;    // samples per 256-thread block pass
+// One 256-thread block takes 4 waves x TILES tiles x 16 samples per pass.
+constexpr int FWD_TILES = 8, BWD_TILES = 4;
 
-__global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a, const float *v)
+template <int TILES>
+__global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
 {
+    constexpr int CHUNK = 4 * TILES * 16;
     __shared__ LdsLayer s;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
@@ -214,12 +219,20 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a, con
             idx[t] = base + (uint64_t)(wave * TILES + t) * 16 + (lane & 15);
             valid[t] = idx[t] < a.n;
             f4 vv = {0.5f, 0.5f, 0.5f, 0.5f};
-            if (valid[t])
-                vv = *reinterpret_cast<const f4 *>(v + 16 * idx[t] + 4 * g);
+            if (valid[t]) {
+                if (a.v_in) {
+                    vv = *reinterpret_cast<const f4 *>(a.v_in + 16 * idx[t] + 4 * g);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (4 * g + i < D)
+                            vv[i] = stream_v(a.key, 12ull * (a.sample0 + idx[t]) + 4 * g + i);
+                }
+            }
             float l = 0.f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float x = vv[i];
+                const float x = 4 * g + i < D ? vv[i] : 0.5f;
                 z[t][i] = logf(x) - log1pf(-x);
                 if (4 * g + i < D)
                     l += logf(x) + log1pf(-x); // log v (1 - v)
@@ -260,11 +273,11 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a, con
             for (int i = 0; i < 4; ++i) {
                 const int j = 4 * g + i;
                 if (j < D) {
-                    const double u = 1. / (1. + exp(-(double)z[t][i]));
+                    const double u = sigmoid_u(z[t][i]);
                     if (a.u_out)
                         a.u_out[12 * idx[t] + j] = u;
-                    if (a.u32_out)
-                        a.u32_out[12 * idx[t] + j] = (float)u;
+                    if (a.z_out)
+                        a.z_out[12 * idx[t] + j] = z[t][i];
                 }
             }
             if (g == 0) {
@@ -277,9 +290,11 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a, con
     }
 }
 
-// ---- inverse: density of given points u -------------------------------------
+// ---- inverse: density of given or uniformly drawn points --------------------
+template <int TILES>
 __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
 {
+    constexpr int CHUNK = 4 * TILES * 16;
     __shared__ LdsLayer s;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
@@ -296,8 +311,15 @@ __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
             for (int i = 0; i < 4; ++i) {
                 const int j = 4 * g + i;
                 double u = 0.5;
-                if (valid[t] && j < D)
-                    u = a.u_in[12 * idx[t] + j];
+                if (valid[t] && j < D) {
+                    if (a.u_in) {
+                        u = fmin(fmax(a.u_in[12 * idx[t] + j], 0x1.0p-60), 0x1.fffffffffffffp-1);
+                    } else {
+                        u = stream_u(a.key, 12ull * (a.sample0 + idx[t]) + j);
+                        if (a.u_out)
+                            a.u_out[12 * idx[t] + j] = u;
+                    }
+                }
                 z[t][i] = (float)(log(u) - log1p(-u));
                 if (j < D)
                     l -= (float)(log(u) + log1p(-u)); // - log u (1 - u)
@@ -345,8 +367,10 @@ __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
 // output.  Walks the layers top-down, recomputing each MLP from the saved
 // layer input, and stores the bf16 activations and deltas (feature-major) for
 // the batch weight-gradient products.
+template <int TILES>
 __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
 {
+    constexpr int CHUNK = 4 * TILES * 16;
     __shared__ LdsLayer s;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
@@ -562,27 +586,21 @@ __global__ void flow_to_bf16_kernel(const float *w, uint16_t *wbf)
 }
 
 // ---- launchers ---------------------------------------------------------------
-static unsigned grid_for(uint64_t n) { return (unsigned)((n + CHUNK - 1) / CHUNK); }
+template <int TILES> static unsigned grid_for(uint64_t n) { return (unsigned)((n + 64 * TILES - 1) / (64 * TILES)); }
 
-void launch_flow_draw(const FlowKernelArgs &a, float *v, hipStream_t st)
+void launch_flow_forward(const FlowKernelArgs &a, hipStream_t st)
 {
-    const uint64_t threads = (a.n + 63) / 64;
-    hipLaunchKernelGGL(flow_draw_kernel, dim3((unsigned)((threads + 127) / 128)), dim3(128), 0, st, a, v);
-}
-
-void launch_flow_forward_v(const FlowKernelArgs &a, const float *v, hipStream_t st)
-{
-    hipLaunchKernelGGL(flow_forward_kernel, dim3(grid_for(a.n)), dim3(256), 0, st, a, v);
+    hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
 }
 
 void launch_flow_inverse(const FlowKernelArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(flow_inverse_kernel, dim3(grid_for(a.n)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
 }
 
 void launch_flow_backward(const FlowBackArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(flow_backward_kernel, dim3(grid_for(a.n)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(flow_backward_kernel<BWD_TILES>, dim3(grid_for<BWD_TILES>(a.n)), dim3(256), 0, st, a);
 }
 
 void launch_flow_wgrad(const uint16_t *save, uint64_t n, float inv_n, float *grad, hipStream_t st)
@@ -604,12 +622,14 @@ void launch_flow_to_bf16(const float *w, uint16_t *wbf, hipStream_t st)
 }
 
 // ---- integrand and its gradient for training --------------------------------
-// f(u) = sum_c lam_c |M|^2_c(u) cut(u) from the hand-written adjoint, and
+// f(u) = sum_c lam_c |M|^2_c(u) cut(u) from the hand-written adjoint (the
+// smooth cut factor of event_adjoint when tau > 0), and
 // gz = dloss/dz at the flow output for loss = log q - log(f + eps):
 //   dloss/du = -(1 - 2u)/(u (1 - u)) - f'(u)/(f + eps),  dz = du * u (1 - u).
 __global__ __launch_bounds__(128) void flow_integrand_kernel(tpd::Params P, double lam0, double lam1, double lam2,
                                                              double lam3, double lam4, uint64_t n, const double *u,
-                                                             double eps, float *gz, float *logf, double *f_out)
+                                                             double eps, double tau, double fcap, int target,
+                                                             float clip, float *gz, float *logf, double *f_out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -620,30 +640,63 @@ __global__ __launch_bounds__(128) void flow_integrand_kernel(tpd::Params P, doub
 #pragma unroll
     for (int j = 0; j < 12; ++j)
         uu[j] = u[12 * i + j];
-    const bool keep = tpd::event_adjoint<double>(A, uu, lam, m, du, dp);
+    bool keep;
+    if (target == 2)
+        keep = tpd::event_adjoint<double, 2>(A, uu, lam, m, du, dp, tau, fcap);
+    else if (target == 1)
+        keep = tpd::event_adjoint<double, 1>(A, uu, lam, m, du, dp, tau);
+    else
+        keep = tpd::event_adjoint<double, 0>(A, uu, lam, m, du, dp);
     double f = 0.;
     if (keep)
         for (int c = 0; c < 5; ++c)
             f += lam[c] * m[c];
-    const double fe = f + eps;
+    // target 2 differentiates log f itself (du already carries the 1/f)
+    const double fe = target == 2 ? (keep ? 1. : INFINITY) : f + eps;
 #pragma unroll
     for (int j = 0; j < 12; ++j) {
         const double w = uu[j] * (1. - uu[j]);
-        gz[16 * i + j] = (float)(-(1. - 2. * uu[j]) - w * du[j] / fe);
+        const float g = (float)(-(1. - 2. * uu[j]) - w * du[j] / fe);
+        gz[16 * i + j] = fminf(fmaxf(g, -clip), clip);
     }
 #pragma unroll
     for (int j = 12; j < 16; ++j)
         gz[16 * i + j] = 0.f;
-    logf[i] = (float)log(fe);
+    logf[i] = (float)(target == 2 ? (keep ? log(f) : -100.) : log(fe)); // dropped events: -100
     if (f_out)
         f_out[i] = f;
 }
 
 void launch_flow_integrand(const tpd::Params &P, const double lam[5], uint64_t n, const double *u, double eps,
-                           float *gz, float *logf, double *f_out, hipStream_t st)
+                           double tau, double fcap, int target, float clip, float *gz, float *logf, double *f_out,
+                           hipStream_t st)
 {
     hipLaunchKernelGGL(flow_integrand_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, P, lam[0], lam[1],
-                       lam[2], lam[3], lam[4], n, u, eps, gz, logf, f_out);
+                       lam[2], lam[3], lam[4], n, u, eps, tau, fcap, target, clip, gz, logf, f_out);
+}
+
+// Mean over the batch of log q - log(f + eps), one 256-thread block, fixed
+// summation order (per-thread strided sums, then a tree).
+__global__ __launch_bounds__(256) void flow_loss_kernel(const float *logq, const float *logf, uint64_t n, float *out)
+{
+    __shared__ double red[256];
+    double acc = 0.;
+    for (uint64_t i = threadIdx.x; i < n; i += 256)
+        acc += (double)logq[i] - (double)logf[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *out = (float)(red[0] / (double)n);
+}
+
+void launch_flow_loss(const float *logq, const float *logf, uint64_t n, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(flow_loss_kernel, dim3(1), dim3(256), 0, st, logq, logf, n, out);
 }
 
 } // namespace tpf

@@ -29,6 +29,11 @@ struct FusedArgs {
     // buffer source: RANF values (int32 in [0, 1e9)) of this launch's batches,
     // batch (b - b_first) at offset 12*batch_size*(b - b_first).
     const int32_t *draws;
+    // weighted source (importance sampling): uniforms u[12] (f64) and weight
+    // 1/q(u) per event, event e of batch (b - b_first) at index
+    // batch_size*(b - b_first) + e; every |M|^2 is multiplied by its weight.
+    const double *u;
+    const double *invq;
     // output: one accumulator per batch, index b - b_first
     void *out;             // tp_acc*
 };
@@ -49,7 +54,8 @@ struct RanfArgs {
     int32_t *out;
 };
 
-void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer*/, int grid, hipStream_t st);
+void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer, 2 weighted (f64 only)*/, int grid,
+                  hipStream_t st);
 int fused_blocks_per_cu(int source, int precision); // resident workgroups per CU (occupancy)
 // Per-event diagnostics: one thread per event, same device code as the loop.
 // Event e draws from offset 12*e (xoshiro: a.s0 with a.mt; buffer: a.draws).

@@ -156,7 +156,8 @@ constexpr int QCAP = 128;
 template <int SOURCE, int PREC>
 __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
 {
-    __shared__ double queue[12][QCAP]; // photon momenta, structure of arrays
+    constexpr int QROWS = SOURCE == 2 ? 13 : 12; // momenta (+ event weight)
+    __shared__ double queue[QROWS][QCAP];       // structure of arrays
     __shared__ double red[1][13];
     const uint32_t lane = threadIdx.x;
     const uint64_t lanemask_lt = (1ull << lane) - 1ull;
@@ -192,23 +193,31 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
             s[3] = st[3];
         }
         const int32_t *src = (SOURCE == 1) ? a.draws + (size_t)lb * 12 * a.batch_size : nullptr;
+        const size_t wbase = (size_t)lb * a.batch_size;
         Acc<PREC> A;
         acc_zero(A);
         uint32_t qn = 0; // queued events (uniform)
         for (uint32_t k = 0; k < iters; ++k) {
             bool pass = false;
             double p[3][4];
+            double wt = 1.;
             if (k < len) {
                 double u[12];
                 if (SOURCE == 0) {
 #pragma unroll
                     for (int j = 0; j < 12; ++j)
                         u[j] = xo_uniform(s);
-                } else {
+                } else if (SOURCE == 1) {
                     const int32_t *e = src + 12 * (size_t)(k * 64 + lane);
 #pragma unroll
                     for (int j = 0; j < 12; ++j)
                         u[j] = (double)e[j] * 1e-9;
+                } else {
+                    const size_t e = wbase + (size_t)(k * 64 + lane);
+#pragma unroll
+                    for (int j = 0; j < 12; ++j)
+                        u[j] = a.u[12 * e + j];
+                    wt = a.invq[e];
                 }
                 pass = generate_keep<PREC>(P, u, p);
             }
@@ -220,6 +229,8 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         queue[4 * i + j][slot] = p[i][j];
+                if (SOURCE == 2)
+                    queue[QROWS - 1][slot] = wt;
             }
             qn += (uint32_t)__popcll(mask);
             if (qn >= 64) {
@@ -231,9 +242,14 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         pq[i][j] = queue[4 * i + j][slot];
+                const double wq = SOURCE == 2 ? queue[QROWS - 1][slot] : 1.;
                 __syncthreads();
                 qn -= 64;
                 matrix_elements<PREC>(P, pq, m);
+                if (SOURCE == 2)
+#pragma unroll
+                    for (int c = 0; c < 5; ++c)
+                        m[c] = m[c] * wq;
                 acc_add(A, P, m);
             }
         }
@@ -247,6 +263,12 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
                     for (int j = 0; j < 4; ++j)
                         pq[i][j] = queue[4 * i + j][lane];
                 matrix_elements<PREC>(P, pq, m);
+                if (SOURCE == 2) {
+                    const double wq = queue[QROWS - 1][lane];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c)
+                        m[c] = m[c] * wq;
+                }
                 acc_add(A, P, m);
             }
         }
@@ -268,6 +290,8 @@ int fused_blocks_per_cu(int source, int precision)
 {
     if (source == 0)
         return precision == 0 ? blocks_per_cu<0, 0>() : blocks_per_cu<0, 1>();
+    if (source == 2)
+        return blocks_per_cu<2, 0>();
     return precision == 0 ? blocks_per_cu<1, 0>() : blocks_per_cu<1, 1>();
 }
 
@@ -279,11 +303,13 @@ void launch_fused(const FusedArgs &a, int source, int grid, hipStream_t st)
             hipLaunchKernelGGL((fused_kernel<0, 0>), g, b, 0, st, a);
         else
             hipLaunchKernelGGL((fused_kernel<0, 1>), g, b, 0, st, a);
-    } else {
+    } else if (source == 1) {
         if (a.precision == 0)
             hipLaunchKernelGGL((fused_kernel<1, 0>), g, b, 0, st, a);
         else
             hipLaunchKernelGGL((fused_kernel<1, 1>), g, b, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((fused_kernel<2, 0>), g, b, 0, st, a);
     }
 }
 

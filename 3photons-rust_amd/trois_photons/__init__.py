@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -74,6 +75,19 @@ class TpConfig(C.Structure):
         (n, C.c_int32) for n in ("num_bins", "impr", "plot", "parsed")]
 
 
+class TpFlowOpts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("batch", C.c_uint64), ("steps", C.c_int32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("adam_eps", C.c_float), ("grad_clip", C.c_float),
+                ("eps_rel", C.c_double), ("cut_width", C.c_double), ("target", C.c_int32),
+                ("cap_rel", C.c_double), ("seed", C.c_uint64)]
+
+
+FLOW_NPARAM = 43968
+FLOW_LAYER = 7328
+FLOW_COND_MASKS = (0x03F, 0xFC0, 0x555, 0xAAA, 0x1C7, 0xE38)
+UNIFORM_KEY_XOR = 0x5555555555555555
+
+
 class TpFinal(C.Structure):
     _fields_ = [("selected_events", C.c_uint64), ("spm2", (C.c_double * 5) * 2), ("vars", (C.c_double * 5) * 2)] + [
         (n, C.c_double) for n in ("sigma", "prec", "variance", "beta_min", "ss_p", "inc_ss_p", "ss_m", "inc_ss_m")]
@@ -98,6 +112,20 @@ def _load() -> C.CDLL:
         "tp_events": (C.c_int, [vp, P(TpParams), C.c_int, vp, C.c_uint32, vp, vp, vp, vp]),
         "tp_adjoint": (C.c_int, [vp, P(TpParams), C.c_int, vp, vp, C.c_uint64, C.c_int, C.c_int, vp, vp, vp]),
         "tp_sync": (C.c_int, [vp, P(TpStats)]),
+        "tp_flow_default_opts": (C.c_int, [P(TpFlowOpts)]),
+        "tp_flow_create": (C.c_int, [vp, C.c_uint64, P(vp)]),
+        "tp_flow_destroy": (C.c_int, [vp]),
+        "tp_flow_get_params": (C.c_int, [vp, vp]),
+        "tp_flow_set_params": (C.c_int, [vp, vp]),
+        "tp_flow_train": (C.c_int, [vp, P(TpParams), P(TpFlowOpts), vp, P(TpStats)]),
+        "tp_flow_sample": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, C.c_int, vp, vp, vp]),
+        "tp_flow_density": (C.c_int, [vp, C.c_uint64, vp, C.c_int, vp]),
+        "tp_flow_grad": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_int, vp, vp]),
+        "tp_flow_integrand": (C.c_int, [vp, P(TpParams), C.c_uint64, vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_float, C.c_int, vp, vp, vp]),
+        "tp_flow_run_device": (C.c_int, [vp, P(TpParams), C.c_double, C.c_uint64, C.c_uint64, vp, vp,
+                                         P(TpStats)]),
+        "tp_flow_run": (C.c_int, [vp, P(TpParams), C.c_double, C.c_uint64, C.c_uint64, vp, P(TpAcc), P(TpStats)]),
         "tp_config_load": (C.c_int, [C.c_char_p, P(TpConfig), C.c_char_p, C.c_int]),
         "tp_config_format": (C.c_int, [P(TpConfig), C.c_char_p, C.c_int]),
         "tp_params_from_config": (C.c_int, [P(TpConfig), P(TpParams)]),
@@ -189,6 +217,8 @@ def merge(accs) -> TpAcc:
     """Left fold in batch order (resacc.rs:133-139)."""
     out = TpAcc.from_buffer_copy(accs[0])
     for a in accs[1:]:
+        if not isinstance(a, TpAcc):  # element of an ACC_DTYPE array
+            a = TpAcc.from_buffer_copy(a)
         lib.tp_acc_merge(C.byref(out), C.byref(a))
     return out
 
@@ -219,9 +249,12 @@ class Engine:
         if rc != TP_OK:
             raise TpError(f"tp_create(device={device}) failed with status {rc}: no usable GPU")
         self.h = h
+        self._flows = weakref.WeakSet()  # flows use the context: closed first
 
     def close(self):
         if self.h:
+            for f in list(self._flows):
+                f.close()
             lib.tp_destroy(self.h)
             self.h = None
 
@@ -296,6 +329,111 @@ class Engine:
         self._check(lib.tp_events(self.h, C.byref(params), kind, C.byref(st), n, ptr(u), ptr(p), ptr(k), ptr(m)),
                     "tp_events")
         return u, p, k, m
+
+
+def flow_default_opts(**kw) -> TpFlowOpts:
+    o = TpFlowOpts()
+    lib.tp_flow_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Flow:
+    """RealNVP proposal on the 12 RAMBO uniforms, bound to an Engine
+    (tp_flow_* entry points).  Host arrays in and out unless a method says
+    otherwise."""
+
+    def __init__(self, engine: Engine, init_seed: int = 0):
+        h = C.c_void_p()
+        self.engine = engine
+        engine._check(lib.tp_flow_create(engine.h, init_seed, C.byref(h)), "tp_flow_create")
+        self.h = h
+        engine._flows.add(self)
+
+    def close(self):
+        if self.h:
+            lib.tp_flow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        self.engine._check(rc, what)
+
+    def get_params(self) -> np.ndarray:
+        w = np.zeros(FLOW_NPARAM, np.float32)
+        self._check(lib.tp_flow_get_params(self.h, _ptr(w)), "tp_flow_get_params")
+        return w
+
+    def set_params(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == FLOW_NPARAM
+        self._check(lib.tp_flow_set_params(self.h, _ptr(w)), "tp_flow_set_params")
+
+    def train(self, params: TpParams, opts: TpFlowOpts):
+        """(loss per step, TpStats)."""
+        loss = np.zeros(max(1, opts.steps), np.float32)
+        st = TpStats()
+        self._check(lib.tp_flow_train(self.h, C.byref(params), C.byref(opts), _ptr(loss), C.byref(st)),
+                    "tp_flow_train")
+        return loss[:opts.steps], st
+
+    def sample(self, n: int, key: int = 0, sample0: int = 0, v: np.ndarray | None = None):
+        """(u[n,12] f64, z[n,12] f32, log q[n] f32)."""
+        u = np.zeros((n, 12)); z = np.zeros((n, 12), np.float32); lq = np.zeros(n, np.float32)
+        vin = None if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(n, 16)
+        self._check(lib.tp_flow_sample(self.h, key, sample0, n, _ptr(vin), 0, _ptr(u), _ptr(z), _ptr(lq)),
+                    "tp_flow_sample")
+        return u, z, lq
+
+    def density(self, u: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 12)
+        lq = np.zeros(len(u), np.float32)
+        self._check(lib.tp_flow_density(self.h, len(u), _ptr(u), 0, _ptr(lq)), "tp_flow_density")
+        return lq
+
+    def grad(self, v: np.ndarray, gz: np.ndarray):
+        """(d/dparams, d/dz0 per sample) of (1/n) sum_i [gz_i . z_i - sum s]."""
+        n = len(v)
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(n, 16)
+        gz = np.ascontiguousarray(gz, dtype=np.float32).reshape(n, 16)
+        g = np.zeros(FLOW_NPARAM, np.float32); gv = np.zeros((n, 16), np.float32)
+        self._check(lib.tp_flow_grad(self.h, n, _ptr(v), _ptr(gz), 0, _ptr(g), _ptr(gv)), "tp_flow_grad")
+        return g, gv
+
+    def integrand(self, params: TpParams, u: np.ndarray, eps: float, tau: float = 0.0, target: int = 0,
+                  clip: float = 1e30, fcap: float = float("inf")):
+        """(f[n] f64, gz[n,16] f32, log g[n] f32) of the training target (tp_flow_integrand)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 12)
+        n = len(u)
+        f = np.zeros(n); gz = np.zeros((n, 16), np.float32); lf = np.zeros(n, np.float32)
+        self._check(lib.tp_flow_integrand(self.h, C.byref(params), n, _ptr(u), eps, tau, fcap, target, clip, 0,
+                                          _ptr(f), _ptr(gz), _ptr(lf)), "tp_flow_integrand")
+        return f, gz, lf
+
+    def run(self, params: TpParams, alpha: float, key: int, num_events: int):
+        """Weighted run: (per-batch accumulators, merged TpAcc, TpStats), host side."""
+        nb = (num_events + BATCH_SIZE - 1) // BATCH_SIZE
+        per = np.zeros(nb, dtype=ACC_DTYPE)
+        merged, stats = TpAcc(), TpStats()
+        self._check(lib.tp_flow_run(self.h, C.byref(params), alpha, key, num_events, per.ctypes.data_as(C.c_void_p),
+                                    C.byref(merged), C.byref(stats)), "tp_flow_run")
+        return per, merged, stats
+
+    def run_device(self, params: TpParams, alpha: float, key: int, num_events: int, per_batch_ptr: int,
+                   merged_ptr: int | None) -> None:
+        self._check(lib.tp_flow_run_device(self.h, C.byref(params), alpha, key, num_events,
+                                           C.c_void_p(per_batch_ptr), C.c_void_p(merged_ptr or 0), None),
+                    "tp_flow_run_device")
 
 
 @dataclass

@@ -178,6 +178,90 @@ int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, const void *rng
  * last tp_run_device when stats is non-null. */
 int tp_sync(void *ctx, tp_stats *stats);
 
+/* ---- Learned proposal ---------------------------------------------------
+ * No reference counterpart: the reference samples RAMBO's 12 uniforms flat
+ * (evgen.rs:174-207).  These entry points add a RealNVP normalising flow on
+ * the 12-dimensional unit cube (6 affine coupling layers, each with a
+ * 12 -> 64 -> 64 -> 24 SiLU MLP on bf16 MFMA with f32 accumulation), trained
+ * on the GPU by reverse KL through the adjoint of the event integrand, and a
+ * weighted event loop that samples the mixture alpha q + (1 - alpha) U and
+ * feeds the same per-batch accumulators (tp_acc) with every |M|^2 multiplied
+ * by 1 / q_mix, so tp_finalize yields the cross-section and its error as for
+ * a plain run.
+ *
+ * Parameters (f32, TP_FLOW_NPARAM values): per layer l, at l * 7328,
+ *   W1[64][16] (inputs 12..15 padding), b1[64], W2[64][64], b2[64],
+ *   W3[32][64], b3[32] (rows 0..11 = s, rows 16..27 = t, others padding).
+ * Layer l keeps the features whose bit is set in
+ *   {0x03F, 0xFC0, 0x555, 0xAAA, 0x1C7, 0xE38}[l]
+ * and maps the others as z <- z exp(1.5 tanh(s_raw)) + t.
+ * The flow's randomness is a counter-based SplitMix64 stream: draw k of key K
+ * is mix64(K + (k + 1) 0x9E3779B97F4A7C15); sample i uses draws 12 i .. 12 i + 11. */
+#define TP_FLOW_NPARAM 43968
+
+typedef struct {
+    double alpha;       /* flow share of the sampling mixture (0.9) */
+    uint64_t batch;     /* samples per training step (65536; multiple of 16) */
+    int32_t steps;      /* training steps (2000) */
+    float lr;           /* Adam step size (3e-3) */
+    float beta1, beta2, adam_eps; /* Adam moments (0.9, 0.999, 1e-8) */
+    float grad_clip;    /* per-sample, per-feature clip of dloss/dz (100) */
+    double eps_rel;     /* eps of log(f + eps), relative to the mean of f over the cube (1e-3) */
+    double cut_width;   /* training integrand: cuts smoothed over this width tau of their
+                           dimensionless margins (0.05) */
+    int32_t target;     /* training integrand g, loss log q - log g:
+                           0: f cut + eps (hard cuts, whose boundaries the gradient cannot see),
+                           1: f S + eps, 2: min(f, fcap) S with log g = log min(f, fcap) + log S (2) */
+    double cap_rel;     /* target 2: fcap relative to the mean of f cut over the cube (10) */
+    uint64_t seed;      /* key of the training stream (1) */
+} tp_flow_opts;
+
+int tp_flow_default_opts(tp_flow_opts *opts);
+/* Flow bound to a context; weights initialised from init_seed (hidden layers
+ * uniform in +-1/sqrt(fan_in), output layer zero: the identity map). */
+int tp_flow_create(void *ctx, uint64_t init_seed, void **flow_out);
+int tp_flow_destroy(void *flow);
+int tp_flow_get_params(void *flow, float *w);       /* host, TP_FLOW_NPARAM */
+int tp_flow_set_params(void *flow, const float *w); /* host, TP_FLOW_NPARAM; resets Adam */
+/* Reverse-KL training: per step, batch samples u ~ q, loss mean(log q(u) -
+ * log g(u)) with g built from f = sum_c sigma_contribs[c] |M|^2_c as opts
+ * target says, gradient through the flow and the hand-written adjoint of f,
+ * one Adam update.
+ * loss (host, steps values) may be null; stats gets the device time. */
+int tp_flow_train(void *flow, const tp_params *params, const tp_flow_opts *opts, float *loss, tp_stats *stats);
+/* n samples of the flow: u[n][12] (f64), z[n][12] = logit(u) (f32) and
+ * log q(u) (f32), from the stream (key, samples sample0 ..) or, when v_in is
+ * non-null, from the given base points v_in[n][16] (f32, columns 12..15
+ * ignored).  Pointers are device pointers when on_device is 1; any output
+ * may be null. */
+int tp_flow_sample(void *flow, uint64_t key, uint64_t sample0, uint64_t n, const float *v_in, int on_device,
+                   double *u, float *z, float *logq);
+/* log q(u) of n given points u[n][12]. */
+int tp_flow_density(void *flow, uint64_t n, const double *u, int on_device, float *logq);
+/* Gradient of (1/n) sum_i [gz_i . z_i - sum_l sum_j s_lij] (z_i the flow
+ * output of base point v_i; the second term is log q's Jacobian part) with
+ * respect to the parameters (grad, TP_FLOW_NPARAM) and, per sample, to
+ * z0_i = logit(v_i) (gv[n][16], without the 1/n).  n must be a multiple of 16. */
+int tp_flow_grad(void *flow, uint64_t n, const float *v_in, const float *gz, int on_device, float *grad, float *gv);
+/* The training integrand at n given points: f cut (target 0) or g (1, 2),
+ * gz = d/dz[-sum log u(1-u) - log g] clipped to +-clip (n x 16, columns
+ * 12..15 zero) and log g, with g as tp_flow_opts.target defines (RAMBO
+ * weight 1, smoothed cuts of width tau, cap fcap). */
+int tp_flow_integrand(void *flow, const tp_params *params, uint64_t n, const double *u, double eps, double tau,
+                      double fcap, int target, float clip, int on_device, double *f, float *gz, float *logf);
+/* Weighted event loop: num_events events in batches of 10 000, the first
+ * round(alpha * n) events of every chunk of batches drawn from the flow (key,
+ * draws continuing across chunks), the rest uniform (key ^ 0x5555...5),
+ * every event weighted by 1 / (alpha q(u) + 1 - alpha); per-batch
+ * accumulators and (if merged_dev is non-null) their in-order fold in device
+ * memory, as tp_run_device.  Returns when enqueued; tp_sync waits. */
+int tp_flow_run_device(void *flow, const tp_params *params, double alpha, uint64_t key, uint64_t num_events,
+                       tp_acc *per_batch_dev, tp_acc *merged_dev, tp_stats *stats);
+/* tp_flow_run_device with host outputs (per_batch_out: ceil(n / 10 000)
+ * accumulators; merged_out may be null), like tp_run; returns when done. */
+int tp_flow_run(void *flow, const tp_params *params, double alpha, uint64_t key, uint64_t num_events,
+                tp_acc *per_batch_out, tp_acc *merged_out, tp_stats *stats);
+
 /* Events per batch (scheduling/mod.rs:21). */
 #define TP_BATCH_SIZE 10000
 
