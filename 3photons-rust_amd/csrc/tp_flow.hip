@@ -12,36 +12,45 @@
 #include "tp_adjoint.h"
 #include "tp_xoshiro.h"
 
+// The network code below is not bound to any reference operation order:
+// let the compiler fuse multiply-adds.  (The physics included above keeps
+// the build's contraction-free setting; it is switched back off before the
+// integrand kernel at the end of this file.)
+#pragma clang fp contract(fast)
+
 namespace tpf {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float x)
+constexpr float LOG2E = 1.4426950408889634f, LN2 = 0.6931471805599453f;
+
+__device__ __forceinline__ uint16_t f2bf(float x) // round to nearest even (v_cvt_pk_bf16_f32)
 {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u); // round to nearest even
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint16_t, (__bf16)x);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ s4 pack4(f4 v)
 {
-    s4 r;
-    r.x = (short)f2bf(v.x);
-    r.y = (short)f2bf(v.y);
-    r.z = (short)f2bf(v.z);
-    r.w = (short)f2bf(v.w);
-    return r;
+    const bf4 b = {(__bf16)v.x, (__bf16)v.y, (__bf16)v.z, (__bf16)v.w};
+    return __builtin_bit_cast(s4, b);
 }
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
-__device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
+// Hardware exp2 / log2 / reciprocal (about 1 ulp): the network's activations
+// are rounded to bf16 anyway.
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * LN2; }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sigm(float x) { return frcp(1.f + fexp(-x)); }
 __device__ __forceinline__ float silu(float x) { return x * sigm(x); }
 __device__ __forceinline__ float dsilu(float x)
 {
     const float s = sigm(x);
     return s * (1.f + x * (1.f - s));
 }
-__device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + log1pf(__expf(-fabsf(x))); }
+__device__ __forceinline__ float ftanh(float x) { return 1.f - 2.f * frcp(fexp(2.f * x) + 1.f); }
+__device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + flog(1.f + fexp(-fabsf(x))); }
 
 // ---- LDS image of one layer (bf16, padded row strides against bank conflicts)
 constexpr int S_W1 = 20, S_W2 = 68, S_W3 = 68, S_W1T = 68, S_W2T = 68, S_W3T = 36;
@@ -91,6 +100,9 @@ __device__ __forceinline__ s4 atile(const uint16_t *m, int stride, int mb, int k
     return *reinterpret_cast<const s4 *>(m + r * stride + c);
 }
 
+// Bias values of rows r0 .. r0 + 3 (the lane's rows of an output tile).
+__device__ __forceinline__ f4 bias4(const LdsLayer &s, int r0) { return *reinterpret_cast<const f4 *>(s.b + r0); }
+
 struct MlpFwd {
     s4 a;          // bf16 input (features 4g..4g+3)
     f4 h1p[4];     // pre-activations, 4 row blocks
@@ -104,13 +116,9 @@ __device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &
 {
     const int g = (threadIdx.x & 63) >> 4;
     r.a = pack4(a_in);
-    const f4 zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-        f4 c = mfma(atile(s.w + L_W1, S_W1, mb, 0), r.a, zero);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            c[i] += s.b[16 * mb + 4 * g + i];
+        const f4 c = mfma(atile(s.w + L_W1, S_W1, mb, 0), r.a, bias4(s, 16 * mb + 4 * g));
         r.h1p[mb] = c;
         f4 h;
 #pragma unroll
@@ -120,13 +128,10 @@ __device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &
     }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-        f4 c = zero;
+        f4 c = bias4(s, H + 16 * mb + 4 * g);
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
             c = mfma(atile(s.w + L_W2, S_W2, mb, kb), r.h1[kb], c);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            c[i] += s.b[H + 16 * mb + 4 * g + i];
         r.h2p[mb] = c;
         f4 h;
 #pragma unroll
@@ -134,16 +139,11 @@ __device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &
             h[i] = silu(c[i]);
         r.h2[mb] = pack4(h);
     }
-    f4 o0 = zero, o1 = zero;
+    f4 o0 = bias4(s, 2 * H + 4 * g), o1 = bias4(s, 2 * H + 16 + 4 * g);
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
         o0 = mfma(atile(s.w + L_W3, S_W3, 0, kb), r.h2[kb], o0);
         o1 = mfma(atile(s.w + L_W3, S_W3, 1, kb), r.h2[kb], o1);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        o0[i] += s.b[2 * H + 4 * g + i];
-        o1[i] += s.b[2 * H + 16 + 4 * g + i];
     }
     r.o0 = o0;
     r.o1 = o1;
@@ -192,11 +192,13 @@ __device__ __forceinline__ double stream_u(uint64_t key, uint64_t k)
 {
     return (double)(stream_u64(key, k) >> 12) * 0x1.0p-52 + 0x1.0p-53;
 }
-// u = sigmoid(z) in f64, kept inside (0, 1) for z beyond +-37.
+// u = sigmoid(z), evaluated in f32 (the density is that of the f32 network;
+// the point moves by at most an f32 rounding), stored as f64 and kept inside
+// (0, 1): at most 1 - 2^-53 where the f32 value rounds to 1.
 __device__ __forceinline__ double sigmoid_u(float z)
 {
-    const double u = 1. / (1. + exp(-(double)z));
-    return fmin(fmax(u, 0x1.0p-60), 0x1.fffffffffffffp-1);
+    const double u = (double)sigm(z);
+    return fmin(fmax(u, 0x1.0p-126), 0x1.fffffffffffffp-1);
 }
 
 // ---- forward ----------------------------------------------------------------
@@ -233,9 +235,10 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float x = 4 * g + i < D ? vv[i] : 0.5f;
-                z[t][i] = logf(x) - log1pf(-x);
+                const float lx = flog(x), l1x = flog(1.f - x);
+                z[t][i] = lx - l1x;
                 if (4 * g + i < D)
-                    l += logf(x) + log1pf(-x); // log v (1 - v)
+                    l += lx + l1x; // log v (1 - v)
             }
             ld[t] = l;
         }
@@ -251,12 +254,12 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
                 MlpFwd r;
                 mlp_forward(s, masked(z[t], cond), r);
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (transformed(cond, 4 * g + i)) {
-                        const float sc = FLOW_SCALE * tanhf(r.o0[i]);
-                        z[t][i] = z[t][i] * __expf(sc) + r.o1[i];
-                        ld[t] -= sc;
-                    }
+                for (int i = 0; i < 4; ++i) { // branch-free: exp(0) = 1 exactly
+                    const bool tr = transformed(cond, 4 * g + i);
+                    const float sc = tr ? FLOW_SCALE * ftanh(r.o0[i]) : 0.f;
+                    z[t][i] = z[t][i] * fexp(sc) + (tr ? r.o1[i] : 0.f);
+                    ld[t] -= sc;
+                }
             }
         }
 #pragma unroll
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
                 if (a.logq_out)
                     a.logq_out[idx[t]] = l;
                 if (a.invq_out)
-                    a.invq_out[idx[t]] = 1. / (a.alpha * exp((double)l) + (1. - a.alpha));
+                    a.invq_out[idx[t]] = 1. / (a.alpha * (double)fexp(l) + (1. - a.alpha));
             }
         }
     }
@@ -313,16 +316,18 @@ __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
                 double u = 0.5;
                 if (valid[t] && j < D) {
                     if (a.u_in) {
-                        u = fmin(fmax(a.u_in[12 * idx[t] + j], 0x1.0p-60), 0x1.fffffffffffffp-1);
+                        u = fmin(fmax(a.u_in[12 * idx[t] + j], 0x1.0p-126), 0x1.fffffffffffffp-1);
                     } else {
                         u = stream_u(a.key, 12ull * (a.sample0 + idx[t]) + j);
                         if (a.u_out)
                             a.u_out[12 * idx[t] + j] = u;
                     }
                 }
-                z[t][i] = (float)(log(u) - log1p(-u));
+                // u and 1 - u formed in f64, logs in f32
+                const float lu = flog((float)u), l1u = flog((float)(1. - u));
+                z[t][i] = lu - l1u;
                 if (j < D)
-                    l -= (float)(log(u) + log1p(-u)); // - log u (1 - u)
+                    l -= lu + l1u; // - log u (1 - u)
             }
             ld[t] = l;
         }
@@ -336,12 +341,12 @@ __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
                 MlpFwd r;
                 mlp_forward(s, masked(z[t], cond), r);
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (transformed(cond, 4 * g + i)) {
-                        const float sc = FLOW_SCALE * tanhf(r.o0[i]);
-                        z[t][i] = (z[t][i] - r.o1[i]) * __expf(-sc);
-                        ld[t] -= sc;
-                    }
+                for (int i = 0; i < 4; ++i) { // branch-free: exp(0) = 1 exactly
+                    const bool tr = transformed(cond, 4 * g + i);
+                    const float sc = tr ? FLOW_SCALE * ftanh(r.o0[i]) : 0.f;
+                    z[t][i] = (z[t][i] - (tr ? r.o1[i] : 0.f)) * fexp(-sc);
+                    ld[t] -= sc;
+                }
             }
         }
 #pragma unroll
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
                 if (a.logq_out)
                     a.logq_out[idx[t]] = l;
                 if (a.invq_out)
-                    a.invq_out[idx[t]] = 1. / (a.alpha * exp((double)l) + (1. - a.alpha));
+                    a.invq_out[idx[t]] = 1. / (a.alpha * (double)fexp(l) + (1. - a.alpha));
             }
         }
     }
@@ -400,8 +405,8 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
                 for (int i = 0; i < 4; ++i) {
                     const int j = 4 * g + i;
                     if (transformed(cond, j)) {
-                        const float th = tanhf(r.o0[i]);
-                        const float es = __expf(FLOW_SCALE * th);
+                        const float th = ftanh(r.o0[i]);
+                        const float es = fexp(FLOW_SCALE * th);
                         const float sbar = gz[t][i] * zin[i] * es - 1.f; // the -sum s term of log q
                         ob0[i] = sbar * FLOW_SCALE * (1.f - th * th);
                         ob1[i] = gz[t][i];
@@ -622,6 +627,9 @@ void launch_flow_to_bf16(const float *w, uint16_t *wbf, hipStream_t st)
 }
 
 // ---- integrand and its gradient for training --------------------------------
+// Physics from here on: no contraction, as in the rest of the build.
+#pragma clang fp contract(off)
+
 // f(u) = sum_c lam_c |M|^2_c(u) cut(u) from the hand-written adjoint (the
 // smooth cut factor of event_adjoint when tau > 0), and
 // gz = dloss/dz at the flow output for loss = log q - log(f + eps):

@@ -5,10 +5,14 @@ stream (seed_from_u64(12345)), 1e9 events = 1e5 batches of 10 000, batch
 accumulators and their in-order fold left in HBM (tp_run_device).  The f64
 path gives `value`; the mixed-precision path (f32 spinor/amplitude algebra)
 is timed on the same stream and reported beside it with its per-quantity
-relative error against the f64 run.  Prints ONE JSON line.
+relative error against the f64 run.  Also reported: the adjoint (2^24
+points) and the learned proposal (training, then the same number of
+weighted events, sigma and GPU-seconds against plain RAMBO at equal
+error).  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--events N]
-                  [--no-mixed] [--no-adjoint] [--cpu-sample N] [--no-cpu]
+                  [--no-mixed] [--no-adjoint] [--no-flow] [--flow-steps S]
+                  [--flow-batch B] [--cpu-sample N] [--no-cpu]
   python bench.py --copy-calibration   # 3 plain 1 GiB device copies, for
                                        # scaling FETCH_SIZE / WRITE_SIZE
 """
@@ -112,6 +116,44 @@ def measure_adjoint(tp, torch, eng, params, n: int) -> dict:
     return res
 
 
+def measure_flow(tp, eng, cfg, plain_value, plain_acc, events, steps, batch) -> dict:
+    """Learned proposal: reverse-KL training on the GPU (steps x batch), then
+    `events` weighted events from the 0.9 flow / 0.1 uniform mixture; sigma,
+    its standard error and the GPU-seconds plain RAMBO needs for the same
+    error, from the plain f64 run of the same size."""
+    c = cfg.with_events(events)
+    pc = tp.params_from_config(c)
+    fl = tp.Flow(eng, init_seed=1)
+    opts = tp.flow_default_opts(steps=steps, batch=batch)
+    t0 = time.perf_counter()
+    loss, st = fl.train(pc, opts)
+    train_s = time.perf_counter() - t0
+    nb = (events + tp.BATCH_SIZE - 1) // tp.BATCH_SIZE
+    fl.run(pc, opts.alpha, 99, min(events, 10 * tp.BATCH_SIZE))  # warm-up
+    t0 = time.perf_counter()
+    _, accf, sf = fl.run(pc, opts.alpha, 12345, events)
+    run_s = time.perf_counter() - t0
+    rf, rp = tp.finalize(c, accf), tp.finalize(c, plain_acc)
+    se_f, se_p = rf.sigma * rf.prec, rp.sigma * rp.prec
+    plain_s = events / plain_value
+    # plain GPU-seconds to reach the flow's standard error (error ~ 1/sqrt(N))
+    plain_equal = plain_s * (se_p / se_f) ** 2
+    fl.close()
+    return {
+        "train": {"steps": steps, "batch": batch, "wall_s": train_s, "device_ms": st.device_ms,
+                  "loss_first": float(loss[:max(1, steps // 20)].mean()),
+                  "loss_last": float(loss[-max(1, steps // 20):].mean())},
+        "events": events, "alpha": opts.alpha,
+        "sigma_flow": rf.sigma, "se_flow": se_f, "sigma_plain": rp.sigma, "se_plain": se_p,
+        "sigma_diff_in_se": (rf.sigma - rp.sigma) / (se_f ** 2 + se_p ** 2) ** 0.5,
+        "variance_gain_per_event": (rp.prec / rf.prec) ** 2,
+        "gpu_s_flow_run": run_s, "gpu_s_flow_kernels": sf.device_ms / 1e3, "gpu_s_plain_same_events": plain_s,
+        "gpu_s_plain_at_flow_error": plain_equal,
+        "gpu_s_ratio_excl_training": plain_equal / run_s,
+        "gpu_s_ratio_incl_training": plain_equal / (run_s + train_s),
+    }
+
+
 def copy_calibration() -> int:
     """Three device-to-device copies of 1 GiB: the reference point that turns
     the FETCH_SIZE / WRITE_SIZE counters of a profiled run into bytes."""
@@ -139,6 +181,9 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-points", type=int, default=1 << 24)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--flow-steps", type=int, default=2000)
+    ap.add_argument("--flow-batch", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the event loop shards by "
@@ -203,6 +248,8 @@ def main() -> int:
         }
     if not args.no_adjoint:
         out["adjoint"] = measure_adjoint(tp, torch, eng, p64, args.adjoint_points)
+    if not args.no_flow:
+        out["flow"] = measure_flow(tp, eng, cfg, value, acc64, args.events, args.flow_steps, args.flow_batch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out))
