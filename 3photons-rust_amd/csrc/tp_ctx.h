@@ -12,6 +12,7 @@
 
 #include "tp.h"
 #include "tp_physics.h"
+#include "tp_rng_host.h"
 
 namespace tpe {
 
@@ -28,7 +29,7 @@ struct Tables { // device copies of GF(2) jump tables for one (seeding, batch si
 struct Ctx {
     int device = 0;
     int cus = 1;                    // compute units of the device
-    int blocks_per_cu[3][2] = {{1, 1}, {1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
+    int blocks_per_cu[4][2] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
@@ -36,12 +37,15 @@ struct Ctx {
     int km = 0;
     std::map<std::string, std::unique_ptr<Tables>> tables;
     std::map<std::string, std::vector<uint32_t>> ranf_polys; // host copies keyed by R
-    DevBuf polys, draws, origin, p0, per_batch, merged;
+    DevBuf polys, draws, origin, p0, per_batch, merged, evq;
     tp_stats last{};
     bool timed = false;
     // Host staging of the RANF origin: must outlive the asynchronous copies.
     int32_t origin0[56];
     int32_t p00 = 0;
+    // faster-evgen: event start positions of the last run and the state after it
+    std::vector<uint64_t> fe_starts;
+    tp::RanfState fe_end{};
 };
 
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws

@@ -160,8 +160,89 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
         return fail(c, TP_ERR_ARG, "unknown rng kind " + std::to_string(rng_kind));
     if (params->precision != TP_PREC_F64 && params->precision != TP_PREC_MIXED)
         return fail(c, TP_ERR_UNSUPPORTED, "precision " + std::to_string(params->precision) + " not available");
+    if (params->faster_evgen && (rng_kind != TP_RNG_RANF || params->precision != TP_PREC_F64))
+        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available with the RANF stream in f64");
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
         return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
+    return TP_OK;
+}
+
+// RANF with `faster-evgen`: the host walks the stream for the event starts
+// (the reference's simulate_event_batch walk), then each chunk of batches is
+// expanded into HBM and simulated from those starts.
+constexpr uint64_t FE_CHUNK_BATCHES = 1024;
+
+int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs, uint64_t num_events,
+                              uint32_t batch_size, void *per_batch_dev)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors below are reused
+    tp::RanfState g;
+    std::memcpy(g.numbers, rs->numbers, sizeof g.numbers);
+    g.index = rs->index;
+    g.seed = rs->seed;
+    const uint64_t p0_master = 55u - rs->index;
+    c->fe_starts.resize(num_events);
+    uint64_t q_end = 0;
+    tp::ranf_walk_faster_evgen(g, num_events, c->fe_starts.data(), &q_end);
+    c->fe_end = g;
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    const uint64_t chunk = std::min<uint64_t>(nb, FE_CHUNK_BATCHES);
+    int rc = ensure(c, c->evq, chunk * batch_size * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->origin, 56 * sizeof(int32_t));
+    if (!rc)
+        rc = ensure(c, c->p0, sizeof(int32_t));
+    if (rc)
+        return rc;
+    const int K = std::max(1, bits_for(q_end / 55 / RANF_R + 2));
+    rc = ensure_polys(c, RANF_R, K);
+    if (rc)
+        return rc;
+    std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
+    c->p00 = static_cast<int32_t>(p0_master);
+    HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
+    bool first = true;
+    for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        const uint64_t b1 = std::min(nb, b0 + chunk);
+        const uint64_t e0 = b0 * batch_size, e1 = std::min<uint64_t>(b1 * batch_size, num_events);
+        const uint64_t q_first = c->fe_starts[e0], q_last = e1 < num_events ? c->fe_starts[e1] : q_end;
+        rc = ensure(c, c->draws, (q_last - q_first) * sizeof(int32_t));
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(c->evq.p, c->fe_starts.data() + e0, (e1 - e0) * sizeof(uint64_t),
+                                 hipMemcpyHostToDevice, c->stream));
+        tpk::RanfArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.origin = static_cast<const int32_t *>(c->origin.p);
+        r.p0 = static_cast<const int32_t *>(c->p0.p);
+        r.polys = static_cast<const uint32_t *>(c->polys.p);
+        r.K = K;
+        r.R = RANF_R;
+        r.out = static_cast<int32_t *>(c->draws.p);
+        r.d_begin = q_first - p0_master;
+        r.d_end = q_last - p0_master;
+        r.seg_first = q_first / 55 / RANF_R;
+        r.nsegs = (q_last - 1) / 55 / RANF_R - r.seg_first + 1;
+        r.piece_stride = 0;
+        if (first)
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        tpk::launch_ranf_expand(r, 1, c->stream);
+        HIPCHK(c, hipGetLastError());
+        a.nb = b1 - b0;
+        a.b_first = b0;
+        a.draws = static_cast<const int32_t *>(c->draws.p);
+        a.qbase = q_first;
+        a.evq = static_cast<const uint64_t *>(c->evq.p);
+        a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+        if (first) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            first = false;
+        }
+        tpk::launch_fused(a, 3, grid_for(c, 3, 0, a.nb), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     return TP_OK;
 }
 
@@ -206,6 +287,11 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
     const tp_ranf_state *rs = static_cast<const tp_ranf_state *>(state0);
     if (rs->index > 55)
         return fail(c, TP_ERR_ARG, "RANF state index out of range");
+    if (params->faster_evgen) {
+        if (seeding != TP_SEED_SKIPAHEAD)
+            return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen runs use skip-ahead seeding");
+        return enqueue_ranf_faster_evgen(c, a, rs, num_events, batch_size, per_batch_dev);
+    }
     const uint64_t chunk = std::min<uint64_t>(nb, RANF_CHUNK_BATCHES);
     const size_t per_batch_draws = 12ull * batch_size;
     rc = ensure(c, c->draws, chunk * per_batch_draws * sizeof(int32_t));
@@ -328,7 +414,7 @@ extern "C" int tp_create(int device, void **ctx_out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cus = prop.multiProcessorCount;
-    for (int src = 0; src < 3; ++src)
+    for (int src = 0; src < 4; ++src)
         for (int prec = 0; prec < 2; ++prec)
             c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
     *ctx_out = c;
@@ -342,7 +428,7 @@ extern "C" int tp_destroy(void *ctx)
         return TP_ERR_ARG;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged})
+    for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq})
         if (b->p)
             (void)hipFree(b->p);
     for (auto &kv : c->tables) {
@@ -445,6 +531,8 @@ extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const
         return TP_OK;
     if (n > (1u << 20))
         return fail(c, TP_ERR_ARG, "tp_events: n exceeds 2^20");
+    if (params->faster_evgen)
+        return fail(c, TP_ERR_UNSUPPORTED, "tp_events: faster-evgen events are not available");
     HIPCHK(c, hipSetDevice(c->device));
     tpk::FusedArgs a;
     std::memset(&a, 0, sizeof a);
@@ -648,8 +736,13 @@ extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kin
         return rc;
     HIPCHK(c, hipMemcpyAsync(out, c->per_batch.p, sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    // `&mut rng` side effect: 12 draws per event (evgen.rs:182-187).
-    if (rng_kind == TP_RNG_XOSHIRO) {
+    // `&mut rng` side effect: 12 draws per event (evgen.rs:182-187), or the
+    // walk of the faster-evgen draws.
+    if (params->faster_evgen) {
+        tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
+        std::memcpy(s->numbers, c->fe_end.numbers, sizeof s->numbers);
+        s->index = c->fe_end.index;
+    } else if (rng_kind == TP_RNG_XOSHIRO) {
         tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, 12ull * n);
     } else {
         tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);

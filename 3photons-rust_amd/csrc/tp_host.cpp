@@ -417,6 +417,8 @@ extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
     p->ev_weight = k.ev_weight;
     p->sort_photons = 1;
     p->precision = TP_PREC_F64;
+    p->faster_evgen = 0;
+    p->reserved = 0;
     return TP_OK;
 }
 

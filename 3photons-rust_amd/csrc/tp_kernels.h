@@ -34,6 +34,11 @@ struct FusedArgs {
     // batch_size*(b - b_first) + e; every |M|^2 is multiplied by its weight.
     const double *u;
     const double *invq;
+    // faster-evgen source (RANF, f64): stream values from position qbase in
+    // draws[], event e of batch (b - b_first) starting at stream position
+    // evq[batch_size*(b - b_first) + e].
+    const uint64_t *evq;
+    uint64_t qbase;
     // output: one accumulator per batch, index b - b_first
     void *out;             // tp_acc*
 };
@@ -54,7 +59,7 @@ struct RanfArgs {
     int32_t *out;
 };
 
-void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer, 2 weighted (f64 only)*/, int grid,
+void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer, 2 weighted, 3 faster-evgen (f64 only)*/, int grid,
                   hipStream_t st);
 int fused_blocks_per_cu(int source, int precision); // resident workgroups per CU (occupancy)
 // Per-event diagnostics: one thread per event, same device code as the loop.

@@ -201,7 +201,10 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a)
             bool pass = false;
             double p[3][4];
             double wt = 1.;
-            if (k < len) {
+            if (k < len && SOURCE == 3) {
+                RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
+                pass = generate_keep_faster_f64(P, w, p);
+            } else if (k < len) {
                 double u[12];
                 if (SOURCE == 0) {
 #pragma unroll
@@ -292,6 +295,8 @@ int fused_blocks_per_cu(int source, int precision)
         return precision == 0 ? blocks_per_cu<0, 0>() : blocks_per_cu<0, 1>();
     if (source == 2)
         return blocks_per_cu<2, 0>();
+    if (source == 3)
+        return blocks_per_cu<3, 0>();
     return precision == 0 ? blocks_per_cu<1, 0>() : blocks_per_cu<1, 1>();
 }
 
@@ -308,8 +313,10 @@ void launch_fused(const FusedArgs &a, int source, int grid, hipStream_t st)
             hipLaunchKernelGGL((fused_kernel<1, 0>), g, b, 0, st, a);
         else
             hipLaunchKernelGGL((fused_kernel<1, 1>), g, b, 0, st, a);
-    } else {
+    } else if (source == 2) {
         hipLaunchKernelGGL((fused_kernel<2, 0>), g, b, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((fused_kernel<3, 0>), g, b, 0, st, a);
     }
 }
 

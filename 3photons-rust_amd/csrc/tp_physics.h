@@ -378,6 +378,74 @@ __device__ __forceinline__ bool generate_keep_f64(const Params &P, const double 
     return keep_f64(P, p);
 }
 
+// ---- faster-evgen (evgen.rs:143-173, 221-249) --------------------------------
+// Reads the RANF stream expanded in HBM.  Stream position q holds the value
+// at index 55 - q % 55 of round q / 55; random_array::<N> first skips the
+// round's leftovers when fewer than N remain, then returns its N values in
+// ascending array index, i.e. descending stream position (ranf.rs:77-104).
+struct RanfWalk {
+    const int32_t *buf; // stream values from position qbase on
+    uint64_t qbase;
+    uint64_t q;         // next position
+    __device__ __forceinline__ void take(int n, double *out)
+    {
+        const uint32_t rem = 55u - (uint32_t)(q % 55u);
+        if (rem < (uint32_t)n)
+            q += rem;
+        const int32_t *v = buf + (q - qbase);
+        for (int k = 0; k < n; ++k)
+            out[k] = (double)v[n - 1 - k] * 1e-9;
+        q += (uint64_t)n;
+    }
+};
+
+// Massless photons from 9 + 6 draws and unit-disc rejection re-rolls, in the
+// reference's operation order (matrix fills column-major, nalgebra sums).
+__device__ __forceinline__ void raw_photons_faster_f64(RanfWalk &w, double q[3][4])
+{
+    double r9[9], r6[6], x[3], y[3], r2[3];
+    w.take(9, r9);
+    w.take(6, r6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        x[i] = 2. * r6[i] - 1.;
+        y[i] = 2. * r6[3 + i] - 1.;
+        r2[i] = (0. + x[i] * x[i]) + y[i] * y[i];
+    }
+    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    for (int i = 0; i < 3; ++i)
+        while (r2[i] > 1. || r2[i] < min2) {
+            double r[2];
+            w.take(2, r);
+            x[i] = 2. * r[0] - 1.;
+            y[i] = 2. * r[1] - 1.;
+            r2[i] = 0. + (x[i] * x[i] + y[i] * y[i]);
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double nrm = 1. / sqrt(r2[i]);
+        const double cx = x[i] * nrm, cy = y[i] * nrm;
+        const double cos_t = 2. * r9[i] - 1.;
+        const double em = r9[3 + i] * r9[6 + i];
+        const double st = sqrt(1. - cos_t * cos_t);
+        const double en = -log(em + 2.2250738585072014e-308);
+        q[i][0] = en * (st * cx);
+        q[i][1] = en * (st * cy);
+        q[i][2] = en * cos_t;
+        q[i][3] = en * 1.;
+    }
+}
+
+__device__ __forceinline__ bool generate_keep_faster_f64(const Params &P, RanfWalk &w, double p[3][4])
+{
+    double q[3][4];
+    raw_photons_faster_f64(w, q);
+    conformal_f64(P, q, p);
+    if (P.sort)
+        sort3(p);
+    return keep_f64(P, p);
+}
+
 // Helicity sums m[5] of an event that passed the cuts.
 __device__ __forceinline__ void matrix_elements_f64(const Params &P, const double p[3][4], double m[5])
 {

@@ -255,4 +255,53 @@ void xoshiro_advance(uint64_t s[4], uint64_t n)
             gf2_apply(t[static_cast<size_t>(k)], s, s);
 }
 
+// ---- faster-evgen stream walk ----------------------------------------------
+namespace {
+struct RanfWalker {
+    RanfState g;
+    uint64_t q; // position of the next value
+    // random_array::<N>: the N values in the order the reference returns them
+    // (ascending array index = descending stream position).
+    void take(int n, double *out)
+    {
+        if ((int)g.index < n) {
+            q += g.index; // leftovers of the round are discarded
+            ranf_round(g.numbers);
+            g.index = 55;
+        }
+        g.index -= (uint32_t)n;
+        for (int k = 0; k < n; ++k)
+            out[k] = (double)g.numbers[g.index + 1 + k] * 1e-9;
+        q += (uint64_t)n;
+    }
+};
+} // namespace
+
+void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end)
+{
+    RanfWalker w{g, 55u - g.index};
+    double r9[9], r6[6], r2v[2];
+    for (uint64_t e = 0; e < num_events; ++e) {
+        if (starts)
+            starts[e] = w.q;
+        w.take(9, r9);
+        w.take(6, r6);
+        // radii of the three points (evgen.rs:228-246); only the accept test
+        // matters here, with the reference's float operations
+        for (int i = 0; i < 3; ++i) {
+            const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
+            double r2 = (0. + x * x) + y * y;
+            const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+            while (r2 > 1. || r2 < min2) {
+                w.take(2, r2v);
+                const double a = 2. * r2v[0] - 1., b = 2. * r2v[1] - 1.;
+                r2 = 0. + (a * a + b * b);
+            }
+        }
+    }
+    g = w.g;
+    if (q_end)
+        *q_end = w.q;
+}
+
 } // namespace tp

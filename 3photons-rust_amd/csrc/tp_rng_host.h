@@ -40,6 +40,18 @@ void ranf_apply_round_jump(const RanfPoly &c, const int32_t *x_in /*[56]*/, int3
 // Advance a generator state by n single draws (same effect as n random()).
 void ranf_advance(RanfState &g, uint64_t n);
 
+// `faster-evgen` consumes a variable number of draws per event (9 + 6, plus
+// 2 per rejected unit-disc point, each random_array::<N> call first
+// discarding a round's leftovers when fewer than N remain: ranf.rs:77-104,
+// evgen.rs:143-173,221-249), so event starts are found by walking the stream
+// as EventGenerator::simulate_event_batch does (evgen.rs:257-267).  Stream
+// positions q count values from the start of the state's current round
+// (value q sits at index 55 - q % 55 of round q / 55), so the state g itself
+// is at q0 = 55 - g.index.  starts[e] receives the position of event e's
+// first random_array::<9> call (before any discard); on return g is the
+// state after the last event and *q_end its position.
+void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end);
+
 // ---- xoshiro256+ -------------------------------------------------------------
 void xoshiro_seed(uint64_t seed, uint64_t s[4]); // seed_from_u64 (SplitMix64 expansion)
 uint64_t xoshiro_next(uint64_t s[4]);

@@ -6,6 +6,7 @@
 //   --standard-random     xoshiro256+ stream instead of RANF
 //   --faster-threading    per-batch jump()/reseed instead of skip-ahead
 //   --no-photon-sorting   keep RAMBO's photon order
+//   --faster-evgen        the reference's faster-evgen generation (RANF, f64)
 //   --mixed               f32 spinor/amplitude arithmetic (tp_precision MIXED)
 //   --config PATH         configuration file (default: ./valeurs)
 //   --device N            HIP device ordinal (default 0)
@@ -25,6 +26,7 @@ int main(int argc, char **argv)
 {
     std::string config = "valeurs";
     int device = 0, rng = TP_RNG_RANF, seeding = TP_SEED_SKIPAHEAD, sort = 1, precision = TP_PREC_F64;
+    int faster_evgen = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--standard-random")
@@ -33,6 +35,8 @@ int main(int argc, char **argv)
             seeding = TP_SEED_JUMP;
         else if (a == "--no-photon-sorting")
             sort = 0;
+        else if (a == "--faster-evgen")
+            faster_evgen = 1;
         else if (a == "--mixed")
             precision = TP_PREC_MIXED;
         else if (a == "--config" && i + 1 < argc)
@@ -41,7 +45,7 @@ int main(int argc, char **argv)
             device = std::atoi(argv[++i]);
         else {
             std::fprintf(stderr, "usage: %s [--standard-random] [--faster-threading] [--no-photon-sorting] "
-                                 "[--mixed] [--config PATH] [--device N]\n",
+                                 "[--faster-evgen] [--mixed] [--config PATH] [--device N]\n",
                          argv[0]);
             return 2;
         }
@@ -65,6 +69,7 @@ int main(int argc, char **argv)
     tp_params_from_config(&cfg, &params);
     params.sort_photons = sort;
     params.precision = precision;
+    params.faster_evgen = faster_evgen;
     std::puts("IBegin"); // evgen.rs:47
     void *ctx = nullptr;
     rc = tp_create(device, &ctx);

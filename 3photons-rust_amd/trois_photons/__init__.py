@@ -36,7 +36,8 @@ class TpParams(C.Structure):
     _fields_ = [("e_total", C.c_double), ("cut_beam", C.c_double), ("cut_pp", C.c_double),
                 ("e_min", C.c_double), ("cut_plane", C.c_double), ("g_a", C.c_double),
                 ("g_beta_p", C.c_double), ("g_beta_m", C.c_double), ("sigma_contribs", C.c_double * 5),
-                ("ev_weight", C.c_double), ("sort_photons", C.c_int32), ("precision", C.c_int32)]
+                ("ev_weight", C.c_double), ("sort_photons", C.c_int32), ("precision", C.c_int32),
+                ("faster_evgen", C.c_int32), ("reserved", C.c_int32)]
 
 
 class TpAcc(C.Structure):
@@ -185,13 +186,15 @@ def _text(fn, *args) -> str:
     return buf.value.decode()
 
 
-def params_from_config(cfg: Configuration, sort: bool = True, precision: int = PREC_F64) -> TpParams:
+def params_from_config(cfg: Configuration, sort: bool = True, precision: int = PREC_F64,
+                       faster_evgen: bool = False) -> TpParams:
     p = TpParams()
     rc = lib.tp_params_from_config(C.byref(cfg.raw), C.byref(p))
     if rc != TP_OK:
         raise TpError(f"tp_params_from_config: {rc}")
     p.sort_photons = 1 if sort else 0
     p.precision = precision
+    p.faster_evgen = 1 if faster_evgen else 0
     return p
 
 
@@ -445,10 +448,11 @@ class RunResult:
 
 
 def run_simulation(cfg: Configuration, rng: str = "ranf", seeding: str = "skipahead", sort: bool = True,
-                   precision: int = PREC_F64, engine: Engine | None = None) -> RunResult:
+                   precision: int = PREC_F64, engine: Engine | None = None,
+                   faster_evgen: bool = False) -> RunResult:
     """scheduling::run_simulation (scheduling/mod.rs:31-59) on the GPU."""
     kind = RNG_KINDS[rng]
-    params = params_from_config(cfg, sort=sort, precision=precision)
+    params = params_from_config(cfg, sort=sort, precision=precision, faster_evgen=faster_evgen)
     st = rng_state(kind)
     own = engine is None
     eng = engine or Engine()

@@ -73,6 +73,9 @@ typedef struct {
     double ev_weight;                     /* RAMBO event weight (evgen.rs:59-62) */
     int32_t sort_photons;                 /* 1 = sort by energy (default), 0 = no-photon-sorting */
     int32_t precision;                    /* tp_precision */
+    int32_t faster_evgen;                 /* 1 = `faster-evgen` generation (evgen.rs:143-173, 221-249;
+                                             RANF stream, f64, skip-ahead seeding), 0 = default */
+    int32_t reserved;                     /* 0 */
 } tp_params;
 
 /* One batch accumulator, field for field ResultsAccumulator (resacc.rs:16-34). */

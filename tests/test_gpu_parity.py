@@ -123,20 +123,61 @@ def test_simulate_batch_is_the_closure(engine, cfg, oracle, valeurs, rng):
             assert st.index == g.index and list(st.numbers)[1:] == list(g.numbers)[1:]
 
 
-GOLDEN = [("features_", "ranf", "skipahead", True), ("features_standard-random", "xoshiro", "skipahead", True),
-          ("features_multi-threading,faster-threading", "ranf", "jump", True),
-          ("features_standard-random,multi-threading,faster-threading", "xoshiro", "jump", True),
-          ("features_no-photon-sorting", "ranf", "skipahead", False)]
+@pytest.mark.parametrize("n_events", [1, 9999, 35017, 123457])
+def test_faster_evgen_per_batch_matches_oracle(engine, cfg, oracle, valeurs, n_events):
+    """faster-evgen (variable draws per event, round leftovers discarded):
+    the host walk's batch starts and the device reading of the expanded
+    stream against the oracle, batch by batch."""
+    print(f"faster-evgen run N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, faster_evgen=True)
+    per, merged, _ = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), n_events)
+    ocfg = oracle.config(valeurs, n_events)
+    om, oper = oracle.run(ocfg, oracle.opts("ranf", threads=4, faster_evgen=True), per_batch=True)
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]], np.float64)
+        acc_close(a, oper[b].as_array(), min(10000, n_events - 10000 * b))
+    acc_close(merged.as_array(), om.as_array(), n_events)
 
 
-@pytest.mark.parametrize("suffix,rng,seeding,sort", GOLDEN)
-def test_full_run_reproduces_golden(engine, cfg, oracle_run, suffix, rng, seeding, sort):
+def test_faster_evgen_simulate_batch_advances_like_the_closure(engine, cfg, oracle, valeurs):
+    params = tp.params_from_config(cfg, faster_evgen=True)
+    st = tp.rng_state(tp.RNG_RANF)
+    g = oracle.rng("ranf")
+    ocfg = oracle.config(valeurs)
+    opts = oracle.opts("ranf", faster_evgen=True)
+    for n in (10000, 7, 1, 23456):
+        acc = engine.simulate_batch(params, tp.RNG_RANF, st, n)
+        oacc = oracle.simulate_batch(ocfg, opts, g, n)
+        acc_close(acc.as_array(), oacc.as_array(), n)
+        assert st.index == g.index and list(st.numbers)[1:] == list(g.numbers)[1:]
+
+
+def test_faster_evgen_unsupported_combinations_fail_cleanly(engine, cfg):
+    params = tp.params_from_config(cfg.with_events(100), faster_evgen=True)
+    with pytest.raises(tp.TpError, match="faster-evgen"):
+        engine.run(params, tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), 100)
+    with pytest.raises(tp.TpError, match="faster-evgen"):
+        engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, tp.rng_state(tp.RNG_RANF), 100)
+
+
+GOLDEN = [("features_", "ranf", "skipahead", True, False),
+          ("features_standard-random", "xoshiro", "skipahead", True, False),
+          ("features_multi-threading,faster-threading", "ranf", "jump", True, False),
+          ("features_standard-random,multi-threading,faster-threading", "xoshiro", "jump", True, False),
+          ("features_no-photon-sorting", "ranf", "skipahead", False, False),
+          ("features_faster-evgen", "ranf", "skipahead", True, True)]
+
+
+@pytest.mark.parametrize("suffix,rng,seeding,sort,fe", GOLDEN)
+def test_full_run_reproduces_golden(engine, cfg, oracle_run, suffix, rng, seeding, sort, fe):
     """N = 1e7 with the shipped valeurs: res.data numerically equal to the
     reference's file at its printed precision, finals within 1e-12 of the
     oracle's."""
     print(f"golden {suffix}", flush=True)
-    res = tp.run_simulation(cfg, rng=rng, seeding=seeding, sort=sort, engine=engine)
-    ocfg, omerged, _, ofin = oracle_run(rng, seeding == "jump", sort)
+    res = tp.run_simulation(cfg, rng=rng, seeding=seeding, sort=sort, engine=engine, faster_evgen=fe)
+    ocfg, omerged, _, ofin = oracle_run(rng, seeding == "jump", sort, fe)
     assert res.merged.selected_events == omerged.selected
     for f in ("sigma", "prec", "variance", "beta_min", "ss_p", "inc_ss_p", "ss_m", "inc_ss_m"):
         a, b = getattr(res.final, f), getattr(ofin, f)
