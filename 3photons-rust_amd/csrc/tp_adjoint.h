@@ -94,11 +94,11 @@ template <class T> __device__ __forceinline__ C<T> tinv(const SpinVal<T> &V, int
 // v_bar += e * gamma * conj(1/v)
 template <class T> __device__ __forceinline__ void add_s(SpinAdj<T> &D, const SpinVal<T> &V, int i, int j, T e, C<T> g)
 {
-    D.sb[i][j] = caddv(D.sb[i][j], cscale(e, cmul(g, cconj(V.is[i][j]))));
+    D.sb[i][j] = caddv(D.sb[i][j], cscale(e, xmul<true>(g, cconj(V.is[i][j]))));
 }
 template <class T> __device__ __forceinline__ void add_t(SpinAdj<T> &D, const SpinVal<T> &V, int i, int j, T e, C<T> g)
 {
-    D.tb[i][j] = caddv(D.tb[i][j], cscale(e, cmul(g, cconj(tinv(V, i, j)))));
+    D.tb[i][j] = caddv(D.tb[i][j], cscale(e, xmul<true>(g, cconj(tinv(V, i, j)))));
 }
 
 template <class T> __device__ __forceinline__ T log_sigmoid(T x) { return -(fmax(-x, (T)0) + log1p(exp(-fabs(x)))); }
@@ -305,7 +305,7 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
 #pragma unroll
         for (int h = 0; h < 6; ++h) {
             const C<T> Ag = cscale(P.g_a, a[h]), Pg = cscale(P.g_bp, bp[h]);
-            const C<T> X = cmul(cscale(two, Ag), cconj(Pg));
+            const C<T> X = xmul<true>(cscale(two, Ag), cconj(Pg));
             sA = sA + (Ag.re * Ag.re + Ag.im * Ag.im);
             sP = sP + (Pg.re * Pg.re + Pg.im * Pg.im);
             sR = sR + X.re;
@@ -361,9 +361,9 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
 #pragma unroll
     for (int h = 0; h < 6; ++h) {
         const int k1 = ak[h][0], k2 = ak[h][1], k3 = ak[h][2];
-        const C<T> abar = caddv(cscale(cA, a[h]), cscale(kappa, cmul(Xbar, bp[h])));
-        const C<T> pbar = caddv(cscale(cP, bp[h]), cscale(kappa, cmul(cconj(Xbar), a[h])));
-        const C<T> ga = cmul(abar, cconj(a[h])), gp = cmul(pbar, cconj(bp[h]));
+        const C<T> abar = caddv(cscale(cA, a[h]), cscale(kappa, xmul<true>(Xbar, bp[h])));
+        const C<T> pbar = caddv(cscale(cP, bp[h]), cscale(kappa, xmul<true>(cconj(Xbar), a[h])));
+        const C<T> ga = xmul<true>(abar, cconj(a[h])), gp = xmul<true>(pbar, cconj(bp[h]));
         if (aty[h] == A_ppm) {
             // a_ppm = ns01 s(0,k3)^2 / (s(0,k1) s(0,k2) s(1,k1) s(1,k2))
             add_s(D, V, 0, k3, two, ga);
@@ -389,32 +389,32 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
     {
         // bm_mmm = nt01 sum_j Y_j^2 with Y = s(k1,0)s(k2,k3), s(k2,0)s(k1,k3), s(k3,0)s(k1,k2)
         const C<T> bbar = cscale(cM, bmm);
-        const C<T> Y1 = cmul(V.s[2][0], V.s[3][4]), Y2 = cmul(V.s[3][0], V.s[2][4]), Y3 = cmul(V.s[4][0], V.s[2][3]);
-        const C<T> d1 = cmul(bbar, cconj(cscale(two, cmul(P.nt01, Y1))));
-        const C<T> d2 = cmul(bbar, cconj(cscale(two, cmul(P.nt01, Y2))));
-        const C<T> d3 = cmul(bbar, cconj(cscale(two, cmul(P.nt01, Y3))));
-        D.sb[2][0] = caddv(D.sb[2][0], cmul(d1, cconj(V.s[3][4])));
-        D.sb[3][4] = caddv(D.sb[3][4], cmul(d1, cconj(V.s[2][0])));
-        D.sb[3][0] = caddv(D.sb[3][0], cmul(d2, cconj(V.s[2][4])));
-        D.sb[2][4] = caddv(D.sb[2][4], cmul(d2, cconj(V.s[3][0])));
-        D.sb[4][0] = caddv(D.sb[4][0], cmul(d3, cconj(V.s[2][3])));
-        D.sb[2][3] = caddv(D.sb[2][3], cmul(d3, cconj(V.s[4][0])));
+        const C<T> Y1 = xmul<true>(V.s[2][0], V.s[3][4]), Y2 = xmul<true>(V.s[3][0], V.s[2][4]), Y3 = xmul<true>(V.s[4][0], V.s[2][3]);
+        const C<T> d1 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y1))));
+        const C<T> d2 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y2))));
+        const C<T> d3 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y3))));
+        D.sb[2][0] = caddv(D.sb[2][0], xmul<true>(d1, cconj(V.s[3][4])));
+        D.sb[3][4] = caddv(D.sb[3][4], xmul<true>(d1, cconj(V.s[2][0])));
+        D.sb[3][0] = caddv(D.sb[3][0], xmul<true>(d2, cconj(V.s[2][4])));
+        D.sb[2][4] = caddv(D.sb[2][4], xmul<true>(d2, cconj(V.s[3][0])));
+        D.sb[4][0] = caddv(D.sb[4][0], xmul<true>(d3, cconj(V.s[2][3])));
+        D.sb[2][3] = caddv(D.sb[2][3], xmul<true>(d3, cconj(V.s[4][0])));
     }
     {
         // bm_ppp = ns01 sum_j Y_j^2 with Y = t(k1,k2)t(k3,1), t(k1,k3)t(k2,1), t(k2,k3)t(k1,1)
         const C<T> bbar = cscale(cM, bpp);
         const C<T> t23 = tval(V, 2, 3), t41 = tval(V, 4, 1), t24 = tval(V, 2, 4), t31 = tval(V, 3, 1);
         const C<T> t34 = tval(V, 3, 4), t21 = tval(V, 2, 1);
-        const C<T> Y1 = cmul(t23, t41), Y2 = cmul(t24, t31), Y3 = cmul(t34, t21);
-        const C<T> d1 = cmul(bbar, cconj(cscale(two, cmul(P.ns01, Y1))));
-        const C<T> d2 = cmul(bbar, cconj(cscale(two, cmul(P.ns01, Y2))));
-        const C<T> d3 = cmul(bbar, cconj(cscale(two, cmul(P.ns01, Y3))));
-        D.tb[2][3] = caddv(D.tb[2][3], cmul(d1, cconj(t41)));
-        D.tb[4][1] = caddv(D.tb[4][1], cmul(d1, cconj(t23)));
-        D.tb[2][4] = caddv(D.tb[2][4], cmul(d2, cconj(t31)));
-        D.tb[3][1] = caddv(D.tb[3][1], cmul(d2, cconj(t24)));
-        D.tb[3][4] = caddv(D.tb[3][4], cmul(d3, cconj(t21)));
-        D.tb[2][1] = caddv(D.tb[2][1], cmul(d3, cconj(t34)));
+        const C<T> Y1 = xmul<true>(t23, t41), Y2 = xmul<true>(t24, t31), Y3 = xmul<true>(t34, t21);
+        const C<T> d1 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y1))));
+        const C<T> d2 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y2))));
+        const C<T> d3 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y3))));
+        D.tb[2][3] = caddv(D.tb[2][3], xmul<true>(d1, cconj(t41)));
+        D.tb[4][1] = caddv(D.tb[4][1], xmul<true>(d1, cconj(t23)));
+        D.tb[2][4] = caddv(D.tb[2][4], xmul<true>(d2, cconj(t31)));
+        D.tb[3][1] = caddv(D.tb[3][1], xmul<true>(d2, cconj(t24)));
+        D.tb[3][4] = caddv(D.tb[3][4], xmul<true>(d3, cconj(t21)));
+        D.tb[2][1] = caddv(D.tb[2][1], xmul<true>(d3, cconj(t34)));
     }
     // fold onto the independent products: s(j,i) = -s(i,j), t(i,j) = -conj(s(i,j)),
     // t(j,i) = conj(s(i,j))

@@ -38,6 +38,9 @@ struct Ctx {
     std::map<std::string, std::unique_ptr<Tables>> tables;
     std::map<std::string, std::vector<uint32_t>> ranf_polys; // host copies keyed by R
     DevBuf polys, draws, origin, p0, per_batch, merged, evq;
+    DevBuf mt128, bt128, f32ev; // all-f32 mode: xoshiro128+ tables, per-event values
+    std::string bt128_key;
+    int bt128_kb = 0;
     tp_stats last{};
     bool timed = false;
     // Host staging of the RANF origin: must outlive the asynchronous copies.
@@ -94,5 +97,6 @@ inline int grid_for(const Ctx *c, int source, int precision, uint64_t nb)
 // Device-side constants of a run, computed exactly as the reference does on
 // the host (coupling.rs, resacc.rs:59-100, evgen.rs:66-69, spinor.rs:37-44).
 tpd::Params make_params(const tp_params *p);
+tpd::FParams make_fparams(const tp_params *p);
 
 } // namespace tpe

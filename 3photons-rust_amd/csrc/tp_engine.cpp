@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "tp.h"
+#include "tp_host.h"
 #include "tp_kernels.h"
 #include "tp_rng_host.h"
 #include "tp_ctx.h"
@@ -55,11 +56,130 @@ tpd::Params make_params(const tp_params *p)
     return P;
 }
 
+// The same constants in f32 arithmetic for the all-f32 mode (the tp_params
+// values are f32 values there, from tp_params_from_config_f32).
+tpd::FParams make_fparams(const tp_params *p)
+{
+    tpd::FParams F;
+    std::memset(&F, 0, sizeof F);
+    F.e_total = (float)p->e_total;
+    F.cut_beam = (float)p->cut_beam;
+    F.cut_pp = (float)p->cut_pp;
+    F.e_min = (float)p->e_min;
+    F.cut_plane = (float)p->cut_plane;
+    F.g_a = (float)p->g_a;
+    F.g_bp = (float)p->g_beta_p;
+    F.g_bm = (float)p->g_beta_m;
+    for (int i = 0; i < 5; ++i)
+        F.sc[i] = (float)p->sigma_contribs[i];
+    const float e = F.e_total;
+    F.el_x = -e / 2.f;
+    F.el_e = e / 2.f;
+    const float pos_x = e / 2.f, pos_e = e / 2.f;
+    F.xx0 = std::sqrt(F.el_e + 0.f);
+    F.f0re = F.el_x / F.xx0;
+    F.f0im = 0.f / F.xx0;
+    F.xx1 = std::sqrt(pos_e + 0.f);
+    F.f1re = pos_x / F.xx1;
+    F.f1im = 0.f / F.xx1;
+    const float s01re = F.f0re * F.xx1 - F.f1re * F.xx0;
+    const float s01im = F.f0im * F.xx1 - F.f1im * F.xx0;
+    const float N = -(2.f * 1.41421356237309504880f); // -RAC8 in f32 (spinor.rs:19)
+    F.ns01re = N * s01re;
+    F.ns01im = N * s01im;
+    F.nt01re = N * -s01re;
+    F.nt01im = N * s01im;
+    F.sort = p->sort_photons ? 1 : 0;
+    return F;
+}
+
 } // namespace tpe
 
 namespace {
 
 using namespace tpe;
+
+int bits_for(uint64_t n);
+
+// xoshiro128+ skip-ahead tables for the all-f32 mode: M^(2^k) (k < 20: draw
+// offsets inside a batch) and B^(2^k) with B = M^(12 batch_size) or the 2^64
+// jump.
+constexpr int KM128 = 20;
+
+int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, DevBuf **bt_out, int *kb_out)
+{
+    if (!c->mt128.p) {
+        const std::vector<tp::Gf2Mat128> &t = tp::xoshiro128_step_pow2();
+        int rc = ensure(c, c->mt128, KM128 * sizeof(tp::Gf2Mat128));
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpy(c->mt128.p, t.data(), KM128 * sizeof(tp::Gf2Mat128), hipMemcpyHostToDevice));
+    }
+    const int kb = std::max(1, bits_for(nb));
+    const std::string key = std::to_string(seeding) + ":" + std::to_string(batch_size);
+    if (c->bt128_key != key || c->bt128_kb < kb) {
+        tp::Gf2Mat128 B;
+        if (seeding == TP_SEED_JUMP)
+            tp::gf2_128_jump64(B);
+        else
+            tp::gf2_128_pow_from_table(tp::xoshiro128_step_pow2(), 12ull * batch_size, B);
+        const std::vector<tp::Gf2Mat128> bt = tp::gf2_128_pow2_table(B, kb);
+        int rc = ensure(c, c->bt128, bt.size() * sizeof(tp::Gf2Mat128));
+        if (rc)
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->bt128.p, bt.data(), bt.size() * sizeof(tp::Gf2Mat128), hipMemcpyHostToDevice));
+        c->bt128_key = key;
+        c->bt128_kb = kb;
+    }
+    *bt_out = &c->bt128;
+    *kb_out = c->bt128_kb;
+    return TP_OK;
+}
+
+constexpr uint64_t F32_CHUNK_BATCHES = 4096; // 4096 * 10000 events * 24 B = 0.98 GB of per-event values
+
+int f32_grid(const Ctx *c, uint64_t nb) { return (int)std::max<uint64_t>(1, std::min<uint64_t>(nb, 16ull * c->cus)); }
+
+// All-f32 run with the xoshiro128+ stream.
+int enqueue_xo128_f32(Ctx *c, const tp_params *params, int seeding, const tp_xoshiro128_state *x,
+                      uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
+{
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    DevBuf *bt = nullptr;
+    int kb = 0;
+    int rc = ensure_tables128(c, seeding, batch_size, nb, &bt, &kb);
+    if (rc)
+        return rc;
+    const uint64_t chunk = std::min<uint64_t>(nb, F32_CHUNK_BATCHES);
+    rc = ensure(c, c->f32ev, chunk * batch_size * 6 * sizeof(float));
+    if (rc)
+        return rc;
+    tpk::F32Args a;
+    std::memset(&a, 0, sizeof a);
+    a.P = make_fparams(params);
+    a.num_events = num_events;
+    a.batch_size = batch_size;
+    a.source = 0;
+    std::memcpy(a.s0, x->s, sizeof a.s0);
+    a.mt = static_cast<const uint32_t *>(c->mt128.p);
+    a.km = KM128;
+    a.bt = static_cast<const uint32_t *>(bt->p);
+    a.kb = kb;
+    a.ev = static_cast<float *>(c->f32ev.p);
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        a.nb = std::min(nb, b0 + chunk) - b0;
+        a.b_first = b0;
+        a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+        tpk::launch_f32_events(a, f32_grid(c, a.nb), c->stream);
+        HIPCHK(c, hipGetLastError());
+        tpk::launch_f32_accumulate(a, f32_grid(c, a.nb), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    return TP_OK;
+}
 
 
 int upload_mats(Ctx *c, DevBuf &b, const std::vector<tp::Gf2Mat> &mats)
@@ -156,10 +276,12 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
 {
     if (!params || !state)
         return fail(c, TP_ERR_ARG, "null params or rng state");
-    if (rng_kind != TP_RNG_RANF && rng_kind != TP_RNG_XOSHIRO)
+    if (rng_kind != TP_RNG_RANF && rng_kind != TP_RNG_XOSHIRO && rng_kind != TP_RNG_XOSHIRO128)
         return fail(c, TP_ERR_ARG, "unknown rng kind " + std::to_string(rng_kind));
-    if (params->precision != TP_PREC_F64 && params->precision != TP_PREC_MIXED)
+    if (params->precision != TP_PREC_F64 && params->precision != TP_PREC_MIXED && params->precision != TP_PREC_F32)
         return fail(c, TP_ERR_UNSUPPORTED, "precision " + std::to_string(params->precision) + " not available");
+    if ((params->precision == TP_PREC_F32) != (rng_kind == TP_RNG_XOSHIRO128) && rng_kind != TP_RNG_RANF)
+        return fail(c, TP_ERR_ARG, "xoshiro128+ is the stream of the all-f32 mode, xoshiro256+ of the others");
     if (params->faster_evgen && (rng_kind != TP_RNG_RANF || params->precision != TP_PREC_F64))
         return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available with the RANF stream in f64");
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
@@ -262,6 +384,22 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
         return rc;
     a.mt = static_cast<const uint64_t *>(c->mt.p);
     a.km = c->km;
+    if (rng_kind == TP_RNG_XOSHIRO128)
+        return enqueue_xo128_f32(c, params, seeding, static_cast<const tp_xoshiro128_state *>(state0), num_events,
+                                 batch_size, per_batch_dev);
+    const bool f32 = params->precision == TP_PREC_F32;
+    tpk::F32Args fa;
+    std::memset(&fa, 0, sizeof fa);
+    if (f32) {
+        fa.P = make_fparams(params);
+        fa.num_events = num_events;
+        fa.batch_size = batch_size;
+        fa.source = 1;
+        rc = ensure(c, c->f32ev, std::min<uint64_t>(nb, RANF_CHUNK_BATCHES) * batch_size * 6 * sizeof(float));
+        if (rc)
+            return rc;
+        fa.ev = static_cast<float *>(c->f32ev.p);
+    }
     if (rng_kind == TP_RNG_XOSHIRO) {
         const tp_xoshiro_state *x = static_cast<const tp_xoshiro_state *>(state0);
         std::memcpy(a.s0, x->s, sizeof a.s0);
@@ -371,7 +509,17 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
             HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
             first_kernel = false;
         }
-        tpk::launch_fused(a, 1, grid_for(c, 1, a.precision, cnb), c->stream);
+        if (f32) {
+            fa.nb = cnb;
+            fa.b_first = b0;
+            fa.draws = a.draws;
+            fa.out = a.out;
+            tpk::launch_f32_events(fa, f32_grid(c, cnb), c->stream);
+            HIPCHK(c, hipGetLastError());
+            tpk::launch_f32_accumulate(fa, f32_grid(c, cnb), c->stream);
+        } else {
+            tpk::launch_fused(a, 1, grid_for(c, 1, a.precision, cnb), c->stream);
+        }
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -428,7 +576,8 @@ extern "C" int tp_destroy(void *ctx)
         return TP_ERR_ARG;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq})
+    for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
+                      &c->mt128, &c->bt128, &c->f32ev})
         if (b->p)
             (void)hipFree(b->p);
     for (auto &kv : c->tables) {
@@ -468,6 +617,10 @@ extern "C" int tp_rng_seed(int rng_kind, uint64_t seed, void *state)
         tp::xoshiro_seed(seed, static_cast<tp_xoshiro_state *>(state)->s);
         return TP_OK;
     }
+    if (rng_kind == TP_RNG_XOSHIRO128) {
+        tp::xoshiro128_seed(seed, static_cast<tp_xoshiro128_state *>(state)->s);
+        return TP_OK;
+    }
     return TP_ERR_ARG;
 }
 
@@ -482,6 +635,10 @@ extern "C" int tp_rng_skip(int rng_kind, void *state, uint64_t n)
         return TP_ERR_ARG;
     if (rng_kind == TP_RNG_XOSHIRO) {
         tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(state)->s, n);
+        return TP_OK;
+    }
+    if (rng_kind == TP_RNG_XOSHIRO128) {
+        tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(state)->s, n);
         return TP_OK;
     }
     if (rng_kind != TP_RNG_RANF)
@@ -511,6 +668,10 @@ extern "C" int tp_rng_jump(int rng_kind, void *state)
         tp::gf2_apply(J, s, s);
         return TP_OK;
     }
+    if (rng_kind == TP_RNG_XOSHIRO128) {
+        tp::xoshiro128_jump(static_cast<tp_xoshiro128_state *>(state)->s);
+        return TP_OK;
+    }
     if (rng_kind != TP_RNG_RANF)
         return TP_ERR_ARG;
     tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
@@ -531,8 +692,8 @@ extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const
         return TP_OK;
     if (n > (1u << 20))
         return fail(c, TP_ERR_ARG, "tp_events: n exceeds 2^20");
-    if (params->faster_evgen)
-        return fail(c, TP_ERR_UNSUPPORTED, "tp_events: faster-evgen events are not available");
+    if (params->faster_evgen || params->precision == TP_PREC_F32)
+        return fail(c, TP_ERR_UNSUPPORTED, "tp_events: per-event values of faster-evgen and all-f32 runs are not available");
     HIPCHK(c, hipSetDevice(c->device));
     tpk::FusedArgs a;
     std::memset(&a, 0, sizeof a);
@@ -744,6 +905,8 @@ extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kin
         s->index = c->fe_end.index;
     } else if (rng_kind == TP_RNG_XOSHIRO) {
         tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, 12ull * n);
+    } else if (rng_kind == TP_RNG_XOSHIRO128) {
+        tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(rng_state)->s, 12ull * n);
     } else {
         tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
         tp::RanfState g;
@@ -780,7 +943,7 @@ extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, i
     if (rc)
         return rc;
     if (merged_dev) {
-        tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream);
+        tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream, params->precision == TP_PREC_F32);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
@@ -830,7 +993,12 @@ extern "C" int tp_run(void *ctx, const tp_params *params, int rng_kind, int seed
     rc = tp_sync(ctx, stats);
     if (rc)
         return rc;
-    if (merged_out) { // ReproducibleAccumulator::get_merged_result (multi_threading.rs:107-126)
+    if (merged_out && params->precision == TP_PREC_F32) {
+        tp_acc m = per_batch_out[0];
+        for (uint64_t b = 1; b < nb; ++b)
+            tp_acc_merge_f32(&m, &per_batch_out[b]);
+        *merged_out = m;
+    } else if (merged_out) { // ReproducibleAccumulator::get_merged_result (multi_threading.rs:107-126)
         tp_acc m = per_batch_out[0];
         for (uint64_t b = 1; b < nb; ++b) {
             m.selected_events += per_batch_out[b].selected_events;

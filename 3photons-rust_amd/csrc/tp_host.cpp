@@ -20,7 +20,7 @@
 
 namespace {
 
-const double PI = 3.14159265358979323846;
+constexpr double PI = 3.14159265358979323846;
 const double SQRT_2 = 1.41421356237309504880;
 
 // ---- Rust-compatible number formatting -----------------------------------
@@ -337,78 +337,96 @@ extern "C" int tp_config_format(const tp_config *c, char *buf, int len)
 
 namespace {
 
-// Constants of ResultsAccumulator::new (resacc.rs:59-100) and
-// EventGenerator::new (evgen.rs:40-77).
-struct AccConsts {
-    double ev_weight, fact_com, norm_weight, propagator, delta, relat_width;
-    double sc[5];
+// Scalar type of a run: double, or float for the all-f32 mode, where the
+// reference parses, computes and prints in f32 (numeric.rs:6-14).
+template <class T> struct Real;
+template <> struct Real<double> {
+    static constexpr double PI_ = PI;
+    static double cv(double x) { return x; }
+    // (2 pi)^-5: the reference folds the powi at compile time with the host pow()
+    static double pow_2pi_m5() { return std::pow(2. * PI, -5.); }
+};
+template <> struct Real<float> {
+    static constexpr float PI_ = 3.14159265358979323846f;
+    static float cv(double x) { return (float)x; }
+    static float pow_2pi_m5() { return (float)std::pow((double)(2.f * PI_), -5.); }
 };
 
-AccConsts acc_consts(const tp_config *c)
+// Constants of ResultsAccumulator::new (resacc.rs:59-100) and
+// EventGenerator::new (evgen.rs:40-77).
+template <class T> struct AccConsts {
+    T ev_weight, fact_com, norm_weight, propagator, delta, relat_width;
+    T sc[5];
+};
+
+template <class T> AccConsts<T> acc_consts(const tp_config *c)
 {
-    AccConsts k;
+    using R = Real<T>;
+    const T e_total = R::cv(c->e_total), conv = R::cv(c->gev2_to_picobarn), g_z0 = R::cv(c->g_z0);
+    const T m_z0 = R::cv(c->m_z0), sin2w = R::cv(c->sin2_weinberg);
+    const T beta_plus = R::cv(c->beta_plus), beta_minus = R::cv(c->beta_minus);
+    AccConsts<T> k;
     // evgen.rs:49-62, NUM_OUTGOING = 3.
-    double z_n = 2. * std::log(PI / 2.);
-    z_n -= 2. * std::log(1.);
-    z_n = z_n - std::log(2.);
-    double ln_weight = (2. * 3. - 4.) * std::log(c->e_total) + z_n;
+    T z_n = (T)2. * std::log(R::PI_ / (T)2.);
+    z_n -= (T)2. * std::log((T)1.);
+    z_n = z_n - std::log((T)2.);
+    T ln_weight = ((T)2. * (T)3. - (T)4.) * std::log(e_total) + z_n;
     k.ev_weight = std::exp(ln_weight);
     // resacc.rs:64-100.
-    double fact_com = 1. / 6. * c->gev2_to_picobarn;
-    double relat_width = c->g_z0 / c->m_z0;
-    double p_aa = 2.;
-    double p_ab = 1. - 4. * c->sin2_weinberg;
-    double p_bb = p_ab + 8. * (c->sin2_weinberg * c->sin2_weinberg);
-    double m2 = c->m_z0 * c->m_z0;
-    double m4 = m2 * m2;
-    double c_aa = fact_com * p_aa;
-    double c_ab = fact_com * p_ab / m2;
-    double c_bb = fact_com * p_bb / m4;
-    double r = c->e_total / c->m_z0;
-    double dzeta = r * r;
-    double delta = (dzeta - 1.) / relat_width;
-    double propagator = 1. / (1. + delta * delta);
-    double n_ev = static_cast<double>(c->num_events);
-    double norm = std::pow(2. * PI, -5.) / n_ev;
-    double norm_weight = k.ev_weight * norm;
-    double com = norm_weight / 4.;
-    double aa = com * c_aa;
-    double bb = com * c_bb * propagator / (relat_width * relat_width);
-    double ab = com * c_ab * 2. * c->beta_plus * propagator / relat_width;
+    T fact_com = (T)1. / (T)6. * conv;
+    T relat_width = g_z0 / m_z0;
+    T p_aa = (T)2.;
+    T p_ab = (T)1. - (T)4. * sin2w;
+    T p_bb = p_ab + (T)8. * (sin2w * sin2w);
+    T m2 = m_z0 * m_z0;
+    T m4 = m2 * m2;
+    T c_aa = fact_com * p_aa;
+    T c_ab = fact_com * p_ab / m2;
+    T c_bb = fact_com * p_bb / m4;
+    T r = e_total / m_z0;
+    T dzeta = r * r;
+    T delta = (dzeta - (T)1.) / relat_width;
+    T propagator = (T)1. / ((T)1. + delta * delta);
+    T n_ev = static_cast<T>(c->num_events);
+    T norm = R::pow_2pi_m5() / n_ev;
+    T norm_weight = k.ev_weight * norm;
+    T com = norm_weight / (T)4.;
+    T aa = com * c_aa;
+    T bb = com * c_bb * propagator / (relat_width * relat_width);
+    T ab = com * c_ab * (T)2. * beta_plus * propagator / relat_width;
     k.fact_com = fact_com;
     k.norm_weight = norm_weight;
     k.propagator = propagator;
     k.delta = delta;
     k.relat_width = relat_width;
     k.sc[0] = aa;
-    k.sc[1] = bb * (c->beta_plus * c->beta_plus);
-    k.sc[2] = bb * (c->beta_minus * c->beta_minus);
+    k.sc[1] = bb * (beta_plus * beta_plus);
+    k.sc[2] = bb * (beta_minus * beta_minus);
     k.sc[3] = ab * delta;
     k.sc[4] = -ab;
     return k;
 }
 
-} // namespace
-
-extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
+template <class T> void params_from_config(const tp_config *c, tp_params *p)
 {
-    if (!c || !p)
-        return TP_ERR_ARG;
+    using R = Real<T>;
     std::memset(p, 0, sizeof *p);
-    AccConsts k = acc_consts(c);
+    AccConsts<T> k = acc_consts<T>(c);
+    const T alpha = R::cv(c->alpha), alpha_z = R::cv(c->alpha_z), sin2w = R::cv(c->sin2_weinberg);
+    const T m_z0 = R::cv(c->m_z0);
     // Couplings::new (coupling.rs:23-33).
-    double e2 = 4. * PI * c->alpha;
-    double e2_z = 4. * PI * c->alpha_z;
-    double cos2w = 1. - c->sin2_weinberg;
-    double m2 = c->m_z0 * c->m_z0;
-    double m4 = m2 * m2;
-    double g_beta = -std::sqrt(e2_z / (4. * cos2w * c->sin2_weinberg)) / m4;
-    double x = std::sqrt(e2);
-    p->e_total = c->e_total;
-    p->cut_beam = c->cut_beam;
-    p->cut_pp = c->cut_pp;
-    p->e_min = c->e_min;
-    p->cut_plane = c->cut_plane;
+    T e2 = (T)4. * R::PI_ * alpha;
+    T e2_z = (T)4. * R::PI_ * alpha_z;
+    T cos2w = (T)1. - sin2w;
+    T m2 = m_z0 * m_z0;
+    T m4 = m2 * m2;
+    T g_beta = -std::sqrt(e2_z / ((T)4. * cos2w * sin2w)) / m4;
+    T x = std::sqrt(e2);
+    p->e_total = R::cv(c->e_total);
+    p->cut_beam = R::cv(c->cut_beam);
+    p->cut_pp = R::cv(c->cut_pp);
+    p->e_min = R::cv(c->e_min);
+    p->cut_plane = R::cv(c->cut_plane);
     p->g_a = -(x * (x * x));
     p->g_beta_p = g_beta;
     p->g_beta_m = g_beta;
@@ -416,56 +434,58 @@ extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
         p->sigma_contribs[i] = k.sc[i];
     p->ev_weight = k.ev_weight;
     p->sort_photons = 1;
-    p->precision = TP_PREC_F64;
+    p->precision = sizeof(T) == 4 ? TP_PREC_F32 : TP_PREC_F64;
     p->faster_evgen = 0;
     p->reserved = 0;
-    return TP_OK;
 }
 
-extern "C" void tp_acc_merge(tp_acc *into, const tp_acc *o)
+template <class T> void acc_merge(tp_acc *into, const tp_acc *o)
 {
     into->selected_events += o->selected_events;
     for (int i = 0; i < 5; ++i)
-        into->spm2[i] += o->spm2[i];
+        into->spm2[i] = (T)into->spm2[i] + (T)o->spm2[i];
     for (int i = 0; i < 5; ++i)
-        into->vars[i] += o->vars[i];
-    into->sigma += o->sigma;
-    into->variance += o->variance;
+        into->vars[i] = (T)into->vars[i] + (T)o->vars[i];
+    into->sigma = (T)into->sigma + (T)o->sigma;
+    into->variance = (T)into->variance + (T)o->variance;
 }
 
-extern "C" int tp_finalize(const tp_config *cfg, const tp_acc *acc, tp_final *out)
+// ResultsAccumulator::finalize (resacc.rs:142-223).
+template <class T> void finalize(const tp_config *cfg, const tp_acc *acc, tp_final *out)
 {
-    if (!cfg || !acc || !out)
-        return TP_ERR_ARG;
-    AccConsts k = acc_consts(cfg);
-    double spm2v[5], varsv[5];
-    double n_ev = static_cast<double>(cfg->num_events);
+    using R = Real<T>;
+    AccConsts<T> k = acc_consts<T>(cfg);
+    const T e_total = R::cv(cfg->e_total), sin2w = R::cv(cfg->sin2_weinberg);
+    const T g_z0 = R::cv(cfg->g_z0), m_z0 = R::cv(cfg->m_z0);
+    T spm2v[5], varsv[5];
+    T n_ev = static_cast<T>(cfg->num_events);
     for (int c = 0; c < 5; ++c) {
-        double v = (acc->vars[c] - acc->spm2[c] * acc->spm2[c] / n_ev) / (n_ev - 1.);
-        varsv[c] = std::sqrt(v / n_ev) / std::fabs(acc->spm2[c] / n_ev);
-        spm2v[c] = acc->spm2[c];
+        const T s = (T)acc->spm2[c], q = (T)acc->vars[c];
+        T v = (q - s * s / n_ev) / (n_ev - (T)1.);
+        varsv[c] = std::sqrt(v / n_ev) / std::fabs(s / n_ev);
+        spm2v[c] = s;
     }
-    double spm2[2][5], vars[2][5];
+    T spm2[2][5], vars[2][5];
     for (int s = 0; s < 2; ++s)
         for (int c = 0; c < 5; ++c) {
             spm2[s][c] = spm2v[c];
             vars[s][c] = varsv[c];
         }
-    double polar_p = -2. * cfg->sin2_weinberg;
-    double polar_m = 1. + polar_p;
-    const double polars[2] = {polar_m, polar_p};
+    T polar_p = (T)-2. * sin2w;
+    T polar_m = (T)1. + polar_p;
+    const T polars[2] = {polar_m, polar_p};
     for (int c = 1; c < 5; ++c)
         for (int s = 0; s < 2; ++s)
             spm2[s][c] *= polars[s];
     for (int c = 1; c < 3; ++c)
         for (int s = 0; s < 2; ++s)
             spm2[s][c] *= polars[s];
-    double incident_flux = 1. / (2. * (cfg->e_total * cfg->e_total));
-    double scale = k.fact_com * incident_flux * k.norm_weight;
+    T incident_flux = (T)1. / ((T)2. * (e_total * e_total));
+    T scale = k.fact_com * incident_flux * k.norm_weight;
     for (int s = 0; s < 2; ++s)
         for (int c = 0; c < 5; ++c)
             spm2[s][c] *= scale;
-    double gm_z0 = cfg->g_z0 * cfg->m_z0;
+    T gm_z0 = g_z0 * m_z0;
     for (int c = 1; c < 5; ++c)
         for (int s = 0; s < 2; ++s)
             spm2[s][c] *= k.propagator / gm_z0;
@@ -474,25 +494,29 @@ extern "C" int tp_finalize(const tp_config *cfg, const tp_acc *acc, tp_final *ou
             spm2[s][c] /= gm_z0;
     for (int s = 0; s < 2; ++s)
         spm2[s][3] *= k.delta;
-    auto colsum = [&](int col) { return (0. + spm2[0][col]) + spm2[1][col]; };
+    auto colsum = [&](int col) { return ((T)0. + spm2[0][col]) + spm2[1][col]; };
     auto inc_num = [&](int col) {
-        double a = spm2[0][col] * vars[0][col], b = spm2[1][col] * vars[1][col];
-        return std::sqrt(0. + (a * a + b * b));
+        T a = spm2[0][col] * vars[0][col], b = spm2[1][col] * vars[1][col];
+        return std::sqrt((T)0. + (a * a + b * b));
     };
-    double beta_min = std::sqrt(colsum(0) / colsum(1));
-    double ss_denom = colsum(0);
-    double ss_norm = 1. / (2. * std::sqrt(ss_denom));
-    double ss_p = colsum(1) * ss_norm;
-    double ss_m = colsum(2) * ss_norm;
-    double inc_common = inc_num(0) / (2. * std::fabs(ss_denom));
-    double inc_ss_p = inc_num(1) / std::fabs(colsum(1)) + inc_common;
-    double inc_ss_m = inc_num(2) / std::fabs(colsum(2)) + inc_common;
-    double variance = (acc->variance - acc->sigma * acc->sigma / n_ev) / (n_ev - 1.);
-    double prec = std::sqrt(variance / n_ev) / std::fabs(acc->sigma / n_ev);
-    double sigma = acc->sigma * incident_flux;
+    T beta_min = std::sqrt(colsum(0) / colsum(1));
+    T ss_denom = colsum(0);
+    T ss_norm = (T)1. / ((T)2. * std::sqrt(ss_denom));
+    T ss_p = colsum(1) * ss_norm;
+    T ss_m = colsum(2) * ss_norm;
+    T inc_common = inc_num(0) / ((T)2. * std::fabs(ss_denom));
+    T inc_ss_p = inc_num(1) / std::fabs(colsum(1)) + inc_common;
+    T inc_ss_m = inc_num(2) / std::fabs(colsum(2)) + inc_common;
+    const T sig = (T)acc->sigma, var = (T)acc->variance;
+    T variance = (var - sig * sig / n_ev) / (n_ev - (T)1.);
+    T prec = std::sqrt(variance / n_ev) / std::fabs(sig / n_ev);
+    T sigma = sig * incident_flux;
     out->selected_events = acc->selected_events;
-    std::memcpy(out->spm2, spm2, sizeof spm2);
-    std::memcpy(out->vars, vars, sizeof vars);
+    for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 5; ++c) {
+            out->spm2[s][c] = spm2[s][c];
+            out->vars[s][c] = vars[s][c];
+        }
     out->sigma = sigma;
     out->prec = prec;
     out->variance = variance;
@@ -501,6 +525,43 @@ extern "C" int tp_finalize(const tp_config *cfg, const tp_acc *acc, tp_final *ou
     out->inc_ss_p = inc_ss_p;
     out->ss_m = ss_m;
     out->inc_ss_m = inc_ss_m;
+}
+
+} // namespace
+
+extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
+{
+    if (!c || !p)
+        return TP_ERR_ARG;
+    params_from_config<double>(c, p);
+    return TP_OK;
+}
+
+extern "C" int tp_params_from_config_f32(const tp_config *c, tp_params *p)
+{
+    if (!c || !p)
+        return TP_ERR_ARG;
+    params_from_config<float>(c, p);
+    return TP_OK;
+}
+
+extern "C" void tp_acc_merge(tp_acc *into, const tp_acc *o) { acc_merge<double>(into, o); }
+
+extern "C" void tp_acc_merge_f32(tp_acc *into, const tp_acc *o) { acc_merge<float>(into, o); }
+
+extern "C" int tp_finalize(const tp_config *cfg, const tp_acc *acc, tp_final *out)
+{
+    if (!cfg || !acc || !out)
+        return TP_ERR_ARG;
+    finalize<double>(cfg, acc, out);
+    return TP_OK;
+}
+
+extern "C" int tp_finalize_f32(const tp_config *cfg, const tp_acc *acc, tp_final *out)
+{
+    if (!cfg || !acc || !out)
+        return TP_ERR_ARG;
+    finalize<float>(cfg, acc, out);
     return TP_OK;
 }
 

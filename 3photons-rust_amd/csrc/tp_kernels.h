@@ -76,7 +76,25 @@ void launch_ranf_expand(const RanfArgs &a, uint64_t npieces, hipStream_t st);
 void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0,
                       hipStream_t st);
 // In-order left fold of nb accumulators (resacc.rs:133-139) into *merged.
-void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st);
+// In-order fold of nb accumulators; f32 = 1 adds in float (all-f32 mode).
+void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st, int f32 = 0);
+
+// ---- all-f32 mode: events, then per-batch sums in event order --------------
+struct F32Args {
+    tpd::FParams P;
+    uint64_t nb, b_first, num_events;
+    uint32_t batch_size;
+    int32_t source;        // 0 xoshiro128+ in registers, 1 RANF values from draws
+    uint32_t s0[4];        // xoshiro128+ state of the run
+    const uint32_t *mt;    // M^(2^k) of xoshiro128+, k < km (128 columns x 4 words)
+    const uint32_t *bt;    // B^(2^k), k < kb: B = M^(12 batch_size) or the 2^64 jump
+    int32_t km, kb;
+    const int32_t *draws;  // RANF values of this launch's batches, 12 per event, batch-major
+    float *ev;             // per event m[5] and keep (0/1), at batch_size*(b - b_first) + e
+    void *out;             // tp_acc per batch, index b - b_first (f32 values)
+};
+void launch_f32_events(const F32Args &a, int grid, hipStream_t st);
+void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st);
 
 } // namespace tpk
 

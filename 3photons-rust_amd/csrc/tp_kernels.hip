@@ -545,6 +545,7 @@ void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *ori
 constexpr int MERGE_TILE = 512;
 constexpr int MERGE_THREADS = 256;
 
+template <class T>
 __global__ __launch_bounds__(MERGE_THREADS) void merge_kernel(const DevAcc *in, uint64_t nb, DevAcc *out)
 {
     __shared__ double tile[12][MERGE_TILE + 1]; // +1: the 12 chains hit distinct banks
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void merge_kernel(const DevAcc *in, 
     if (tid == 0)
         cnt = 0;
     const uint64_t *words = reinterpret_cast<const uint64_t *>(in); // 13 words per accumulator
-    double t = 0.;
+    T t = (T)0;
     unsigned long long my_cnt = 0;
     for (uint64_t base = 0; base < nb; base += MERGE_TILE) {
         const uint64_t m = (nb - base) < (uint64_t)MERGE_TILE ? (nb - base) : (uint64_t)MERGE_TILE;
@@ -569,7 +570,7 @@ __global__ __launch_bounds__(MERGE_THREADS) void merge_kernel(const DevAcc *in, 
         if (tid < 12) {
             uint64_t e = 0;
             if (base == 0) {
-                t = tile[tid][0];
+                t = (T)tile[tid][0];
                 e = 1;
             }
             // 16 independent LDS reads in flight, then 16 dependent adds.
@@ -580,25 +581,139 @@ __global__ __launch_bounds__(MERGE_THREADS) void merge_kernel(const DevAcc *in, 
                     v[u] = tile[tid][e + u];
 #pragma unroll
                 for (int u = 0; u < 16; ++u)
-                    t = t + v[u];
+                    t = t + (T)v[u];
             }
             for (; e < m; ++e)
-                t = t + tile[tid][e];
+                t = t + (T)tile[tid][e];
         }
         __syncthreads();
     }
     atomicAdd(&cnt, my_cnt);
     __syncthreads();
     if (tid < 12)
-        reinterpret_cast<double *>(out)[1 + tid] = t;
+        reinterpret_cast<double *>(out)[1 + tid] = (double)t;
     if (tid == 0)
         out->selected = cnt;
 }
 
-void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st)
+void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t st, int f32)
 {
-    hipLaunchKernelGGL(merge_kernel, dim3(1), dim3(MERGE_THREADS), 0, st,
-                       reinterpret_cast<const DevAcc *>(per_batch), nb, reinterpret_cast<DevAcc *>(merged));
+    if (f32)
+        hipLaunchKernelGGL(merge_kernel<float>, dim3(1), dim3(MERGE_THREADS), 0, st,
+                           reinterpret_cast<const DevAcc *>(per_batch), nb, reinterpret_cast<DevAcc *>(merged));
+    else
+        hipLaunchKernelGGL(merge_kernel<double>, dim3(1), dim3(MERGE_THREADS), 0, st,
+                           reinterpret_cast<const DevAcc *>(per_batch), nb, reinterpret_cast<DevAcc *>(merged));
+}
+
+// ---- all-f32 mode --------------------------------------------------------------
+// One wavefront per batch (grid-stride).  Each lane takes a contiguous run of
+// the batch's events (one skip-ahead of the xoshiro128+ stream per run) and
+// writes each event's m[5] and cut decision; the batch sums are then formed
+// in event order, as the reference's sequential loop does in f32
+// (resacc.rs:121-129), where a tree sum would round differently.
+template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F32Args a)
+{
+    const uint32_t lane = threadIdx.x;
+    for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
+        const uint64_t b = a.b_first + lb;
+        const uint64_t left = a.num_events - b * a.batch_size;
+        const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
+        const uint32_t q = n / 64, r = n % 64;
+        const uint32_t len = q + (lane < r ? 1u : 0u);
+        const uint32_t e0 = lane * q + (lane < r ? lane : r);
+        uint32_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
+        if (SOURCE == 0) {
+            for (int k = 0; k < a.kb; ++k)
+                if ((b >> k) & 1ull) // uniform across the wavefront
+                    gf2_apply128(a.bt + 512 * k, s);
+            const uint64_t off = 12ull * e0;
+            for (int k = 0; k < a.km; ++k)
+                if ((off >> k) & 1ull)
+                    gf2_apply128(a.mt + 512 * k, s);
+        }
+        for (uint32_t i = 0; i < len; ++i) {
+            const uint64_t e = lb * a.batch_size + e0 + i;
+            float u[12], m[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            if (SOURCE == 0) {
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    u[j] = xo128_uniform(s);
+            } else {
+                const int32_t *d = a.draws + 12 * e;
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    u[j] = (float)d[j] * 1e-9f; // (n as f32) * INV_MODULO (ranf.rs:99)
+            }
+            const bool keep = event_f32(a.P, u, m);
+            float *o = a.ev + 6 * e;
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                o[c] = m[c];
+            o[5] = keep ? 1.f : 0.f;
+        }
+    }
+}
+
+constexpr int F32_TILE = 512;
+
+__global__ __launch_bounds__(64) void f32_accumulate_kernel(F32Args a)
+{
+    __shared__ float tile[F32_TILE * 6];
+    const uint32_t lane = threadIdx.x;
+    DevAcc *out = reinterpret_cast<DevAcc *>(a.out);
+    for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
+        const uint64_t b = a.b_first + lb;
+        const uint64_t left = a.num_events - b * a.batch_size;
+        const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
+        const float *ev = a.ev + 6 * lb * a.batch_size;
+        float t = 0.f; // lane j < 12: sum j of (spm2[5], vars[5], sigma, variance)
+        uint32_t cnt = 0;
+        for (uint32_t base = 0; base < n; base += F32_TILE) {
+            const uint32_t mm = n - base < (uint32_t)F32_TILE ? n - base : (uint32_t)F32_TILE;
+            __syncthreads();
+            for (uint32_t f = lane; f < 6 * mm; f += 64)
+                tile[f] = ev[6 * (size_t)base + f];
+            __syncthreads();
+            if (lane < 13) {
+                for (uint32_t i = 0; i < mm; ++i) {
+                    const float *x = tile + 6 * i;
+                    if (x[5] == 0.f)
+                        continue;
+                    float v;
+                    if (lane < 5) {
+                        v = x[lane];
+                    } else if (lane < 10) {
+                        v = x[lane - 5] * x[lane - 5];
+                    } else if (lane < 12) {
+                        const float w = weight_f32(a.P, x);
+                        v = lane == 10 ? w : w * w;
+                    } else {
+                        ++cnt;
+                        continue;
+                    }
+                    t = t + v;
+                }
+            }
+        }
+        if (lane < 12)
+            reinterpret_cast<double *>(out + lb)[1 + lane] = (double)t;
+        else if (lane == 12)
+            out[lb].selected = cnt;
+    }
+}
+
+void launch_f32_events(const F32Args &a, int grid, hipStream_t st)
+{
+    if (a.source == 0)
+        hipLaunchKernelGGL(f32_events_kernel<0>, dim3(grid), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(f32_events_kernel<1>, dim3(grid), dim3(64), 0, st, a);
+}
+
+void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(f32_accumulate_kernel, dim3(grid), dim3(64), 0, st, a);
 }
 
 } // namespace tpk

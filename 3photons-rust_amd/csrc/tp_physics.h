@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace tpd {
 
 // Per-run constants, computed on the host with the same IEEE operations.
@@ -203,75 +205,104 @@ __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T 
         }
 }
 
+// Complex products of the amplitude algebra.  With FAST (f32 only) they
+// contract multiply-adds into fma and divide by a hardware reciprocal (the
+// mixed-precision path and the f32 adjoint); otherwise they are the
+// reference's plain operations (the f64 and all-f32 parity paths).
+__device__ __forceinline__ C<float> cmul_fast(C<float> a, C<float> b)
+{
+    return C<float>{fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ C<float> cdiv_fast(C<float> a, C<float> b)
+{
+    const float inv = __builtin_amdgcn_rcpf(fmaf(b.re, b.re, b.im * b.im));
+    return C<float>{fmaf(a.re, b.re, a.im * b.im) * inv, fmaf(a.im, b.re, -(a.re * b.im)) * inv};
+}
+template <bool FAST, class T> __device__ __forceinline__ C<T> xmul(C<T> a, C<T> b)
+{
+    if constexpr (FAST && std::is_same<T, float>::value)
+        return cmul_fast(a, b);
+    else
+        return cmul(a, b);
+}
+template <bool FAST, class T> __device__ __forceinline__ C<T> xdiv(C<T> a, C<T> b)
+{
+    if constexpr (FAST && std::is_same<T, float>::value)
+        return cdiv_fast(a, b);
+    else
+        return cdiv(a, b);
+}
+template <bool FAST, class T> __device__ __forceinline__ C<T> xsq(C<T> z) { return xmul<FAST>(z, z); }
+
 // -RAC8*x*y/z is ((-RAC8*x)*y)/z; the (-RAC8)*s(e-,e+) and (-RAC8)*t(e-,e+)
 // factors are per-run constants (ns01, nt01).
-template <class T>
+template <class T, bool FAST = true>
 __device__ __forceinline__ C<T> a_ppm(const Spin<T> &S, C<T> ns01, int k1, int k2, int k3)
 {
-    C<T> num = cmul(ns01, csq(S.s[0][k3]));
-    C<T> den = cmul(cmul(cmul(S.s[0][k1], S.s[0][k2]), S.s[1][k1]), S.s[1][k2]);
-    return cdiv(num, den);
+    C<T> num = xmul<FAST>(ns01, xsq<FAST>(S.s[0][k3]));
+    C<T> den = xmul<FAST>(xmul<FAST>(xmul<FAST>(S.s[0][k1], S.s[0][k2]), S.s[1][k1]), S.s[1][k2]);
+    return xdiv<FAST>(num, den);
 }
-template <class T>
+template <class T, bool FAST = true>
 __device__ __forceinline__ C<T> a_pmm(const Spin<T> &S, C<T> nt01, int k1, int k2, int k3)
 {
-    C<T> num = cmul(nt01, csq(tof(S.s[1][k1])));
-    C<T> den = cmul(cmul(cmul(tof(S.s[1][k2]), tof(S.s[1][k3])), tof(S.s[0][k2])), tof(S.s[0][k3]));
-    return cdiv(num, den);
+    C<T> num = xmul<FAST>(nt01, xsq<FAST>(tof(S.s[1][k1])));
+    C<T> den = xmul<FAST>(xmul<FAST>(xmul<FAST>(tof(S.s[1][k2]), tof(S.s[1][k3])), tof(S.s[0][k2])), tof(S.s[0][k3]));
+    return xdiv<FAST>(num, den);
 }
-template <class T>
+template <class T, bool FAST = true>
 __device__ __forceinline__ C<T> bp_ppm(const Spin<T> &S, C<T> nt01, int k1, int k2, int k3)
 {
-    return cmul(nt01, csq(cmul(tof(S.s[k1][k2]), S.s[k3][0])));
+    return xmul<FAST>(nt01, xsq<FAST>(xmul<FAST>(tof(S.s[k1][k2]), S.s[k3][0])));
 }
-template <class T>
+template <class T, bool FAST = true>
 __device__ __forceinline__ C<T> bp_pmm(const Spin<T> &S, C<T> ns01, int k1, int k2, int k3)
 {
-    return cmul(ns01, csq(cmul(tof(S.s[k1][1]), S.s[k2][k3])));
+    return xmul<FAST>(ns01, xsq<FAST>(xmul<FAST>(tof(S.s[k1][1]), S.s[k2][k3])));
 }
-template <class T> __device__ __forceinline__ C<T> bm_ppp(const Spin<T> &S, C<T> ns01)
+template <class T, bool FAST = true> __device__ __forceinline__ C<T> bm_ppp(const Spin<T> &S, C<T> ns01)
 {
     const int k1 = 2, k2 = 3, k3 = 4;
-    C<T> s = cadd(cadd(csq(cmul(tof(S.s[k1][k2]), tof(S.s[k3][1]))), csq(cmul(tof(S.s[k1][k3]), tof(S.s[k2][1])))),
-                  csq(cmul(tof(S.s[k2][k3]), tof(S.s[k1][1]))));
-    return cmul(ns01, s);
+    C<T> s = cadd(cadd(xsq<FAST>(xmul<FAST>(tof(S.s[k1][k2]), tof(S.s[k3][1]))), xsq<FAST>(xmul<FAST>(tof(S.s[k1][k3]), tof(S.s[k2][1])))),
+                  xsq<FAST>(xmul<FAST>(tof(S.s[k2][k3]), tof(S.s[k1][1]))));
+    return xmul<FAST>(ns01, s);
 }
-template <class T> __device__ __forceinline__ C<T> bm_mmm(const Spin<T> &S, C<T> nt01)
+template <class T, bool FAST = true> __device__ __forceinline__ C<T> bm_mmm(const Spin<T> &S, C<T> nt01)
 {
     const int k1 = 2, k2 = 3, k3 = 4;
-    C<T> s = cadd(cadd(csq(cmul(S.s[k1][0], S.s[k2][k3])), csq(cmul(S.s[k2][0], S.s[k1][k3]))),
-                  csq(cmul(S.s[k3][0], S.s[k1][k2])));
-    return cmul(nt01, s);
+    C<T> s = cadd(cadd(xsq<FAST>(xmul<FAST>(S.s[k1][0], S.s[k2][k3])), xsq<FAST>(xmul<FAST>(S.s[k2][0], S.s[k1][k3]))),
+                  xsq<FAST>(xmul<FAST>(S.s[k3][0], S.s[k1][k2])));
+    return xmul<FAST>(nt01, s);
 }
 
 // Helicity sums of the five matrix-element contributions (matelems.rs:53-86):
 // m[0] = sum|A g_a|^2, m[1] = sum|B+ g_b+|^2, m[2] = sum|B- g_b-|^2,
 // m[3] + i m[4] = sum (2 A g_a) conj(B+ g_b+).  Helicities MMP..PPM carry A and
 // B+ (spinor.rs:72-98), MMM and PPP carry B- only (spinor.rs:100-113).
-template <class T>
+template <class T, bool FAST = true>
 __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> nt01, T g_a, T g_bp, T g_bm,
                                               T m[5])
 {
     // (A, B+) for h = MMP, MPM, MPP, PMM, PMP, PPM.
     C<T> a[6], b[6];
-    a[0] = a_pmm(S, nt01, 4, 2, 3);
-    b[0] = bp_pmm(S, ns01, 4, 2, 3);
-    a[1] = a_pmm(S, nt01, 3, 2, 4);
-    b[1] = bp_pmm(S, ns01, 3, 2, 4);
-    a[2] = a_ppm(S, ns01, 3, 4, 2);
-    b[2] = bp_ppm(S, nt01, 3, 4, 2);
-    a[3] = a_pmm(S, nt01, 2, 3, 4);
-    b[3] = bp_pmm(S, ns01, 2, 3, 4);
-    a[4] = a_ppm(S, ns01, 2, 4, 3);
-    b[4] = bp_ppm(S, nt01, 2, 4, 3);
-    a[5] = a_ppm(S, ns01, 2, 3, 4);
-    b[5] = bp_ppm(S, nt01, 2, 3, 4);
+    a[0] = a_pmm<T, FAST>(S, nt01, 4, 2, 3);
+    b[0] = bp_pmm<T, FAST>(S, ns01, 4, 2, 3);
+    a[1] = a_pmm<T, FAST>(S, nt01, 3, 2, 4);
+    b[1] = bp_pmm<T, FAST>(S, ns01, 3, 2, 4);
+    a[2] = a_ppm<T, FAST>(S, ns01, 3, 4, 2);
+    b[2] = bp_ppm<T, FAST>(S, nt01, 3, 4, 2);
+    a[3] = a_pmm<T, FAST>(S, nt01, 2, 3, 4);
+    b[3] = bp_pmm<T, FAST>(S, ns01, 2, 3, 4);
+    a[4] = a_ppm<T, FAST>(S, ns01, 2, 4, 3);
+    b[4] = bp_ppm<T, FAST>(S, nt01, 2, 4, 3);
+    a[5] = a_ppm<T, FAST>(S, ns01, 2, 3, 4);
+    b[5] = bp_ppm<T, FAST>(S, nt01, 2, 3, 4);
     T sA = 0, sP = 0, sR = 0, sI = 0;
 #pragma unroll
     for (int h = 0; h < 6; ++h) {
         C<T> A = mk<T>(a[h].re * g_a, a[h].im * g_a);
         C<T> Pp = mk<T>(b[h].re * g_bp, b[h].im * g_bp);
-        C<T> X = cmul(mk<T>((T)2. * A.re, (T)2. * A.im), mk<T>(Pp.re, -Pp.im));
+        C<T> X = xmul<FAST>(mk<T>((T)2. * A.re, (T)2. * A.im), mk<T>(Pp.re, -Pp.im));
         T nA = A.re * A.re + A.im * A.im;
         T nP = Pp.re * Pp.re + Pp.im * Pp.im;
         if (h == 0) {
@@ -286,8 +317,8 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> 
             sI = sI + X.im;
         }
     }
-    C<T> bmm = bm_mmm(S, nt01);
-    C<T> bpp = bm_ppp(S, ns01);
+    C<T> bmm = bm_mmm<T, FAST>(S, nt01);
+    C<T> bpp = bm_ppp<T, FAST>(S, ns01);
     C<T> Bm = mk<T>(bmm.re * g_bm, bmm.im * g_bm);
     C<T> Bp = mk<T>(bpp.re * g_bm, bpp.im * g_bm);
     m[0] = sA;
@@ -299,21 +330,12 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> 
 
 // ---- mixed precision: f32 spinor/amplitude algebra ------------------------
 //
-// The f32 instantiations contract multiply-adds into fma and divide by a
-// hardware reciprocal (1 ulp): this path trades the reference's rounding for
-// speed and is judged by its error against the f64 path on the same stream.
+// The mixed path runs the amplitudes with FAST complex products (fma and a
+// hardware reciprocal, 1 ulp): it trades the reference's rounding for speed
+// and is judged by its error against the f64 path on the same stream.
 // After the cuts every spinor product of the shipped configuration has
 // |s|^2 = 2 p_i.p_j >= 2.5 GeV^2, and every amplitude denominator stays far
 // inside the f32 range.
-template <> __device__ __forceinline__ C<float> cmul(C<float> a, C<float> b)
-{
-    return C<float>{fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re)};
-}
-template <> __device__ __forceinline__ C<float> cdiv(C<float> a, C<float> b)
-{
-    const float inv = __builtin_amdgcn_rcpf(fmaf(b.re, b.re, b.im * b.im));
-    return C<float>{fmaf(a.re, b.re, a.im * b.im) * inv, fmaf(a.im, b.re, -(a.re * b.im)) * inv};
-}
 
 // ez[k] = E_k + Z_k is formed in f64 by the caller: for a photon close to -Z
 // the f32 difference of two rounded values could go negative.
@@ -501,6 +523,129 @@ __device__ __forceinline__ bool event_f64(const Params &P, const double u[12], d
         return false;
     matrix_elements_f64(P, p, m);
     return true;
+}
+
+// ---- all-f32 mode (numeric.rs:6-14, feature f32) ----------------------------
+// Every quantity in f32 with the reference's operation order and plain IEEE
+// operations (no contraction: the translation unit is built with
+// -ffp-contract=off; FAST = false in the amplitude algebra).  Run constants
+// are computed on the host in f32 arithmetic.
+struct FParams {
+    float e_total, cut_beam, cut_pp, e_min, cut_plane;
+    float g_a, g_bp, g_bm;
+    float sc[5];
+    float el_x, el_e;
+    float xx0, f0re, f0im, xx1, f1re, f1im;
+    float ns01re, ns01im, nt01re, nt01im;
+    int sort;
+};
+
+// sin, cos and ln of an f32 argument are evaluated in f64 and rounded once:
+// the correctly rounded f32 result in all but astronomically rare cases,
+// which is what the host libm's f32 functions return (within their < 1 ulp
+// error bound) far more often than a faster f32 device routine would.
+__device__ __forceinline__ void raw_photon_f32(float uc, float uphi, float ue1, float ue2, float q[4])
+{
+    const float two_pi = 2.f * 3.14159265358979323846f;
+    float cos_t = 2.f * uc - 1.f;
+    float phi = two_pi * uphi;
+    float em = ue1 * ue2;
+    double sd, cd;
+    sincos((double)phi, &sd, &cd);
+    const float sphi = (float)sd, cphi = (float)cd;
+    float st = sqrtf(1.f - cos_t * cos_t);
+    float en = -(float)log((double)(em + 1.17549435e-38f));
+    q[0] = en * (st * sphi);
+    q[1] = en * (st * cphi);
+    q[2] = en * cos_t;
+    q[3] = en;
+}
+
+__device__ __forceinline__ void conformal_f32(const FParams &P, const float q[3][4], float p[3][4])
+{
+    float r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r[k] = (q[0][k] + q[1][k]) + q[2][k];
+    float rn2 = r[3] * r[3] - ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
+    float alpha = P.e_total / rn2;
+    float rn = sqrtf(rn2);
+    float beta = 1.f / (rn + r[3]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float rq = q[i][0] * r[0];
+        rq = q[i][1] * r[1] + rq;
+        rq = q[i][2] * r[2] + rq;
+        p[i][3] = alpha * (r[3] * q[i][3] - rq);
+        float b = beta * rq - q[i][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            p[i][k] = alpha * (rn * q[i][k] + b * r[k]);
+    }
+}
+
+__device__ __forceinline__ bool keep_f32(const FParams &P, const float p[3][4])
+{
+    float emin;
+    if (P.sort) {
+        emin = p[2][3];
+    } else {
+        emin = p[0][3];
+        emin = (emin < p[1][3]) ? emin : p[1][3];
+        emin = (emin < p[2][3]) ? emin : p[2][3];
+    }
+    bool ok = !(emin < P.e_min);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float num = p[i][0] * P.el_x;
+        float den = p[i][3] * P.el_e;
+        ok = ok && !(fabsf(num) > P.cut_beam * den);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 3; ++j) {
+            float num = (p[i][0] * p[j][0] + p[i][1] * p[j][1]) + p[i][2] * p[j][2];
+            float den = p[i][3] * p[j][3];
+            ok = ok && !(num > P.cut_pp * den);
+        }
+    if (P.cut_plane > 0.f) {
+        float n0 = p[0][1] * p[1][2] - p[0][2] * p[1][1];
+        float n1 = p[0][2] * p[1][0] - p[0][0] * p[1][2];
+        float n2 = p[0][0] * p[1][1] - p[0][1] * p[1][0];
+        float num = P.el_x * n0;
+        float nn = sqrtf((n0 * n0 + n1 * n1) + n2 * n2);
+        ok = ok && !(fabsf(num) < P.cut_plane * (P.el_e * nn));
+    }
+    return ok;
+}
+
+// One f32 event from 12 uniforms: cut decision and helicity sums m[5].
+__device__ __forceinline__ bool event_f32(const FParams &P, const float u[12], float m[5])
+{
+    float q[3][4], p[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        raw_photon_f32(u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3], q[i]);
+    conformal_f32(P, q, p);
+    if (P.sort)
+        sort3(p);
+    if (!keep_f32(P, p))
+        return false;
+    Spin<float> S;
+    spinor_products<float>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
+    helicity_sums<float, false>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+    return true;
+}
+
+__device__ __forceinline__ float weight_f32(const FParams &P, const float m[5])
+{
+    float w = m[0] * P.sc[0];
+    w = w + m[1] * P.sc[1];
+    w = w + m[2] * P.sc[2];
+    w = w + m[3] * P.sc[3];
+    w = w + m[4] * P.sc[4];
+    return w;
 }
 
 } // namespace tpd

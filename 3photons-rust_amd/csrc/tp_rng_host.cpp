@@ -255,6 +255,141 @@ void xoshiro_advance(uint64_t s[4], uint64_t n)
             gf2_apply(t[static_cast<size_t>(k)], s, s);
 }
 
+// ---- xoshiro128+ ---------------------------------------------------------------
+
+void xoshiro128_seed(uint64_t seed, uint32_t s[4])
+{
+    uint64_t x = seed;
+    const uint64_t a = splitmix64(x), b = splitmix64(x);
+    s[0] = (uint32_t)a;
+    s[1] = (uint32_t)(a >> 32);
+    s[2] = (uint32_t)b;
+    s[3] = (uint32_t)(b >> 32);
+}
+
+static inline uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+static inline void xo128_update(uint32_t *s)
+{
+    const uint32_t t = s[1] << 9;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl32(s[3], 11);
+}
+
+uint32_t xoshiro128_next(uint32_t s[4])
+{
+    const uint32_t r = s[0] + s[3];
+    xo128_update(s);
+    return r;
+}
+
+static const uint32_t XO128_JUMP[4] = {0x8764000bu, 0xf542d2d3u, 0x6fa035c3u, 0x77f2db5bu};
+
+static void xo128_jump_state(uint32_t *s)
+{
+    uint32_t a[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 4; ++w)
+        for (int b = 0; b < 32; ++b) {
+            if (XO128_JUMP[w] & (1u << b))
+                for (int q = 0; q < 4; ++q)
+                    a[q] ^= s[q];
+            xo128_update(s);
+        }
+    std::memcpy(s, a, sizeof a);
+}
+
+void xoshiro128_jump(uint32_t s[4]) { xo128_jump_state(s); }
+
+void gf2_128_identity(Gf2Mat128 &m)
+{
+    std::memset(&m, 0, sizeof m);
+    for (int i = 0; i < 128; ++i)
+        m.col[i][i / 32] = 1u << (i % 32);
+}
+
+void gf2_128_step(Gf2Mat128 &m)
+{
+    gf2_128_identity(m);
+    for (int i = 0; i < 128; ++i)
+        xo128_update(m.col[i]);
+}
+
+void gf2_128_jump64(Gf2Mat128 &m)
+{
+    gf2_128_identity(m);
+    for (int i = 0; i < 128; ++i)
+        xo128_jump_state(m.col[i]);
+}
+
+void gf2_128_apply(const Gf2Mat128 &m, const uint32_t in[4], uint32_t out[4])
+{
+    uint32_t r[4] = {0, 0, 0, 0};
+    for (int w = 0; w < 4; ++w) {
+        uint32_t x = in[w];
+        while (x) {
+            const int b = __builtin_ctz(x);
+            x &= x - 1;
+            const uint32_t *c = m.col[w * 32 + b];
+            r[0] ^= c[0];
+            r[1] ^= c[1];
+            r[2] ^= c[2];
+            r[3] ^= c[3];
+        }
+    }
+    std::memcpy(out, r, sizeof r);
+}
+
+void gf2_128_mul(const Gf2Mat128 &a, const Gf2Mat128 &b, Gf2Mat128 &out)
+{
+    Gf2Mat128 tmp;
+    for (int i = 0; i < 128; ++i)
+        gf2_128_apply(a, b.col[i], tmp.col[i]);
+    out = tmp;
+}
+
+std::vector<Gf2Mat128> gf2_128_pow2_table(const Gf2Mat128 &base, int count)
+{
+    std::vector<Gf2Mat128> t(static_cast<size_t>(count));
+    if (count == 0)
+        return t;
+    t[0] = base;
+    for (int k = 1; k < count; ++k)
+        gf2_128_mul(t[k - 1], t[k - 1], t[k]);
+    return t;
+}
+
+void gf2_128_pow_from_table(const std::vector<Gf2Mat128> &table, uint64_t n, Gf2Mat128 &out)
+{
+    gf2_128_identity(out);
+    for (size_t k = 0; n && k < table.size(); ++k, n >>= 1)
+        if (n & 1)
+            gf2_128_mul(table[k], out, out);
+}
+
+const std::vector<Gf2Mat128> &xoshiro128_step_pow2()
+{
+    static std::once_flag once;
+    static std::vector<Gf2Mat128> table;
+    std::call_once(once, [] {
+        Gf2Mat128 m;
+        gf2_128_step(m);
+        table = gf2_128_pow2_table(m, 64);
+    });
+    return table;
+}
+
+void xoshiro128_advance(uint32_t s[4], uint64_t n)
+{
+    const std::vector<Gf2Mat128> &t = xoshiro128_step_pow2();
+    for (int k = 0; n; ++k, n >>= 1)
+        if (n & 1)
+            gf2_128_apply(t[static_cast<size_t>(k)], s, s);
+}
+
 // ---- faster-evgen stream walk ----------------------------------------------
 namespace {
 struct RanfWalker {

@@ -73,4 +73,23 @@ const std::vector<Gf2Mat> &xoshiro_step_pow2();
 // Advance a state by n steps with the cached tables.
 void xoshiro_advance(uint64_t s[4], uint64_t n);
 
+// ---- xoshiro128+ (the generator of the all-f32 mode, standard.rs with the
+// f32 feature; rand_xoshiro 0.6.0): same structure on 4 x 32-bit words ------
+void xoshiro128_seed(uint64_t seed, uint32_t s[4]); // seed_from_u64: SplitMix64 bytes, little-endian
+uint32_t xoshiro128_next(uint32_t s[4]);
+void xoshiro128_jump(uint32_t s[4]);                // the published 2^64 jump()
+
+struct Gf2Mat128 {
+    uint32_t col[128][4];
+};
+void gf2_128_identity(Gf2Mat128 &m);
+void gf2_128_step(Gf2Mat128 &m);
+void gf2_128_jump64(Gf2Mat128 &m);
+void gf2_128_mul(const Gf2Mat128 &a, const Gf2Mat128 &b, Gf2Mat128 &out);
+void gf2_128_apply(const Gf2Mat128 &m, const uint32_t in[4], uint32_t out[4]);
+std::vector<Gf2Mat128> gf2_128_pow2_table(const Gf2Mat128 &base, int count);
+void gf2_128_pow_from_table(const std::vector<Gf2Mat128> &table, uint64_t n, Gf2Mat128 &out);
+const std::vector<Gf2Mat128> &xoshiro128_step_pow2(); // M^(2^k), k = 0..63 (cached)
+void xoshiro128_advance(uint32_t s[4], uint64_t n);
+
 } // namespace tp

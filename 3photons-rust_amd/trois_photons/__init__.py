@@ -23,7 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libtp.so")
 
 BATCH_SIZE = 10000
-RNG_RANF, RNG_XOSHIRO = 0, 1
+RNG_RANF, RNG_XOSHIRO, RNG_XOSHIRO128 = 0, 1, 2
 SEED_SKIPAHEAD, SEED_JUMP = 0, 1
 PREC_F64, PREC_MIXED, PREC_F32 = 0, 1, 2
 TP_OK = 0
@@ -61,6 +61,10 @@ class RanfState(C.Structure):
 
 class XoshiroState(C.Structure):
     _fields_ = [("s", C.c_uint64 * 4)]
+
+
+class Xoshiro128State(C.Structure):
+    _fields_ = [("s", C.c_uint32 * 4)]
 
 
 class TpStats(C.Structure):
@@ -132,6 +136,9 @@ def _load() -> C.CDLL:
         "tp_params_from_config": (C.c_int, [P(TpConfig), P(TpParams)]),
         "tp_acc_merge": (None, [P(TpAcc), P(TpAcc)]),
         "tp_finalize": (C.c_int, [P(TpConfig), P(TpAcc), P(TpFinal)]),
+        "tp_params_from_config_f32": (C.c_int, [P(TpConfig), P(TpParams)]),
+        "tp_acc_merge_f32": (None, [P(TpAcc), P(TpAcc)]),
+        "tp_finalize_f32": (C.c_int, [P(TpConfig), P(TpAcc), P(TpFinal)]),
         "tp_format_res_data": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
         "tp_format_report": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
         "tp_dump_results": (C.c_int, [P(TpConfig), P(TpFinal), C.c_double, C.c_char_p]),
@@ -188,8 +195,11 @@ def _text(fn, *args) -> str:
 
 def params_from_config(cfg: Configuration, sort: bool = True, precision: int = PREC_F64,
                        faster_evgen: bool = False) -> TpParams:
+    """Run constants; precision PREC_F32 computes them in f32 like the
+    reference's f32 feature (tp_params_from_config_f32)."""
     p = TpParams()
-    rc = lib.tp_params_from_config(C.byref(cfg.raw), C.byref(p))
+    fn = lib.tp_params_from_config_f32 if precision == PREC_F32 else lib.tp_params_from_config
+    rc = fn(C.byref(cfg.raw), C.byref(p))
     if rc != TP_OK:
         raise TpError(f"tp_params_from_config: {rc}")
     p.sort_photons = 1 if sort else 0
@@ -199,7 +209,7 @@ def params_from_config(cfg: Configuration, sort: bool = True, precision: int = P
 
 
 def rng_state(kind: int, seed: int | None = None):
-    st = RanfState() if kind == RNG_RANF else XoshiroState()
+    st = {RNG_RANF: RanfState, RNG_XOSHIRO: XoshiroState, RNG_XOSHIRO128: Xoshiro128State}[kind]()
     rc = lib.tp_rng_init(kind, C.byref(st)) if seed is None else lib.tp_rng_seed(kind, seed, C.byref(st))
     if rc != TP_OK:
         raise TpError(f"rng init: {rc}")
@@ -216,19 +226,21 @@ def rng_jump(kind: int, st) -> None:
         raise TpError("tp_rng_jump")
 
 
-def merge(accs) -> TpAcc:
-    """Left fold in batch order (resacc.rs:133-139)."""
+def merge(accs, f32: bool = False) -> TpAcc:
+    """Left fold in batch order (resacc.rs:133-139), in f32 for f32 runs."""
+    fn = lib.tp_acc_merge_f32 if f32 else lib.tp_acc_merge
     out = TpAcc.from_buffer_copy(accs[0])
     for a in accs[1:]:
         if not isinstance(a, TpAcc):  # element of an ACC_DTYPE array
             a = TpAcc.from_buffer_copy(a)
-        lib.tp_acc_merge(C.byref(out), C.byref(a))
+        fn(C.byref(out), C.byref(a))
     return out
 
 
-def finalize(cfg: Configuration, acc: TpAcc) -> TpFinal:
+def finalize(cfg: Configuration, acc: TpAcc, f32: bool = False) -> TpFinal:
     f = TpFinal()
-    if lib.tp_finalize(C.byref(cfg.raw), C.byref(acc), C.byref(f)) != TP_OK:
+    fn = lib.tp_finalize_f32 if f32 else lib.tp_finalize
+    if fn(C.byref(cfg.raw), C.byref(acc), C.byref(f)) != TP_OK:
         raise TpError("tp_finalize")
     return f
 
@@ -452,6 +464,8 @@ def run_simulation(cfg: Configuration, rng: str = "ranf", seeding: str = "skipah
                    faster_evgen: bool = False) -> RunResult:
     """scheduling::run_simulation (scheduling/mod.rs:31-59) on the GPU."""
     kind = RNG_KINDS[rng]
+    if kind == RNG_XOSHIRO and precision == PREC_F32:
+        kind = RNG_XOSHIRO128  # the f32 feature's generator (standard.rs:8-11)
     params = params_from_config(cfg, sort=sort, precision=precision, faster_evgen=faster_evgen)
     st = rng_state(kind)
     own = engine is None
@@ -461,4 +475,4 @@ def run_simulation(cfg: Configuration, rng: str = "ranf", seeding: str = "skipah
     finally:
         if own:
             eng.close()
-    return RunResult(per, merged, finalize(cfg, merged), stats)
+    return RunResult(per, merged, finalize(cfg, merged, f32=precision == PREC_F32), stats)

@@ -47,8 +47,9 @@ typedef enum {
 } tp_status;
 
 /* Random stream (src/random/mod.rs:13-16): RANF is the default, xoshiro the
- * `standard-random` feature (xoshiro256+ in f64 runs). */
-typedef enum { TP_RNG_RANF = 0, TP_RNG_XOSHIRO = 1 } tp_rng_kind;
+ * `standard-random` feature: xoshiro256+ in f64 runs, xoshiro128+ in all-f32
+ * runs (standard.rs:8-11 picks the generator from the float type). */
+typedef enum { TP_RNG_RANF = 0, TP_RNG_XOSHIRO = 1, TP_RNG_XOSHIRO128 = 2 } tp_rng_kind;
 
 /* How consecutive batches get their streams (multi_threading.rs:58-70):
  * skip-ahead = reproducible default (batch b starts 120 000*b draws in),
@@ -60,7 +61,13 @@ typedef enum { TP_SEED_SKIPAHEAD = 0, TP_SEED_JUMP = 1 } tp_seeding;
  *   TP_PREC_F64   -- IEEE double, no contraction, reference operation order.
  *   TP_PREC_MIXED -- f64 RAMBO, cuts and weight; f32 spinor products and
  *                    helicity amplitudes; Kahan-compensated f64 batch sums.
- *   TP_PREC_F32   -- everything in float (tp_adjoint only). */
+ *   TP_PREC_F32   -- the reference's `f32` feature (numeric.rs:6-14):
+ *                    everything in float in the reference's order, batch
+ *                    sums in event order (RANF or xoshiro128+ streams, run
+ *                    constants from tp_params_from_config_f32); the tp_acc
+ *                    fields then hold f32 values, to be folded with
+ *                    tp_acc_merge_f32 and finalised with tp_finalize_f32.
+ *                    tp_adjoint accepts it too (f32 arithmetic). */
 typedef enum { TP_PREC_F64 = 0, TP_PREC_MIXED = 1, TP_PREC_F32 = 2 } tp_precision;
 
 /* Everything the per-event path reads, all computed on the host exactly as
@@ -99,6 +106,11 @@ typedef struct {
 typedef struct {
     uint64_t s[4];
 } tp_xoshiro_state;
+
+/* xoshiro128+: rand_xoshiro's [u32; 4] (the all-f32 mode's generator). */
+typedef struct {
+    uint32_t s[4];
+} tp_xoshiro128_state;
 
 /* Optional timing of a call, measured with HIP events on the context stream. */
 typedef struct {

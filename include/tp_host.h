@@ -54,6 +54,14 @@ void tp_acc_merge(tp_acc *into, const tp_acc *other);
 /* ResultsAccumulator::finalize (resacc.rs:142-223). */
 int tp_finalize(const tp_config *cfg, const tp_acc *merged, tp_final *out);
 
+/* The same three with the reference's `f32` feature (numeric.rs:6-14): the
+ * configuration values rounded to f32 and every constant, sum and final
+ * quantity computed in f32 arithmetic (results stored as doubles holding f32
+ * values); tp_params_from_config_f32 sets precision = TP_PREC_F32. */
+int tp_params_from_config_f32(const tp_config *cfg, tp_params *params);
+void tp_acc_merge_f32(tp_acc *into, const tp_acc *other);
+int tp_finalize_f32(const tp_config *cfg, const tp_acc *merged, tp_final *out);
+
 /* Text of res.data (output.rs:258-338) into buf; returns bytes needed. */
 int tp_format_res_data(const tp_config *cfg, const tp_final *res, char *buf, int buf_len);
 

@@ -1,0 +1,100 @@
+"""The all-f32 mode on the GPU (the reference's `f32` feature): RANF values
+as (n as f32) * 1e-9 and the xoshiro128+ stream, the per-event path in f32
+with the reference's operations, batch sums in event order, f32 fold and
+finalize, against the f32 oracle batch by batch and against the reference's
+shipped f32 outputs at N = 1e7."""
+import math
+
+import numpy as np
+import pytest
+
+import trois_photons as tp
+from conftest import golden_path
+from numdiff import printed_match
+from resdata import golden_numbers, printed_values
+
+pytestmark = pytest.mark.gpu
+
+KIND = {"ranf": tp.RNG_RANF, "xoshiro": tp.RNG_XOSHIRO128}
+
+
+@pytest.fixture(scope="module")
+def engine():
+    print("creating engine on cuda:0", flush=True)
+    with tp.Engine(0) as e:
+        yield e
+
+
+@pytest.fixture(scope="module")
+def cfg(valeurs):
+    return tp.Configuration.load(valeurs)
+
+
+def close_f32(a, b, n_events):
+    """Selected counts exact; sums to the f32 rounding that one-ulp
+    differences of sinf/cosf/logf (ocml against glibc) leave."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    assert a[0] == b[0], (a[0], b[0], n_events)
+    rel = np.abs(a[1:] - b[1:]) / np.maximum(np.abs(b[1:]), 1e-300)
+    worst = float(rel.max()) if len(rel) else 0.0
+    # I_MX (index 5 of the sums) cancels: compare it to the size of its terms
+    rel[4] = abs(a[5] - b[5]) / max(abs(b[10]) ** 0.5 * max(b[0], 1) ** 0.5, 1e-300)
+    assert rel.max() <= 2e-5, (n_events, rel)
+    return worst
+
+
+@pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
+@pytest.mark.parametrize("jump", [False, True])
+@pytest.mark.parametrize("n_events", [1, 9999, 35017])
+def test_f32_per_batch_matches_oracle(engine, cfg, oracle32, valeurs, rng, jump, n_events):
+    print(f"f32 run rng={rng} jump={jump} N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, precision=tp.PREC_F32)
+    st = tp.rng_state(KIND[rng])
+    per, merged, _ = engine.run(params, KIND[rng], tp.SEED_JUMP if jump else tp.SEED_SKIPAHEAD, st, n_events)
+    ocfg = oracle32.config(valeurs, n_events)
+    om, oper = oracle32.run(ocfg, oracle32.opts(rng, faster_threading=jump, threads=4), per_batch=True)
+    worst = 0.0
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        worst = max(worst, close_f32(a, oper[b].as_array(), n_events))
+        assert all(float(np.float32(x)) == x for x in a[1:]), "f32 values"
+    close_f32(merged.as_array(), om.as_array(), n_events)
+    print(f"  worst relative difference of a batch sum: {worst:.2e}")
+
+
+def test_f32_simulate_batch_advances_the_stream(engine, cfg, oracle32, valeurs):
+    params = tp.params_from_config(cfg, precision=tp.PREC_F32)
+    st = tp.rng_state(tp.RNG_XOSHIRO128)
+    g = oracle32.rng("xoshiro")
+    ocfg = oracle32.config(valeurs)
+    opts = oracle32.opts("xoshiro")
+    for n in (10000, 7, 1, 23456):
+        acc = engine.simulate_batch(params, tp.RNG_XOSHIRO128, st, n)
+        close_f32(acc.as_array(), oracle32.simulate_batch(ocfg, opts, g, n).as_array(), n)
+        assert list(st.s) == list(g.xs32)
+
+
+@pytest.mark.parametrize("name,rng", [("res.data-features_f32", "ranf"),
+                                      ("res.data-features_standard-random,f32", "xoshiro")])
+def test_f32_full_run_reproduces_golden(engine, cfg, name, rng):
+    """N = 1e7: every printed field of the reference's f32 res.data (CI
+    compares these files with numdiff -a 1.1e-8, ci.yml:180-183)."""
+    print(f"golden {name}", flush=True)
+    res = tp.run_simulation(cfg, rng=rng, precision=tp.PREC_F32, engine=engine)
+    golden = golden_numbers(open(golden_path(name)).read())
+    vals = printed_values(cfg, res.final, f32=True)
+    assert len(vals) == len(golden)
+    bad = [(i, v, g) for i, (v, g) in enumerate(zip(vals, golden))
+           if not (math.isnan(v) and g == "NaN") and not (abs(v - float(g)) <= 1.1e-8 or printed_match(v, g))]
+    assert not bad, bad
+
+
+def test_f32_rejects_mismatched_streams(engine, cfg):
+    p32 = tp.params_from_config(cfg.with_events(10), precision=tp.PREC_F32)
+    with pytest.raises(tp.TpError, match="xoshiro128"):
+        engine.run(p32, tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), 10)
+    p64 = tp.params_from_config(cfg.with_events(10))
+    with pytest.raises(tp.TpError, match="xoshiro128"):
+        engine.run(p64, tp.RNG_XOSHIRO128, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO128), 10)

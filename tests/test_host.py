@@ -130,3 +130,39 @@ def test_cli_config_errors(tmp_path, valeurs):
     assert r.stdout.startswith("ITOT           : 0\n")  # echoed before the check, like config.rs:108
     r = subprocess.run([exe, "--bogus"], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+# ---- all-f32 mode host side (the reference's f32 feature) ---------------------
+
+def test_f32_params_match_oracle_constants(oracle32, valeurs):
+    """tp_params_from_config_f32: the couplings and sigma_contribs computed in
+    f32 arithmetic, bit for bit the f32 oracle's."""
+    import ctypes
+    cfg = tp.Configuration.load(valeurs)
+    p = tp.params_from_config(cfg, precision=tp.PREC_F32)
+    assert p.precision == tp.PREC_F32
+
+    class K(ctypes.Structure):
+        _fields_ = [(n, ctypes.c_float) for n in ("ev_weight", "g_a", "g_bp", "g_bm", "fact_com", "norm_weight",
+                                                   "propagator", "delta_z0")] + [("sc", ctypes.c_float * 5)]
+    k = K()
+    oracle32.lib.or_consts_f32(ctypes.byref(oracle32.config(valeurs)), ctypes.byref(k))
+    assert (p.g_a, p.g_beta_p, p.g_beta_m) == (k.g_a, k.g_bp, k.g_bm)
+    assert list(p.sigma_contribs) == list(k.sc)
+    assert p.e_total == float(np.float32(cfg.e_total))
+
+
+def test_f32_merge_and_finalize_match_oracle(oracle32, valeurs):
+    """f32 fold of per-batch accumulators and f32 finalize against the f32
+    oracle's own, bit for bit, on a 2e5-event run."""
+    ocfg = oracle32.config(valeurs, 200_000)
+    merged, per = oracle32.run(ocfg, oracle32.opts("ranf", threads=4), per_batch=True)
+    cfg = tp.Configuration.load(valeurs).with_events(200_000)
+    m = tp.merge([to_tp_acc(a) for a in per], f32=True)
+    assert m.as_array().tolist() == merged.as_array().tolist()
+    fin = tp.finalize(cfg, m, f32=True)
+    ofin = oracle32.finalize(ocfg, merged)
+    for f in ("sigma", "prec", "variance", "beta_min", "ss_p", "inc_ss_p", "ss_m", "inc_ss_m"):
+        assert getattr(fin, f) == getattr(ofin, f), f
+    assert np.array_equal(np.array(fin.spm2), np.array(ofin.spm2))
+    assert np.array_equal(np.array(fin.vars), np.array(ofin.vars))

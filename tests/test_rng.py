@@ -93,3 +93,26 @@ def test_seeded(oracle):
     g = oracle.rng("ranf", 1234567)
     st = tp.rng_state(tp.RNG_RANF, 1234567)
     assert ranf_equal(st, g)
+
+
+# ---- xoshiro128+ (the all-f32 mode's stream) ----------------------------------
+
+def test_xoshiro128_seed_skip_jump_match_oracle(oracle32):
+    """seed_from_u64 (SplitMix64 bytes), GF(2) skip-ahead = stepping, and the
+    published 2^64 jump, against the f32 oracle's generator."""
+    for seed in (None, 0, 42):
+        g = oracle32.rng("xoshiro", seed)
+        st = tp.rng_state(tp.RNG_XOSHIRO128, seed)
+        assert list(st.s) == list(g.xs32)
+    for n in (0, 1, 12, 127, 128, 120000, 12 * 10_000 * 37 + 11):
+        g = oracle32.rng("xoshiro")
+        st = tp.rng_state(tp.RNG_XOSHIRO128)
+        oracle32.skip(g, n)
+        tp.rng_skip(tp.RNG_XOSHIRO128, st, n)
+        assert list(st.s) == list(g.xs32), n
+    g = oracle32.rng("xoshiro")
+    st = tp.rng_state(tp.RNG_XOSHIRO128)
+    for _ in range(3):
+        oracle32.jump(g)
+        tp.rng_jump(tp.RNG_XOSHIRO128, st)
+        assert list(st.s) == list(g.xs32)
