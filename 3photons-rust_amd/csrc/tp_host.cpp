@@ -25,9 +25,14 @@ const double SQRT_2 = 1.41421356237309504880;
 
 // ---- Rust-compatible number formatting -----------------------------------
 
-// Display for f64 (`{}`): shortest digits that round-trip, never an
-// exponent.  The reference's shipped outputs print negative zero as "0".
-std::string rust_display(double x)
+// Display (`{}`): shortest digits that round-trip (to f64, or to f32 for
+// f32 values), never an exponent.  The reference's shipped outputs print
+// negative zero as "0".
+template <class T> bool round_trips(const char *buf, T x);
+template <> bool round_trips<double>(const char *buf, double x) { return std::strtod(buf, nullptr) == x; }
+template <> bool round_trips<float>(const char *buf, float x) { return std::strtof(buf, nullptr) == x; }
+
+template <class T> std::string rust_display_t(T x)
 {
     if (std::isnan(x))
         return "NaN";
@@ -37,9 +42,10 @@ std::string rust_display(double x)
         return "0";
     char buf[64];
     int p;
-    for (p = 1; p <= 17; ++p) {
-        std::snprintf(buf, sizeof buf, "%.*e", p - 1, x);
-        if (std::strtod(buf, nullptr) == x)
+    const int maxp = sizeof(T) == 4 ? 9 : 17;
+    for (p = 1; p <= maxp; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*e", p - 1, (double)x);
+        if (round_trips<T>(buf, x))
             break;
     }
     // buf = [-]d[.ddd]e[+-]XX
@@ -109,20 +115,21 @@ std::string pad_right(const std::string &s, size_t w)
     return s.size() >= w ? s : s + std::string(w - s.size(), ' ');
 }
 
-// write_engineering (output.rs:226-269), %g-like with SIG_DIGITS = 14.
-std::string write_engineering(double x, int sig_digits)
+// write_engineering (output.rs:226-269), %g-like with SIG_DIGITS significant
+// digits, the logarithm taken in the value's own type.
+template <class T> std::string write_engineering_t(T x, int sig_digits)
 {
     int precision = sig_digits - 1;
     if (x == 0)
         return "0";
-    double log_x = std::log10(std::fabs(x));
-    if (log_x >= -3. && log_x <= static_cast<double>(sig_digits)) {
+    T log_x = std::log10(std::fabs(x));
+    if (log_x >= (T)-3. && log_x <= static_cast<T>(sig_digits)) {
         long p = static_cast<long>(precision) - static_cast<long>(std::trunc(log_x));
-        if (log_x < 0.)
+        if (log_x < (T)0.)
             p += 1;
         if (p < 0)
             p = 0;
-        std::string s = rust_fixed(x, static_cast<int>(p));
+        std::string s = rust_fixed((double)x, static_cast<int>(p));
         if (s.find('.') != std::string::npos) {
             while (!s.empty() && s.back() == '0')
                 s.pop_back();
@@ -131,10 +138,11 @@ std::string write_engineering(double x, int sig_digits)
         }
         return s;
     }
-    return rust_exp(x, precision);
+    return rust_exp((double)x, precision);
 }
 
-const int SIG_DIGITS = 15 - 1; // f64::DIGITS - 1 (output.rs:217-222)
+// f64::DIGITS - 1 or f32::DIGITS - 1 (output.rs:217-222)
+template <class T> constexpr int sig_digits() { return sizeof(T) == 4 ? 6 - 1 : 15 - 1; }
 
 struct Out {
     std::string s;
@@ -311,29 +319,33 @@ extern "C" int tp_config_load(const char *path, tp_config *cfg, char *err, int e
     return TP_OK;
 }
 
-extern "C" int tp_config_format(const tp_config *c, char *buf, int len)
+template <class T> int config_format(const tp_config *c, char *buf, int len)
 {
     Out o;
     o.line("ITOT           : " + std::to_string(c->num_events));
-    o.line("ETOT           : " + rust_display(c->e_total));
-    o.line("oCutpar.ACUT   : " + rust_display(c->cut_beam));
-    o.line("oCutpar.BCUT   : " + rust_display(c->cut_pp));
-    o.line("oCutpar.EMIN   : " + rust_display(c->e_min));
-    o.line("oCutpar.SINCUT : " + rust_display(c->cut_plane));
-    o.line("ALPHA          : " + rust_display(c->alpha));
-    o.line("ALPHAZ         : " + rust_display(c->alpha_z));
-    o.line("CONVERS        : " + rust_display(c->gev2_to_picobarn));
-    o.line("oParam.MZ0     : " + rust_display(c->m_z0));
-    o.line("oParam.GZ0     : " + rust_display(c->g_z0));
-    o.line("SIN2W          : " + rust_display(c->sin2_weinberg));
-    o.line("BREPEM         : " + rust_display(c->branching_ep_em));
-    o.line("BETAPLUS       : " + rust_display(c->beta_plus));
-    o.line("BETAMOINS      : " + rust_display(c->beta_minus));
+    o.line("ETOT           : " + rust_display_t<T>((T)c->e_total));
+    o.line("oCutpar.ACUT   : " + rust_display_t<T>((T)c->cut_beam));
+    o.line("oCutpar.BCUT   : " + rust_display_t<T>((T)c->cut_pp));
+    o.line("oCutpar.EMIN   : " + rust_display_t<T>((T)c->e_min));
+    o.line("oCutpar.SINCUT : " + rust_display_t<T>((T)c->cut_plane));
+    o.line("ALPHA          : " + rust_display_t<T>((T)c->alpha));
+    o.line("ALPHAZ         : " + rust_display_t<T>((T)c->alpha_z));
+    o.line("CONVERS        : " + rust_display_t<T>((T)c->gev2_to_picobarn));
+    o.line("oParam.MZ0     : " + rust_display_t<T>((T)c->m_z0));
+    o.line("oParam.GZ0     : " + rust_display_t<T>((T)c->g_z0));
+    o.line("SIN2W          : " + rust_display_t<T>((T)c->sin2_weinberg));
+    o.line("BREPEM         : " + rust_display_t<T>((T)c->branching_ep_em));
+    o.line("BETAPLUS       : " + rust_display_t<T>((T)c->beta_plus));
+    o.line("BETAMOINS      : " + rust_display_t<T>((T)c->beta_minus));
     o.line("NBIN           : " + std::to_string(c->num_bins));
     o.line(std::string("oParam.IMPR    : ") + (c->impr ? "true" : "false"));
     o.line(std::string("PLOT           : ") + (c->plot ? "true" : "false"));
     return copy_out(o.s, buf, len);
 }
+
+extern "C" int tp_config_format(const tp_config *c, char *buf, int len) { return config_format<double>(c, buf, len); }
+
+extern "C" int tp_config_format_f32(const tp_config *c, char *buf, int len) { return config_format<float>(c, buf, len); }
 
 namespace {
 
@@ -565,152 +577,190 @@ extern "C" int tp_finalize_f32(const tp_config *cfg, const tp_acc *acc, tp_final
     return TP_OK;
 }
 
-extern "C" int tp_format_res_data(const tp_config *cfg, const tp_final *res, char *buf, int len)
+namespace {
+
+// powi of a constant, which the compiler folds with the host pow() in f64
+// and rounds to the value's type.
+template <class T> T cpow(T x, int n) { return (T)std::pow((double)x, (double)n); }
+
+// Text of res.data (output.rs:258-338) in the run's float type.
+template <class T> std::string res_data_text(const tp_config *cfg, const tp_final *res)
 {
     Out o;
-    auto F = [](double x) { return write_engineering(x, SIG_DIGITS); };
+    auto F = [](T x) { return write_engineering_t<T>(x, sig_digits<T>()); };
+    auto C = [](double x) { return (T)x; };
     o.kv("Nombre d'evenements", std::to_string(cfg->num_events));
     o.kv("... apres coupure", std::to_string(res->selected_events));
-    o.kv("energie dans le CdM      (GeV)", F(cfg->e_total));
-    o.kv("coupure / cos(photon,faisceau)", F(cfg->cut_beam));
-    o.kv("coupure / cos(photon,photon)", F(cfg->cut_pp));
-    o.kv("coupure / sin(normale,faisceau)", F(cfg->cut_plane));
-    o.kv("coupure sur l'energie    (GeV)", F(cfg->e_min));
-    o.kv("1/(constante de structure fine)", F(1. / cfg->alpha));
-    o.kv("1/(structure fine au pic)", F(1. / cfg->alpha_z));
-    o.kv("facteur de conversion GeV-2/pb", F(cfg->gev2_to_picobarn));
-    o.kv("Masse du Z0              (GeV)", F(cfg->m_z0));
-    o.kv("Largeur du Z0            (GeV)", F(cfg->g_z0));
-    o.kv("Sinus^2 Theta Weinberg", F(cfg->sin2_weinberg));
-    o.kv("Taux de branchement Z--->e+e-", F(cfg->branching_ep_em));
-    o.kv("Beta plus", F(cfg->beta_plus));
-    o.kv("Beta moins", F(cfg->beta_minus));
+    o.kv("energie dans le CdM      (GeV)", F(C(cfg->e_total)));
+    o.kv("coupure / cos(photon,faisceau)", F(C(cfg->cut_beam)));
+    o.kv("coupure / cos(photon,photon)", F(C(cfg->cut_pp)));
+    o.kv("coupure / sin(normale,faisceau)", F(C(cfg->cut_plane)));
+    o.kv("coupure sur l'energie    (GeV)", F(C(cfg->e_min)));
+    o.kv("1/(constante de structure fine)", F((T)1. / C(cfg->alpha)));
+    o.kv("1/(structure fine au pic)", F((T)1. / C(cfg->alpha_z)));
+    o.kv("facteur de conversion GeV-2/pb", F(C(cfg->gev2_to_picobarn)));
+    o.kv("Masse du Z0              (GeV)", F(C(cfg->m_z0)));
+    o.kv("Largeur du Z0            (GeV)", F(C(cfg->g_z0)));
+    o.kv("Sinus^2 Theta Weinberg", F(C(cfg->sin2_weinberg)));
+    o.kv("Taux de branchement Z--->e+e-", F(C(cfg->branching_ep_em)));
+    o.kv("Beta plus", F(C(cfg->beta_plus)));
+    o.kv("Beta moins", F(C(cfg->beta_minus)));
     o.line(" ---------------------------------------------");
-    o.kv("Section Efficace          (pb)", F(res->sigma));
-    o.kv("Ecart-Type                (pb)", F(res->sigma * res->prec));
-    o.kv("Precision Relative", F(res->prec));
+    o.kv("Section Efficace          (pb)", F(C(res->sigma)));
+    o.kv("Ecart-Type                (pb)", F(C(res->sigma) * C(res->prec)));
+    o.kv("Precision Relative", F(C(res->prec)));
     o.line(" ---------------------------------------------");
-    o.kv("Beta minimum", F(res->beta_min));
-    o.kv("Stat. Significance  B+(pb-1/2)", F(res->ss_p));
-    o.kv("Incert. Stat. Sign. B+(pb-1/2)", F(res->ss_p * res->inc_ss_p));
-    o.kv("Stat. Significance  B-(pb-1/2)", F(res->ss_m));
-    o.kv("Incert. Stat. Sign. B-(pb-1/2)", F(res->ss_m * res->inc_ss_m));
+    o.kv("Beta minimum", F(C(res->beta_min)));
+    o.kv("Stat. Significance  B+(pb-1/2)", F(C(res->ss_p)));
+    o.kv("Incert. Stat. Sign. B+(pb-1/2)", F(C(res->ss_p) * C(res->inc_ss_p)));
+    o.kv("Stat. Significance  B-(pb-1/2)", F(C(res->ss_m)));
+    o.kv("Incert. Stat. Sign. B-(pb-1/2)", F(C(res->ss_m) * C(res->inc_ss_m)));
     o.s += "\n";
-    const int decs = 7;      // clamp_max(SIG_DIGITS - 1, 7)
-    const size_t width = 15; // decimals + 8
+    const int decs = std::min(sig_digits<T>() - 1, 7); // clamp_max(SIG_DIGITS - 1, 7)
+    const size_t width = static_cast<size_t>(decs) + 8;
     for (int sp = 0; sp < 2; ++sp) {
         for (int k = 0; k < 5; ++k) {
-            double v = res->spm2[sp][k];
+            const T v = C(res->spm2[sp][k]), r = C(res->vars[sp][k]);
             o.s += pad_left(std::to_string(sp + 1), 3) + pad_left(std::to_string(k + 1), 3) +
-                   pad_left(rust_exp(v, decs), width) +
-                   pad_left(rust_exp(std::fabs(v) * res->vars[sp][k], decs), width) +
-                   pad_left(rust_exp(res->vars[sp][k], decs), width) + "\n";
+                   pad_left(rust_exp((double)v, decs), width) + pad_left(rust_exp((double)(std::fabs(v) * r), decs), width) +
+                   pad_left(rust_exp((double)r, decs), width) + "\n";
         }
         o.s += "\n";
     }
     for (int k = 0; k < 5; ++k) {
-        double tmp1 = (0. + res->spm2[0][k]) + res->spm2[1][k];
-        double a = res->spm2[0][k] * res->vars[0][k], b = res->spm2[1][k] * res->vars[1][k];
-        double tmp2 = std::sqrt(0. + (a * a + b * b));
-        o.s += "   " + pad_left(std::to_string(k + 1), 3) + pad_left(rust_exp(tmp1 / 4., decs), width) +
-               pad_left(rust_exp(tmp2 / 4., decs), width) +
-               pad_left(rust_exp(tmp2 / std::fabs(tmp1), decs), width) + "\n";
+        const T s0 = C(res->spm2[0][k]), s1 = C(res->spm2[1][k]);
+        T tmp1 = ((T)0. + s0) + s1;
+        T a = s0 * C(res->vars[0][k]), b = s1 * C(res->vars[1][k]);
+        T tmp2 = std::sqrt((T)0. + (a * a + b * b));
+        o.s += "   " + pad_left(std::to_string(k + 1), 3) + pad_left(rust_exp((double)(tmp1 / (T)4.), decs), width) +
+               pad_left(rust_exp((double)(tmp2 / (T)4.), decs), width) +
+               pad_left(rust_exp((double)(tmp2 / std::fabs(tmp1)), decs), width) + "\n";
     }
-    return copy_out(o.s, buf, len);
+    return o.s;
 }
 
-extern "C" int tp_format_report(const tp_config *cfg, const tp_final *res, char *buf, int len)
+// FinalResults::eric + fawzi (resfin.rs:66-194) in the run's float type.
+template <class T> std::string report_text(const tp_config *cfg, const tp_final *res)
 {
     Out o;
-    const double(&spm2)[2][5] = res->spm2;
-    const double(&vars)[2][5] = res->vars;
+    const T PI_T = (T)PI;
+    auto C = [](double x) { return (T)x; };
+    T spm2[2][5], vars[2][5];
+    for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 5; ++c) {
+            spm2[s][c] = C(res->spm2[s][c]);
+            vars[s][c] = C(res->vars[s][c]);
+        }
+    const T m_z0 = C(cfg->m_z0), g_z0 = C(cfg->g_z0), conv = C(cfg->gev2_to_picobarn);
+    const T br = C(cfg->branching_ep_em), e = C(cfg->e_total);
+    auto D = [](T x) { return (double)x; };
     // FinalResults::eric (resfin.rs:66-97).
-    double mu_th = cfg->branching_ep_em * cfg->gev2_to_picobarn /
-                   (8. * 9. * 5. * std::pow(PI, 2.) * cfg->m_z0 * cfg->g_z0);
-    double sigma0[2], alpha0[2], beta0[2], lambda0[2], mu0[2];
+    T mu_th = br * conv / ((T)8. * (T)9. * (T)5. * cpow<T>(PI_T, 2) * m_z0 * g_z0);
+    T sigma0[2], alpha0[2], beta0[2], lambda0[2], mu0[2];
     for (int s = 0; s < 2; ++s) {
-        sigma0[s] = spm2[s][0] / 2.;
-        alpha0[s] = spm2[s][4] / 2.;
-        beta0[s] = -spm2[s][3] / 2.;
-        lambda0[s] = (spm2[s][2] - spm2[s][1]) / 2.;
-        mu0[s] = (spm2[s][2] + spm2[s][1]) / 2.;
+        sigma0[s] = spm2[s][0] / (T)2.;
+        alpha0[s] = spm2[s][4] / (T)2.;
+        beta0[s] = -spm2[s][3] / (T)2.;
+        lambda0[s] = (spm2[s][2] - spm2[s][1]) / (T)2.;
+        mu0[s] = (spm2[s][2] + spm2[s][1]) / (T)2.;
     }
-    double mu_num = ((((0. + spm2[0][1]) + spm2[1][1]) + spm2[0][2]) + spm2[1][2]) / 4.;
+    T mu_num = (((((T)0. + spm2[0][1]) + spm2[1][1]) + spm2[0][2]) + spm2[1][2]) / (T)4.;
     o.s += "\n";
     o.line("       :        -          +");
-    o.line("sigma0  : " + rust_fixed(sigma0[0], 6) + " | " + rust_fixed(sigma0[1], 6));
-    o.line("alpha0  : " + rust_exp(alpha0[0], 5) + " | " + rust_exp(alpha0[1], 4));
-    o.line("beta0   : " + rust_display(beta0[0]) + " | " + rust_display(beta0[1]));
-    o.line("lambda0 : " + rust_fixed(lambda0[0], 4) + " | " + rust_fixed(lambda0[1], 4));
-    o.line("mu0     : " + rust_fixed(mu0[0], 4) + " | " + rust_fixed(mu0[1], 5));
-    o.line("mu/lamb : " + rust_fixed(mu0[0] / lambda0[0], 5) + " | " + rust_fixed(mu0[1] / lambda0[1], 5));
-    o.line("mu (num): " + rust_fixed(mu_num, 4));
-    o.line("rapport : " + rust_fixed(mu_num / mu_th, 6));
-    o.line("mu (th) : " + rust_fixed(mu_th, 4));
+    o.line("sigma0  : " + rust_fixed(D(sigma0[0]), 6) + " | " + rust_fixed(D(sigma0[1]), 6));
+    o.line("alpha0  : " + rust_exp(D(alpha0[0]), 5) + " | " + rust_exp(D(alpha0[1]), 4));
+    o.line("beta0   : " + rust_display_t<T>(beta0[0]) + " | " + rust_display_t<T>(beta0[1]));
+    o.line("lambda0 : " + rust_fixed(D(lambda0[0]), 4) + " | " + rust_fixed(D(lambda0[1]), 4));
+    o.line("mu0     : " + rust_fixed(D(mu0[0]), 4) + " | " + rust_fixed(D(mu0[1]), 5));
+    o.line("mu/lamb : " + rust_fixed(D(mu0[0] / lambda0[0]), 5) + " | " + rust_fixed(D(mu0[1] / lambda0[1]), 5));
+    o.line("mu (num): " + rust_fixed(D(mu_num), 4));
+    o.line("rapport : " + rust_fixed(D(mu_num / mu_th), 6));
+    o.line("mu (th) : " + rust_fixed(D(mu_th), 4));
     // FinalResults::fawzi (resfin.rs:101-194).
-    double e = cfg->e_total;
-    double mre = cfg->m_z0 / e;
-    double gre = cfg->g_z0 * cfg->m_z0 / (e * e);
-    double x = 1. - mre * mre;
-    double den = x * x + gre * gre;
-    double sdz_re = x / den, sdz_im = -gre / den;
-    double del = (1. - cfg->cut_pp) / 2.;
-    double eps = 2. * cfg->e_min / e;
-    double bra = cfg->m_z0 / (3. * 6. * std::pow(PI, 3.) * 16. * 120.);
-    double r = e / cfg->m_z0;
-    double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
-    double sig = 12. * PI / (cfg->m_z0 * cfg->m_z0) * cfg->branching_ep_em * cfg->g_z0 * bra / (e * e) * r8 *
-                 (sdz_re * sdz_re + sdz_im * sdz_im) * cfg->gev2_to_picobarn;
-    double eps_2 = eps * eps;
-    double eps_4 = eps_2 * eps_2;
-    double del_2 = del * del;
-    double del_3 = del * del_2;
-    double f1 = 1. - 15. * eps_4 - 9. / 7. * (1. - 70. * eps_4) * del_2 + 6. / 7. * (1. + 70. * eps_4) * del_3;
-    double g1 = 1. - 30. * eps_4 - 9. / 7. * (1. - 70. * eps_4) * del - 90. * eps_4 * del_2 -
-                1. / 7. * (1. - 420. * eps_4) * del_3;
-    double g2 = 1. - 25. * eps_4 - 6. / 7. * (1. - 70. * eps_4) * del - 3. / 7. * (1. + 210. * eps_4) * del_2 -
-                8. / 21. * (1. - 52.5 * eps_4) * del_3;
-    double g3 = 1. - 195. / 11. * eps_4 - 18. / 77. * (1. - 7. * eps_4) * del -
-                9. / 11. * (9. / 7. - 70. * eps_4) * del_2 - 8. / 11. * (1. - 105. / 11. * eps_4) * del_3;
-    double c = cfg->cut_plane;
-    double c3 = c * (c * c);
-    double ff = f1 * (1. - c3);
-    double gg = g1 - 27. / 16. * g2 * c + 11. / 16. * g3 * c3;
-    double sig_p = sig * (ff + 2. * gg);
-    double sig_m = sig_p + 2. * sig * gg;
-    double mc_p = ((0. + spm2[0][1]) + spm2[1][1]) / 4.;
-    double mc_m = ((0. + spm2[0][2]) + spm2[1][2]) / 4.;
+    T mre = m_z0 / e;
+    T gre = g_z0 * m_z0 / (e * e);
+    T x = (T)1. - mre * mre;
+    T den = x * x + gre * gre;
+    T sdz_re = x / den, sdz_im = -gre / den;
+    T del = ((T)1. - C(cfg->cut_pp)) / (T)2.;
+    T eps = (T)2. * C(cfg->e_min) / e;
+    T bra = m_z0 / ((T)3. * (T)6. * cpow<T>(PI_T, 3) * (T)16. * (T)120.);
+    T r = e / m_z0;
+    T r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    T sig = (T)12. * PI_T / (m_z0 * m_z0) * br * g_z0 * bra / (e * e) * r8 * (sdz_re * sdz_re + sdz_im * sdz_im) * conv;
+    T eps_2 = eps * eps;
+    T eps_4 = eps_2 * eps_2;
+    T del_2 = del * del;
+    T del_3 = del * del_2;
+    T f1 = (T)1. - (T)15. * eps_4 - (T)9. / (T)7. * ((T)1. - (T)70. * eps_4) * del_2 +
+           (T)6. / (T)7. * ((T)1. + (T)70. * eps_4) * del_3;
+    T g1 = (T)1. - (T)30. * eps_4 - (T)9. / (T)7. * ((T)1. - (T)70. * eps_4) * del - (T)90. * eps_4 * del_2 -
+           (T)1. / (T)7. * ((T)1. - (T)420. * eps_4) * del_3;
+    T g2 = (T)1. - (T)25. * eps_4 - (T)6. / (T)7. * ((T)1. - (T)70. * eps_4) * del -
+           (T)3. / (T)7. * ((T)1. + (T)210. * eps_4) * del_2 - (T)8. / (T)21. * ((T)1. - (T)52.5 * eps_4) * del_3;
+    T g3 = (T)1. - (T)195. / (T)11. * eps_4 - (T)18. / (T)77. * ((T)1. - (T)7. * eps_4) * del -
+           (T)9. / (T)11. * ((T)9. / (T)7. - (T)70. * eps_4) * del_2 -
+           (T)8. / (T)11. * ((T)1. - (T)105. / (T)11. * eps_4) * del_3;
+    T c = C(cfg->cut_plane);
+    T c3 = c * (c * c);
+    T ff = f1 * ((T)1. - c3);
+    T gg = g1 - (T)27. / (T)16. * g2 * c + (T)11. / (T)16. * g3 * c3;
+    T sig_p = sig * (ff + (T)2. * gg);
+    T sig_m = sig_p + (T)2. * sig * gg;
+    T mc_p = (((T)0. + spm2[0][1]) + spm2[1][1]) / (T)4.;
+    T mc_m = (((T)0. + spm2[0][2]) + spm2[1][2]) / (T)4.;
     auto incr = [&](int col) {
-        double a = spm2[0][col] * vars[0][col], b = spm2[1][col] * vars[1][col];
-        return std::sqrt(0. + (a * a + b * b)) / std::fabs((0. + spm2[0][col]) + spm2[1][col]);
+        T a = spm2[0][col] * vars[0][col], b = spm2[1][col] * vars[1][col];
+        return std::sqrt((T)0. + (a * a + b * b)) / std::fabs(((T)0. + spm2[0][col]) + spm2[1][col]);
     };
-    double incr_p = incr(1), incr_m = incr(2);
+    T incr_p = incr(1), incr_m = incr(2);
     o.s += "\n";
     o.line("s (pb) :   Sig_cut_Th    Sig_Th      Rapport");
     o.line("       :   Sig_Num");
     o.line("       :   Ecart_relatif  Incertitude");
     o.s += "\n";
-    o.line("s+(pb) : " + rust_fixed(sig_p, 5) + " | " + rust_fixed(sig * 3., 5) + " | " +
-           rust_fixed(sig_p / (3. * sig), 6));
-    o.line("       : " + rust_fixed(mc_p, 5));
-    o.line("       : " + rust_fixed(mc_p / sig_p - 1., 6) + " | " + rust_fixed(incr_p, 8) + " | " +
-           rust_fixed((mc_p / sig_p - 1.) / incr_p, 2));
+    o.line("s+(pb) : " + rust_fixed(D(sig_p), 5) + " | " + rust_fixed(D(sig * (T)3.), 5) + " | " +
+           rust_fixed(D(sig_p / ((T)3. * sig)), 6));
+    o.line("       : " + rust_fixed(D(mc_p), 5));
+    o.line("       : " + rust_fixed(D(mc_p / sig_p - (T)1.), 6) + " | " + rust_fixed(D(incr_p), 8) + " | " +
+           rust_fixed(D((mc_p / sig_p - (T)1.) / incr_p), 2));
     o.s += "\n";
-    o.line("s-(pb) : " + rust_fixed(sig_m, 5) + " | " + rust_fixed(sig * 5., 4) + " | " +
-           rust_fixed(sig_m / (5. * sig), 6));
-    o.line("       : " + rust_fixed(mc_m, 5));
-    o.line("       : " + rust_fixed(mc_m / sig_m - 1., 6) + " | " + rust_fixed(incr_m, 9) + " | " +
-           rust_fixed((mc_m / sig_m - 1.) / incr_m, 2));
+    o.line("s-(pb) : " + rust_fixed(D(sig_m), 5) + " | " + rust_fixed(D(sig * (T)5.), 4) + " | " +
+           rust_fixed(D(sig_m / ((T)5. * sig)), 6));
+    o.line("       : " + rust_fixed(D(mc_m), 5));
+    o.line("       : " + rust_fixed(D(mc_m / sig_m - (T)1.), 6) + " | " + rust_fixed(D(incr_m), 9) + " | " +
+           rust_fixed(D((mc_m / sig_m - (T)1.) / incr_m), 2));
     o.s += "\n";
-    return copy_out(o.s, buf, len);
+    return o.s;
 }
 
-extern "C" int tp_dump_results(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
+} // namespace
+
+extern "C" int tp_format_res_data(const tp_config *cfg, const tp_final *res, char *buf, int len)
+{
+    return copy_out(res_data_text<double>(cfg, res), buf, len);
+}
+
+extern "C" int tp_format_res_data_f32(const tp_config *cfg, const tp_final *res, char *buf, int len)
+{
+    return copy_out(res_data_text<float>(cfg, res), buf, len);
+}
+
+extern "C" int tp_format_report(const tp_config *cfg, const tp_final *res, char *buf, int len)
+{
+    return copy_out(report_text<double>(cfg, res), buf, len);
+}
+
+extern "C" int tp_format_report_f32(const tp_config *cfg, const tp_final *res, char *buf, int len)
+{
+    return copy_out(report_text<float>(cfg, res), buf, len);
+}
+
+template <class T> int dump_results(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
 {
     if (!cfg || !res)
         return TP_ERR_ARG;
-    std::string report(static_cast<size_t>(tp_format_report(cfg, res, nullptr, 0)), '\0');
-    tp_format_report(cfg, res, &report[0], static_cast<int>(report.size()));
+    const std::string report = report_text<T>(cfg, res);
     std::fputs(report.c_str(), stdout);
     std::fflush(stdout);
     // "[day]-[month repr:short]-[year repr:last_two] [hour]:[minute]:[second]" (output.rs:232-236), UTC.
@@ -728,16 +778,15 @@ extern "C" int tp_dump_results(const tp_config *cfg, const tp_final *res, double
         o.s += std::string(" ") + stamp + "\n";
         o.line(" ---------------------------------------------");
         o.kv("Temps ecoule", "???");
-        o.kv("Temps ecoule utilisateur", write_engineering(elapsed_s, SIG_DIGITS));
+        o.kv("Temps ecoule utilisateur", write_engineering_t<T>((T)elapsed_s, sig_digits<T>()));
         o.kv("Temps ecoule systeme", "???");
         o.kv("Temps ecoule par evenement",
-             write_engineering(elapsed_s / static_cast<double>(cfg->num_events), SIG_DIGITS));
+             write_engineering_t<T>((T)elapsed_s / static_cast<T>(cfg->num_events), sig_digits<T>()));
         std::fputs(o.s.c_str(), f);
         std::fclose(f);
     }
     {
-        std::string text(static_cast<size_t>(tp_format_res_data(cfg, res, nullptr, 0)), '\0');
-        tp_format_res_data(cfg, res, &text[0], static_cast<int>(text.size()));
+        const std::string text = res_data_text<T>(cfg, res);
         FILE *f = std::fopen((base + "res.data").c_str(), "w");
         if (!f)
             return TP_ERR_IO;
@@ -748,17 +797,28 @@ extern "C" int tp_dump_results(const tp_config *cfg, const tp_final *res, double
         FILE *f = std::fopen((base + "pil.mc").c_str(), "a");
         if (!f)
             return TP_ERR_IO;
-        auto colsum = [&](int col) { return (0. + res->spm2[0][col]) + res->spm2[1][col]; };
-        double res1 = colsum(0);
-        double res2 = colsum(1) * (cfg->beta_plus * cfg->beta_plus);
-        double res3 = colsum(2) * (cfg->beta_minus * cfg->beta_minus);
-        double res4 = colsum(3) * cfg->beta_plus;
-        std::string l = rust_display(cfg->e_total) + " " + rust_display(res1 / 4.) + " " +
-                        rust_display(res2 / 4.) + " " + rust_display(res3 / 4.) + " " +
-                        rust_display(res4 / 4.) + " " + rust_display((res1 + res2 + res3 + res4) / 4.) +
-                        " " + rust_display(res->sigma);
+        auto colsum = [&](int col) { return ((T)0. + (T)res->spm2[0][col]) + (T)res->spm2[1][col]; };
+        const T bp = (T)cfg->beta_plus, bm = (T)cfg->beta_minus;
+        T res1 = colsum(0);
+        T res2 = colsum(1) * (bp * bp);
+        T res3 = colsum(2) * (bm * bm);
+        T res4 = colsum(3) * bp;
+        auto Dp = [](T x) { return rust_display_t<T>(x); };
+        std::string l = Dp((T)cfg->e_total) + " " + Dp(res1 / (T)4.) + " " + Dp(res2 / (T)4.) + " " +
+                        Dp(res3 / (T)4.) + " " + Dp(res4 / (T)4.) + " " + Dp((res1 + res2 + res3 + res4) / (T)4.) +
+                        " " + Dp((T)res->sigma);
         std::fprintf(f, "%s\n%s\n", stamp, l.c_str());
         std::fclose(f);
     }
     return TP_OK;
+}
+
+extern "C" int tp_dump_results(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
+{
+    return dump_results<double>(cfg, res, elapsed_s, dir);
+}
+
+extern "C" int tp_dump_results_f32(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
+{
+    return dump_results<float>(cfg, res, elapsed_s, dir);
 }

@@ -8,6 +8,8 @@
 //   --no-photon-sorting   keep RAMBO's photon order
 //   --faster-evgen        the reference's faster-evgen generation (RANF, f64)
 //   --mixed               f32 spinor/amplitude arithmetic (tp_precision MIXED)
+//   --f32                 the reference's f32 feature: everything in f32,
+//                         xoshiro128+ with --standard-random, f32 outputs
 //   --config PATH         configuration file (default: ./valeurs)
 //   --device N            HIP device ordinal (default 0)
 // Outputs are the reference's: stdout report, res.data, res.times, pil.mc in
@@ -39,13 +41,15 @@ int main(int argc, char **argv)
             faster_evgen = 1;
         else if (a == "--mixed")
             precision = TP_PREC_MIXED;
+        else if (a == "--f32")
+            precision = TP_PREC_F32;
         else if (a == "--config" && i + 1 < argc)
             config = argv[++i];
         else if (a == "--device" && i + 1 < argc)
             device = std::atoi(argv[++i]);
         else {
             std::fprintf(stderr, "usage: %s [--standard-random] [--faster-threading] [--no-photon-sorting] "
-                                 "[--faster-evgen] [--mixed] [--config PATH] [--device N]\n",
+                                 "[--faster-evgen] [--mixed] [--f32] [--config PATH] [--device N]\n",
                          argv[0]);
             return 2;
         }
@@ -55,8 +59,9 @@ int main(int argc, char **argv)
     char err[512];
     int rc = tp_config_load(config.c_str(), &cfg, err, sizeof err);
     if (cfg.parsed) { // the reference echoes the configuration before its ensure! checks
-        std::vector<char> text(static_cast<size_t>(tp_config_format(&cfg, nullptr, 0)));
-        tp_config_format(&cfg, text.data(), static_cast<int>(text.size()));
+        auto format = precision == TP_PREC_F32 ? tp_config_format_f32 : tp_config_format;
+        std::vector<char> text(static_cast<size_t>(format(&cfg, nullptr, 0)));
+        format(&cfg, text.data(), static_cast<int>(text.size()));
         std::fputs(text.data(), stdout);
     }
     if (rc != TP_OK) {
@@ -66,7 +71,13 @@ int main(int argc, char **argv)
     // ### SIMULATION INITIALIZATION ### (main.rs:85-97)
     auto t0 = std::chrono::steady_clock::now();
     tp_params params;
-    tp_params_from_config(&cfg, &params);
+    if (precision == TP_PREC_F32) {
+        tp_params_from_config_f32(&cfg, &params);
+        if (rng == TP_RNG_XOSHIRO)
+            rng = TP_RNG_XOSHIRO128; // the f32 feature's generator (standard.rs:8-11)
+    } else {
+        tp_params_from_config(&cfg, &params);
+    }
     params.sort_photons = sort;
     params.precision = precision;
     params.faster_evgen = faster_evgen;
@@ -81,6 +92,7 @@ int main(int argc, char **argv)
     union {
         tp_ranf_state ranf;
         tp_xoshiro_state xo;
+        tp_xoshiro128_state xo128;
     } state;
     tp_rng_init(rng, &state);
     const uint64_t nb = (cfg.num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
@@ -93,10 +105,10 @@ int main(int argc, char **argv)
         return 1;
     }
     tp_final res;
-    tp_finalize(&cfg, &merged, &res);
+    (precision == TP_PREC_F32 ? tp_finalize_f32 : tp_finalize)(&cfg, &merged, &res);
     // ### RESULTS DISPLAY AND STORAGE ### (main.rs:136-141)
     double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    rc = tp_dump_results(&cfg, &res, elapsed, nullptr);
+    rc = (precision == TP_PREC_F32 ? tp_dump_results_f32 : tp_dump_results)(&cfg, &res, elapsed, nullptr);
     tp_destroy(ctx);
     if (rc != TP_OK) {
         std::fprintf(stderr, "Error: failed to output the results\n");

@@ -137,6 +137,10 @@ def _load() -> C.CDLL:
         "tp_acc_merge": (None, [P(TpAcc), P(TpAcc)]),
         "tp_finalize": (C.c_int, [P(TpConfig), P(TpAcc), P(TpFinal)]),
         "tp_params_from_config_f32": (C.c_int, [P(TpConfig), P(TpParams)]),
+        "tp_config_format_f32": (C.c_int, [P(TpConfig), C.c_char_p, C.c_int]),
+        "tp_format_res_data_f32": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
+        "tp_format_report_f32": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
+        "tp_dump_results_f32": (C.c_int, [P(TpConfig), P(TpFinal), C.c_double, C.c_char_p]),
         "tp_acc_merge_f32": (None, [P(TpAcc), P(TpAcc)]),
         "tp_finalize_f32": (C.c_int, [P(TpConfig), P(TpAcc), P(TpFinal)]),
         "tp_format_res_data": (C.c_int, [P(TpConfig), P(TpFinal), C.c_char_p, C.c_int]),
@@ -182,8 +186,8 @@ class Configuration:
         raw.num_events = n
         return Configuration(raw)
 
-    def display(self) -> str:
-        return _text(lib.tp_config_format, C.byref(self.raw))
+    def display(self, f32: bool = False) -> str:
+        return _text(lib.tp_config_format_f32 if f32 else lib.tp_config_format, C.byref(self.raw))
 
 
 def _text(fn, *args) -> str:
@@ -245,12 +249,12 @@ def finalize(cfg: Configuration, acc: TpAcc, f32: bool = False) -> TpFinal:
     return f
 
 
-def format_res_data(cfg: Configuration, res: TpFinal) -> str:
-    return _text(lib.tp_format_res_data, C.byref(cfg.raw), C.byref(res))
+def format_res_data(cfg: Configuration, res: TpFinal, f32: bool = False) -> str:
+    return _text(lib.tp_format_res_data_f32 if f32 else lib.tp_format_res_data, C.byref(cfg.raw), C.byref(res))
 
 
-def format_report(cfg: Configuration, res: TpFinal) -> str:
-    return _text(lib.tp_format_report, C.byref(cfg.raw), C.byref(res))
+def format_report(cfg: Configuration, res: TpFinal, f32: bool = False) -> str:
+    return _text(lib.tp_format_report_f32 if f32 else lib.tp_format_report, C.byref(cfg.raw), C.byref(res))
 
 
 # ---- engine -------------------------------------------------------------------

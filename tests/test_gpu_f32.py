@@ -98,3 +98,20 @@ def test_f32_rejects_mismatched_streams(engine, cfg):
     p64 = tp.params_from_config(cfg.with_events(10))
     with pytest.raises(tp.TpError, match="xoshiro128"):
         engine.run(p64, tp.RNG_XOSHIRO128, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO128), 10)
+
+
+def test_f32_cli_end_to_end(tmp_path, valeurs):
+    """trois_photons --f32: the reference's f32 res.data and stdout."""
+    import os
+    import shutil
+    import subprocess
+    from conftest import PKG
+    from numdiff import compare_texts
+    exe = os.path.join(PKG, "build", "trois_photons")
+    shutil.copy(valeurs, tmp_path / "valeurs")
+    print("cli --f32 run", flush=True)
+    r = subprocess.run([exe, "--f32"], cwd=tmp_path, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    golden = open(golden_path("res.data-features_f32")).read()
+    assert compare_texts(golden, (tmp_path / "res.data").read_text(), digits_slack=1.5) == []
+    assert compare_texts(open(golden_path("stdout.log-features_f32")).read(), r.stdout, digits_slack=1.5) == []
