@@ -4,8 +4,8 @@ Workload (one step): e+e- -> 3 photons at the shipped valeurs, xoshiro256+
 stream (seed_from_u64(12345)), 1e9 events = 1e5 batches of 10 000, batch
 accumulators and their in-order fold left in HBM (tp_run_device).  The f64
 path gives `value`; the mixed-precision path (f32 spinor/amplitude algebra)
-is timed on the same stream and reported beside it with its per-quantity
-relative error against the f64 run.  Also reported: the adjoint (2^24
+is timed on 1e10 events of the same stream (1e6 batches) and reported beside
+it with its per-quantity relative error against an f64 run of those events.  Also reported: the adjoint (2^24
 points) and the learned proposal (training, then the same number of
 weighted events, sigma and GPU-seconds against plain RAMBO at equal
 error).  Prints ONE JSON line.
@@ -176,6 +176,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--events", type=int, default=1_000_000_000)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-events", type=int, default=10_000_000_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -199,8 +200,9 @@ def main() -> int:
 
     cfg = tp.Configuration.load(VALEURS).with_events(args.events)
     nb = (args.events + tp.BATCH_SIZE - 1) // tp.BATCH_SIZE
+    nb_max = (max(args.events, 0 if args.no_mixed else args.mixed_events) + tp.BATCH_SIZE - 1) // tp.BATCH_SIZE
     dev = torch.device("cuda:0")
-    per_batch = torch.empty(nb * 104, dtype=torch.uint8, device=dev)
+    per_batch = torch.empty(nb_max * 104, dtype=torch.uint8, device=dev)
     merged = torch.empty(104, dtype=torch.uint8, device=dev)
     eng = tp.Engine(0)
 
@@ -228,19 +230,27 @@ def main() -> int:
                      "kernel": "fused_kernel<0,0> (f64, xoshiro)", "kernel_ms": k_ms, "flops_per_event": F_ALG},
     }
     if not args.no_mixed:
-        pm = tp.params_from_config(cfg, precision=tp.PREC_MIXED)
-        v32, dt32, k32, acc32 = measure(tp, torch, eng, pm, args.events, args.steps, args.warmup, per_batch, merged)
-        f64, f32 = tp.finalize(cfg, acc64), tp.finalize(cfg, acc32)
+        # mixed precision on 1e10 events, against an f64 run of the same stream
+        nm = args.mixed_events
+        cm = cfg.with_events(nm)
+        pm = tp.params_from_config(cm, precision=tp.PREC_MIXED)
+        v32, dt32, k32, acc32 = measure(tp, torch, eng, pm, nm, args.steps, args.warmup, per_batch, merged)
+        if nm == args.events:
+            acc64m = acc64
+        else:
+            _, _, _, acc64m = measure(tp, torch, eng, tp.params_from_config(cm), nm, 1, 0, per_batch, merged)
+        f64, f32 = tp.finalize(cm, acc64m), tp.finalize(cm, acc32)
         rel = {q: abs(getattr(f32, q) - getattr(f64, q)) / abs(getattr(f64, q))
                for q in ("sigma", "prec", "variance", "beta_min", "ss_p", "ss_m")}
         names = ["A", "B+", "B-", "R_MX", "I_MX"]
         for k in range(5):
-            rel[f"spm2[{names[k]}]"] = abs(acc32.spm2[k] - acc64.spm2[k]) / abs(acc64.spm2[k])
-            rel[f"vars[{names[k]}]"] = abs(acc32.vars[k] - acc64.vars[k]) / abs(acc64.vars[k])
-        a32 = F_ALG * args.events / (k32 * 1e-3) / 1e12
+            rel[f"spm2[{names[k]}]"] = abs(acc32.spm2[k] - acc64m.spm2[k]) / abs(acc64m.spm2[k])
+            rel[f"vars[{names[k]}]"] = abs(acc32.vars[k] - acc64m.vars[k]) / abs(acc64m.vars[k])
+        a32 = F_ALG * nm / (k32 * 1e-3) / 1e12
         out["fp32_mixed"] = {
             "value": v32, "unit": "events/s", "ms_per_step": dt32 * 1e3 / args.steps, "dtype": "f32+f64",
-            "selected_events": int(acc32.selected_events), "selected_events_f64": int(acc64.selected_events),
+            "events": nm, "batches": (nm + tp.BATCH_SIZE - 1) // tp.BATCH_SIZE,
+            "selected_events": int(acc32.selected_events), "selected_events_f64": int(acc64m.selected_events),
             "rel_err_vs_f64": rel,
             "roofline": {"bound": "valu", "achieved": a32, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": a32 / FP32_PEAK_TFLOPS, "kernel": "fused_kernel<0,1> (mixed, xoshiro)",
