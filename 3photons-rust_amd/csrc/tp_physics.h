@@ -283,25 +283,12 @@ template <class T, bool FAST = true>
 __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> nt01, T g_a, T g_bp, T g_bm,
                                               T m[5])
 {
-    // (A, B+) for h = MMP, MPM, MPP, PMM, PMP, PPM.
-    C<T> a[6], b[6];
-    a[0] = a_pmm<T, FAST>(S, nt01, 4, 2, 3);
-    b[0] = bp_pmm<T, FAST>(S, ns01, 4, 2, 3);
-    a[1] = a_pmm<T, FAST>(S, nt01, 3, 2, 4);
-    b[1] = bp_pmm<T, FAST>(S, ns01, 3, 2, 4);
-    a[2] = a_ppm<T, FAST>(S, ns01, 3, 4, 2);
-    b[2] = bp_ppm<T, FAST>(S, nt01, 3, 4, 2);
-    a[3] = a_pmm<T, FAST>(S, nt01, 2, 3, 4);
-    b[3] = bp_pmm<T, FAST>(S, ns01, 2, 3, 4);
-    a[4] = a_ppm<T, FAST>(S, ns01, 2, 4, 3);
-    b[4] = bp_ppm<T, FAST>(S, nt01, 2, 4, 3);
-    a[5] = a_ppm<T, FAST>(S, ns01, 2, 3, 4);
-    b[5] = bp_ppm<T, FAST>(S, nt01, 2, 3, 4);
+    // (A, B+) for h = MMP, MPM, MPP, PMM, PMP, PPM, each folded into the sums
+    // as soon as it is formed (same order of additions as a loop over h).
     T sA = 0, sP = 0, sR = 0, sI = 0;
-#pragma unroll
-    for (int h = 0; h < 6; ++h) {
-        C<T> A = mk<T>(a[h].re * g_a, a[h].im * g_a);
-        C<T> Pp = mk<T>(b[h].re * g_bp, b[h].im * g_bp);
+    auto fold = [&](int h, C<T> a, C<T> b) {
+        C<T> A = mk<T>(a.re * g_a, a.im * g_a);
+        C<T> Pp = mk<T>(b.re * g_bp, b.im * g_bp);
         C<T> X = xmul<FAST>(mk<T>((T)2. * A.re, (T)2. * A.im), mk<T>(Pp.re, -Pp.im));
         T nA = A.re * A.re + A.im * A.im;
         T nP = Pp.re * Pp.re + Pp.im * Pp.im;
@@ -316,7 +303,13 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> 
             sR = sR + X.re;
             sI = sI + X.im;
         }
-    }
+    };
+    fold(0, a_pmm<T, FAST>(S, nt01, 4, 2, 3), bp_pmm<T, FAST>(S, ns01, 4, 2, 3));
+    fold(1, a_pmm<T, FAST>(S, nt01, 3, 2, 4), bp_pmm<T, FAST>(S, ns01, 3, 2, 4));
+    fold(2, a_ppm<T, FAST>(S, ns01, 3, 4, 2), bp_ppm<T, FAST>(S, nt01, 3, 4, 2));
+    fold(3, a_pmm<T, FAST>(S, nt01, 2, 3, 4), bp_pmm<T, FAST>(S, ns01, 2, 3, 4));
+    fold(4, a_ppm<T, FAST>(S, ns01, 2, 4, 3), bp_ppm<T, FAST>(S, nt01, 2, 4, 3));
+    fold(5, a_ppm<T, FAST>(S, ns01, 2, 3, 4), bp_ppm<T, FAST>(S, nt01, 2, 3, 4));
     C<T> bmm = bm_mmm<T, FAST>(S, nt01);
     C<T> bpp = bm_ppp<T, FAST>(S, ns01);
     C<T> Bm = mk<T>(bmm.re * g_bm, bmm.im * g_bm);
