@@ -21,6 +21,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -72,8 +73,23 @@ def traffic_from_profiles(name: str):
     return None
 
 
-def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged):
-    kind = tp.RNG_XOSHIRO
+def finite_or_none(x):
+    """JSON has no NaN: a non-finite figure is printed as null."""
+    return x if math.isfinite(x) else None
+
+
+def json_safe(o):
+    if isinstance(o, float):
+        return finite_or_none(o)
+    if isinstance(o, dict):
+        return {k: json_safe(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [json_safe(v) for v in o]
+    return o
+
+
+def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged, kind=None):
+    kind = tp.RNG_XOSHIRO if kind is None else kind
 
     def step():
         st = tp.rng_state(kind)
@@ -180,6 +196,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--adjoint-points", type=int, default=1 << 24)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
@@ -256,13 +273,35 @@ def main() -> int:
                          "frac": a32 / FP32_PEAK_TFLOPS, "kernel": "fused_kernel<0,1> (mixed, xoshiro)",
                          "kernel_ms": k32, "flops_per_event": F_ALG},
         }
+    if not args.no_variants:
+        # the reference's default stream (RANF, expanded into HBM per chunk of
+        # batches) and its all-f32 build (xoshiro128+, sums in event order)
+        vr, dtr, kr, accr = measure(tp, torch, eng, p64, args.events, 2, 1, per_batch, merged, tp.RNG_RANF)
+        p32 = tp.params_from_config(cfg, precision=tp.PREC_F32)
+        v3, dt3, k3, acc3 = measure(tp, torch, eng, p32, args.events, 2, 1, per_batch, merged, tp.RNG_XOSHIRO128)
+        f64r, f32r = tp.finalize(cfg, acc64), tp.finalize(cfg, acc3, f32=True)  # different streams
+        # spinor.rs:40 takes sqrt(E + p_z): in f32 a photon along -z can round
+        # to E < |p_z|, and that event's NaN reaches its batch's sums (about
+        # 4.5 per 1e8 events, as in the reference's own f32 build)
+        nb = (args.events + 9999) // 10000
+        sums = per_batch[:nb * 104].view(torch.float64).reshape(nb, 13)[:, 1:]
+        nan_batches = int((~torch.isfinite(sums).all(dim=1)).sum().item())
+        out["variants"] = {
+            "ranf_f64": {"value": vr, "unit": "events/s", "ms_per_step": dtr * 1e3 / 2,
+                         "kernels_ms": kr, "selected_events": int(accr.selected_events)},
+            "all_f32_xoshiro128": {"value": v3, "unit": "events/s", "ms_per_step": dt3 * 1e3 / 2, "kernels_ms": k3,
+                                   "selected_events": int(acc3.selected_events),
+                                   "sigma_pb": finite_or_none(f32r.sigma), "prec": finite_or_none(f32r.prec),
+                                   "nonfinite_batches": nan_batches,
+                                   "sigma_pb_f64_xoshiro256": f64r.sigma},
+        }
     if not args.no_adjoint:
         out["adjoint"] = measure_adjoint(tp, torch, eng, p64, args.adjoint_points)
     if not args.no_flow:
         out["flow"] = measure_flow(tp, eng, cfg, value, acc64, args.events, args.flow_steps, args.flow_batch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
-    print(json.dumps(out))
+    print(json.dumps(json_safe(out), allow_nan=False))
     eng.close()
     return 0
 

@@ -5,6 +5,7 @@ xoshiro256+/SplitMix64 seeding, jump variants), RAMBO (both generators), the
 cuts, the amplitudes, the accumulation and finalize."""
 import math
 
+import numpy as np
 import pytest
 
 from conftest import golden_path
@@ -68,3 +69,38 @@ def test_sequential_equals_multithreaded(oracle, valeurs):
         a, _ = oracle.run(cfg, oracle.opts(rng, threads=1))
         b, _ = oracle.run(cfg, oracle.opts(rng, threads=4))
         assert a.as_array().tolist() == b.as_array().tolist()
+
+
+# Two batches of the f32 xoshiro128+ stream (seed_from_u64(0), batch b starts
+# 120000 b draws in) that hold one event each whose sums come out NaN.
+F32_NAN_BATCHES = ((1158, 6517), (1880, 6323))
+
+
+@pytest.mark.parametrize("batch,event", F32_NAN_BATCHES)
+def test_oracle_f32_spinor_nan(oracle, oracle32, valeurs, batch, event):
+    """spinor.rs:40 takes sqrt(E + p_z) of every particle.  In f32, a photon
+    emitted along -z can round to E < |p_z|; the square root is NaN, the event
+    still passes the cuts (the beam runs along x) and its NaN reaches every
+    sum, so the reference's f32 build gives NaN sigma once N reaches ~1e8
+    (45 batches of 1e5 at N = 1e9).  The same uniforms in f64 give a finite
+    event, so the NaN is the f32 rounding and not a slip in the oracle."""
+    cfg32, cfg64 = oracle32.config(valeurs), oracle.config(valeurs)
+    g = oracle32.rng("xoshiro")
+    oracle32.skip(g, 120000 * batch)
+    u = oracle32.uniforms(g, 120000).reshape(-1, 12)
+    p, keep, _, m, _ = oracle32.events(cfg32, oracle32.opts("xoshiro"), u)
+    bad = np.where(~np.isfinite(m).all(axis=1))[0]
+    assert bad.tolist() == [event]
+    assert keep[event]
+    photons = p[event, 2:]
+    assert np.isfinite(photons).all()
+    e_plus_z = photons[:, 3] + photons[:, 2]          # one f32 rounding, as the reference
+    assert (e_plus_z < 0).sum() == 1
+    with np.errstate(invalid="ignore"):
+        assert np.isnan(np.sqrt(e_plus_z[e_plus_z < 0])).all()
+    p64, keep64, _, m64, _ = oracle.events(cfg64, oracle.opts("xoshiro"), u[event:event + 1].astype(np.float64))
+    assert keep64[0] and np.isfinite(m64).all() and (p64[0, 2:, 3] + p64[0, 2:, 2] >= 0).all()
+    g = oracle32.rng("xoshiro")
+    oracle32.skip(g, 120000 * batch)
+    acc = oracle32.simulate_batch(cfg32, oracle32.opts("xoshiro"), g, 10000).as_array()
+    assert np.isnan(acc[1:]).all() and acc[0] == keep[:10000].sum()
