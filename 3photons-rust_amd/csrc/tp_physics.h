@@ -2,7 +2,10 @@
 // RAMBO (evgen), cuts (evcut), spinor products (spinor), helicity amplitudes
 // and matrix elements (matelems), per-event weight (resacc integrate).
 //
-// Everything lives in registers; there is no per-event memory traffic.  The
+// Everything lives in registers; the only per-event memory reads are the
+// L1-resident log and sin/cos tables of tp_math.h, whose routines return the
+// correctly rounded value but within ~0.002 ulp of a rounding boundary (the
+// reference calls the host libm; the two agree on > 99.8 % of arguments).  The
 // f64 path keeps the reference's operation order exactly (the translation
 // unit is compiled with -ffp-contract=off); it differs from the reference only
 // where the reference computes known zeros or exact negations:
@@ -18,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "tp_math.h"
 
 #include <type_traits>
 
@@ -46,13 +51,30 @@ template <class T> __device__ __forceinline__ C<T> cmul(C<T> a, C<T> b)
 {
     return C<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
+// IEEE division and square root: in f64 through tpm::div_n / tpm::sqrt_n
+// (the same correctly rounded results in the event loop's range, fewer
+// instructions), in f32 the plain operations.
+template <class T> __device__ __forceinline__ T dv(T a, T b)
+{
+    if constexpr (std::is_same<T, double>::value)
+        return tpm::div_n(a, b);
+    else
+        return a / b;
+}
+template <class T> __device__ __forceinline__ T sq(T x)
+{
+    if constexpr (std::is_same<T, double>::value)
+        return tpm::sqrt_n(x);
+    else
+        return sqrt(x);
+}
 // num-complex naive division (spinor.rs divides with it).
 template <class T> __device__ __forceinline__ C<T> cdiv(C<T> a, C<T> b)
 {
     T n = b.re * b.re + b.im * b.im;
     T re = a.re * b.re + a.im * b.im;
     T im = a.im * b.re - a.re * b.im;
-    return C<T>{re / n, im / n};
+    return C<T>{dv(re, n), dv(im, n)};
 }
 template <class T> __device__ __forceinline__ C<T> cadd(C<T> a, C<T> b) { return C<T>{a.re + b.re, a.im + b.im}; }
 template <class T> __device__ __forceinline__ C<T> csq(C<T> z) { return cmul(z, z); }
@@ -70,9 +92,9 @@ __device__ __forceinline__ void raw_photon_f64(double uc, double uphi, double ue
     double phi = two_pi * uphi;
     double em = ue1 * ue2;
     double sphi, cphi;
-    sincos(phi, &sphi, &cphi);
-    double st = sqrt(1. - cos_t * cos_t);
-    double en = -log(em + 2.2250738585072014e-308);
+    tpm::sincos_f64(phi, sphi, cphi);
+    double st = sq(1. - cos_t * cos_t);
+    double en = -tpm::log_pos(em + 2.2250738585072014e-308);
     q[0] = en * (st * sphi);
     q[1] = en * (st * cphi);
     q[2] = en * cos_t;
@@ -88,9 +110,9 @@ __device__ __forceinline__ void conformal_f64(const Params &P, const double q[3]
     for (int k = 0; k < 4; ++k)
         r[k] = (q[0][k] + q[1][k]) + q[2][k];
     double rn2 = r[3] * r[3] - ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
-    double alpha = P.e_total / rn2;
-    double rn = sqrt(rn2);
-    double beta = 1. / (rn + r[3]);
+    double alpha = dv(P.e_total, rn2);
+    double rn = sq(rn2);
+    double beta = dv(1., rn + r[3]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         double rq = q[i][0] * r[0];
@@ -157,7 +179,7 @@ __device__ __forceinline__ bool keep_f64(const Params &P, const double p[3][4])
         double n1 = p[0][2] * p[1][0] - p[0][0] * p[1][2];
         double n2 = p[0][0] * p[1][1] - p[0][1] * p[1][0];
         double num = P.el_x * n0;
-        double nn = sqrt((n0 * n0 + n1 * n1) + n2 * n2);
+        double nn = sq((n0 * n0 + n1 * n1) + n2 * n2);
         ok = ok && !(fabs(num) < P.cut_plane * (P.el_e * nn));
     }
     return ok;
@@ -178,13 +200,13 @@ __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T 
     C<T> fx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        xx[k] = sqrt(p[k][3] + p[k][2]);
+        xx[k] = sq(p[k][3] + p[k][2]);
         // f32::MIN_POSITIVE / f64::MIN_POSITIVE
         const T minpos = sizeof(T) == 8 ? (T)2.2250738585072014e-308 : (T)1.17549435e-38f;
         if (xx[k] > minpos)
-            fx[k] = mk<T>(p[k][0] / xx[k], p[k][1] / xx[k]);
+            fx[k] = mk<T>(dv(p[k][0], xx[k]), dv(p[k][1], xx[k]));
         else
-            fx[k] = mk<T>(sqrt((T)2. * p[k][3]), (T)0.);
+            fx[k] = mk<T>(sq((T)2. * p[k][3]), (T)0.);
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -438,12 +460,12 @@ __device__ __forceinline__ void raw_photons_faster_f64(RanfWalk &w, double q[3][
         }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const double nrm = 1. / sqrt(r2[i]);
+        const double nrm = dv(1., sq(r2[i]));
         const double cx = x[i] * nrm, cy = y[i] * nrm;
         const double cos_t = 2. * r9[i] - 1.;
         const double em = r9[3 + i] * r9[6 + i];
-        const double st = sqrt(1. - cos_t * cos_t);
-        const double en = -log(em + 2.2250738585072014e-308);
+        const double st = sq(1. - cos_t * cos_t);
+        const double en = -tpm::log_pos(em + 2.2250738585072014e-308);
         q[i][0] = en * (st * cx);
         q[i][1] = en * (st * cy);
         q[i][2] = en * cos_t;
@@ -544,10 +566,10 @@ __device__ __forceinline__ void raw_photon_f32(float uc, float uphi, float ue1, 
     float phi = two_pi * uphi;
     float em = ue1 * ue2;
     double sd, cd;
-    sincos((double)phi, &sd, &cd);
+    tpm::sincos_f64((double)phi, sd, cd);
     const float sphi = (float)sd, cphi = (float)cd;
     float st = sqrtf(1.f - cos_t * cos_t);
-    float en = -(float)log((double)(em + 1.17549435e-38f));
+    float en = -(float)tpm::log_pos((double)(em + 1.17549435e-38f));
     q[0] = en * (st * sphi);
     q[1] = en * (st * cphi);
     q[2] = en * cos_t;

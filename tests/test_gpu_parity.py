@@ -58,9 +58,11 @@ def test_events_match_oracle(engine, cfg, oracle, valeurs, rng, sort):
     dp = np.abs(p - ph) / np.spacing(escale)
     print(f"momenta ulp(E): max {dp.max():.1f} p99 {np.percentile(dp, 99):.1f} p99.9 {np.percentile(dp, 99.9):.1f} "
           f"exact {np.mean(dp == 0):.3f}")
-    # Differences start as 1-ulp differences of sin/cos/log (device vs glibc)
-    # and grow only where RAMBO's conformal map cancels (M^2 = R_E^2 - |R|^2).
-    assert np.mean(dp == 0) > 0.8 and np.percentile(dp, 99) <= 4 and dp.max() <= 1024
+    # sin/cos/log are correctly rounded on both sides but for the rare
+    # arguments next to a rounding boundary (about 1 in 700 calls), where the
+    # device and glibc can differ by 1 ulp; RAMBO's conformal map
+    # (M^2 = R_E^2 - |R|^2) can amplify such a difference.
+    assert np.mean(dp == 0) > 0.99 and np.percentile(dp, 99) <= 1 and dp.max() <= 256
     k = okeep
     mk, omk = m[k], om[k]
     rel = np.abs(mk[:, :3] - omk[:, :3]) / np.abs(omk[:, :3])
@@ -68,6 +70,7 @@ def test_events_match_oracle(engine, cfg, oracle, valeurs, rng, sort):
     relx = np.abs(mk[:, 3:] - omk[:, 3:]) / scale_x[:, None]
     print(f"m rel: max {rel.max():.2e} mixed {relx.max():.2e}; exact fraction {np.mean(mk == omk):.3f}")
     assert rel.max() < 1e-11 and relx.max() < 1e-11
+    assert np.mean(mk == omk) > 0.98
     assert np.all(m[~k] == 0)
 
 
