@@ -204,16 +204,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
         for (uint32_t k = 0; k < iters; ++k) {
             bool pass = false;
             double p[3][4];
+            int d[3]; // sort slots of the three photons
             double wt = 1.;
             if (k < len && SOURCE == 3) {
                 RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
-                pass = generate_keep_faster_f64(P, w, p);
+                pass = generate_faster_f64(P, w, p, d);
             } else if (k < len) {
                 double u[12];
                 if (SOURCE == 0) {
 #pragma unroll
                     for (int j = 0; j < 12; ++j)
-                        u[j] = xo_uniform(s);
+                        u[j] = xo_uniform53(s);
                 } else if (SOURCE == 1) {
                     const int32_t *e = src + 12 * (size_t)(k * 64 + lane);
 #pragma unroll
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                         u[j] = a.u[12 * e + j];
                     wt = a.invq[e];
                 }
-                pass = generate_keep<PREC>(P, u, p);
+                pass = generate_f64<SOURCE == 0>(P, u, p, d);
             }
             const uint64_t mask = __ballot(pass);
             if (pass) {
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        queue[4 * i + j][slot] = p[i][j];
+                        queue[4 * d[i] + j][slot] = p[i][j];
                 if (SOURCE == 2)
                     queue[QROWS - 1][slot] = wt;
             }
@@ -343,14 +344,20 @@ __global__ __launch_bounds__(256) void events_kernel(FusedArgs a, uint32_t n, Ev
                 gf2_apply(a.mt + 1024 * k, s);
 #pragma unroll
         for (int j = 0; j < 12; ++j)
-            u[j] = xo_uniform(s);
+            u[j] = xo_uniform53(s); // u 2^53, as the fused kernel draws them
     } else {
 #pragma unroll
         for (int j = 0; j < 12; ++j)
             u[j] = (double)a.draws[12ull * e + j] * 1e-9;
     }
-    double p[3][4], m[5] = {0., 0., 0., 0., 0.};
-    const bool k = generate_keep<PREC>(a.P, u, p);
+    double pu[3][4], p[3][4], m[5] = {0., 0., 0., 0., 0.};
+    int d[3];
+    const bool k = generate_f64<SOURCE == 0>(a.P, u, pu, d);
+    to_slots(pu, d, p);
+    if (SOURCE == 0)
+#pragma unroll
+        for (int j = 0; j < 12; ++j)
+            u[j] = u[j] * 0x1.0p-53;
     if (k)
         matrix_elements<PREC>(a.P, p, m);
     if (!k) {

@@ -85,12 +85,21 @@ template <class T> __device__ __forceinline__ C<T> tof(C<T> s) { return C<T>{-s.
 // ---- RAMBO: evgen.rs:174-207 (generate_raw) and 89-118 (generate) ---------
 
 // Massless momentum of one photon from (cos theta, phi, e1, e2) uniforms.
+// SCALED: the uniforms come as u 2^53, the integers xoshiro256+'s x >> 11
+// holds (xo_uniform53), and the 2^-53 factors fold into the operations
+// exactly: 2u - 1 = fma(u', 2^-52, -1) (2u is exact, so the reference's two
+// roundings are one), 2 pi u = (2 pi 2^-53) u' and u1 u2 = (u1' u2') 2^-106,
+// each rounded once as before.
+template <bool SCALED>
 __device__ __forceinline__ void raw_photon_f64(double uc, double uphi, double ue1, double ue2, double q[4])
 {
+    constexpr double S = SCALED ? 0x1.0p-53 : 1.;
     const double two_pi = 2. * 3.14159265358979323846;
-    double cos_t = 2. * uc - 1.;
-    double phi = two_pi * uphi;
+    double cos_t = __builtin_fma(uc, 2. * S, -1.);
+    double phi = (two_pi * S) * uphi;
     double em = ue1 * ue2;
+    if constexpr (SCALED)
+        em = em * (S * S);
     double sphi, cphi;
     tpm::sincos_f64(phi, sphi, cphi);
     double st = sq(1. - cos_t * cos_t);
@@ -148,23 +157,71 @@ template <class T> __device__ __forceinline__ void sort3(T p[3][4])
         swap4(p[1], p[2]);
 }
 
-// ---- Cuts: evcut.rs:42-96 -------------------------------------------------
-__device__ __forceinline__ bool keep_f64(const Params &P, const double p[3][4])
+// Slot of each photon after the reference's energy sort (evgen.rs:109-118:
+// strict '>' swaps over the slot pairs (0,1), (0,2), (1,2)), from pairwise
+// comparisons of the unsorted energies.  It reproduces the swap sequence
+// exactly, ties and NaN included: step 2 compares photon 2 with the winner a
+// of step 1, step 3 compares the loser b with whatever step 2 left in slot 2.
+// The kernels keep the photons where RAMBO made them and apply the slots where
+// they store the event (no register shuffles).
+__device__ __forceinline__ void sort_slots(const double p[3][4], int sort, int d[3])
 {
+    if (!sort) {
+        d[0] = 0;
+        d[1] = 1;
+        d[2] = 2;
+        return;
+    }
+    const double e0 = p[0][3], e1 = p[1][3], e2 = p[2][3];
+    const bool c01 = e1 > e0, c02 = e2 > e0, c12 = e2 > e1, g01 = e0 > e1;
+    const bool s02 = c01 ? c12 : c02;                        // step 2 swaps
+    const bool s12 = s02 ? (c01 || g01) : (c01 ? c02 : c12); // step 3 swaps
+    const int da = s02 ? (s12 ? 1 : 2) : 0;                  // winner of step 1
+    const int db = s12 ? 2 : 1;                              // loser of step 1
+    d[0] = c01 ? db : da;
+    d[1] = c01 ? da : db;
+    d[2] = s02 ? 0 : (s12 ? 1 : 2);
+}
+
+// p sorted into its slots (for outputs that show the sorted event).
+template <class T> __device__ __forceinline__ void to_slots(const T p[3][4], const int d[3], T ps[3][4])
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const T x0 = p[0][j], x1 = p[1][j], x2 = p[2][j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            ps[k][j] = d[0] == k ? x0 : (d[1] == k ? x1 : x2);
+    }
+}
+
+// ---- Cuts: evcut.rs:42-96 -------------------------------------------------
+// On the unsorted photons with their slots d: every test but the energy and
+// plane cuts treats the photons symmetrically (each dot product is the same
+// IEEE expression for (i,j) and (j,i)), the energy cut takes the slot-2
+// energy as the reference does after sorting, and the plane cut's normal
+// n = p_slot0 x p_slot1 only changes sign (exactly) when its two photons are
+// taken in the other order, which |num| and |n| do not see.  All tests are
+// evaluated (no early exit), which changes nothing: they have no side effects.
+__device__ __forceinline__ bool keep_f64(const Params &P, const double p[3][4], const int d[3])
+{
+    // (selects between loaded values, never between array elements: the
+    // latter can turn into an address select that moves p to scratch)
+    const double E0 = p[0][3], E1 = p[1][3], E2 = p[2][3];
     double emin;
     if (P.sort) {
-        emin = p[2][3];
+        emin = d[0] == 2 ? E0 : (d[1] == 2 ? E1 : E2);
     } else {
-        emin = p[0][3];
-        emin = (emin < p[1][3]) ? emin : p[1][3];
-        emin = (emin < p[2][3]) ? emin : p[2][3];
+        emin = E0;
+        emin = (emin < E1) ? emin : E1;
+        emin = (emin < E2) ? emin : E2;
     }
     bool ok = !(emin < P.e_min);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         double num = p[i][0] * P.el_x;
         double den = p[i][3] * P.el_e;
-        ok = ok && !(fabs(num) > P.cut_beam * den);
+        ok = ok & !(fabs(num) > P.cut_beam * den);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -172,15 +229,23 @@ __device__ __forceinline__ bool keep_f64(const Params &P, const double p[3][4])
         for (int j = i + 1; j < 3; ++j) {
             double num = (p[i][0] * p[j][0] + p[i][1] * p[j][1]) + p[i][2] * p[j][2];
             double den = p[i][3] * p[j][3];
-            ok = ok && !(num > P.cut_pp * den);
+            ok = ok & !(num > P.cut_pp * den);
         }
     if (P.cut_plane > 0.) {
-        double n0 = p[0][1] * p[1][2] - p[0][2] * p[1][1];
-        double n1 = p[0][2] * p[1][0] - p[0][0] * p[1][2];
-        double n2 = p[0][0] * p[1][1] - p[0][1] * p[1][0];
+        // the two photons outside slot 2
+        double a[3], b[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double x0 = p[0][k], x1 = p[1][k], x2 = p[2][k];
+            a[k] = d[0] == 2 ? x1 : x0;
+            b[k] = d[2] == 2 ? x1 : x2;
+        }
+        double n0 = a[1] * b[2] - a[2] * b[1];
+        double n1 = a[2] * b[0] - a[0] * b[2];
+        double n2 = a[0] * b[1] - a[1] * b[0];
         double num = P.el_x * n0;
         double nn = sq((n0 * n0 + n1 * n1) + n2 * n2);
-        ok = ok && !(fabs(num) < P.cut_plane * (P.el_e * nn));
+        ok = ok & !(fabs(num) < P.cut_plane * (P.el_e * nn));
     }
     return ok;
 }
@@ -403,16 +468,18 @@ __device__ __forceinline__ double weight_f64(const Params &P, const double m[5])
 }
 
 // RAMBO event from 12 uniforms (photons sorted) and its cut decision.
-__device__ __forceinline__ bool generate_keep_f64(const Params &P, const double u[12], double p[3][4])
+// RAMBO event from 12 uniforms: photons as RAMBO makes them (unsorted), their
+// sort slots d, and the cut decision.
+template <bool SCALED = false>
+__device__ __forceinline__ bool generate_f64(const Params &P, const double u[12], double p[3][4], int d[3])
 {
     double q[3][4];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        raw_photon_f64(u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3], q[i]);
+        raw_photon_f64<SCALED>(u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3], q[i]);
     conformal_f64(P, q, p);
-    if (P.sort)
-        sort3(p);
-    return keep_f64(P, p);
+    sort_slots(p, P.sort, d);
+    return keep_f64(P, p, d);
 }
 
 // ---- faster-evgen (evgen.rs:143-173, 221-249) --------------------------------
@@ -473,14 +540,13 @@ __device__ __forceinline__ void raw_photons_faster_f64(RanfWalk &w, double q[3][
     }
 }
 
-__device__ __forceinline__ bool generate_keep_faster_f64(const Params &P, RanfWalk &w, double p[3][4])
+__device__ __forceinline__ bool generate_faster_f64(const Params &P, RanfWalk &w, double p[3][4], int d[3])
 {
     double q[3][4];
     raw_photons_faster_f64(w, q);
     conformal_f64(P, q, p);
-    if (P.sort)
-        sort3(p);
-    return keep_f64(P, p);
+    sort_slots(p, P.sort, d);
+    return keep_f64(P, p, d);
 }
 
 // Helicity sums m[5] of an event that passed the cuts.
@@ -517,27 +583,12 @@ __device__ __forceinline__ void matrix_elements_mixed(const Params &P, const dou
         m[c] = (double)mf[c];
 }
 
-template <int PREC>
-__device__ __forceinline__ bool generate_keep(const Params &P, const double u[12], double p[3][4])
-{
-    return generate_keep_f64(P, u, p);
-}
-
 template <int PREC> __device__ __forceinline__ void matrix_elements(const Params &P, const double p[3][4], double m[5])
 {
     if (PREC == 0)
         matrix_elements_f64(P, p, m);
     else
         matrix_elements_mixed(P, p, m);
-}
-
-// Whole f64 event from 12 uniforms: returns false if cut; else m[5].
-__device__ __forceinline__ bool event_f64(const Params &P, const double u[12], double p[3][4], double m[5])
-{
-    if (!generate_keep_f64(P, u, p))
-        return false;
-    matrix_elements_f64(P, p, m);
-    return true;
 }
 
 // ---- all-f32 mode (numeric.rs:6-14, feature f32) ----------------------------

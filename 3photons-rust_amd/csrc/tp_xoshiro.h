@@ -8,19 +8,32 @@
 
 namespace tpd {
 
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+// rotl(x, 45) = rotr(x, 19) and x << 17 on the two 32-bit halves with
+// v_alignbit_b32 (a 64-bit rotate would otherwise take a 64-bit shift plus
+// two 32-bit operations).
+__device__ __forceinline__ uint64_t pack64(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
+__device__ __forceinline__ uint64_t rotl45(uint64_t x)
+{
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return pack64(__builtin_amdgcn_alignbit(hi, lo, 19), __builtin_amdgcn_alignbit(lo, hi, 19));
+}
+__device__ __forceinline__ uint64_t shl17(uint64_t x)
+{
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return pack64(lo << 17, __builtin_amdgcn_alignbit(hi, lo, 15));
+}
 
 // next_u64: output s0 + s3 before the state update.
 __device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
 {
-    uint64_t r = s[0] + s[3];
-    uint64_t t = s[1] << 17;
+    const uint64_t r = s[0] + s[3];
+    const uint64_t t = shl17(s[1]);
     s[2] ^= s[0];
     s[3] ^= s[1];
     s[1] ^= s[2];
     s[0] ^= s[3];
     s[2] ^= t;
-    s[3] = rotl64(s[3], 45);
+    s[3] = rotl45(s[3]);
     return r;
 }
 
@@ -32,6 +45,16 @@ __device__ __forceinline__ double xo_uniform(uint64_t s[4])
     const uint64_t x = xo_next(s);
     const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
     return fma((double)hi, 0x1.0p-32, (double)(lo >> 11) * 0x1.0p-53);
+}
+
+// The same uniform times 2^53, i.e. the integer u64 >> 11 as an exact double
+// (hi * 2^21 + (lo >> 11) < 2^53), for code that folds the 2^-53 into its
+// own operations.
+__device__ __forceinline__ double xo_uniform53(uint64_t s[4])
+{
+    const uint64_t x = xo_next(s);
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    return fma((double)hi, 0x1.0p21, (double)(lo >> 11));
 }
 
 // s <- M s.  Read through the constant address space so that the uniform
