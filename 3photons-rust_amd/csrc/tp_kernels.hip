@@ -157,6 +157,20 @@ constexpr int QCAP = 128;
 // queue (12.4 KB per workgroup) caps a CU at 12 resident workgroups anyway,
 // and the third wave hides the transcendental and LDS latencies better than
 // the 2-wave, spill-free allocation (measured 3 % faster at f64).
+// The run constants re-read from the kernel-argument segment (FusedArgs::P
+// sits at its start) through a pointer the compiler cannot see through, at
+// each use site: scalar loads, which cost no VALU slot, instead of ~40 SGPRs
+// held across the whole event loop, which the allocator spills to VGPR lanes
+// and reads back with one VALU v_readlane per dword.
+static_assert(offsetof(FusedArgs, P) == 0, "Params first in FusedArgs");
+typedef const __attribute__((address_space(4))) Params KParams;
+__device__ __forceinline__ const Params &fresh_params()
+{
+    KParams *k = (KParams *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return *(const Params *)k;
+}
+
 template <int SOURCE, int PREC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void fused_kernel(FusedArgs a)
 {
@@ -208,7 +222,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
             double wt = 1.;
             if (k < len && SOURCE == 3) {
                 RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
-                pass = generate_faster_f64(P, w, p, d);
+                pass = generate_faster_f64(fresh_params(), w, p, d);
             } else if (k < len) {
                 double u[12];
                 if (SOURCE == 0) {
@@ -227,7 +241,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                         u[j] = a.u[12 * e + j];
                     wt = a.invq[e];
                 }
-                pass = generate_f64<SOURCE == 0>(P, u, p, d);
+                pass = generate_f64<SOURCE == 0>(fresh_params(), u, p, d);
             }
             const uint64_t mask = __ballot(pass);
             if (pass) {
@@ -253,12 +267,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                 const double wq = SOURCE == 2 ? queue[QROWS - 1][slot] : 1.;
                 __syncthreads();
                 qn -= 64;
-                matrix_elements<PREC>(P, pq, m);
+                matrix_elements<PREC>(fresh_params(), pq, m);
                 if (SOURCE == 2)
 #pragma unroll
                     for (int c = 0; c < 5; ++c)
                         m[c] = m[c] * wq;
-                acc_add(A, P, m);
+                acc_add(A, fresh_params(), m);
             }
         }
         if (qn > 0) { // partial flush
@@ -270,14 +284,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         pq[i][j] = queue[4 * i + j][lane];
-                matrix_elements<PREC>(P, pq, m);
+                matrix_elements<PREC>(fresh_params(), pq, m);
                 if (SOURCE == 2) {
                     const double wq = queue[QROWS - 1][lane];
 #pragma unroll
                     for (int c = 0; c < 5; ++c)
                         m[c] = m[c] * wq;
                 }
-                acc_add(A, P, m);
+                acc_add(A, fresh_params(), m);
             }
         }
         __syncthreads();

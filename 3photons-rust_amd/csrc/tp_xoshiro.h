@@ -53,7 +53,11 @@ __device__ __forceinline__ double xo_uniform(uint64_t s[4])
 __device__ __forceinline__ double xo_uniform53(uint64_t s[4])
 {
     const uint64_t x = xo_next(s);
-    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    uint32_t hi = (uint32_t)(x >> 32);
+    const uint32_t lo = (uint32_t)x;
+    // (an empty asm on hi: without it the compiler matches the two 32-bit
+    // conversions as a 64-bit one and adds a +0.0 term)
+    asm("" : "+v"(hi));
     return fma((double)hi, 0x1.0p21, (double)(lo >> 11));
 }
 
