@@ -179,7 +179,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     __shared__ double red[1][13];
     const uint32_t lane = threadIdx.x;
     const uint64_t lanemask_lt = (1ull << lane) - 1ull;
-    const Params &P = a.P;
     uint64_t st[4] = {0, 0, 0, 0}; // xoshiro state at this lane's chunk start
     bool have = false;
     DevAcc *out = reinterpret_cast<DevAcc *>(a.out);
@@ -567,49 +566,96 @@ void launch_ranf_seed(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *ori
 // workgroup; 12 lanes then run the 12 chains out of LDS, so the chain pays
 // the add latency, not the HBM latency.  The event count is an integer sum
 // and is reduced in any order.
-constexpr int MERGE_TILE = 512;
+constexpr int MERGE_TILE = 256;
 constexpr int MERGE_THREADS = 256;
 
+// In-order left fold of the per-batch accumulators (resacc.rs:133-139): the
+// 12 sums are 12 chains of dependent adds that must run in batch order, so
+// their length (nb adds) is the floor.  Wave 0 (lanes 0..11) runs the chains
+// over one LDS tile while waves 1..3 load the next tile from HBM into the
+// other buffer; the selected counts are integers and are summed by the loaders
+// in any order.
 template <class T>
 __global__ __launch_bounds__(MERGE_THREADS) void merge_kernel(const DevAcc *in, uint64_t nb, DevAcc *out)
 {
-    __shared__ double tile[12][MERGE_TILE + 1]; // +1: the 12 chains hit distinct banks
+    __shared__ double tile[2][12][MERGE_TILE + 1]; // +1: the 12 chains hit distinct banks
     __shared__ unsigned long long cnt;
     const int tid = threadIdx.x;
     if (tid == 0)
         cnt = 0;
     const uint64_t *words = reinterpret_cast<const uint64_t *>(in); // 13 words per accumulator
-    T t = (T)0;
+    const uint64_t ntiles = (nb + MERGE_TILE - 1) / MERGE_TILE;
     unsigned long long my_cnt = 0;
-    for (uint64_t base = 0; base < nb; base += MERGE_TILE) {
-        const uint64_t m = (nb - base) < (uint64_t)MERGE_TILE ? (nb - base) : (uint64_t)MERGE_TILE;
-        for (uint64_t f = tid; f < 13 * m; f += MERGE_THREADS) {
-            const uint64_t e = f / 13, w = f % 13;
-            const uint64_t v = words[13 * base + f];
-            if (w == 0)
-                my_cnt += v;
-            else
-                tile[w - 1][e] = __longlong_as_double((long long)v);
+    // tile k -> buffer k & 1, loaded by threads [first, MERGE_THREADS): each
+    // thread issues all its loads before its first LDS store, so a tile costs
+    // one memory latency rather than one per word.
+    constexpr int PER = (13 * MERGE_TILE + (MERGE_THREADS - 64) - 1) / (MERGE_THREADS - 64);
+    auto load = [&](uint64_t k, int first) {
+        const uint64_t base = k * MERGE_TILE;
+        const uint32_t m = (uint32_t)((nb - base) < (uint64_t)MERGE_TILE ? (nb - base) : (uint64_t)MERGE_TILE);
+        const uint32_t nw = 13 * m, step = (uint32_t)(MERGE_THREADS - first);
+        const uint64_t *src = words + 13 * base;
+        double(*dst)[MERGE_TILE + 1] = tile[k & 1];
+        uint64_t v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const uint32_t f = (uint32_t)(tid - first) + (uint32_t)i * step;
+            v[i] = f < nw ? src[f] : 0ull;
         }
-        __syncthreads();
-        if (tid < 12) {
-            uint64_t e = 0;
-            if (base == 0) {
-                t = (T)tile[tid][0];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const uint32_t f = (uint32_t)(tid - first) + (uint32_t)i * step;
+            if (f < nw) {
+                const uint32_t e = f / 13, w = f - 13 * e;
+                if (w == 0)
+                    my_cnt += v[i];
+                else
+                    dst[w - 1][e] = __longlong_as_double((long long)v[i]);
+            }
+        }
+    };
+    if (ntiles > 0)
+        load(0, 0);
+    __syncthreads();
+    T t = (T)0;
+    for (uint64_t k = 0; k < ntiles; ++k) {
+        if (tid >= 64) {
+            if (k + 1 < ntiles)
+                load(k + 1, 64);
+        } else if (tid < 12) {
+            const uint64_t base = k * MERGE_TILE;
+            const uint32_t m = (uint32_t)((nb - base) < (uint64_t)MERGE_TILE ? (nb - base) : (uint64_t)MERGE_TILE);
+            const double *row = tile[k & 1][tid];
+            uint32_t e = 0;
+            if (k == 0) {
+                t = (T)row[0];
                 e = 1;
             }
-            // 16 independent LDS reads in flight, then 16 dependent adds.
-            for (; e + 16 <= m; e += 16) {
+            // Software-pipelined: the LDS reads of the next 16 values are in
+            // flight while the current 16 go through the add chain.
+            if (e + 16 <= m) {
                 double v[16];
 #pragma unroll
                 for (int u = 0; u < 16; ++u)
-                    v[u] = tile[tid][e + u];
+                    v[u] = row[e + u];
+                for (e += 16; e + 16 <= m; e += 16) {
+                    double w[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        w[u] = row[e + u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        t = t + (T)v[u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        v[u] = w[u];
+                }
 #pragma unroll
                 for (int u = 0; u < 16; ++u)
                     t = t + (T)v[u];
             }
             for (; e < m; ++e)
-                t = t + (T)tile[tid][e];
+                t = t + (T)row[e];
         }
         __syncthreads();
     }
