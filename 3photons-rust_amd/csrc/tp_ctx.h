@@ -52,7 +52,7 @@ struct Ctx {
 };
 
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws
-constexpr int RANF_R = 128;         // RANF rounds per expansion segment
+constexpr int RANF_R = 2048;        // RANF rounds per expansion segment (one jump per segment)
 constexpr uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
 
 inline int fail(Ctx *c, int code, const std::string &msg)

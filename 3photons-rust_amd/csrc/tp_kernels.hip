@@ -430,25 +430,36 @@ __device__ __forceinline__ int32_t norm_mod(int32_t v)
 //   i <= 24:       X[i] - X[i+31]
 //   25 <= i <= 48: X[i] - X[i-24] + X[i+7]
 //   i >= 49:       X[i] - X[i-24] + X[i-48] - X[i-17]        (all mod 1e9)
+// One RANF round in closed form (lane i - 1 holds X[i], i = 1..55):
+//   i <= 24:  X'[i] = X[i] - X[i+31]
+//   25..48:   X'[i] = X[i] - X'[i-24] = X[i] - X[i-24] + X[i+7]
+//   49..55:   X'[i] = X[i] - X[i-24] + X[i-48] - X[i-17]
+// (ranf.rs:86-97 applies the first two lines as in-place updates; the third
+// is the second with X'[i-24] expanded once more).  Every lane evaluates
+// (w[a] - w[b]) + (w[c] - w[d]) mod 1e9 with its own slots, slot RANF_ZERO
+// holding 0 where a case has fewer terms; each value is the unique
+// representative in [0, 1e9), so the sum order does not matter.
+constexpr int RANF_ZERO = 111;
+__device__ __forceinline__ uint32_t mod_sub(uint32_t a, uint32_t b) // (a - b) mod 1e9, a, b in [0, 1e9)
+{
+    const uint32_t t = a - b;
+    return min(t, t + (uint32_t)MODULO);
+}
 __device__ __forceinline__ int32_t ranf_next_round(int32_t x, int32_t *win)
 {
     const int lane = threadIdx.x;
+    const int i = lane + 1;
+    const int sa = lane < 55 ? i - 1 : RANF_ZERO;
+    const int sb = lane < 55 ? (i <= 24 ? i + 30 : i - 25) : RANF_ZERO;
+    const int sc = (lane < 55 && i > 24) ? (i <= 48 ? i + 6 : i - 49) : RANF_ZERO;
+    const int sd = (lane < 55 && i > 48) ? i - 18 : RANF_ZERO;
     if (lane < 55)
         win[lane] = x;
     __syncthreads();
-    int32_t v = x;
-    const int i = lane + 1;
-    if (lane < 55) {
-        if (i <= 24)
-            v = win[i - 1] - win[i + 30];
-        else if (i <= 48)
-            v = norm_mod(win[i - 1] - win[i - 25]) + win[i + 6];
-        else
-            v = norm_mod(win[i - 1] - win[i - 25]) + norm_mod(win[i - 49] - win[i - 18]);
-        v = norm_mod(v);
-    }
+    const uint32_t wa = (uint32_t)win[sa], wb = (uint32_t)win[sb], wc = (uint32_t)win[sc], wd = (uint32_t)win[sd];
     __syncthreads();
-    return v;
+    const uint32_t v = mod_sub(wa, wb) + mod_sub(wc, wd); // [0, 2e9)
+    return (int32_t)min(v, v - (uint32_t)MODULO);
 }
 
 // X_{r+m} from X_r with c = x^(55 m) mod P: X_{r+m}[i] = sum_k c_k Y[i-1+k]
@@ -482,26 +493,34 @@ __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
     const int lane = threadIdx.x;
     const uint64_t piece = blockIdx.x / a.nsegs;
     const uint64_t seg = a.seg_first + blockIdx.x % a.nsegs;
-    const uint64_t p0 = (uint64_t)a.p0[piece];
+    const uint64_t p0 = (uint64_t)__builtin_amdgcn_readfirstlane(a.p0[piece]); // uniform: scalar compares below
     const uint64_t q_begin = p0 + a.d_begin, q_end = p0 + a.d_end;
     const uint64_t r_begin = seg * (uint64_t)a.R;
     if (r_begin * 55 >= q_end)
         return; // whole workgroup
+    if (lane == 0)
+        win[RANF_ZERO] = 0;
     int32_t x = lane < 55 ? a.origin[56 * piece + lane + 1] : 0;
     for (int k = 0; k < a.K; ++k)
         if ((seg >> k) & 1ull)
             x = ranf_jump(x, a.polys + 55 * k, win);
     int32_t *out = a.out + piece * a.piece_stride;
+    // X[lane+1] of round r is emitted at stream position 55 r + 54 - lane
+    const int64_t off = (int64_t)(54 - lane) - (int64_t)q_begin;
     for (int j = 0; j < a.R; ++j) {
-        const uint64_t r = r_begin + j;
-        if (r * 55 >= q_end)
+        const uint64_t q0 = 55 * (r_begin + j);
+        if (q0 >= q_end)
             break;
         if (lane < 55) {
-            const uint64_t qq = 55 * r + (uint64_t)(54 - lane); // X[lane+1] is emitted at 55 - (lane+1)
-            if (qq >= q_begin && qq < q_end)
-                out[qq - q_begin] = x;
+            if (q0 >= q_begin && q0 + 55 <= q_end) { // the whole round is in range (uniform)
+                out[(int64_t)q0 + off] = x;
+            } else {
+                const uint64_t qq = q0 + (uint64_t)(54 - lane);
+                if (qq >= q_begin && qq < q_end)
+                    out[qq - q_begin] = x;
+            }
         }
-        if ((r + 1) * 55 < q_end)
+        if (q0 + 55 < q_end)
             x = ranf_next_round(x, win);
     }
 }
@@ -726,11 +745,17 @@ template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F3
     }
 }
 
-constexpr int F32_TILE = 512;
+constexpr int F32_TILE = 256;
 
+// In-order f32 batch sums (resacc.rs:121-129 with Float = f32).  For each
+// tile of events all 64 lanes form the 12 summands (m_c, m_c^2, w, w^2) in
+// parallel, writing -0.0 for a rejected event (x + (-0.0) == x for every x,
+// so a rejected event changes no sum, as when the reference skips it); then
+// lanes 0..11 add their summand column in event order, which is the order
+// that fixes the f32 rounding.  Selected events are counted with ballots.
 __global__ __launch_bounds__(64) void f32_accumulate_kernel(F32Args a)
 {
-    __shared__ float tile[F32_TILE * 6];
+    __shared__ float terms[12][F32_TILE + 1];
     const uint32_t lane = threadIdx.x;
     DevAcc *out = reinterpret_cast<DevAcc *>(a.out);
     for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
@@ -742,34 +767,45 @@ __global__ __launch_bounds__(64) void f32_accumulate_kernel(F32Args a)
         uint32_t cnt = 0;
         for (uint32_t base = 0; base < n; base += F32_TILE) {
             const uint32_t mm = n - base < (uint32_t)F32_TILE ? n - base : (uint32_t)F32_TILE;
-            __syncthreads();
-            for (uint32_t f = lane; f < 6 * mm; f += 64)
-                tile[f] = ev[6 * (size_t)base + f];
-            __syncthreads();
-            if (lane < 13) {
-                for (uint32_t i = 0; i < mm; ++i) {
-                    const float *x = tile + 6 * i;
-                    if (x[5] == 0.f)
-                        continue;
-                    float v;
-                    if (lane < 5) {
-                        v = x[lane];
-                    } else if (lane < 10) {
-                        v = x[lane - 5] * x[lane - 5];
-                    } else if (lane < 12) {
-                        const float w = weight_f32(a.P, x);
-                        v = lane == 10 ? w : w * w;
-                    } else {
-                        ++cnt;
-                        continue;
-                    }
-                    t = t + v;
+            __syncthreads(); // the previous tile has been summed
+            for (uint32_t e = lane; e < mm; e += 64) {
+                const float *x = ev + 6 * (size_t)(base + e);
+                float m[5];
+#pragma unroll
+                for (int c = 0; c < 5; ++c)
+                    m[c] = x[c];
+                const bool keep = x[5] != 0.f;
+                cnt += (uint32_t)__popcll(__ballot(keep));
+                const float w = weight_f32(a.P, m);
+                const float z = -0.f;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    terms[c][e] = keep ? m[c] : z;
+                    terms[5 + c][e] = keep ? m[c] * m[c] : z;
                 }
+                terms[10][e] = keep ? w : z;
+                terms[11][e] = keep ? w * w : z;
+            }
+            __syncthreads();
+            if (lane < 12) {
+                const float *row = terms[lane];
+                uint32_t e = 0;
+                for (; e + 16 <= mm; e += 16) {
+                    float v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        v[u] = row[e + u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        t = t + v[u];
+                }
+                for (; e < mm; ++e)
+                    t = t + row[e];
             }
         }
         if (lane < 12)
             reinterpret_cast<double *>(out + lb)[1 + lane] = (double)t;
-        else if (lane == 12)
+        if (lane == 0) // lane 0 runs every pass of the tile loop, so its count is complete
             out[lb].selected = cnt;
     }
 }
