@@ -55,8 +55,11 @@ template <class T> __host__ __device__ inline AdjParams<T> adj_params(const Para
     return A;
 }
 
-__device__ __forceinline__ void tsincos(double x, double *s, double *c) { sincos(x, s, c); }
+// f64: the event loop's correctly rounded routines (tp_math.h); f32: the device library.
+__device__ __forceinline__ void tsincos(double x, double *s, double *c) { tpm::sincos_f64(x, *s, *c); }
 __device__ __forceinline__ void tsincos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ double tlog(double x) { return tpm::log_pos(x); }
+__device__ __forceinline__ float tlog(float x) { return logf(x); }
 template <class T> __device__ __forceinline__ T minpos() { return sizeof(T) == 8 ? (T)2.2250738585072014e-308 : (T)1.17549435e-38f; }
 
 template <class T> __device__ __forceinline__ C<T> cconj(C<T> z) { return C<T>{z.re, -z.im}; }
@@ -65,7 +68,7 @@ template <class T> __device__ __forceinline__ C<T> caddv(C<T> a, C<T> b) { retur
 template <class T> __device__ __forceinline__ C<T> cinv(C<T> z)
 {
     const T n = z.re * z.re + z.im * z.im;
-    return C<T>{z.re / n, -z.im / n};
+    return C<T>{dv(z.re, n), dv(-z.im, n)};
 }
 // Re(a * conj(b))
 template <class T> __device__ __forceinline__ T cdot(C<T> a, C<T> b) { return a.re * b.re + a.im * b.im; }
@@ -134,8 +137,8 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
         const T ph = two_pi * u[4 * i + 1];
         em[i] = u[4 * i + 2] * u[4 * i + 3];
         tsincos(ph, &sp[i], &cp[i]);
-        st[i] = sqrt(one - ct[i] * ct[i]);
-        en[i] = -log(em[i] + minpos<T>());
+        st[i] = sq(one - ct[i] * ct[i]);
+        en[i] = -tlog(em[i] + minpos<T>());
         q[i][0] = en[i] * (st[i] * sp[i]);
         q[i][1] = en[i] * (st[i] * cp[i]);
         q[i][2] = en[i] * ct[i];
@@ -147,9 +150,9 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
     for (int k = 0; k < 4; ++k)
         R[k] = (q[0][k] + q[1][k]) + q[2][k];
     const T rn2 = R[3] * R[3] - ((R[0] * R[0] + R[1] * R[1]) + R[2] * R[2]);
-    const T alpha = P.e_total / rn2;
-    const T rn = sqrt(rn2);
-    const T beta = one / (rn + R[3]);
+    const T alpha = dv(P.e_total, rn2);
+    const T rn = sq(rn2);
+    const T beta = dv(one, rn + R[3]);
     T rq[3], b[3], p[3][4];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -246,9 +249,9 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
     bool small[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        xx[k] = sqrt(ps[k][3] + ps[k][2]);
+        xx[k] = sq(ps[k][3] + ps[k][2]);
         small[k] = !(xx[k] > minpos<T>());
-        fx[k] = small[k] ? C<T>{sqrt(two * ps[k][3]), (T)0} : C<T>{ps[k][0] / xx[k], ps[k][1] / xx[k]};
+        fx[k] = small[k] ? C<T>{sq(two * ps[k][3]), (T)0} : C<T>{dv(ps[k][0], xx[k]), dv(ps[k][1], xx[k])};
     }
     SpinVal<T> V;
     const C<T> f0 = C<T>{P.f0re, P.f0im}, f1 = C<T>{P.f1re, P.f1im};
