@@ -164,7 +164,7 @@ template <class T> __device__ __forceinline__ void sort3(T p[3][4])
 // of step 1, step 3 compares the loser b with whatever step 2 left in slot 2.
 // The kernels keep the photons where RAMBO made them and apply the slots where
 // they store the event (no register shuffles).
-__device__ __forceinline__ void sort_slots(const double p[3][4], int sort, int d[3])
+__host__ __device__ __forceinline__ void sort_slots(const double p[3][4], int sort, int d[3])
 {
     if (!sort) {
         d[0] = 0;
