@@ -41,6 +41,8 @@ struct Ctx {
     DevBuf mt128, bt128, f32ev; // all-f32 mode: xoshiro128+ tables, per-event values
     std::string bt128_key;
     int bt128_kb = 0;
+    DevBuf st128;               // all-f32 mode: B^grid, the grid-stride advance
+    std::string st128_key;
     tp_stats last{};
     bool timed = false;
     // Host staging of the RANF origin: must outlive the asynchronous copies.

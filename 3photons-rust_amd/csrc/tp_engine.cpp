@@ -106,7 +106,7 @@ int bits_for(uint64_t n);
 // jump.
 constexpr int KM128 = 20;
 
-int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, DevBuf **bt_out, int *kb_out)
+int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int grid, DevBuf **bt_out, int *kb_out)
 {
     if (!c->mt128.p) {
         const std::vector<tp::Gf2Mat128> &t = tp::xoshiro128_step_pow2();
@@ -132,12 +132,30 @@ int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, DevB
         c->bt128_key = key;
         c->bt128_kb = kb;
     }
+    const std::string skey = key + ":" + std::to_string(grid);
+    if (c->st128_key != skey) {
+        tp::Gf2Mat128 B, S;
+        if (seeding == TP_SEED_JUMP)
+            tp::gf2_128_jump64(B);
+        else
+            tp::gf2_128_pow_from_table(tp::xoshiro128_step_pow2(), 12ull * batch_size, B);
+        tp::gf2_128_pow_from_table(tp::gf2_128_pow2_table(B, std::max(1, bits_for((uint64_t)grid))), (uint64_t)grid, S);
+        int rc = ensure(c, c->st128, sizeof S);
+        if (rc)
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->st128.p, &S, sizeof S, hipMemcpyHostToDevice));
+        c->st128_key = skey;
+    }
     *bt_out = &c->bt128;
     *kb_out = c->bt128_kb;
     return TP_OK;
 }
 
-constexpr uint64_t F32_CHUNK_BATCHES = 4096; // 4096 * 10000 events * 24 B = 0.98 GB of per-event values
+// 16384 * 10000 events * 24 B = 3.9 GB of per-event values; with a grid of
+// 16 workgroups per CU each lane steps through 4 batches of a launch by the
+// grid-stride matrix instead of a full skip-ahead per batch.
+constexpr uint64_t F32_CHUNK_BATCHES = 16384;
 
 int f32_grid(const Ctx *c, uint64_t nb) { return (int)std::max<uint64_t>(1, std::min<uint64_t>(nb, 16ull * c->cus)); }
 
@@ -148,7 +166,8 @@ int enqueue_xo128_f32(Ctx *c, const tp_params *params, int seeding, const tp_xos
     const uint64_t nb = (num_events + batch_size - 1) / batch_size;
     DevBuf *bt = nullptr;
     int kb = 0;
-    int rc = ensure_tables128(c, seeding, batch_size, nb, &bt, &kb);
+    const int sgrid = f32_grid(c, F32_CHUNK_BATCHES);
+    int rc = ensure_tables128(c, seeding, batch_size, nb, sgrid, &bt, &kb);
     if (rc)
         return rc;
     const uint64_t chunk = std::min<uint64_t>(nb, F32_CHUNK_BATCHES);
@@ -166,6 +185,8 @@ int enqueue_xo128_f32(Ctx *c, const tp_params *params, int seeding, const tp_xos
     a.km = KM128;
     a.bt = static_cast<const uint32_t *>(bt->p);
     a.kb = kb;
+    a.stride = static_cast<const uint32_t *>(c->st128.p);
+    a.stride_grid = sgrid;
     a.ev = static_cast<float *>(c->f32ev.p);
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
@@ -577,7 +598,7 @@ extern "C" int tp_destroy(void *ctx)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
-                      &c->mt128, &c->bt128, &c->f32ev})
+                      &c->mt128, &c->bt128, &c->st128, &c->f32ev})
         if (b->p)
             (void)hipFree(b->p);
     for (auto &kv : c->tables) {

@@ -88,7 +88,9 @@ struct F32Args {
     uint32_t s0[4];        // xoshiro128+ state of the run
     const uint32_t *mt;    // M^(2^k) of xoshiro128+, k < km (128 columns x 4 words)
     const uint32_t *bt;    // B^(2^k), k < kb: B = M^(12 batch_size) or the 2^64 jump
+    const uint32_t *stride;// B^stride_grid: a lane's next batch of a grid-stride loop
     int32_t km, kb;
+    int32_t stride_grid;
     const int32_t *draws;  // RANF values of this launch's batches, 12 per event, batch-major
     float *ev;             // per event m[5] and keep (0/1), at batch_size*(b - b_first) + e
     void *out;             // tp_acc per batch, index b - b_first (f32 values)

@@ -705,6 +705,8 @@ void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t 
 template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F32Args a)
 {
     const uint32_t lane = threadIdx.x;
+    uint32_t st[4] = {0, 0, 0, 0}; // xoshiro128+ state at this lane's chunk of the previous batch
+    bool have = false;
     for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
         const uint64_t b = a.b_first + lb;
         const uint64_t left = a.num_events - b * a.batch_size;
@@ -714,13 +716,29 @@ template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F3
         const uint32_t e0 = lane * q + (lane < r ? lane : r);
         uint32_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
         if (SOURCE == 0) {
-            for (int k = 0; k < a.kb; ++k)
-                if ((b >> k) & 1ull) // uniform across the wavefront
-                    gf2_apply128(a.bt + 512 * k, s);
-            const uint64_t off = 12ull * e0;
-            for (int k = 0; k < a.km; ++k)
-                if ((off >> k) & 1ull)
-                    gf2_apply128(a.mt + 512 * k, s);
+            // M^(12 e0) B^b s0; the next batch of the grid-stride loop is B^grid
+            // times the same (all factors are powers of the step matrix, so
+            // they commute), unless its event split differs (the last batch)
+            if (have && n == a.batch_size && (int)gridDim.x == a.stride_grid) {
+                gf2_apply128(a.stride, st);
+            } else {
+                st[0] = a.s0[0];
+                st[1] = a.s0[1];
+                st[2] = a.s0[2];
+                st[3] = a.s0[3];
+                for (int k = 0; k < a.kb; ++k)
+                    if ((b >> k) & 1ull) // uniform across the wavefront
+                        gf2_apply128(a.bt + 512 * k, st);
+                const uint64_t off = 12ull * e0;
+                for (int k = 0; k < a.km; ++k)
+                    if ((off >> k) & 1ull)
+                        gf2_apply128(a.mt + 512 * k, st);
+            }
+            have = n == a.batch_size;
+            s[0] = st[0];
+            s[1] = st[1];
+            s[2] = st[2];
+            s[3] = st[3];
         }
         for (uint32_t i = 0; i < len; ++i) {
             const uint64_t e = lb * a.batch_size + e0 + i;

@@ -142,3 +142,24 @@ def test_f32_spinor_nan_batches_match_oracle(engine, cfg, oracle32, valeurs, bat
         else:
             assert np.isfinite(a).all()
             close_f32(a, o, 10000)
+
+
+def test_f32_grid_stride_batches_match_oracle(engine, cfg, oracle32, valeurs):
+    """N = 1e8 (10 000 batches, more than the launch's workgroups): each lane
+    reaches its later batches by the grid-stride matrix of xoshiro128+ instead
+    of a skip-ahead from the run's start.  The last batches, and one early
+    one, against the f32 oracle started at those batches."""
+    n_events = 100_000_000
+    print(f"f32 run N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, precision=tp.PREC_F32)
+    per, _, _ = engine.run(params, tp.RNG_XOSHIRO128, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO128), n_events)
+    ocfg, opts = oracle32.config(valeurs, n_events), oracle32.opts("xoshiro")  # the weight carries 1/N
+    nb = n_events // 10000
+    for b in (7, nb - 3, nb - 2, nb - 1):
+        g = oracle32.rng("xoshiro")
+        oracle32.skip(g, 120000 * b)
+        o = oracle32.simulate_batch(ocfg, opts, g, 10000).as_array()
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        close_f32(a, o, 10000)
