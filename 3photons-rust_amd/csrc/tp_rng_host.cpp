@@ -8,17 +8,24 @@ namespace tp {
 
 // ---- RANF ------------------------------------------------------------------
 
+// The reference's two loops, with x[i] += 1e9 where the difference is
+// negative done without a branch; the second loop is cut at i = 49 so that
+// no block reads what it writes (x[i - 24] for i < 49 comes from the first
+// loop, for i >= 49 from the first part of the second), which lets the
+// compiler vectorise each block.
 void ranf_round(int32_t *x)
 {
     for (int i = 1; i < 25; ++i) {
-        x[i] -= x[i + 31];
-        if (x[i] < 0)
-            x[i] += RANF_MODULO;
+        const int32_t v = x[i] - x[i + 31];
+        x[i] = v + ((v >> 31) & RANF_MODULO);
     }
-    for (int i = 25; i < 56; ++i) {
-        x[i] -= x[i - 24];
-        if (x[i] < 0)
-            x[i] += RANF_MODULO;
+    for (int i = 25; i < 49; ++i) {
+        const int32_t v = x[i] - x[i - 24];
+        x[i] = v + ((v >> 31) & RANF_MODULO);
+    }
+    for (int i = 49; i < 56; ++i) {
+        const int32_t v = x[i] - x[i - 24];
+        x[i] = v + ((v >> 31) & RANF_MODULO);
     }
 }
 
@@ -396,8 +403,8 @@ struct RanfWalker {
     RanfState g;
     uint64_t q; // position of the next value
     // random_array::<N>: the N values in the order the reference returns them
-    // (ascending array index = descending stream position).
-    void take(int n, double *out)
+    // (ascending array index = descending stream position), as raw integers.
+    const int32_t *take(int n)
     {
         if ((int)g.index < n) {
             q += g.index; // leftovers of the round are discarded
@@ -405,9 +412,8 @@ struct RanfWalker {
             g.index = 55;
         }
         g.index -= (uint32_t)n;
-        for (int k = 0; k < n; ++k)
-            out[k] = (double)g.numbers[g.index + 1 + k] * 1e-9;
         q += (uint64_t)n;
+        return g.numbers + g.index + 1;
     }
 };
 } // namespace
@@ -415,21 +421,25 @@ struct RanfWalker {
 void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end)
 {
     RanfWalker w{g, 55u - g.index};
-    double r9[9], r6[6], r2v[2];
     for (uint64_t e = 0; e < num_events; ++e) {
         if (starts)
             starts[e] = w.q;
-        w.take(9, r9);
-        w.take(6, r6);
+        w.take(9); // only their positions matter here
+        const int32_t *r6 = w.take(6);
+        double px[3], py[3];
+        for (int i = 0; i < 3; ++i) {
+            px[i] = (double)r6[i] * 1e-9;
+            py[i] = (double)r6[3 + i] * 1e-9;
+        }
         // radii of the three points (evgen.rs:228-246); only the accept test
         // matters here, with the reference's float operations
         for (int i = 0; i < 3; ++i) {
-            const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
+            const double x = 2. * px[i] - 1., y = 2. * py[i] - 1.;
             double r2 = (0. + x * x) + y * y;
             const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
             while (r2 > 1. || r2 < min2) {
-                w.take(2, r2v);
-                const double a = 2. * r2v[0] - 1., b = 2. * r2v[1] - 1.;
+                const int32_t *r2v = w.take(2);
+                const double a = 2. * ((double)r2v[0] * 1e-9) - 1., b = 2. * ((double)r2v[1] * 1e-9) - 1.;
                 r2 = 0. + (a * a + b * b);
             }
         }
