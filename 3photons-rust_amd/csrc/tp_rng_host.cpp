@@ -1,7 +1,10 @@
 // tp_rng_host.cpp -- host-side stream algebra (see tp_rng_host.h).
 #include "tp_rng_host.h"
 
+#include <algorithm>
 #include <cstring>
+#include <thread>
+#include <vector>
 #include <mutex>
 
 namespace tp {
@@ -418,12 +421,15 @@ struct RanfWalker {
 };
 } // namespace
 
-void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end)
+namespace {
+// Event starts from the walker's position onward, until n events are recorded
+// or the position reaches q_stop (when q_stop is not 0).
+void walk_events(RanfWalker &w, uint64_t n, uint64_t q_stop, std::vector<uint64_t> &out)
 {
-    RanfWalker w{g, 55u - g.index};
-    for (uint64_t e = 0; e < num_events; ++e) {
-        if (starts)
-            starts[e] = w.q;
+    for (uint64_t e = 0; e < n; ++e) {
+        if (q_stop && w.q >= q_stop)
+            return;
+        out.push_back(w.q);
         w.take(9); // only their positions matter here
         const int32_t *r6 = w.take(6);
         double px[3], py[3];
@@ -444,9 +450,126 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
             }
         }
     }
-    g = w.g;
+}
+
+// A walker whose next value is at stream position q (q >= q0, the position
+// of g0's next value).
+RanfWalker walker_at(const RanfState &g0, uint64_t q0, uint64_t q)
+{
+    RanfState g = g0;
+    ranf_advance(g, q - q0);
+    return RanfWalker{g, q};
+}
+} // namespace
+
+// The walk is sequential by nature, but a walker's future depends only on its
+// stream position at an event start.  Long walks therefore split the stream
+// at round boundaries: each thread starts at its boundary as if an event
+// started there, and walks its segment plus an overlap into the next.  The
+// true walk from the previous segment reaches, within a few events, an event
+// start the next thread also recorded; from there on both agree, so the
+// segments are joined at that point (a sequential walk covers any gap).
+void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end, int max_threads)
+{
+    const uint64_t q0 = 55u - g.index;
+    unsigned hw = std::thread::hardware_concurrency();
+    const int T = (int)std::min<unsigned>((unsigned)std::max(1, max_threads), hw ? hw : 1u);
+    // The true walk as pieces [begin, end) of the lists, in stream order.
+    struct Piece {
+        size_t list, begin, end;
+    };
+    std::vector<std::vector<uint64_t>> lists;
+    std::vector<Piece> pieces;
+    if (num_events < 4000000 || T < 2) {
+        lists.resize(1);
+        RanfWalker w{g, q0};
+        lists[0].reserve(num_events + 1);
+        walk_events(w, num_events + 1, 0, lists[0]);
+        pieces.push_back({0, 0, lists[0].size()});
+    } else {
+        // ~17.9 draws per event; segments of whole rounds
+        const uint64_t est = q0 + (uint64_t)((double)num_events * 18.5);
+        const uint64_t seg = (est / (uint64_t)T / 55 + 1) * 55;
+        const uint64_t overlap = 64 * 55;
+        lists.resize((size_t)T);
+        std::vector<std::thread> th;
+        for (int k = 0; k < T; ++k)
+            th.emplace_back([&, k] {
+                const uint64_t qs = k == 0 ? q0 : (uint64_t)k * seg;
+                RanfWalker w = k == 0 ? RanfWalker{g, q0} : walker_at(g, q0, qs);
+                lists[(size_t)k].reserve((size_t)(seg / 17));
+                walk_events(w, ~0ull, k + 1 < T ? (uint64_t)(k + 1) * seg + overlap : est + overlap, lists[(size_t)k]);
+            });
+        for (auto &t : th)
+            t.join();
+        // The true walk continues in lists[cur] from index cur_begin; find where
+        // it meets list k's speculative starts past that segment's boundary.
+        size_t cur = 0, cur_begin = 0;
+        uint64_t have = 0;
+        for (int k = 1; k < T && have <= num_events; ++k) {
+            std::vector<uint64_t> &a = lists[cur];
+            const std::vector<uint64_t> &b = lists[(size_t)k];
+            const uint64_t qs = (uint64_t)k * seg;
+            size_t i = 0, j = 0;
+            auto meet = [&]() {
+                i = (size_t)(std::lower_bound(a.begin() + (std::ptrdiff_t)cur_begin, a.end(), qs) - a.begin());
+                j = 0;
+                while (i < a.size() && j < b.size()) {
+                    if (a[i] == b[j])
+                        return true;
+                    if (a[i] < b[j])
+                        ++i;
+                    else
+                        ++j;
+                }
+                return false;
+            };
+            bool joined = meet();
+            if (!joined && !b.empty()) { // extend the true walk until it meets b or passes it
+                RanfWalker w = walker_at(g, q0, a.back());
+                uint64_t stop = a.back();
+                bool first = true;
+                while (!joined && a.back() <= b.back()) {
+                    stop += overlap;
+                    std::vector<uint64_t> ext;
+                    walk_events(w, ~0ull, stop, ext);
+                    a.insert(a.end(), ext.begin() + (first ? 1 : 0), ext.end()); // the first repeats a.back()
+                    first = false;
+                    joined = meet();
+                }
+            }
+            if (!joined) // the true walk has covered this segment itself
+                continue;
+            pieces.push_back({cur, cur_begin, i});
+            have += i - cur_begin;
+            cur = (size_t)k;
+            cur_begin = j;
+        }
+        pieces.push_back({cur, cur_begin, lists[cur].size()});
+    }
+    uint64_t total = 0;
+    for (const Piece &p : pieces)
+        total += p.end - p.begin;
+    if (total < num_events + 1) { // estimate too short: finish sequentially
+        const Piece &p = pieces.back();
+        RanfWalker w = walker_at(g, q0, lists[p.list][p.end - 1]);
+        lists.emplace_back();
+        walk_events(w, num_events + 1 - total + 1, 0, lists.back());
+        pieces.push_back({lists.size() - 1, 1, lists.back().size()}); // [0] repeats the previous start
+    }
+    uint64_t qe = 0, off = 0; // the start of event num_events is where the last one ended
+    for (const Piece &p : pieces) {
+        const uint64_t *d = lists[p.list].data() + p.begin;
+        const uint64_t cnt = p.end - p.begin;
+        if (off <= num_events && num_events < off + cnt)
+            qe = d[num_events - off];
+        if (starts && off < num_events)
+            std::memcpy(starts + off, d, (size_t)std::min<uint64_t>(cnt, num_events - off) * sizeof(uint64_t));
+        off += cnt;
+    }
+    ranf_advance(g, qe - q0);
     if (q_end)
-        *q_end = w.q;
+        *q_end = qe;
 }
 
 } // namespace tp

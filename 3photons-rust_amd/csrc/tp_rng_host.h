@@ -50,7 +50,11 @@ void ranf_advance(RanfState &g, uint64_t n);
 // is at q0 = 55 - g.index.  starts[e] receives the position of event e's
 // first random_array::<9> call (before any discard); on return g is the
 // state after the last event and *q_end its position.
-void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end);
+// faster-evgen stream walk: the start position of each event; g ends at the
+// position after the last event.  Walks of 4e6 events or more use up to
+// max_threads threads (1: sequential); the result does not depend on it.
+void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end,
+                            int max_threads = 16);
 
 // ---- xoshiro256+ -------------------------------------------------------------
 void xoshiro_seed(uint64_t seed, uint64_t s[4]); // seed_from_u64 (SplitMix64 expansion)

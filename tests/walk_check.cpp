@@ -1,0 +1,28 @@
+// walk_check.cpp -- TEST INFRASTRUCTURE ONLY: the faster-evgen stream walk
+// (3photons-rust_amd/csrc/tp_rng_host.cpp) split over threads against the
+// same walk done sequentially: event starts, end position and final state.
+//
+//   g++ -O2 -std=c++17 -pthread -shared -fPIC -I<csrc> walk_check.cpp <csrc>/tp_rng_host.cpp
+#include "tp_rng_host.h"
+
+#include <cstring>
+#include <vector>
+
+// 0 when the threaded walk of n events from seed / index equals the
+// sequential one.
+extern "C" int tp_walk_mismatch(uint64_t n, int seed, uint32_t index, int threads)
+{
+    tp::RanfState g0;
+    tp::ranf_seeded(g0, seed);
+    g0.index = index;
+    tp::RanfState a = g0, b = g0;
+    std::vector<uint64_t> sa(n), sb(n);
+    uint64_t qa = 0, qb = 0;
+    tp::ranf_walk_faster_evgen(a, n, sa.data(), &qa, 1);
+    tp::ranf_walk_faster_evgen(b, n, sb.data(), &qb, threads);
+    int bad = 0;
+    bad += qa != qb;
+    bad += sa != sb;
+    bad += a.index != b.index || std::memcmp(a.numbers, b.numbers, sizeof a.numbers) != 0;
+    return bad;
+}
