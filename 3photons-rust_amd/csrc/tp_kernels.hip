@@ -140,8 +140,9 @@ __device__ __forceinline__ void block_reduce_store(double v[12], uint32_t n, Dev
 // ---- the fused event loop ------------------------------------------------
 //
 // One wavefront per batch.  Each iteration every lane generates one event
-// (uniforms -> RAMBO -> sort -> cuts); the photon momenta of the events that
-// pass are appended to a queue in LDS (ballot + prefix count), and whenever 64
+// (uniforms -> RAMBO -> sort slots -> cuts); the photon momenta of the events
+// that pass are appended to a queue in LDS (ballot + prefix count), each photon
+// into the row of its sort slot, and whenever 64
 // are queued the whole wavefront takes them and evaluates spinors, amplitudes
 // and weights on full lanes.  Cut events therefore cost no amplitude work, and
 // the amplitude code never runs with masked-off lanes except for the final
@@ -153,10 +154,10 @@ __device__ __forceinline__ void block_reduce_store(double v[12], uint32_t n, Dev
 // from HBM lane l takes events l, l+64, ... (coalesced loads).
 constexpr int QCAP = 128;
 
-// Three wavefronts per SIMD (168 VGPRs, a few cold values spilled): the LDS
-// queue (12.4 KB per workgroup) caps a CU at 12 resident workgroups anyway,
-// and the third wave hides the transcendental and LDS latencies better than
-// the 2-wave, spill-free allocation (measured 3 % faster at f64).
+// Three wavefronts per SIMD (163 VGPRs at f64, no scratch): the LDS queue
+// (12.4 KB per workgroup) caps a CU at 12 resident workgroups anyway, and the
+// third wave hides the transcendental-table, LDS and f64 latencies better than
+// two (measured 3 % faster at f64).
 // The run constants re-read from the kernel-argument segment (FusedArgs::P
 // sits at its start) through a pointer the compiler cannot see through, at
 // each use site: scalar loads, which cost no VALU slot, instead of ~40 SGPRs
