@@ -2,6 +2,8 @@
 // points (internal; not part of the C ABI).
 #pragma once
 
+#include "tp_guard.h"
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,6 +58,9 @@ struct Ctx {
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws
 constexpr int RANF_R = 2048;        // RANF rounds per expansion segment (one jump per segment)
 constexpr uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
+
+// Where tp_guard stores an exception's message for tp_last_error.
+inline std::string *ctx_err(void *ctx) { return ctx ? &static_cast<Ctx *>(ctx)->err : nullptr; }
 
 inline int fail(Ctx *c, int code, const std::string &msg)
 {

@@ -559,60 +559,64 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 
 extern "C" int tp_create(int device, void **ctx_out)
 {
-    if (!ctx_out)
-        return TP_ERR_ARG;
-    *ctx_out = nullptr;
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
-        return TP_ERR_HIP;
-    if (device < 0 || device >= count)
-        return TP_ERR_ARG;
-    if (hipSetDevice(device) != hipSuccess)
-        return TP_ERR_HIP;
-    Ctx *c = new Ctx;
-    c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return TP_ERR_HIP;
-    }
-    for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
+    return tp_guard(nullptr, [&]() -> int {
+        if (!ctx_out)
+            return TP_ERR_ARG;
+        *ctx_out = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            return TP_ERR_HIP;
+        if (device < 0 || device >= count)
+            return TP_ERR_ARG;
+        if (hipSetDevice(device) != hipSuccess)
+            return TP_ERR_HIP;
+        Ctx *c = new Ctx;
+        c->device = device;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
             delete c;
             return TP_ERR_HIP;
         }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-        c->cus = prop.multiProcessorCount;
-    for (int src = 0; src < 4; ++src)
-        for (int prec = 0; prec < 2; ++prec)
-            c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
-    *ctx_out = c;
-    return TP_OK;
+        for (auto &e : c->ev)
+            if (hipEventCreate(&e) != hipSuccess) {
+                delete c;
+                return TP_ERR_HIP;
+            }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->cus = prop.multiProcessorCount;
+        for (int src = 0; src < 4; ++src)
+            for (int prec = 0; prec < 2; ++prec)
+                c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
+        *ctx_out = c;
+        return TP_OK;
+    });
 }
 
 extern "C" int tp_destroy(void *ctx)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
-                      &c->mt128, &c->bt128, &c->st128, &c->f32ev})
-        if (b->p)
-            (void)hipFree(b->p);
-    for (auto &kv : c->tables) {
-        if (kv.second->bt.p)
-            (void)hipFree(kv.second->bt.p);
-        if (kv.second->stride.p)
-            (void)hipFree(kv.second->stride.p);
-    }
-    for (auto &e : c->ev)
-        if (e)
-            (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-    return TP_OK;
+    return tp_guard(nullptr, [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
+                          &c->mt128, &c->bt128, &c->st128, &c->f32ev})
+            if (b->p)
+                (void)hipFree(b->p);
+        for (auto &kv : c->tables) {
+            if (kv.second->bt.p)
+                (void)hipFree(kv.second->bt.p);
+            if (kv.second->stride.p)
+                (void)hipFree(kv.second->stride.p);
+        }
+        for (auto &e : c->ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return TP_OK;
+    });
 }
 
 extern "C" const char *tp_last_error(const void *ctx)
@@ -623,414 +627,434 @@ extern "C" const char *tp_last_error(const void *ctx)
 
 extern "C" int tp_rng_seed(int rng_kind, uint64_t seed, void *state)
 {
-    if (!state)
+    return tp_guard(nullptr, [&]() -> int {
+        if (!state)
+            return TP_ERR_ARG;
+        if (rng_kind == TP_RNG_RANF) {
+            tp::RanfState g;
+            tp::ranf_seeded(g, static_cast<int32_t>(seed));
+            tp_ranf_state *o = static_cast<tp_ranf_state *>(state);
+            std::memcpy(o->numbers, g.numbers, sizeof o->numbers);
+            o->index = g.index;
+            o->seed = g.seed;
+            return TP_OK;
+        }
+        if (rng_kind == TP_RNG_XOSHIRO) {
+            tp::xoshiro_seed(seed, static_cast<tp_xoshiro_state *>(state)->s);
+            return TP_OK;
+        }
+        if (rng_kind == TP_RNG_XOSHIRO128) {
+            tp::xoshiro128_seed(seed, static_cast<tp_xoshiro128_state *>(state)->s);
+            return TP_OK;
+        }
         return TP_ERR_ARG;
-    if (rng_kind == TP_RNG_RANF) {
-        tp::RanfState g;
-        tp::ranf_seeded(g, static_cast<int32_t>(seed));
-        tp_ranf_state *o = static_cast<tp_ranf_state *>(state);
-        std::memcpy(o->numbers, g.numbers, sizeof o->numbers);
-        o->index = g.index;
-        o->seed = g.seed;
-        return TP_OK;
-    }
-    if (rng_kind == TP_RNG_XOSHIRO) {
-        tp::xoshiro_seed(seed, static_cast<tp_xoshiro_state *>(state)->s);
-        return TP_OK;
-    }
-    if (rng_kind == TP_RNG_XOSHIRO128) {
-        tp::xoshiro128_seed(seed, static_cast<tp_xoshiro128_state *>(state)->s);
-        return TP_OK;
-    }
-    return TP_ERR_ARG;
+    });
 }
 
 extern "C" int tp_rng_init(int rng_kind, void *state)
 {
-    return tp_rng_seed(rng_kind, rng_kind == TP_RNG_RANF ? static_cast<uint64_t>(tp::RANF_SEED) : 12345u, state);
+    return tp_guard(nullptr, [&]() -> int {
+        return tp_rng_seed(rng_kind, rng_kind == TP_RNG_RANF ? static_cast<uint64_t>(tp::RANF_SEED) : 12345u, state);
+    });
 }
 
 extern "C" int tp_rng_skip(int rng_kind, void *state, uint64_t n)
 {
-    if (!state)
-        return TP_ERR_ARG;
-    if (rng_kind == TP_RNG_XOSHIRO) {
-        tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(state)->s, n);
+    return tp_guard(nullptr, [&]() -> int {
+        if (!state)
+            return TP_ERR_ARG;
+        if (rng_kind == TP_RNG_XOSHIRO) {
+            tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(state)->s, n);
+            return TP_OK;
+        }
+        if (rng_kind == TP_RNG_XOSHIRO128) {
+            tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(state)->s, n);
+            return TP_OK;
+        }
+        if (rng_kind != TP_RNG_RANF)
+            return TP_ERR_ARG;
+        tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
+        if (s->index > 55)
+            return TP_ERR_ARG;
+        tp::RanfState g;
+        std::memcpy(g.numbers, s->numbers, sizeof g.numbers);
+        g.index = s->index;
+        g.seed = s->seed;
+        tp::ranf_advance(g, n);
+        std::memcpy(s->numbers, g.numbers, sizeof g.numbers);
+        s->index = g.index;
         return TP_OK;
-    }
-    if (rng_kind == TP_RNG_XOSHIRO128) {
-        tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(state)->s, n);
-        return TP_OK;
-    }
-    if (rng_kind != TP_RNG_RANF)
-        return TP_ERR_ARG;
-    tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
-    if (s->index > 55)
-        return TP_ERR_ARG;
-    tp::RanfState g;
-    std::memcpy(g.numbers, s->numbers, sizeof g.numbers);
-    g.index = s->index;
-    g.seed = s->seed;
-    tp::ranf_advance(g, n);
-    std::memcpy(s->numbers, g.numbers, sizeof g.numbers);
-    s->index = g.index;
-    return TP_OK;
+    });
 }
 
 extern "C" int tp_rng_jump(int rng_kind, void *state)
 {
-    if (!state)
-        return TP_ERR_ARG;
-    if (rng_kind == TP_RNG_XOSHIRO) {
-        static tp::Gf2Mat J;
-        static std::once_flag once;
-        std::call_once(once, [] { tp::gf2_jump128(J); });
-        uint64_t *s = static_cast<tp_xoshiro_state *>(state)->s;
-        tp::gf2_apply(J, s, s);
-        return TP_OK;
-    }
-    if (rng_kind == TP_RNG_XOSHIRO128) {
-        tp::xoshiro128_jump(static_cast<tp_xoshiro128_state *>(state)->s);
-        return TP_OK;
-    }
-    if (rng_kind != TP_RNG_RANF)
-        return TP_ERR_ARG;
-    tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
-    const int32_t seed = static_cast<int32_t>(static_cast<uint32_t>(s->seed) + 123456u);
-    return tp_rng_seed(TP_RNG_RANF, static_cast<uint64_t>(static_cast<uint32_t>(seed)), state);
+    return tp_guard(nullptr, [&]() -> int {
+        if (!state)
+            return TP_ERR_ARG;
+        if (rng_kind == TP_RNG_XOSHIRO) {
+            static tp::Gf2Mat J;
+            static std::once_flag once;
+            std::call_once(once, [] { tp::gf2_jump128(J); });
+            uint64_t *s = static_cast<tp_xoshiro_state *>(state)->s;
+            tp::gf2_apply(J, s, s);
+            return TP_OK;
+        }
+        if (rng_kind == TP_RNG_XOSHIRO128) {
+            tp::xoshiro128_jump(static_cast<tp_xoshiro128_state *>(state)->s);
+            return TP_OK;
+        }
+        if (rng_kind != TP_RNG_RANF)
+            return TP_ERR_ARG;
+        tp_ranf_state *s = static_cast<tp_ranf_state *>(state);
+        const int32_t seed = static_cast<int32_t>(static_cast<uint32_t>(s->seed) + 123456u);
+        return tp_rng_seed(TP_RNG_RANF, static_cast<uint64_t>(static_cast<uint32_t>(seed)), state);
+    });
 }
 
 extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const void *rng_state, uint32_t n,
                          double *u, double *p, int32_t *keep, double *m)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    int rc = check_common(c, params, rng_kind, rng_state);
-    if (rc)
-        return rc;
-    if (n == 0)
-        return TP_OK;
-    if (n > (1u << 20))
-        return fail(c, TP_ERR_ARG, "tp_events: n exceeds 2^20");
-    if (params->faster_evgen || params->precision == TP_PREC_F32)
-        return fail(c, TP_ERR_UNSUPPORTED, "tp_events: per-event values of faster-evgen and all-f32 runs are not available");
-    HIPCHK(c, hipSetDevice(c->device));
-    tpk::FusedArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.P = make_params(params);
-    a.precision = params->precision;
-    rc = ensure_mt(c);
-    if (rc)
-        return rc;
-    a.mt = static_cast<const uint64_t *>(c->mt.p);
-    a.km = c->km;
-    int source = 0;
-    if (rng_kind == TP_RNG_XOSHIRO) {
-        std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
-    } else {
-        // Expand 12 n draws from the given state: one piece, skip-ahead.
-        const tp_ranf_state *rs = static_cast<const tp_ranf_state *>(rng_state);
-        if (rs->index > 55)
-            return fail(c, TP_ERR_ARG, "RANF state index out of range");
-        const uint64_t D = 12ull * n;
-        rc = ensure(c, c->draws, D * sizeof(int32_t));
-        if (!rc)
-            rc = ensure(c, c->origin, 56 * sizeof(int32_t));
-        if (!rc)
-            rc = ensure(c, c->p0, sizeof(int32_t));
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        int rc = check_common(c, params, rng_kind, rng_state);
         if (rc)
             return rc;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
-        c->p00 = static_cast<int32_t>(55u - rs->index);
-        const uint64_t max_rounds = (c->p00 + D + 54) / 55 + 1;
-        const int K = std::max(1, bits_for(max_rounds / RANF_R + 1));
-        rc = ensure_polys(c, RANF_R, K);
+        if (n == 0)
+            return TP_OK;
+        if (n > (1u << 20))
+            return fail(c, TP_ERR_ARG, "tp_events: n exceeds 2^20");
+        if (params->faster_evgen || params->precision == TP_PREC_F32)
+            return fail(c, TP_ERR_UNSUPPORTED, "tp_events: per-event values of faster-evgen and all-f32 runs are not available");
+        HIPCHK(c, hipSetDevice(c->device));
+        tpk::FusedArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.P = make_params(params);
+        a.precision = params->precision;
+        rc = ensure_mt(c);
         if (rc)
             return rc;
-        HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
-        tpk::RanfArgs r;
-        std::memset(&r, 0, sizeof r);
-        r.origin = static_cast<const int32_t *>(c->origin.p);
-        r.p0 = static_cast<const int32_t *>(c->p0.p);
-        r.polys = static_cast<const uint32_t *>(c->polys.p);
-        r.K = K;
-        r.R = RANF_R;
-        r.out = static_cast<int32_t *>(c->draws.p);
-        r.d_begin = 0;
-        r.d_end = D;
-        r.seg_first = 0;
-        r.nsegs = (c->p00 + D - 1) / 55 / RANF_R + 1;
-        tpk::launch_ranf_expand(r, 1, c->stream);
-        HIPCHK(c, hipGetLastError());
-        a.draws = static_cast<const int32_t *>(c->draws.p);
-        source = 1;
-    }
-    DevBuf bu, bp, bk, bm;
-    tpk::EventOut o{nullptr, nullptr, nullptr, nullptr};
-    struct Item {
-        void *host;
-        DevBuf *buf;
-        size_t bytes;
-    } items[4] = {{u, &bu, 12ull * n * sizeof(double)},
-                  {p, &bp, 12ull * n * sizeof(double)},
-                  {keep, &bk, n * sizeof(int32_t)},
-                  {m, &bm, 5ull * n * sizeof(double)}};
-    for (Item &it : items)
-        if (it.host && (rc = ensure(c, *it.buf, it.bytes)) != TP_OK)
-            break;
-    if (rc == TP_OK) {
-        o.u = static_cast<double *>(bu.p);
-        o.p = static_cast<double *>(bp.p);
-        o.keep = static_cast<int32_t *>(bk.p);
-        o.m = static_cast<double *>(bm.p);
-        tpk::launch_events(a, source, n, o, c->stream);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-            rc = fail(c, TP_ERR_HIP, std::string("events kernel: ") + hipGetErrorString(e));
+        a.mt = static_cast<const uint64_t *>(c->mt.p);
+        a.km = c->km;
+        int source = 0;
+        if (rng_kind == TP_RNG_XOSHIRO) {
+            std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
+        } else {
+            // Expand 12 n draws from the given state: one piece, skip-ahead.
+            const tp_ranf_state *rs = static_cast<const tp_ranf_state *>(rng_state);
+            if (rs->index > 55)
+                return fail(c, TP_ERR_ARG, "RANF state index out of range");
+            const uint64_t D = 12ull * n;
+            rc = ensure(c, c->draws, D * sizeof(int32_t));
+            if (!rc)
+                rc = ensure(c, c->origin, 56 * sizeof(int32_t));
+            if (!rc)
+                rc = ensure(c, c->p0, sizeof(int32_t));
+            if (rc)
+                return rc;
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
+            c->p00 = static_cast<int32_t>(55u - rs->index);
+            const uint64_t max_rounds = (c->p00 + D + 54) / 55 + 1;
+            const int K = std::max(1, bits_for(max_rounds / RANF_R + 1));
+            rc = ensure_polys(c, RANF_R, K);
+            if (rc)
+                return rc;
+            HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
+            tpk::RanfArgs r;
+            std::memset(&r, 0, sizeof r);
+            r.origin = static_cast<const int32_t *>(c->origin.p);
+            r.p0 = static_cast<const int32_t *>(c->p0.p);
+            r.polys = static_cast<const uint32_t *>(c->polys.p);
+            r.K = K;
+            r.R = RANF_R;
+            r.out = static_cast<int32_t *>(c->draws.p);
+            r.d_begin = 0;
+            r.d_end = D;
+            r.seg_first = 0;
+            r.nsegs = (c->p00 + D - 1) / 55 / RANF_R + 1;
+            tpk::launch_ranf_expand(r, 1, c->stream);
+            HIPCHK(c, hipGetLastError());
+            a.draws = static_cast<const int32_t *>(c->draws.p);
+            source = 1;
+        }
+        DevBuf bu, bp, bk, bm;
+        tpk::EventOut o{nullptr, nullptr, nullptr, nullptr};
+        struct Item {
+            void *host;
+            DevBuf *buf;
+            size_t bytes;
+        } items[4] = {{u, &bu, 12ull * n * sizeof(double)},
+                      {p, &bp, 12ull * n * sizeof(double)},
+                      {keep, &bk, n * sizeof(int32_t)},
+                      {m, &bm, 5ull * n * sizeof(double)}};
         for (Item &it : items)
-            if (rc == TP_OK && it.host) {
-                e = hipMemcpyAsync(it.host, it.buf->p, it.bytes, hipMemcpyDeviceToHost, c->stream);
-                if (e != hipSuccess)
-                    rc = fail(c, TP_ERR_HIP, std::string("events copy: ") + hipGetErrorString(e));
-            }
-        e = hipStreamSynchronize(c->stream);
-        if (rc == TP_OK && e != hipSuccess)
-            rc = fail(c, TP_ERR_HIP, std::string("events sync: ") + hipGetErrorString(e));
-    }
-    for (Item &it : items)
-        if (it.buf->p)
-            (void)hipFree(it.buf->p);
-    return rc;
+            if (it.host && (rc = ensure(c, *it.buf, it.bytes)) != TP_OK)
+                break;
+        if (rc == TP_OK) {
+            o.u = static_cast<double *>(bu.p);
+            o.p = static_cast<double *>(bp.p);
+            o.keep = static_cast<int32_t *>(bk.p);
+            o.m = static_cast<double *>(bm.p);
+            tpk::launch_events(a, source, n, o, c->stream);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                rc = fail(c, TP_ERR_HIP, std::string("events kernel: ") + hipGetErrorString(e));
+            for (Item &it : items)
+                if (rc == TP_OK && it.host) {
+                    e = hipMemcpyAsync(it.host, it.buf->p, it.bytes, hipMemcpyDeviceToHost, c->stream);
+                    if (e != hipSuccess)
+                        rc = fail(c, TP_ERR_HIP, std::string("events copy: ") + hipGetErrorString(e));
+                }
+            e = hipStreamSynchronize(c->stream);
+            if (rc == TP_OK && e != hipSuccess)
+                rc = fail(c, TP_ERR_HIP, std::string("events sync: ") + hipGetErrorString(e));
+        }
+        for (Item &it : items)
+            if (it.buf->p)
+                (void)hipFree(it.buf->p);
+        return rc;
+    });
 }
 
 extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, const void *rng_state, const void *u_in,
                           uint64_t n, int precision, int on_device, void *F, void *dF_du, void *dF_dp)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    if (!params)
-        return fail(c, TP_ERR_ARG, "null params");
-    if (precision != TP_PREC_F64 && precision != TP_PREC_F32)
-        return fail(c, TP_ERR_ARG, "tp_adjoint: precision must be TP_PREC_F64 or TP_PREC_F32");
-    if (!u_in && rng_kind != TP_RNG_XOSHIRO)
-        return fail(c, TP_ERR_UNSUPPORTED, "tp_adjoint draws its points from a xoshiro stream or u_in");
-    if (!u_in && !rng_state)
-        return fail(c, TP_ERR_ARG, "null rng state");
-    if (n == 0)
-        return TP_OK;
-    if (!u_in && 12 * n >= (1ull << KM))
-        return fail(c, TP_ERR_ARG, "tp_adjoint: n exceeds the stream offset range");
-    HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_mt(c);
-    if (rc)
-        return rc;
-    tpk::AdjArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.P = make_params(params);
-    a.w = params->ev_weight;
-    a.n = n;
-    a.per_thread = n >= (1u << 20) ? 16 : 1;
-    a.mt = static_cast<const uint64_t *>(c->mt.p);
-    a.km = c->km;
-    if (!u_in)
-        std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
-    const size_t es = precision == TP_PREC_F32 ? sizeof(float) : sizeof(double);
-    const size_t bytes[3] = {5 * n * es, 60 * n * es, 60 * n * es};
-    void *outs[3] = {F, dF_du, dF_dp};
-    DevBuf tmp[4];
-    if (on_device) {
-        a.u_in = static_cast<const double *>(u_in);
-        a.F = F;
-        a.dF_du = dF_du;
-        a.dF_dp = dF_dp;
-    } else {
-        if (u_in) {
-            rc = ensure(c, tmp[3], 12 * n * sizeof(double));
-            if (rc)
-                return rc;
-            HIPCHK(c, hipMemcpyAsync(tmp[3].p, u_in, 12 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-            a.u_in = static_cast<const double *>(tmp[3].p);
-        }
-        for (int k = 0; k < 3 && rc == TP_OK; ++k)
-            if (outs[k])
-                rc = ensure(c, tmp[k], bytes[k]);
-        a.F = F ? tmp[0].p : nullptr;
-        a.dF_du = dF_du ? tmp[1].p : nullptr;
-        a.dF_dp = dF_dp ? tmp[2].p : nullptr;
-    }
-    if (rc == TP_OK) {
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-        tpk::launch_adjoint(a, precision, c->stream);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-            rc = fail(c, TP_ERR_HIP, std::string("adjoint kernel: ") + hipGetErrorString(e));
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    }
-    if (rc == TP_OK && !on_device)
-        for (int k = 0; k < 3 && rc == TP_OK; ++k)
-            if (outs[k]) {
-                hipError_t e = hipMemcpyAsync(outs[k], tmp[k].p, bytes[k], hipMemcpyDeviceToHost, c->stream);
-                if (e != hipSuccess)
-                    rc = fail(c, TP_ERR_HIP, std::string("adjoint copy: ") + hipGetErrorString(e));
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        if (!params)
+            return fail(c, TP_ERR_ARG, "null params");
+        if (precision != TP_PREC_F64 && precision != TP_PREC_F32)
+            return fail(c, TP_ERR_ARG, "tp_adjoint: precision must be TP_PREC_F64 or TP_PREC_F32");
+        if (!u_in && rng_kind != TP_RNG_XOSHIRO)
+            return fail(c, TP_ERR_UNSUPPORTED, "tp_adjoint draws its points from a xoshiro stream or u_in");
+        if (!u_in && !rng_state)
+            return fail(c, TP_ERR_ARG, "null rng state");
+        if (n == 0)
+            return TP_OK;
+        if (!u_in && 12 * n >= (1ull << KM))
+            return fail(c, TP_ERR_ARG, "tp_adjoint: n exceeds the stream offset range");
+        HIPCHK(c, hipSetDevice(c->device));
+        int rc = ensure_mt(c);
+        if (rc)
+            return rc;
+        tpk::AdjArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.P = make_params(params);
+        a.w = params->ev_weight;
+        a.n = n;
+        a.per_thread = n >= (1u << 20) ? 16 : 1;
+        a.mt = static_cast<const uint64_t *>(c->mt.p);
+        a.km = c->km;
+        if (!u_in)
+            std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
+        const size_t es = precision == TP_PREC_F32 ? sizeof(float) : sizeof(double);
+        const size_t bytes[3] = {5 * n * es, 60 * n * es, 60 * n * es};
+        void *outs[3] = {F, dF_du, dF_dp};
+        DevBuf tmp[4];
+        if (on_device) {
+            a.u_in = static_cast<const double *>(u_in);
+            a.F = F;
+            a.dF_du = dF_du;
+            a.dF_dp = dF_dp;
+        } else {
+            if (u_in) {
+                rc = ensure(c, tmp[3], 12 * n * sizeof(double));
+                if (rc)
+                    return rc;
+                HIPCHK(c, hipMemcpyAsync(tmp[3].p, u_in, 12 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+                a.u_in = static_cast<const double *>(tmp[3].p);
             }
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (rc == TP_OK && e != hipSuccess)
-        rc = fail(c, TP_ERR_HIP, std::string("adjoint sync: ") + hipGetErrorString(e));
-    if (rc == TP_OK) {
-        c->last.kernel_ms = elapsed(c->ev[1], c->ev[2]);
-        c->last.device_ms = c->last.kernel_ms;
-        c->last.events = n;
-        c->last.batches = 0;
-    }
-    for (DevBuf &b : tmp)
-        if (b.p)
-            (void)hipFree(b.p);
-    return rc;
+            for (int k = 0; k < 3 && rc == TP_OK; ++k)
+                if (outs[k])
+                    rc = ensure(c, tmp[k], bytes[k]);
+            a.F = F ? tmp[0].p : nullptr;
+            a.dF_du = dF_du ? tmp[1].p : nullptr;
+            a.dF_dp = dF_dp ? tmp[2].p : nullptr;
+        }
+        if (rc == TP_OK) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            tpk::launch_adjoint(a, precision, c->stream);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                rc = fail(c, TP_ERR_HIP, std::string("adjoint kernel: ") + hipGetErrorString(e));
+            HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        }
+        if (rc == TP_OK && !on_device)
+            for (int k = 0; k < 3 && rc == TP_OK; ++k)
+                if (outs[k]) {
+                    hipError_t e = hipMemcpyAsync(outs[k], tmp[k].p, bytes[k], hipMemcpyDeviceToHost, c->stream);
+                    if (e != hipSuccess)
+                        rc = fail(c, TP_ERR_HIP, std::string("adjoint copy: ") + hipGetErrorString(e));
+                }
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (rc == TP_OK && e != hipSuccess)
+            rc = fail(c, TP_ERR_HIP, std::string("adjoint sync: ") + hipGetErrorString(e));
+        if (rc == TP_OK) {
+            c->last.kernel_ms = elapsed(c->ev[1], c->ev[2]);
+            c->last.device_ms = c->last.kernel_ms;
+            c->last.events = n;
+            c->last.batches = 0;
+        }
+        for (DevBuf &b : tmp)
+            if (b.p)
+                (void)hipFree(b.p);
+        return rc;
+    });
 }
 
 extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kind, void *rng_state, uint32_t n,
                                  tp_acc *out)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    int rc = check_common(c, params, rng_kind, rng_state);
-    if (rc)
-        return rc;
-    if (!out)
-        return fail(c, TP_ERR_ARG, "null output");
-    if (n > 64u * TP_BATCH_SIZE)
-        return fail(c, TP_ERR_ARG, "n exceeds 640000 events per call");
-    HIPCHK(c, hipSetDevice(c->device));
-    if (n == 0) { // the reference's closure returns an empty accumulator
-        std::memset(out, 0, sizeof *out);
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        int rc = check_common(c, params, rng_kind, rng_state);
+        if (rc)
+            return rc;
+        if (!out)
+            return fail(c, TP_ERR_ARG, "null output");
+        if (n > 64u * TP_BATCH_SIZE)
+            return fail(c, TP_ERR_ARG, "n exceeds 640000 events per call");
+        HIPCHK(c, hipSetDevice(c->device));
+        if (n == 0) { // the reference's closure returns an empty accumulator
+            std::memset(out, 0, sizeof *out);
+            return TP_OK;
+        }
+        rc = ensure(c, c->per_batch, sizeof(tp_acc));
+        if (rc)
+            return rc;
+        rc = enqueue_run(c, params, rng_kind, TP_SEED_SKIPAHEAD, rng_state, n, n, c->per_batch.p);
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(out, c->per_batch.p, sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // `&mut rng` side effect: 12 draws per event (evgen.rs:182-187), or the
+        // walk of the faster-evgen draws.
+        if (params->faster_evgen) {
+            tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
+            std::memcpy(s->numbers, c->fe_end.numbers, sizeof s->numbers);
+            s->index = c->fe_end.index;
+        } else if (rng_kind == TP_RNG_XOSHIRO) {
+            tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, 12ull * n);
+        } else if (rng_kind == TP_RNG_XOSHIRO128) {
+            tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(rng_state)->s, 12ull * n);
+        } else {
+            tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
+            tp::RanfState g;
+            std::memcpy(g.numbers, s->numbers, sizeof g.numbers);
+            g.index = s->index;
+            g.seed = s->seed;
+            tp::ranf_advance(g, 12ull * n);
+            std::memcpy(s->numbers, g.numbers, sizeof g.numbers);
+            s->index = g.index;
+        }
         return TP_OK;
-    }
-    rc = ensure(c, c->per_batch, sizeof(tp_acc));
-    if (rc)
-        return rc;
-    rc = enqueue_run(c, params, rng_kind, TP_SEED_SKIPAHEAD, rng_state, n, n, c->per_batch.p);
-    if (rc)
-        return rc;
-    HIPCHK(c, hipMemcpyAsync(out, c->per_batch.p, sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    // `&mut rng` side effect: 12 draws per event (evgen.rs:182-187), or the
-    // walk of the faster-evgen draws.
-    if (params->faster_evgen) {
-        tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
-        std::memcpy(s->numbers, c->fe_end.numbers, sizeof s->numbers);
-        s->index = c->fe_end.index;
-    } else if (rng_kind == TP_RNG_XOSHIRO) {
-        tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, 12ull * n);
-    } else if (rng_kind == TP_RNG_XOSHIRO128) {
-        tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(rng_state)->s, 12ull * n);
-    } else {
-        tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
-        tp::RanfState g;
-        std::memcpy(g.numbers, s->numbers, sizeof g.numbers);
-        g.index = s->index;
-        g.seed = s->seed;
-        tp::ranf_advance(g, 12ull * n);
-        std::memcpy(s->numbers, g.numbers, sizeof g.numbers);
-        s->index = g.index;
-    }
-    return TP_OK;
+    });
 }
 
 extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, int seeding, const void *rng_state0,
                              uint64_t num_events, tp_acc *per_batch_dev, tp_acc *merged_dev, tp_stats *stats)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    int rc = check_common(c, params, rng_kind, rng_state0);
-    if (rc)
-        return rc;
-    if (seeding != TP_SEED_SKIPAHEAD && seeding != TP_SEED_JUMP)
-        return fail(c, TP_ERR_ARG, "unknown seeding " + std::to_string(seeding));
-    if (num_events == 0)
-        return fail(c, TP_ERR_ARG, "Must simulate at least one event");
-    if (!per_batch_dev)
-        return fail(c, TP_ERR_ARG, "null per-batch output");
-    HIPCHK(c, hipSetDevice(c->device));
-    const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
-    // Event 0 and 1 brackets the first kernel; 2 follows the event loop.
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    rc = enqueue_run(c, params, rng_kind, seeding, rng_state0, num_events, TP_BATCH_SIZE, per_batch_dev);
-    if (rc)
-        return rc;
-    if (merged_dev) {
-        tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream, params->precision == TP_PREC_F32);
-        HIPCHK(c, hipGetLastError());
-    }
-    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    c->last.events = num_events;
-    c->last.batches = nb;
-    c->timed = true;
-    if (stats)
-        *stats = c->last;
-    return TP_OK;
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        int rc = check_common(c, params, rng_kind, rng_state0);
+        if (rc)
+            return rc;
+        if (seeding != TP_SEED_SKIPAHEAD && seeding != TP_SEED_JUMP)
+            return fail(c, TP_ERR_ARG, "unknown seeding " + std::to_string(seeding));
+        if (num_events == 0)
+            return fail(c, TP_ERR_ARG, "Must simulate at least one event");
+        if (!per_batch_dev)
+            return fail(c, TP_ERR_ARG, "null per-batch output");
+        HIPCHK(c, hipSetDevice(c->device));
+        const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
+        // Event 0 and 1 brackets the first kernel; 2 follows the event loop.
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        rc = enqueue_run(c, params, rng_kind, seeding, rng_state0, num_events, TP_BATCH_SIZE, per_batch_dev);
+        if (rc)
+            return rc;
+        if (merged_dev) {
+            tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream, params->precision == TP_PREC_F32);
+            HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        c->last.events = num_events;
+        c->last.batches = nb;
+        c->timed = true;
+        if (stats)
+            *stats = c->last;
+        return TP_OK;
+    });
 }
 
 extern "C" int tp_sync(void *ctx, tp_stats *stats)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->timed) {
-        c->last.device_ms = elapsed(c->ev[0], c->ev[3]);
-        c->last.kernel_ms = elapsed(c->ev[1], c->ev[2]);
-    }
-    if (stats)
-        *stats = c->last;
-    return TP_OK;
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->timed) {
+            c->last.device_ms = elapsed(c->ev[0], c->ev[3]);
+            c->last.kernel_ms = elapsed(c->ev[1], c->ev[2]);
+        }
+        if (stats)
+            *stats = c->last;
+        return TP_OK;
+    });
 }
 
 extern "C" int tp_run(void *ctx, const tp_params *params, int rng_kind, int seeding, const void *rng_state0,
                       uint64_t num_events, tp_acc *per_batch_out, tp_acc *merged_out, tp_stats *stats)
 {
-    Ctx *c = static_cast<Ctx *>(ctx);
-    if (!c)
-        return TP_ERR_ARG;
-    if (!per_batch_out)
-        return fail(c, TP_ERR_ARG, "null per-batch output");
-    if (num_events == 0)
-        return fail(c, TP_ERR_ARG, "Must simulate at least one event");
-    const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
-    int rc = ensure(c, c->per_batch, nb * sizeof(tp_acc));
-    if (rc)
-        return rc;
-    rc = tp_run_device(ctx, params, rng_kind, seeding, rng_state0, num_events, static_cast<tp_acc *>(c->per_batch.p),
-                       nullptr, nullptr);
-    if (rc)
-        return rc;
-    HIPCHK(c, hipMemcpyAsync(per_batch_out, c->per_batch.p, nb * sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
-    rc = tp_sync(ctx, stats);
-    if (rc)
-        return rc;
-    if (merged_out && params->precision == TP_PREC_F32) {
-        tp_acc m = per_batch_out[0];
-        for (uint64_t b = 1; b < nb; ++b)
-            tp_acc_merge_f32(&m, &per_batch_out[b]);
-        *merged_out = m;
-    } else if (merged_out) { // ReproducibleAccumulator::get_merged_result (multi_threading.rs:107-126)
-        tp_acc m = per_batch_out[0];
-        for (uint64_t b = 1; b < nb; ++b) {
-            m.selected_events += per_batch_out[b].selected_events;
-            for (int i = 0; i < 5; ++i)
-                m.spm2[i] += per_batch_out[b].spm2[i];
-            for (int i = 0; i < 5; ++i)
-                m.vars[i] += per_batch_out[b].vars[i];
-            m.sigma += per_batch_out[b].sigma;
-            m.variance += per_batch_out[b].variance;
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        if (!per_batch_out)
+            return fail(c, TP_ERR_ARG, "null per-batch output");
+        if (num_events == 0)
+            return fail(c, TP_ERR_ARG, "Must simulate at least one event");
+        const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
+        int rc = ensure(c, c->per_batch, nb * sizeof(tp_acc));
+        if (rc)
+            return rc;
+        rc = tp_run_device(ctx, params, rng_kind, seeding, rng_state0, num_events, static_cast<tp_acc *>(c->per_batch.p),
+                           nullptr, nullptr);
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(per_batch_out, c->per_batch.p, nb * sizeof(tp_acc), hipMemcpyDeviceToHost, c->stream));
+        rc = tp_sync(ctx, stats);
+        if (rc)
+            return rc;
+        if (merged_out && params->precision == TP_PREC_F32) {
+            tp_acc m = per_batch_out[0];
+            for (uint64_t b = 1; b < nb; ++b)
+                tp_acc_merge_f32(&m, &per_batch_out[b]);
+            *merged_out = m;
+        } else if (merged_out) { // ReproducibleAccumulator::get_merged_result (multi_threading.rs:107-126)
+            tp_acc m = per_batch_out[0];
+            for (uint64_t b = 1; b < nb; ++b) {
+                m.selected_events += per_batch_out[b].selected_events;
+                for (int i = 0; i < 5; ++i)
+                    m.spm2[i] += per_batch_out[b].spm2[i];
+                for (int i = 0; i < 5; ++i)
+                    m.vars[i] += per_batch_out[b].vars[i];
+                m.sigma += per_batch_out[b].sigma;
+                m.variance += per_batch_out[b].variance;
+            }
+            *merged_out = m;
         }
-        *merged_out = m;
-    }
-    return TP_OK;
+        return TP_OK;
+    });
 }

@@ -16,6 +16,7 @@
 #include <ctime>
 #include <string>
 
+#include "tp_guard.h"
 #include "tp_host.h"
 
 namespace {
@@ -249,74 +250,76 @@ int cfg_error(char *err, int len, const std::string &msg)
 
 extern "C" int tp_config_load(const char *path, tp_config *cfg, char *err, int err_len)
 {
-    static const char *names[18] = {"num_events", "e_total", "beam_photons_cut", "photon_photon_cut",
-                                    "e_min", "beam_photon_plane_cut", "alpha", "alpha_z",
-                                    "gev2_to_picobarn", "m_z0", "g_z0", "sin2_weinberg",
-                                    "branching_ep_em", "beta_plus", "beta_moins", "num_bins",
-                                    "impr", "plot"};
-    if (!path || !cfg)
-        return cfg_error(err, err_len, "null argument");
-    std::memset(cfg, 0, sizeof *cfg);
-    FILE *f = std::fopen(path, "rb");
-    if (!f)
-        return cfg_error(err, err_len, std::string("failed to load the configuration: cannot open ") + path);
-    std::string text;
-    char chunk[4096];
-    size_t got;
-    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0)
-        text.append(chunk, got);
-    std::fclose(f);
-    // str::lines() then split_whitespace().next() per line (config.rs:69-71).
-    std::string toks[18];
-    int n = 0;
-    size_t pos = 0;
-    while (n < 18 && pos <= text.size()) {
-        size_t eol = text.find('\n', pos);
-        std::string line = text.substr(pos, eol == std::string::npos ? std::string::npos : eol - pos);
-        pos = (eol == std::string::npos) ? text.size() + 1 : eol + 1;
-        size_t a = 0;
-        while (a < line.size() && std::isspace(static_cast<unsigned char>(line[a])))
-            ++a;
-        if (a == line.size())
-            continue;
-        size_t b = a;
-        while (b < line.size() && !std::isspace(static_cast<unsigned char>(line[b])))
-            ++b;
-        toks[n++] = line.substr(a, b - a);
-    }
-    if (n < 18)
-        return cfg_error(err, err_len, std::string("missing configuration of ") + names[n]);
-    tp_config c;
-    std::memset(&c, 0, sizeof c);
-    double *dst[14] = {&c.e_total, &c.cut_beam, &c.cut_pp, &c.e_min, &c.cut_plane, &c.alpha,
-                       &c.alpha_z, &c.gev2_to_picobarn, &c.m_z0, &c.g_z0, &c.sin2_weinberg,
-                       &c.branching_ep_em, &c.beta_plus, &c.beta_minus};
-    if (!parse_u64(toks[0], &c.num_events))
-        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[0]);
-    for (int i = 0; i < 14; ++i)
-        if (!parse_f64(toks[1 + i], dst[i]))
-            return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[1 + i]);
-    if (!parse_i32(toks[15], &c.num_bins))
-        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[15]);
-    if (!parse_bool(toks[16], &c.impr))
-        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[16]);
-    if (!parse_bool(toks[17], &c.plot))
-        return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[17]);
-    c.parsed = 1;
-    *cfg = c;
-    // The ensure! checks of config.rs:110-124 come after the echo.
-    if (c.num_events == 0)
-        return cfg_error(err, err_len, "Please simulate at least one event");
-    if (c.plot)
-        return cfg_error(err, err_len, "Plotting is not supported by this version");
-    if (c.impr)
-        return cfg_error(err, err_len,
-                         "Individual result printing is not supported. This debugging feature has a "
-                         "run-time performance cost even when unused. It should be implemented at "
-                         "compile-time instead.");
-    if (err && err_len > 0)
-        err[0] = 0;
-    return TP_OK;
+    return tp_guard(nullptr, [&]() -> int {
+        static const char *names[18] = {"num_events", "e_total", "beam_photons_cut", "photon_photon_cut",
+                                        "e_min", "beam_photon_plane_cut", "alpha", "alpha_z",
+                                        "gev2_to_picobarn", "m_z0", "g_z0", "sin2_weinberg",
+                                        "branching_ep_em", "beta_plus", "beta_moins", "num_bins",
+                                        "impr", "plot"};
+        if (!path || !cfg)
+            return cfg_error(err, err_len, "null argument");
+        std::memset(cfg, 0, sizeof *cfg);
+        FILE *f = std::fopen(path, "rb");
+        if (!f)
+            return cfg_error(err, err_len, std::string("failed to load the configuration: cannot open ") + path);
+        std::string text;
+        char chunk[4096];
+        size_t got;
+        while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0)
+            text.append(chunk, got);
+        std::fclose(f);
+        // str::lines() then split_whitespace().next() per line (config.rs:69-71).
+        std::string toks[18];
+        int n = 0;
+        size_t pos = 0;
+        while (n < 18 && pos <= text.size()) {
+            size_t eol = text.find('\n', pos);
+            std::string line = text.substr(pos, eol == std::string::npos ? std::string::npos : eol - pos);
+            pos = (eol == std::string::npos) ? text.size() + 1 : eol + 1;
+            size_t a = 0;
+            while (a < line.size() && std::isspace(static_cast<unsigned char>(line[a])))
+                ++a;
+            if (a == line.size())
+                continue;
+            size_t b = a;
+            while (b < line.size() && !std::isspace(static_cast<unsigned char>(line[b])))
+                ++b;
+            toks[n++] = line.substr(a, b - a);
+        }
+        if (n < 18)
+            return cfg_error(err, err_len, std::string("missing configuration of ") + names[n]);
+        tp_config c;
+        std::memset(&c, 0, sizeof c);
+        double *dst[14] = {&c.e_total, &c.cut_beam, &c.cut_pp, &c.e_min, &c.cut_plane, &c.alpha,
+                           &c.alpha_z, &c.gev2_to_picobarn, &c.m_z0, &c.g_z0, &c.sin2_weinberg,
+                           &c.branching_ep_em, &c.beta_plus, &c.beta_minus};
+        if (!parse_u64(toks[0], &c.num_events))
+            return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[0]);
+        for (int i = 0; i < 14; ++i)
+            if (!parse_f64(toks[1 + i], dst[i]))
+                return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[1 + i]);
+        if (!parse_i32(toks[15], &c.num_bins))
+            return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[15]);
+        if (!parse_bool(toks[16], &c.impr))
+            return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[16]);
+        if (!parse_bool(toks[17], &c.plot))
+            return cfg_error(err, err_len, std::string("could not parse configuration of ") + names[17]);
+        c.parsed = 1;
+        *cfg = c;
+        // The ensure! checks of config.rs:110-124 come after the echo.
+        if (c.num_events == 0)
+            return cfg_error(err, err_len, "Please simulate at least one event");
+        if (c.plot)
+            return cfg_error(err, err_len, "Plotting is not supported by this version");
+        if (c.impr)
+            return cfg_error(err, err_len,
+                             "Individual result printing is not supported. This debugging feature has a "
+                             "run-time performance cost even when unused. It should be implemented at "
+                             "compile-time instead.");
+        if (err && err_len > 0)
+            err[0] = 0;
+        return TP_OK;
+    });
 }
 
 template <class T> int config_format(const tp_config *c, char *buf, int len)
@@ -343,9 +346,9 @@ template <class T> int config_format(const tp_config *c, char *buf, int len)
     return copy_out(o.s, buf, len);
 }
 
-extern "C" int tp_config_format(const tp_config *c, char *buf, int len) { return config_format<double>(c, buf, len); }
+extern "C" int tp_config_format(const tp_config *c, char *buf, int len) { return tp_guard(nullptr, [&]() -> int { return config_format<double>(c, buf, len); }); }
 
-extern "C" int tp_config_format_f32(const tp_config *c, char *buf, int len) { return config_format<float>(c, buf, len); }
+extern "C" int tp_config_format_f32(const tp_config *c, char *buf, int len) { return tp_guard(nullptr, [&]() -> int { return config_format<float>(c, buf, len); }); }
 
 namespace {
 
@@ -543,18 +546,22 @@ template <class T> void finalize(const tp_config *cfg, const tp_acc *acc, tp_fin
 
 extern "C" int tp_params_from_config(const tp_config *c, tp_params *p)
 {
-    if (!c || !p)
-        return TP_ERR_ARG;
-    params_from_config<double>(c, p);
-    return TP_OK;
+    return tp_guard(nullptr, [&]() -> int {
+        if (!c || !p)
+            return TP_ERR_ARG;
+        params_from_config<double>(c, p);
+        return TP_OK;
+    });
 }
 
 extern "C" int tp_params_from_config_f32(const tp_config *c, tp_params *p)
 {
-    if (!c || !p)
-        return TP_ERR_ARG;
-    params_from_config<float>(c, p);
-    return TP_OK;
+    return tp_guard(nullptr, [&]() -> int {
+        if (!c || !p)
+            return TP_ERR_ARG;
+        params_from_config<float>(c, p);
+        return TP_OK;
+    });
 }
 
 extern "C" void tp_acc_merge(tp_acc *into, const tp_acc *o) { acc_merge<double>(into, o); }
@@ -563,18 +570,22 @@ extern "C" void tp_acc_merge_f32(tp_acc *into, const tp_acc *o) { acc_merge<floa
 
 extern "C" int tp_finalize(const tp_config *cfg, const tp_acc *acc, tp_final *out)
 {
-    if (!cfg || !acc || !out)
-        return TP_ERR_ARG;
-    finalize<double>(cfg, acc, out);
-    return TP_OK;
+    return tp_guard(nullptr, [&]() -> int {
+        if (!cfg || !acc || !out)
+            return TP_ERR_ARG;
+        finalize<double>(cfg, acc, out);
+        return TP_OK;
+    });
 }
 
 extern "C" int tp_finalize_f32(const tp_config *cfg, const tp_acc *acc, tp_final *out)
 {
-    if (!cfg || !acc || !out)
-        return TP_ERR_ARG;
-    finalize<float>(cfg, acc, out);
-    return TP_OK;
+    return tp_guard(nullptr, [&]() -> int {
+        if (!cfg || !acc || !out)
+            return TP_ERR_ARG;
+        finalize<float>(cfg, acc, out);
+        return TP_OK;
+    });
 }
 
 namespace {
@@ -738,22 +749,30 @@ template <class T> std::string report_text(const tp_config *cfg, const tp_final 
 
 extern "C" int tp_format_res_data(const tp_config *cfg, const tp_final *res, char *buf, int len)
 {
-    return copy_out(res_data_text<double>(cfg, res), buf, len);
+    return tp_guard(nullptr, [&]() -> int {
+        return copy_out(res_data_text<double>(cfg, res), buf, len);
+    });
 }
 
 extern "C" int tp_format_res_data_f32(const tp_config *cfg, const tp_final *res, char *buf, int len)
 {
-    return copy_out(res_data_text<float>(cfg, res), buf, len);
+    return tp_guard(nullptr, [&]() -> int {
+        return copy_out(res_data_text<float>(cfg, res), buf, len);
+    });
 }
 
 extern "C" int tp_format_report(const tp_config *cfg, const tp_final *res, char *buf, int len)
 {
-    return copy_out(report_text<double>(cfg, res), buf, len);
+    return tp_guard(nullptr, [&]() -> int {
+        return copy_out(report_text<double>(cfg, res), buf, len);
+    });
 }
 
 extern "C" int tp_format_report_f32(const tp_config *cfg, const tp_final *res, char *buf, int len)
 {
-    return copy_out(report_text<float>(cfg, res), buf, len);
+    return tp_guard(nullptr, [&]() -> int {
+        return copy_out(report_text<float>(cfg, res), buf, len);
+    });
 }
 
 template <class T> int dump_results(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
@@ -815,10 +834,14 @@ template <class T> int dump_results(const tp_config *cfg, const tp_final *res, d
 
 extern "C" int tp_dump_results(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
 {
-    return dump_results<double>(cfg, res, elapsed_s, dir);
+    return tp_guard(nullptr, [&]() -> int {
+        return dump_results<double>(cfg, res, elapsed_s, dir);
+    });
 }
 
 extern "C" int tp_dump_results_f32(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir)
 {
-    return dump_results<float>(cfg, res, elapsed_s, dir);
+    return tp_guard(nullptr, [&]() -> int {
+        return dump_results<float>(cfg, res, elapsed_s, dir);
+    });
 }

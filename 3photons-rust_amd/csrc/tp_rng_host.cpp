@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 #include <mutex>
@@ -480,28 +481,41 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
     };
     std::vector<std::vector<uint64_t>> lists;
     std::vector<Piece> pieces;
-    if (num_events < 4000000 || T < 2) {
-        lists.resize(1);
+    auto sequential = [&]() {
+        lists.assign(1, {});
         RanfWalker w{g, q0};
         lists[0].reserve(num_events + 1);
         walk_events(w, num_events + 1, 0, lists[0]);
-        pieces.push_back({0, 0, lists[0].size()});
+        pieces.assign(1, Piece{0, 0, lists[0].size()});
+    };
+    // ~17.9 draws per event; T segments of whole rounds, each walked 64 rounds past its end
+    const uint64_t est = q0 + (uint64_t)((double)num_events * 18.5);
+    const uint64_t seg = (est / (uint64_t)T / 55 + 1) * 55;
+    const uint64_t overlap = 64 * 55;
+    if (num_events < 4000000 || T < 2) {
+        sequential();
     } else {
-        // ~17.9 draws per event; segments of whole rounds
-        const uint64_t est = q0 + (uint64_t)((double)num_events * 18.5);
-        const uint64_t seg = (est / (uint64_t)T / 55 + 1) * 55;
-        const uint64_t overlap = 64 * 55;
         lists.resize((size_t)T);
         std::vector<std::thread> th;
-        for (int k = 0; k < T; ++k)
-            th.emplace_back([&, k] {
-                const uint64_t qs = k == 0 ? q0 : (uint64_t)k * seg;
-                RanfWalker w = k == 0 ? RanfWalker{g, q0} : walker_at(g, q0, qs);
-                lists[(size_t)k].reserve((size_t)(seg / 17));
-                walk_events(w, ~0ull, k + 1 < T ? (uint64_t)(k + 1) * seg + overlap : est + overlap, lists[(size_t)k]);
-            });
+        bool started = true;
+        try {
+            for (int k = 0; k < T; ++k)
+                th.emplace_back([&, k] {
+                    const uint64_t qs = k == 0 ? q0 : (uint64_t)k * seg;
+                    RanfWalker w = k == 0 ? RanfWalker{g, q0} : walker_at(g, q0, qs);
+                    lists[(size_t)k].reserve((size_t)(seg / 17));
+                    walk_events(w, ~0ull, k + 1 < T ? (uint64_t)(k + 1) * seg + overlap : est + overlap,
+                                lists[(size_t)k]);
+                });
+        } catch (const std::system_error &) { // no thread to be had: walk alone
+            started = false;
+        }
         for (auto &t : th)
             t.join();
+        if (!started)
+            sequential();
+    }
+    if (pieces.empty()) {
         // The true walk continues in lists[cur] from index cur_begin; find where
         // it meets list k's speculative starts past that segment's boundary.
         size_t cur = 0, cur_begin = 0;

@@ -43,7 +43,8 @@ typedef enum {
     TP_ERR_NOMEM = -3,       /* device or host allocation failed */
     TP_ERR_UNSUPPORTED = -4, /* valid request this build does not implement */
     TP_ERR_CONFIG = -5,      /* configuration file missing/garbled/rejected */
-    TP_ERR_IO = -6           /* output file could not be written */
+    TP_ERR_IO = -6,          /* output file could not be written */
+    TP_ERR_INTERNAL = -7     /* unexpected failure inside the library (message in tp_last_error) */
 } tp_status;
 
 /* Random stream (src/random/mod.rs:13-16): RANF is the default, xoshiro the
