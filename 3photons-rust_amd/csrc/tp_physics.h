@@ -412,7 +412,7 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> 
 //
 // The mixed path runs the amplitudes with FAST complex products (fma and a
 // hardware reciprocal, 1 ulp): it trades the reference's rounding for speed
-// and is judged by its error against the f64 path on the same stream.
+// and is measured by its error against the f64 path on the same stream.
 // After the cuts every spinor product of the shipped configuration has
 // |s|^2 = 2 p_i.p_j >= 2.5 GeV^2, and every amplitude denominator stays far
 // inside the f32 range.

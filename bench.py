@@ -205,8 +205,8 @@ def main() -> int:
     ap.add_argument("--flow-batch", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the event loop shards by "
-              "batch range across GPUs as described in DESIGN.md, but no multi-GPU run is part of it.",
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the event loop would "
+              "shard by contiguous batch ranges, one range per GPU, but no multi-GPU run is part of it.",
               file=sys.stderr)
         return 2
     if args.copy_calibration:
