@@ -849,7 +849,7 @@ void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st)
 
 namespace tpk {
 
-template <class T> __global__ __launch_bounds__(128) void adjoint_kernel(AdjArgs a)
+template <class T> __device__ __forceinline__ void adjoint_body(const AdjArgs &a)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t first = t * a.per_thread;
@@ -889,14 +889,23 @@ template <class T> __global__ __launch_bounds__(128) void adjoint_kernel(AdjArgs
     }
 }
 
+// f64: 256 VGPRs + ~208 AGPRs, one wavefront per SIMD (two would spill
+// 768 B per lane and run 60 % slower).  f32: 256 VGPRs + 3 AGPRs would also
+// leave one; asking for two costs a 16-byte spill and runs 31 % faster.
+__global__ __launch_bounds__(128) void adjoint_kernel_f64(AdjArgs a) { adjoint_body<double>(a); }
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void adjoint_kernel_f32(AdjArgs a)
+{
+    adjoint_body<float>(a);
+}
+
 void launch_adjoint(const AdjArgs &a, int precision, hipStream_t st)
 {
     const uint64_t threads = (a.n + a.per_thread - 1) / a.per_thread;
     const dim3 g((unsigned)((threads + 127) / 128)), b(128);
     if (precision == 2)
-        hipLaunchKernelGGL(adjoint_kernel<float>, g, b, 0, st, a);
+        hipLaunchKernelGGL(adjoint_kernel_f32, g, b, 0, st, a);
     else
-        hipLaunchKernelGGL(adjoint_kernel<double>, g, b, 0, st, a);
+        hipLaunchKernelGGL(adjoint_kernel_f64, g, b, 0, st, a);
 }
 
 } // namespace tpk
