@@ -124,10 +124,17 @@ template <class T> __device__ __forceinline__ T sigmoid(T x) { return (T)1 / ((T
 //           outside an event lies; the cap keeps the soft and collinear
 //           singularities just outside the cuts from dominating (events with
 //           a photon of zero energy or f <= 0 are dropped).
-template <class T, int SOFT = 0>
-__device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[12], const T lam[5], T m[5], T du[12],
-                                              T dp[12], T tau = (T)0, T fcap = (T)INFINITY)
+//
+// event_adjoint_sweeps runs the forward pass once and then NLAM reverse
+// sweeps over the stored forward state: NLAM 1 with the given lam, NLAM 5
+// (SOFT 0 only) with lam = e_0 .. e_4, the five coupling components.  After
+// each sweep it calls sink(c, m, du, dp); a cut event calls it NLAM times
+// with zeros.
+template <class T, int SOFT, int NLAM, class Sink>
+__device__ __forceinline__ bool event_adjoint_sweeps(const AdjParams<T> &P, const T u[12], const T *lam_in, T m[5],
+                                                     T du[12], T dp[12], T tau, T fcap, Sink &&sink)
 {
+    static_assert(NLAM == 1 || (NLAM == 5 && SOFT == 0), "five sweeps need the plain integrand");
     const T one = (T)1, two = (T)2, two_pi = (T)(2. * 3.14159265358979323846);
     // ---- forward: massless photons from the uniforms (evgen.rs:174-207)
     T ct[3], sp[3], cp[3], st[3], en[3], em[3], q[3][4];
@@ -241,6 +248,8 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
 #pragma unroll
         for (int j = 0; j < 12; ++j)
             du[j] = dp[j] = (T)0;
+        for (int c = 0; c < NLAM; ++c)
+            sink(c, m, du, dp);
         return false;
     }
     // ---- forward: spinor products (spinor.rs:31-54) on the sorted photons
@@ -321,254 +330,269 @@ __device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[1
         m[3] = sR;
         m[4] = sI;
     }
-    // SOFT 1: L = S f, f = sum_c lam_c m_c: the amplitude sweep below runs with
-    // lam * S and f S dlogS/dp joins the momentum adjoint.  SOFT 2: L = log f +
-    // log S: the sweep runs with lam / f and dlogS/dp joins.
-    T lm[5];
-    T fS = (T)0;
-    {
-        const T S = SOFT ? exp(lsig) : (T)1;
-        T f = (T)0;
+#pragma unroll 1
+    for (int sw = 0; sw < NLAM; ++sw) {
+        T lam[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c)
-            f = f + lam[c] * m[c];
-        if (SOFT == 2 && !(f > (T)0 && f < (T)INFINITY)) {
+        for (int k = 0; k < 5; ++k)
+            lam[k] = NLAM == 1 ? lam_in[k] : (k == sw ? one : (T)0);
+        // SOFT 1: L = S f, f = sum_c lam_c m_c: the amplitude sweep below runs with
+        // lam * S and f S dlogS/dp joins the momentum adjoint.  SOFT 2: L = log f +
+        // log S: the sweep runs with lam / f and dlogS/dp joins.
+        T lm[5];
+        T fS = (T)0;
+        {
+            const T S = SOFT ? exp(lsig) : (T)1;
+            T f = (T)0;
 #pragma unroll
             for (int c = 0; c < 5; ++c)
-                m[c] = (T)0;
+                f = f + lam[c] * m[c];
+            if (SOFT == 2 && !(f > (T)0 && f < (T)INFINITY)) {
 #pragma unroll
-            for (int j = 0; j < 12; ++j)
-                du[j] = dp[j] = (T)0;
-            return false;
+                for (int c = 0; c < 5; ++c)
+                    m[c] = (T)0;
+#pragma unroll
+                for (int j = 0; j < 12; ++j)
+                    du[j] = dp[j] = (T)0;
+                return false;
+            }
+            const bool capped = SOFT == 2 && f > fcap;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                lm[c] = SOFT == 2 ? (capped ? (T)0 : lam[c] / f) : lam[c] * S;
+                if (SOFT)
+                    m[c] = capped ? m[c] * (fcap / f) * S : m[c] * S;
+            }
+            fS = SOFT == 2 ? (T)1 : f * S;
         }
-        const bool capped = SOFT == 2 && f > fcap;
+        // ---- reverse: amplitudes -> spinor products
+        SpinAdj<T> D;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            lm[c] = SOFT == 2 ? (capped ? (T)0 : lam[c] / f) : lam[c] * S;
-            if (SOFT)
-                m[c] = capped ? m[c] * (fcap / f) * S : m[c] * S;
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                D.sb[i][j] = D.tb[i][j] = C<T>{(T)0, (T)0};
+        const T kappa = two * P.g_a * P.g_bp;
+        const C<T> Xbar = C<T>{lm[3], lm[4]};
+        const T cA = two * lm[0] * P.g_a * P.g_a, cP = two * lm[1] * P.g_bp * P.g_bp;
+        const T cM = two * lm[2] * P.g_bm * P.g_bm;
+#pragma unroll
+        for (int h = 0; h < 6; ++h) {
+            const int k1 = ak[h][0], k2 = ak[h][1], k3 = ak[h][2];
+            const C<T> abar = caddv(cscale(cA, a[h]), cscale(kappa, xmul<true>(Xbar, bp[h])));
+            const C<T> pbar = caddv(cscale(cP, bp[h]), cscale(kappa, xmul<true>(cconj(Xbar), a[h])));
+            const C<T> ga = xmul<true>(abar, cconj(a[h])), gp = xmul<true>(pbar, cconj(bp[h]));
+            if (aty[h] == A_ppm) {
+                // a_ppm = ns01 s(0,k3)^2 / (s(0,k1) s(0,k2) s(1,k1) s(1,k2))
+                add_s(D, V, 0, k3, two, ga);
+                add_s(D, V, 0, k1, -one, ga);
+                add_s(D, V, 0, k2, -one, ga);
+                add_s(D, V, 1, k1, -one, ga);
+                add_s(D, V, 1, k2, -one, ga);
+                // bp_ppm = nt01 (t(k1,k2) s(k3,0))^2
+                add_t(D, V, k1, k2, two, gp);
+                add_s(D, V, k3, 0, two, gp);
+            } else {
+                // a_pmm = nt01 t(1,k1)^2 / (t(1,k2) t(1,k3) t(0,k2) t(0,k3))
+                add_t(D, V, 1, k1, two, ga);
+                add_t(D, V, 1, k2, -one, ga);
+                add_t(D, V, 1, k3, -one, ga);
+                add_t(D, V, 0, k2, -one, ga);
+                add_t(D, V, 0, k3, -one, ga);
+                // bp_pmm = ns01 (t(k1,1) s(k2,k3))^2
+                add_t(D, V, k1, 1, two, gp);
+                add_s(D, V, k2, k3, two, gp);
+            }
         }
-        fS = SOFT == 2 ? (T)1 : f * S;
-    }
-    // ---- reverse: amplitudes -> spinor products
-    SpinAdj<T> D;
-#pragma unroll
-    for (int i = 0; i < 5; ++i)
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-            D.sb[i][j] = D.tb[i][j] = C<T>{(T)0, (T)0};
-    const T kappa = two * P.g_a * P.g_bp;
-    const C<T> Xbar = C<T>{lm[3], lm[4]};
-    const T cA = two * lm[0] * P.g_a * P.g_a, cP = two * lm[1] * P.g_bp * P.g_bp;
-    const T cM = two * lm[2] * P.g_bm * P.g_bm;
-#pragma unroll
-    for (int h = 0; h < 6; ++h) {
-        const int k1 = ak[h][0], k2 = ak[h][1], k3 = ak[h][2];
-        const C<T> abar = caddv(cscale(cA, a[h]), cscale(kappa, xmul<true>(Xbar, bp[h])));
-        const C<T> pbar = caddv(cscale(cP, bp[h]), cscale(kappa, xmul<true>(cconj(Xbar), a[h])));
-        const C<T> ga = xmul<true>(abar, cconj(a[h])), gp = xmul<true>(pbar, cconj(bp[h]));
-        if (aty[h] == A_ppm) {
-            // a_ppm = ns01 s(0,k3)^2 / (s(0,k1) s(0,k2) s(1,k1) s(1,k2))
-            add_s(D, V, 0, k3, two, ga);
-            add_s(D, V, 0, k1, -one, ga);
-            add_s(D, V, 0, k2, -one, ga);
-            add_s(D, V, 1, k1, -one, ga);
-            add_s(D, V, 1, k2, -one, ga);
-            // bp_ppm = nt01 (t(k1,k2) s(k3,0))^2
-            add_t(D, V, k1, k2, two, gp);
-            add_s(D, V, k3, 0, two, gp);
-        } else {
-            // a_pmm = nt01 t(1,k1)^2 / (t(1,k2) t(1,k3) t(0,k2) t(0,k3))
-            add_t(D, V, 1, k1, two, ga);
-            add_t(D, V, 1, k2, -one, ga);
-            add_t(D, V, 1, k3, -one, ga);
-            add_t(D, V, 0, k2, -one, ga);
-            add_t(D, V, 0, k3, -one, ga);
-            // bp_pmm = ns01 (t(k1,1) s(k2,k3))^2
-            add_t(D, V, k1, 1, two, gp);
-            add_s(D, V, k2, k3, two, gp);
+        {
+            // bm_mmm = nt01 sum_j Y_j^2 with Y = s(k1,0)s(k2,k3), s(k2,0)s(k1,k3), s(k3,0)s(k1,k2)
+            const C<T> bbar = cscale(cM, bmm);
+            const C<T> Y1 = xmul<true>(V.s[2][0], V.s[3][4]), Y2 = xmul<true>(V.s[3][0], V.s[2][4]), Y3 = xmul<true>(V.s[4][0], V.s[2][3]);
+            const C<T> d1 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y1))));
+            const C<T> d2 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y2))));
+            const C<T> d3 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y3))));
+            D.sb[2][0] = caddv(D.sb[2][0], xmul<true>(d1, cconj(V.s[3][4])));
+            D.sb[3][4] = caddv(D.sb[3][4], xmul<true>(d1, cconj(V.s[2][0])));
+            D.sb[3][0] = caddv(D.sb[3][0], xmul<true>(d2, cconj(V.s[2][4])));
+            D.sb[2][4] = caddv(D.sb[2][4], xmul<true>(d2, cconj(V.s[3][0])));
+            D.sb[4][0] = caddv(D.sb[4][0], xmul<true>(d3, cconj(V.s[2][3])));
+            D.sb[2][3] = caddv(D.sb[2][3], xmul<true>(d3, cconj(V.s[4][0])));
         }
-    }
-    {
-        // bm_mmm = nt01 sum_j Y_j^2 with Y = s(k1,0)s(k2,k3), s(k2,0)s(k1,k3), s(k3,0)s(k1,k2)
-        const C<T> bbar = cscale(cM, bmm);
-        const C<T> Y1 = xmul<true>(V.s[2][0], V.s[3][4]), Y2 = xmul<true>(V.s[3][0], V.s[2][4]), Y3 = xmul<true>(V.s[4][0], V.s[2][3]);
-        const C<T> d1 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y1))));
-        const C<T> d2 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y2))));
-        const C<T> d3 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.nt01, Y3))));
-        D.sb[2][0] = caddv(D.sb[2][0], xmul<true>(d1, cconj(V.s[3][4])));
-        D.sb[3][4] = caddv(D.sb[3][4], xmul<true>(d1, cconj(V.s[2][0])));
-        D.sb[3][0] = caddv(D.sb[3][0], xmul<true>(d2, cconj(V.s[2][4])));
-        D.sb[2][4] = caddv(D.sb[2][4], xmul<true>(d2, cconj(V.s[3][0])));
-        D.sb[4][0] = caddv(D.sb[4][0], xmul<true>(d3, cconj(V.s[2][3])));
-        D.sb[2][3] = caddv(D.sb[2][3], xmul<true>(d3, cconj(V.s[4][0])));
-    }
-    {
-        // bm_ppp = ns01 sum_j Y_j^2 with Y = t(k1,k2)t(k3,1), t(k1,k3)t(k2,1), t(k2,k3)t(k1,1)
-        const C<T> bbar = cscale(cM, bpp);
-        const C<T> t23 = tval(V, 2, 3), t41 = tval(V, 4, 1), t24 = tval(V, 2, 4), t31 = tval(V, 3, 1);
-        const C<T> t34 = tval(V, 3, 4), t21 = tval(V, 2, 1);
-        const C<T> Y1 = xmul<true>(t23, t41), Y2 = xmul<true>(t24, t31), Y3 = xmul<true>(t34, t21);
-        const C<T> d1 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y1))));
-        const C<T> d2 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y2))));
-        const C<T> d3 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y3))));
-        D.tb[2][3] = caddv(D.tb[2][3], xmul<true>(d1, cconj(t41)));
-        D.tb[4][1] = caddv(D.tb[4][1], xmul<true>(d1, cconj(t23)));
-        D.tb[2][4] = caddv(D.tb[2][4], xmul<true>(d2, cconj(t31)));
-        D.tb[3][1] = caddv(D.tb[3][1], xmul<true>(d2, cconj(t24)));
-        D.tb[3][4] = caddv(D.tb[3][4], xmul<true>(d3, cconj(t21)));
-        D.tb[2][1] = caddv(D.tb[2][1], xmul<true>(d3, cconj(t34)));
-    }
-    // fold onto the independent products: s(j,i) = -s(i,j), t(i,j) = -conj(s(i,j)),
-    // t(j,i) = conj(s(i,j))
-    C<T> g0[3], g1[3], gkl[3][3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int K = 2 + k;
-        g0[k] = caddv(caddv(D.sb[0][K], neg(D.sb[K][0])), caddv(neg(cconj(D.tb[0][K])), cconj(D.tb[K][0])));
-        g1[k] = caddv(caddv(D.sb[1][K], neg(D.sb[K][1])), caddv(neg(cconj(D.tb[1][K])), cconj(D.tb[K][1])));
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = k + 1; l < 3; ++l) {
-            const int K = 2 + k, L = 2 + l;
-            gkl[k][l] =
-                caddv(caddv(D.sb[K][L], neg(D.sb[L][K])), caddv(neg(cconj(D.tb[K][L])), cconj(D.tb[L][K])));
+        {
+            // bm_ppp = ns01 sum_j Y_j^2 with Y = t(k1,k2)t(k3,1), t(k1,k3)t(k2,1), t(k2,k3)t(k1,1)
+            const C<T> bbar = cscale(cM, bpp);
+            const C<T> t23 = tval(V, 2, 3), t41 = tval(V, 4, 1), t24 = tval(V, 2, 4), t31 = tval(V, 3, 1);
+            const C<T> t34 = tval(V, 3, 4), t21 = tval(V, 2, 1);
+            const C<T> Y1 = xmul<true>(t23, t41), Y2 = xmul<true>(t24, t31), Y3 = xmul<true>(t34, t21);
+            const C<T> d1 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y1))));
+            const C<T> d2 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y2))));
+            const C<T> d3 = xmul<true>(bbar, cconj(cscale(two, xmul<true>(P.ns01, Y3))));
+            D.tb[2][3] = caddv(D.tb[2][3], xmul<true>(d1, cconj(t41)));
+            D.tb[4][1] = caddv(D.tb[4][1], xmul<true>(d1, cconj(t23)));
+            D.tb[2][4] = caddv(D.tb[2][4], xmul<true>(d2, cconj(t31)));
+            D.tb[3][1] = caddv(D.tb[3][1], xmul<true>(d2, cconj(t24)));
+            D.tb[3][4] = caddv(D.tb[3][4], xmul<true>(d3, cconj(t21)));
+            D.tb[2][1] = caddv(D.tb[2][1], xmul<true>(d3, cconj(t34)));
         }
-    // ---- reverse: spinor products -> sorted photon momenta
-    T xxb[3] = {(T)0, (T)0, (T)0};
-    C<T> fxb[3];
+        // fold onto the independent products: s(j,i) = -s(i,j), t(i,j) = -conj(s(i,j)),
+        // t(j,i) = conj(s(i,j))
+        C<T> g0[3], g1[3], gkl[3][3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        // s0k = f0 xx_k - fx_k xx0 ; s1k = f1 xx_k - fx_k xx1
-        xxb[k] = xxb[k] + cdot(g0[k], f0) + cdot(g1[k], f1);
-        fxb[k] = caddv(cscale(-P.xx0, g0[k]), cscale(-P.xx1, g1[k]));
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = k + 1; l < 3; ++l) {
-            // skl = fx_k xx_l - fx_l xx_k
-            fxb[k] = caddv(fxb[k], cscale(xx[l], gkl[k][l]));
-            fxb[l] = caddv(fxb[l], cscale(-xx[k], gkl[k][l]));
-            xxb[l] = xxb[l] + cdot(gkl[k][l], fx[k]);
-            xxb[k] = xxb[k] - cdot(gkl[k][l], fx[l]);
+        for (int k = 0; k < 3; ++k) {
+            const int K = 2 + k;
+            g0[k] = caddv(caddv(D.sb[0][K], neg(D.sb[K][0])), caddv(neg(cconj(D.tb[0][K])), cconj(D.tb[K][0])));
+            g1[k] = caddv(caddv(D.sb[1][K], neg(D.sb[K][1])), caddv(neg(cconj(D.tb[1][K])), cconj(D.tb[K][1])));
         }
-    T pb[3][4]; // adjoint of the sorted momenta
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        pb[k][0] = pb[k][1] = pb[k][2] = pb[k][3] = (T)0;
-        if (small[k]) {
-            pb[k][3] = pb[k][3] + fxb[k].re / fx[k].re;
-        } else {
-            pb[k][0] = fxb[k].re / xx[k];
-            pb[k][1] = fxb[k].im / xx[k];
-            xxb[k] = xxb[k] - (fxb[k].re * fx[k].re + fxb[k].im * fx[k].im) / xx[k];
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = k + 1; l < 3; ++l) {
+                const int K = 2 + k, L = 2 + l;
+                gkl[k][l] =
+                    caddv(caddv(D.sb[K][L], neg(D.sb[L][K])), caddv(neg(cconj(D.tb[K][L])), cconj(D.tb[L][K])));
+            }
+        // ---- reverse: spinor products -> sorted photon momenta
+        T xxb[3] = {(T)0, (T)0, (T)0};
+        C<T> fxb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            // s0k = f0 xx_k - fx_k xx0 ; s1k = f1 xx_k - fx_k xx1
+            xxb[k] = xxb[k] + cdot(g0[k], f0) + cdot(g1[k], f1);
+            fxb[k] = caddv(cscale(-P.xx0, g0[k]), cscale(-P.xx1, g1[k]));
         }
-        // sqrt(E + Z) has no derivative at a photon exactly along -Z
-        const T h = small[k] ? (T)0 : xxb[k] / (two * xx[k]);
-        pb[k][3] = pb[k][3] + h;
-        pb[k][2] = pb[k][2] + h;
-    }
-    if (SOFT) { // fS * dlogS/dp of the sorted momenta
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = k + 1; l < 3; ++l) {
+                // skl = fx_k xx_l - fx_l xx_k
+                fxb[k] = caddv(fxb[k], cscale(xx[l], gkl[k][l]));
+                fxb[l] = caddv(fxb[l], cscale(-xx[k], gkl[k][l]));
+                xxb[l] = xxb[l] + cdot(gkl[k][l], fx[k]);
+                xxb[k] = xxb[k] - cdot(gkl[k][l], fx[l]);
+            }
+        T pb[3][4]; // adjoint of the sorted momenta
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            pb[k][0] = pb[k][1] = pb[k][2] = pb[k][3] = (T)0;
+            if (small[k]) {
+                pb[k][3] = pb[k][3] + fxb[k].re / fx[k].re;
+            } else {
+                pb[k][0] = fxb[k].re / xx[k];
+                pb[k][1] = fxb[k].im / xx[k];
+                xxb[k] = xxb[k] - (fxb[k].re * fx[k].re + fxb[k].im * fx[k].im) / xx[k];
+            }
+            // sqrt(E + Z) has no derivative at a photon exactly along -Z
+            const T h = small[k] ? (T)0 : xxb[k] / (two * xx[k]);
+            pb[k][3] = pb[k][3] + h;
+            pb[k][2] = pb[k][2] + h;
+        }
+        if (SOFT) { // fS * dlogS/dp of the sorted momenta
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const T E = ps[i][3];
+                pb[i][3] = pb[i][3] + fS * gE[i] / P.e_min;
+                pb[i][0] = pb[i][0] - fS * gB[i] * ((ps[i][0] >= (T)0 ? one : -one) / E);
+                pb[i][3] = pb[i][3] + fS * gB[i] * (fabs(ps[i][0]) / (E * E));
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
+                const T EE = ps[i][3] * ps[j][3];
+                const T cs = ((ps[i][0] * ps[j][0] + ps[i][1] * ps[j][1]) + ps[i][2] * ps[j][2]) / EE;
+                const T g = fS * gP[k];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T pic = ps[i][c], pjc = ps[j][c];
+                    pb[i][c] = pb[i][c] - g * pjc / EE;
+                    pb[j][c] = pb[j][c] - g * pic / EE;
+                }
+                pb[i][3] = pb[i][3] + g * cs / ps[i][3];
+                pb[j][3] = pb[j][3] + g * cs / ps[j][3];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                dp[4 * k + j] = P.w * pb[k][j];
+        // ---- reverse: sort
+        T pbu[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                pbu[i][j] = perm[0] == i ? pb[0][j] : (perm[1] == i ? pb[1][j] : pb[2][j]);
+        // ---- reverse: conformal map
+        T qb[3][4], Rb[4] = {(T)0, (T)0, (T)0, (T)0};
+        T alb = (T)0, rnb = (T)0, beb = (T)0;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const T E = ps[i][3];
-            pb[i][3] = pb[i][3] + fS * gE[i] / P.e_min;
-            pb[i][0] = pb[i][0] - fS * gB[i] * ((ps[i][0] >= (T)0 ? one : -one) / E);
-            pb[i][3] = pb[i][3] + fS * gB[i] * (fabs(ps[i][0]) / (E * E));
-        }
+            T bb = (T)0, rqb = (T)0;
+            qb[i][0] = qb[i][1] = qb[i][2] = qb[i][3] = (T)0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int i = k == 2 ? 1 : 0, j = k == 0 ? 1 : 2;
-            const T EE = ps[i][3] * ps[j][3];
-            const T cs = ((ps[i][0] * ps[j][0] + ps[i][1] * ps[j][1]) + ps[i][2] * ps[j][2]) / EE;
-            const T g = fS * gP[k];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T pic = ps[i][c], pjc = ps[j][c];
-                pb[i][c] = pb[i][c] - g * pjc / EE;
-                pb[j][c] = pb[j][c] - g * pic / EE;
+            for (int k = 0; k < 3; ++k) {
+                alb = alb + pbu[i][k] * (rn * q[i][k] + b[i] * R[k]);
+                const T t = alpha * pbu[i][k];
+                rnb = rnb + t * q[i][k];
+                qb[i][k] = qb[i][k] + t * rn;
+                bb = bb + t * R[k];
+                Rb[k] = Rb[k] + t * b[i];
             }
-            pb[i][3] = pb[i][3] + g * cs / ps[i][3];
-            pb[j][3] = pb[j][3] + g * cs / ps[j][3];
+            alb = alb + pbu[i][3] * (R[3] * q[i][3] - rq[i]);
+            const T t = alpha * pbu[i][3];
+            Rb[3] = Rb[3] + t * q[i][3];
+            qb[i][3] = qb[i][3] + t * R[3];
+            rqb = rqb - t;
+            beb = beb + bb * rq[i];
+            rqb = rqb + beta * bb;
+            qb[i][3] = qb[i][3] - bb;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                qb[i][k] = qb[i][k] + rqb * R[k];
+                Rb[k] = Rb[k] + rqb * q[i][k];
+            }
         }
-    }
+        const T db = -beb * beta * beta;
+        rnb = rnb + db;
+        Rb[3] = Rb[3] + db;
+        T rn2b = rnb / (two * rn);
+        rn2b = rn2b - alb * alpha / rn2;
+        Rb[3] = Rb[3] + two * R[3] * rn2b;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 3; ++k)
+            Rb[k] = Rb[k] - two * R[k] * rn2b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            dp[4 * k + j] = P.w * pb[k][j];
-    // ---- reverse: sort
-    T pbu[3][4];
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 4; ++k)
+                qb[i][k] = qb[i][k] + Rb[k];
+        // ---- reverse: generation
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            pbu[i][j] = perm[0] == i ? pb[0][j] : (perm[1] == i ? pb[1][j] : pb[2][j]);
-    // ---- reverse: conformal map
-    T qb[3][4], Rb[4] = {(T)0, (T)0, (T)0, (T)0};
-    T alb = (T)0, rnb = (T)0, beb = (T)0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        T bb = (T)0, rqb = (T)0;
-        qb[i][0] = qb[i][1] = qb[i][2] = qb[i][3] = (T)0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            alb = alb + pbu[i][k] * (rn * q[i][k] + b[i] * R[k]);
-            const T t = alpha * pbu[i][k];
-            rnb = rnb + t * q[i][k];
-            qb[i][k] = qb[i][k] + t * rn;
-            bb = bb + t * R[k];
-            Rb[k] = Rb[k] + t * b[i];
+        for (int i = 0; i < 3; ++i) {
+            const T enb = qb[i][0] * (st[i] * sp[i]) + qb[i][1] * (st[i] * cp[i]) + qb[i][2] * ct[i] + qb[i][3];
+            const T stb = en[i] * (qb[i][0] * sp[i] + qb[i][1] * cp[i]);
+            const T spb = en[i] * st[i] * qb[i][0];
+            const T cpb = en[i] * st[i] * qb[i][1];
+            T ctb = en[i] * qb[i][2];
+            if (st[i] > (T)0)
+                ctb = ctb - stb * ct[i] / st[i];
+            const T phb = spb * cp[i] - cpb * sp[i];
+            const T emb = -enb / (em[i] + minpos<T>());
+            du[4 * i] = P.w * (two * ctb);
+            du[4 * i + 1] = P.w * (two_pi * phb);
+            du[4 * i + 2] = P.w * (emb * u[4 * i + 3]);
+            du[4 * i + 3] = P.w * (emb * u[4 * i + 2]);
         }
-        alb = alb + pbu[i][3] * (R[3] * q[i][3] - rq[i]);
-        const T t = alpha * pbu[i][3];
-        Rb[3] = Rb[3] + t * q[i][3];
-        qb[i][3] = qb[i][3] + t * R[3];
-        rqb = rqb - t;
-        beb = beb + bb * rq[i];
-        rqb = rqb + beta * bb;
-        qb[i][3] = qb[i][3] - bb;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            qb[i][k] = qb[i][k] + rqb * R[k];
-            Rb[k] = Rb[k] + rqb * q[i][k];
-        }
-    }
-    const T db = -beb * beta * beta;
-    rnb = rnb + db;
-    Rb[3] = Rb[3] + db;
-    T rn2b = rnb / (two * rn);
-    rn2b = rn2b - alb * alpha / rn2;
-    Rb[3] = Rb[3] + two * R[3] * rn2b;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        Rb[k] = Rb[k] - two * R[k] * rn2b;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            qb[i][k] = qb[i][k] + Rb[k];
-    // ---- reverse: generation
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const T enb = qb[i][0] * (st[i] * sp[i]) + qb[i][1] * (st[i] * cp[i]) + qb[i][2] * ct[i] + qb[i][3];
-        const T stb = en[i] * (qb[i][0] * sp[i] + qb[i][1] * cp[i]);
-        const T spb = en[i] * st[i] * qb[i][0];
-        const T cpb = en[i] * st[i] * qb[i][1];
-        T ctb = en[i] * qb[i][2];
-        if (st[i] > (T)0)
-            ctb = ctb - stb * ct[i] / st[i];
-        const T phb = spb * cp[i] - cpb * sp[i];
-        const T emb = -enb / (em[i] + minpos<T>());
-        du[4 * i] = P.w * (two * ctb);
-        du[4 * i + 1] = P.w * (two_pi * phb);
-        du[4 * i + 2] = P.w * (emb * u[4 * i + 3]);
-        du[4 * i + 3] = P.w * (emb * u[4 * i + 2]);
+        sink(sw, m, du, dp);
     }
     return true;
+}
+
+template <class T, int SOFT = 0>
+__device__ __forceinline__ bool event_adjoint(const AdjParams<T> &P, const T u[12], const T lam[5], T m[5], T du[12],
+                                              T dp[12], T tau = (T)0, T fcap = (T)INFINITY)
+{
+    return event_adjoint_sweeps<T, SOFT, 1>(P, u, lam, m, du, dp, tau, fcap, [](int, const T *, const T *, const T *) {});
 }
 
 } // namespace tpd

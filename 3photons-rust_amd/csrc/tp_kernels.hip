@@ -870,22 +870,21 @@ template <class T> __device__ __forceinline__ void adjoint_body(const AdjArgs &a
 #pragma unroll
         for (int j = 0; j < 12; ++j)
             u[j] = a.u_in ? (T)a.u_in[12 * i + j] : (T)xo_uniform(s);
-        for (int c = 0; c < 5; ++c) {
-            T lam[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};
-            lam[c] = (T)1;
-            T m[5], du[12], dp[12];
-            event_adjoint<T>(P, u, lam, m, du, dp);
-            if (F)
-                F[5 * i + c] = P.w * m[c];
-            if (Gu)
+        // one forward pass, then one reverse sweep per coupling component
+        T m[5], du[12], dp[12];
+        event_adjoint_sweeps<T, 0, 5>(P, u, nullptr, m, du, dp, (T)0, (T)INFINITY,
+                                      [&](int c, const T *mc, const T *duc, const T *dpc) {
+                                          if (F)
+                                              F[5 * i + c] = P.w * (c == 0 ? mc[0] : c == 1 ? mc[1] : c == 2 ? mc[2] : c == 3 ? mc[3] : mc[4]);
+                                          if (Gu)
 #pragma unroll
-                for (int j = 0; j < 12; ++j)
-                    Gu[60 * i + 12 * c + j] = du[j];
-            if (Gp)
+                                              for (int j = 0; j < 12; ++j)
+                                                  Gu[60 * i + 12 * c + j] = duc[j];
+                                          if (Gp)
 #pragma unroll
-                for (int j = 0; j < 12; ++j)
-                    Gp[60 * i + 12 * c + j] = dp[j];
-        }
+                                              for (int j = 0; j < 12; ++j)
+                                                  Gp[60 * i + 12 * c + j] = dpc[j];
+                                      });
     }
 }
 
