@@ -888,9 +888,9 @@ template <class T> __device__ __forceinline__ void adjoint_body(const AdjArgs &a
     }
 }
 
-// f64: 256 VGPRs + ~208 AGPRs, one wavefront per SIMD (two would spill
-// 768 B per lane and run 60 % slower).  f32: 256 VGPRs + 3 AGPRs would also
-// leave one; asking for two costs a 16-byte spill and runs 31 % faster.
+// f64: 255 VGPRs + 240 AGPRs, one wavefront per SIMD (two would spill
+// hundreds of bytes per lane).  f32: 256 VGPRs + 20 AGPRs would also leave
+// one; asking for two costs a 72-byte spill and runs 19 % faster.
 __global__ __launch_bounds__(128) void adjoint_kernel_f64(AdjArgs a) { adjoint_body<double>(a); }
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void adjoint_kernel_f32(AdjArgs a)
 {
