@@ -73,8 +73,11 @@ void launch_flow_forward(const FlowKernelArgs &a, hipStream_t st);
 // log q (and 1/q_mix) of given or uniformly drawn points.
 void launch_flow_inverse(const FlowKernelArgs &a, hipStream_t st);
 void launch_flow_backward(const FlowBackArgs &a, hipStream_t st);
-// Weight and bias gradients over the batch: grad[L][LP] = (1/scale) sum_n ...
-void launch_flow_wgrad(const uint16_t *save, uint64_t n, float inv_n, float *grad, hipStream_t st);
+// Weight and bias gradients over the batch (n a multiple of 16):
+// grad[L][LP] = inv_n sum_n ..., through a partial-sum buffer `part` of
+// flow_wgrad_part_bytes(n) bytes.
+size_t flow_wgrad_part_bytes(uint64_t n);
+void launch_flow_wgrad(const uint16_t *save, uint64_t n, float inv_n, float *part, float *grad, hipStream_t st);
 void launch_flow_adam(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr, float beta1,
                       float beta2, float eps, hipStream_t st);
 void launch_flow_to_bf16(const float *w, uint16_t *wbf, hipStream_t st);

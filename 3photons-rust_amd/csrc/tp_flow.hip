@@ -203,7 +203,10 @@ __device__ __forceinline__ double sigmoid_u(float z)
 
 // ---- forward ----------------------------------------------------------------
 // One 256-thread block takes 4 waves x TILES tiles x 16 samples per pass.
-constexpr int FWD_TILES = 8, BWD_TILES = 4;
+// Tiles per wavefront: 8 for long runs; batches below FWD_SMALL_N samples
+// take FWD_TILES_SMALL so that the grid still covers every CU.
+constexpr int FWD_TILES = 8, FWD_TILES_SMALL = 2, BWD_TILES = 4;
+constexpr uint64_t FWD_SMALL_N = 1u << 20;
 
 template <int TILES>
 __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
@@ -477,93 +480,148 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
     }
 }
 
-// ---- batch weight gradients -------------------------------------------------
-// dW[m][k] = inv_n sum_n delta[m][n] act[k][n] on MFMA with the sample index
-// as the reduction dimension; one 256-thread block per 16x16 output tile,
-// each wavefront over a quarter of the samples, partial tiles combined in a
-// fixed order.  Tiles: per layer 4 (W1), 16 (W2), 8 (W3).
-struct WTile {
-    int delta_row, act_row, param_off, ld, mb, kb;
+// ---- batch weight and bias gradients -----------------------------------------
+// dW[m][k] = inv_n sum_n delta[m][n] act[k][n] and db[m] = inv_n sum_n delta[m][n]
+// on MFMA, with the sample index as the reduction dimension.  One 256-thread
+// block per (layer, sample range): chunks of 64 samples of all 304 saved rows
+// are staged through LDS (the next chunk is loaded into registers while the
+// current one is multiplied), so every saved value is read from HBM once.
+// The layer's 38 output tiles -- 28 weight tiles and 10 bias tiles, whose
+// B operand is a column of ones -- are spread over the four wavefronts.  Each
+// block writes partial sums for all of its layer's parameters, and
+// flow_grad_reduce_kernel adds the partials of each parameter in block order,
+// so the result does not depend on scheduling.
+constexpr int GR_CHUNK = 64, GR_STRIDE = 68, GR_TILES = 38, GR_PER_WAVE = 10;
+constexpr int GR_PIECES = SAVE_F * (GR_CHUNK / 8), GR_LOADS = (GR_PIECES + 255) / 256;
+constexpr int GR_MIN_CHUNKS = 8, GR_MAX_SPLIT = 128;
+
+struct GTile {
+    int delta_row, act_row, param_off, ld; // act_row < 0: bias tile (ones operand)
 };
 
-__device__ __forceinline__ WTile wtile(int t)
+// Tiles of one layer: W1 4 row blocks, W2 4 x 4, W3 2 x 4, then b1 4, b2 4,
+// b3 2 row blocks.  param_off is the parameter of the tile's element (0, 0).
+__device__ __forceinline__ GTile gtile(int t)
 {
-    WTile w;
-    if (t < 4) {
-        w = {SAVE_H1B, SAVE_A, OFF_W1, DP, t, 0};
-    } else if (t < 20) {
-        const int u = t - 4;
-        w = {SAVE_H2B, SAVE_H1, OFF_W2, H, u / 4, u % 4};
-    } else {
-        const int u = t - 20;
-        w = {SAVE_OB, SAVE_H2, OFF_W3, H, u / 4, u % 4};
+    if (t < 4)
+        return {SAVE_H1B + 16 * t, SAVE_A, OFF_W1 + 16 * t * DP, DP};
+    if (t < 20) {
+        const int mb = (t - 4) / 4, kb = (t - 4) % 4;
+        return {SAVE_H2B + 16 * mb, SAVE_H1 + 16 * kb, OFF_W2 + 16 * mb * H + 16 * kb, H};
     }
-    return w;
+    if (t < 28) {
+        const int mb = (t - 20) / 4, kb = (t - 20) % 4;
+        return {SAVE_OB + 16 * mb, SAVE_H2 + 16 * kb, OFF_W3 + 16 * mb * H + 16 * kb, H};
+    }
+    if (t < 32)
+        return {SAVE_H1B + 16 * (t - 28), -1, OFF_B1 + 16 * (t - 28), 0};
+    if (t < 36)
+        return {SAVE_H2B + 16 * (t - 32), -1, OFF_B2 + 16 * (t - 32), 0};
+    return {SAVE_OB + 16 * (t - 36), -1, OFF_B3 + 16 * (t - 36), 0};
 }
 
-__global__ __launch_bounds__(256) void flow_wgrad_kernel(const uint16_t *save, uint64_t n, float inv_n, float *grad)
+// Sample-range blocks per layer for n samples.
+static int wgrad_split(uint64_t n)
 {
-    __shared__ f4 part[4][64];
-    const int layer = blockIdx.x / 28, tt = blockIdx.x % 28;
-    const WTile w = wtile(tt);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t chunks = (n + GR_CHUNK - 1) / GR_CHUNK;
+    const uint64_t s = (chunks + GR_MIN_CHUNKS - 1) / GR_MIN_CHUNKS;
+    return (int)(s < 1 ? 1 : s > GR_MAX_SPLIT ? GR_MAX_SPLIT : s);
+}
+
+__global__ __launch_bounds__(256) void flow_wgrad_kernel(const uint16_t *save, uint64_t n, int split, float *part)
+{
+    __shared__ uint16_t lds[SAVE_F * GR_STRIDE];
+    const int layer = blockIdx.x / split, ks = blockIdx.x % split;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, r16 = lane & 15;
+    const uint64_t chunks = (n + GR_CHUNK - 1) / GR_CHUNK, per = (chunks + split - 1) / split;
+    const uint64_t c0 = (uint64_t)ks * per, c1 = c0 + per < chunks ? c0 + per : chunks;
     const uint16_t *sv = save + (size_t)layer * SAVE_F * n;
-    const uint16_t *drow = sv + (size_t)(w.delta_row + 16 * w.mb + (lane & 15)) * n;
-    const uint16_t *arow = sv + (size_t)(w.act_row + 16 * w.kb + (lane & 15)) * n;
-    f4 c = {0.f, 0.f, 0.f, 0.f};
-    const uint64_t per = ((n / 16 + 3) / 4) * 16;
-    const uint64_t n0 = wave * per, n1 = n0 + per < n ? n0 + per : n;
-    for (uint64_t s0 = n0; s0 + 16 <= n1; s0 += 16) {
-        const uint64_t col = s0 + 4 * (lane >> 4);
-        const s4 A = *reinterpret_cast<const s4 *>(drow + col);
-        const s4 B = *reinterpret_cast<const s4 *>(arow + col);
-        c = mfma(A, B, c);
-    }
-    part[wave][lane] = c;
-    __syncthreads();
-    if (wave == 0) {
-        f4 t = part[0][lane];
+    // piece p = 8 samples (16 B) of row p / 8; n is a multiple of 16, so a
+    // piece lies wholly inside or wholly past the batch
+    uint4 buf[GR_LOADS];
+    auto load = [&](uint64_t c) {
 #pragma unroll
-        for (int q = 1; q < 4; ++q)
-            t += part[q][lane];
-        float *gl = grad + (size_t)layer * LP + w.param_off;
+        for (int k = 0; k < GR_LOADS; ++k) {
+            const int p = threadIdx.x + 256 * k;
+            const uint64_t smp = c * GR_CHUNK + 8 * (p & 7);
+            buf[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (p < GR_PIECES && smp < n)
+                buf[k] = *reinterpret_cast<const uint4 *>(sv + (size_t)(p >> 3) * n + smp);
+        }
+    };
+    f4 acc[GR_PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < GR_PER_WAVE; ++j)
+        acc[j] = (f4){0.f, 0.f, 0.f, 0.f};
+    const short one = (short)0x3F80; // bf16 1.0
+    const s4 ones = {one, one, one, one};
+    if (c0 < c1)
+        load(c0);
+    for (uint64_t c = c0; c < c1; ++c) {
+        __syncthreads(); // the previous chunk's products are done
+#pragma unroll
+        for (int k = 0; k < GR_LOADS; ++k) {
+            const int p = threadIdx.x + 256 * k;
+            if (p < GR_PIECES) {
+                uint2 *d = reinterpret_cast<uint2 *>(lds + (p >> 3) * GR_STRIDE + 8 * (p & 7));
+                d[0] = make_uint2(buf[k].x, buf[k].y);
+                d[1] = make_uint2(buf[k].z, buf[k].w);
+            }
+        }
+        __syncthreads();
+        if (c + 1 < c1)
+            load(c + 1);
+#pragma unroll
+        for (int kk = 0; kk < GR_CHUNK / 16; ++kk)
+#pragma unroll
+            for (int j = 0; j < GR_PER_WAVE; ++j) {
+                const int t = wave + 4 * j;
+                if (t < GR_TILES) {
+                    const GTile w = gtile(t);
+                    const int col = 16 * kk + 4 * g;
+                    const s4 A = *reinterpret_cast<const s4 *>(lds + (w.delta_row + r16) * GR_STRIDE + col);
+                    const s4 B =
+                        w.act_row < 0 ? ones : *reinterpret_cast<const s4 *>(lds + (w.act_row + r16) * GR_STRIDE + col);
+                    acc[j] = mfma(A, B, acc[j]);
+                }
+            }
+    }
+    float *pl = part + ((size_t)ks * L + layer) * LP;
+#pragma unroll
+    for (int j = 0; j < GR_PER_WAVE; ++j) {
+        const int t = wave + 4 * j;
+        if (t >= GR_TILES)
+            continue;
+        const GTile w = gtile(t);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = 16 * w.mb + 4 * (lane >> 4) + i, colk = 16 * w.kb + (lane & 15);
-            gl[row * w.ld + colk] = t[i] * inv_n;
+            const int row = 4 * g + i; // C layout: lane holds rows 4g.. of column r16
+            if (w.act_row >= 0)
+                pl[w.param_off + row * w.ld + r16] = acc[j][i];
+            else if (r16 == 0)
+                pl[w.param_off + row] = acc[j][i];
         }
     }
 }
 
-// Bias gradients: sums over samples of one delta row, fixed-shape reduction.
-__global__ __launch_bounds__(256) void flow_bgrad_kernel(const uint16_t *save, uint64_t n, float inv_n, float *grad)
+// grad[i] = inv_n sum over the sample-range blocks, in block order (loads
+// in groups of 16 so that they are in flight together).
+__global__ __launch_bounds__(256) void flow_grad_reduce_kernel(const float *part, int split, float inv_n, float *grad)
 {
-    __shared__ float red[256];
-    const int layer = blockIdx.x / (H + H + O), r = blockIdx.x % (H + H + O);
-    int row, off;
-    if (r < H) {
-        row = SAVE_H1B + r;
-        off = OFF_B1 + r;
-    } else if (r < 2 * H) {
-        row = SAVE_H2B + (r - H);
-        off = OFF_B2 + (r - H);
-    } else {
-        row = SAVE_OB + (r - 2 * H);
-        off = OFF_B3 + (r - 2 * H);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM)
+        return;
+    float sum = 0.f;
+    for (int k0 = 0; k0 < split; k0 += 16) {
+        float v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            v[j] = k0 + j < split ? part[(size_t)(k0 + j) * NPARAM + i] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            sum += v[j];
     }
-    const uint16_t *d = save + (size_t)layer * SAVE_F * n + (size_t)row * n;
-    float acc = 0.f;
-    for (uint64_t i = threadIdx.x; i < n; i += 256)
-        acc += bf2f(d[i]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        grad[(size_t)layer * LP + off] = red[0] * inv_n;
+    grad[i] = sum * inv_n;
 }
 
 __global__ void flow_adam_kernel(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr,
@@ -595,12 +653,20 @@ template <int TILES> static unsigned grid_for(uint64_t n) { return (unsigned)((n
 
 void launch_flow_forward(const FlowKernelArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
+    if (a.n < FWD_SMALL_N)
+        hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES_SMALL>, dim3(grid_for<FWD_TILES_SMALL>(a.n)), dim3(256), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
 }
 
 void launch_flow_inverse(const FlowKernelArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
+    if (a.n < FWD_SMALL_N)
+        hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES_SMALL>, dim3(grid_for<FWD_TILES_SMALL>(a.n)), dim3(256), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
 }
 
 void launch_flow_backward(const FlowBackArgs &a, hipStream_t st)
@@ -608,10 +674,14 @@ void launch_flow_backward(const FlowBackArgs &a, hipStream_t st)
     hipLaunchKernelGGL(flow_backward_kernel<BWD_TILES>, dim3(grid_for<BWD_TILES>(a.n)), dim3(256), 0, st, a);
 }
 
-void launch_flow_wgrad(const uint16_t *save, uint64_t n, float inv_n, float *grad, hipStream_t st)
+size_t flow_wgrad_part_bytes(uint64_t n) { return (size_t)wgrad_split(n) * NPARAM * sizeof(float); }
+
+void launch_flow_wgrad(const uint16_t *save, uint64_t n, float inv_n, float *part, float *grad, hipStream_t st)
 {
-    hipLaunchKernelGGL(flow_wgrad_kernel, dim3(L * 28), dim3(256), 0, st, save, n, inv_n, grad);
-    hipLaunchKernelGGL(flow_bgrad_kernel, dim3(L * (H + H + O)), dim3(256), 0, st, save, n, inv_n, grad);
+    const int split = wgrad_split(n);
+    hipLaunchKernelGGL(flow_wgrad_kernel, dim3(L * split), dim3(256), 0, st, save, n, split, part);
+    hipLaunchKernelGGL(flow_grad_reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, part, split, inv_n,
+                       grad);
 }
 
 void launch_flow_adam(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr, float beta1,
@@ -683,17 +753,25 @@ void launch_flow_integrand(const tpd::Params &P, const double lam[5], uint64_t n
                        lam[2], lam[3], lam[4], n, u, eps, tau, fcap, target, clip, gz, logf, f_out);
 }
 
-// Mean over the batch of log q - log(f + eps), one 256-thread block, fixed
-// summation order (per-thread strided sums, then a tree).
-__global__ __launch_bounds__(256) void flow_loss_kernel(const float *logq, const float *logf, uint64_t n, float *out)
+// Mean over the batch of log q - log(f + eps), one 1024-thread block, fixed
+// summation order (per-thread strided sums of 4-wide loads, then a tree).
+__global__ __launch_bounds__(1024) void flow_loss_kernel(const float *logq, const float *logf, uint64_t n, float *out)
 {
-    __shared__ double red[256];
+    __shared__ double red[1024];
     double acc = 0.;
-    for (uint64_t i = threadIdx.x; i < n; i += 256)
+    const uint64_t n4 = n / 4;
+    const float4 *q4 = reinterpret_cast<const float4 *>(logq), *f4p = reinterpret_cast<const float4 *>(logf);
+#pragma unroll 4
+    for (uint64_t i = threadIdx.x; i < n4; i += 1024) {
+        const float4 q = q4[i], f = f4p[i];
+        acc += (((double)q.x - (double)f.x) + ((double)q.y - (double)f.y)) +
+               (((double)q.z - (double)f.z) + ((double)q.w - (double)f.w));
+    }
+    for (uint64_t i = 4 * n4 + threadIdx.x; i < n; i += 1024)
         acc += (double)logq[i] - (double)logf[i];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = 512; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o)
             red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
@@ -704,7 +782,7 @@ __global__ __launch_bounds__(256) void flow_loss_kernel(const float *logq, const
 
 void launch_flow_loss(const float *logq, const float *logf, uint64_t n, float *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(flow_loss_kernel, dim3(1), dim3(256), 0, st, logq, logf, n, out);
+    hipLaunchKernelGGL(flow_loss_kernel, dim3(1), dim3(1024), 0, st, logq, logf, n, out);
 }
 
 } // namespace tpf

@@ -23,6 +23,7 @@ struct Flow {
     Ctx *c = nullptr;
     DevBuf w, wbf, m, v, grad;            // parameters, bf16 copy, Adam moments, gradient
     DevBuf zsave, save, u, logq, gz, logf; // training batch
+    DevBuf gpart;                         // per-block partial weight gradients
     DevBuf f, loss, vin, io;              // integrand values, per-step loss, staging
     DevBuf ru, rinvq;                     // weighted-run chunk
     int step = 0;                         // Adam steps taken
@@ -220,7 +221,7 @@ extern "C" int tp_flow_destroy(void *flow)
         (void)hipSetDevice(f->c->device);
         (void)hipStreamSynchronize(f->c->stream);
         for (DevBuf *b : {&f->w, &f->wbf, &f->m, &f->v, &f->grad, &f->zsave, &f->save, &f->u, &f->logq, &f->gz, &f->logf,
-                          &f->f, &f->loss, &f->vin, &f->io, &f->ru, &f->rinvq})
+                          &f->gpart, &f->f, &f->loss, &f->vin, &f->io, &f->ru, &f->rinvq})
             if (b->p)
                 (void)hipFree(b->p);
         delete f;
@@ -337,6 +338,8 @@ int ensure_train(Flow *f, uint64_t n)
         rc = ensure(f->c, f->logf, n * sizeof(float));
     if (!rc)
         rc = ensure(f->c, f->f, n * sizeof(double));
+    if (!rc)
+        rc = ensure(f->c, f->gpart, tpf::flow_wgrad_part_bytes(n));
     return rc;
 }
 
@@ -354,8 +357,8 @@ int enqueue_grad(Flow *f, uint64_t n, float *gv)
     b.gv = gv;
     tpf::launch_flow_backward(b, f->c->stream);
     FHIP(f, hipGetLastError());
-    tpf::launch_flow_wgrad(static_cast<const uint16_t *>(f->save.p), n, 1.f / (float)n, static_cast<float *>(f->grad.p),
-                           f->c->stream);
+    tpf::launch_flow_wgrad(static_cast<const uint16_t *>(f->save.p), n, 1.f / (float)n, static_cast<float *>(f->gpart.p),
+                           static_cast<float *>(f->grad.p), f->c->stream);
     FHIP(f, hipGetLastError());
     return TP_OK;
 }
