@@ -43,11 +43,12 @@ __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x
 __device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * LN2; }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sigm(float x) { return frcp(1.f + fexp(-x)); }
-__device__ __forceinline__ float silu(float x) { return x * sigm(x); }
-__device__ __forceinline__ float dsilu(float x)
+// SiLU and its derivative from one sigmoid (the forward-only kernels drop d).
+__device__ __forceinline__ void silu_d(float x, float &y, float &d)
 {
     const float s = sigm(x);
-    return s * (1.f + x * (1.f - s));
+    y = x * s;
+    d = s * (1.f + x * (1.f - s));
 }
 __device__ __forceinline__ float ftanh(float x) { return 1.f - 2.f * frcp(fexp(2.f * x) + 1.f); }
 __device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + flog(1.f + fexp(-fabsf(x))); }
@@ -105,9 +106,9 @@ __device__ __forceinline__ f4 bias4(const LdsLayer &s, int r0) { return *reinter
 
 struct MlpFwd {
     s4 a;          // bf16 input (features 4g..4g+3)
-    f4 h1p[4];     // pre-activations, 4 row blocks
+    f4 h1d[4];     // SiLU'(pre-activation), 4 row blocks (backward only)
     s4 h1[4];
-    f4 h2p[4];
+    f4 h2d[4];
     s4 h2[4];
     f4 o0, o1;     // s raw (rows 0..15), t raw (rows 16..31)
 };
@@ -119,11 +120,15 @@ __device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
         const f4 c = mfma(atile(s.w + L_W1, S_W1, mb, 0), r.a, bias4(s, 16 * mb + 4 * g));
-        r.h1p[mb] = c;
-        f4 h;
+        f4 h, d;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            h[i] = silu(c[i]);
+        for (int i = 0; i < 4; ++i) {
+            float y, dy;
+            silu_d(c[i], y, dy);
+            h[i] = y;
+            d[i] = dy;
+        }
+        r.h1d[mb] = d;
         r.h1[mb] = pack4(h);
     }
 #pragma unroll
@@ -132,11 +137,15 @@ __device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
             c = mfma(atile(s.w + L_W2, S_W2, mb, kb), r.h1[kb], c);
-        r.h2p[mb] = c;
-        f4 h;
+        f4 h, d;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            h[i] = silu(c[i]);
+        for (int i = 0; i < 4; ++i) {
+            float y, dy;
+            silu_d(c[i], y, dy);
+            h[i] = y;
+            d[i] = dy;
+        }
+        r.h2d[mb] = d;
         r.h2[mb] = pack4(h);
     }
     f4 o0 = bias4(s, 2 * H + 4 * g), o1 = bias4(s, 2 * H + 16 + 4 * g);
@@ -401,8 +410,24 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
             for (int t = 0; t < TILES; ++t) {
                 f4 zin = valid[t] ? *reinterpret_cast<const f4 *>(a.zsave + ((size_t)l * a.n + idx[t]) * 16 + 4 * g)
                                   : (f4){0.f, 0.f, 0.f, 0.f};
+                // rows row0 + 4g .. + 3 of this lane's sample, stored as soon as
+                // they are formed (short live ranges)
+                const bool ok = valid[t];
+                const uint64_t n = idx[t], N = a.n;
+                auto put = [&](int row0, s4 v) {
+                    if (ok)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            sv[(size_t)(row0 + 4 * g + i) * N + n] = (uint16_t)v[i];
+                };
                 MlpFwd r;
                 mlp_forward(s, masked(zin, cond), r);
+                put(SAVE_A, r.a);
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb) {
+                    put(SAVE_H1 + 16 * mb, r.h1[mb]);
+                    put(SAVE_H2 + 16 * mb, r.h2[mb]);
+                }
                 f4 ob0, ob1, zb;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -421,6 +446,8 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
                     }
                 }
                 const s4 obb0 = pack4(ob0), obb1 = pack4(ob1);
+                put(SAVE_OB, obb0);
+                put(SAVE_OB + 16, obb1);
                 const f4 zero = {0.f, 0.f, 0.f, 0.f};
                 s4 h2b[4], h1b[4];
 #pragma unroll
@@ -429,8 +456,9 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
                     c = mfma(atile(s.w + L_W3T, S_W3T, mb, 1), obb1, c);
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        c[i] *= dsilu(r.h2p[mb][i]);
+                        c[i] *= r.h2d[mb][i];
                     h2b[mb] = pack4(c);
+                    put(SAVE_H2B + 16 * mb, h2b[mb]);
                 }
 #pragma unroll
                 for (int mb = 0; mb < 4; ++mb) {
@@ -440,8 +468,9 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
                         c = mfma(atile(s.w + L_W2T, S_W2T, mb, kb), h2b[kb], c);
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        c[i] *= dsilu(r.h1p[mb][i]);
+                        c[i] *= r.h1d[mb][i];
                     h1b[mb] = pack4(c);
+                    put(SAVE_H1B + 16 * mb, h1b[mb]);
                 }
                 f4 ab = zero;
 #pragma unroll
@@ -452,24 +481,6 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
                     if (conditioning(cond, 4 * g + i))
                         zb[i] += ab[i];
                 gz[t] = zb;
-                if (valid[t]) {
-                    const uint64_t n = idx[t];
-                    const uint64_t N = a.n;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int row = 4 * g + i;
-                        sv[(SAVE_A + row) * N + n] = (uint16_t)r.a[i];
-                        sv[(SAVE_OB + row) * N + n] = (uint16_t)obb0[i];
-                        sv[(SAVE_OB + 16 + row) * N + n] = (uint16_t)obb1[i];
-#pragma unroll
-                        for (int mb = 0; mb < 4; ++mb) {
-                            sv[(SAVE_H1 + 16 * mb + row) * N + n] = (uint16_t)r.h1[mb][i];
-                            sv[(SAVE_H2 + 16 * mb + row) * N + n] = (uint16_t)r.h2[mb][i];
-                            sv[(SAVE_H2B + 16 * mb + row) * N + n] = (uint16_t)h2b[mb][i];
-                            sv[(SAVE_H1B + 16 * mb + row) * N + n] = (uint16_t)h1b[mb][i];
-                        }
-                    }
-                }
             }
         }
         if (a.gv)
