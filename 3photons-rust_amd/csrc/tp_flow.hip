@@ -212,9 +212,10 @@ __device__ __forceinline__ double sigmoid_u(float z)
 
 // ---- forward ----------------------------------------------------------------
 // One 256-thread block takes 4 waves x TILES tiles x 16 samples per pass.
-// Tiles per wavefront: 8 for long runs; batches below FWD_SMALL_N samples
-// take FWD_TILES_SMALL so that the grid still covers every CU.
-constexpr int FWD_TILES = 8, FWD_TILES_SMALL = 2, BWD_TILES = 4;
+// Tiles per wavefront: 4 for long runs (two wavefronts per SIMD fit; 8 tiles
+// leave one); batches below FWD_SMALL_N samples take FWD_TILES_SMALL so that
+// the grid still covers every CU.
+constexpr int FWD_TILES = 4, FWD_TILES_SMALL = 2, BWD_TILES = 4;
 constexpr uint64_t FWD_SMALL_N = 1u << 20;
 
 template <int TILES>
