@@ -57,7 +57,10 @@ struct Ctx {
 
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws
 constexpr int RANF_R = 2048;        // RANF rounds per expansion segment (one jump per segment)
-constexpr uint64_t RANF_CHUNK_BATCHES = 4096; // 4096 * 120000 int32 = 1.97 GB of stream per launch
+// Batches per RANF expansion + event-loop launch: two full strides of the
+// 3072-wavefront persistent grid (one wavefront per batch), so no launch ends
+// on a partly idle stride; 6144 * 120000 int32 = 2.95 GB of stream.
+constexpr uint64_t RANF_CHUNK_BATCHES = 6144;
 
 // Where tp_guard stores an exception's message for tp_last_error.
 inline std::string *ctx_err(void *ctx) { return ctx ? &static_cast<Ctx *>(ctx)->err : nullptr; }

@@ -35,7 +35,9 @@ std::string *flow_err(void *flow)
     return f && f->c ? &f->c->err : nullptr;
 }
 
-constexpr uint64_t RUN_CHUNK_BATCHES = 1000;           // 1e7 events per chunk: 0.96 GB of u
+// Batches per weighted-run chunk: two full strides of the 3072-wavefront
+// event-loop grid (6.1e7 events: 5.9 GB of u).
+constexpr uint64_t RUN_CHUNK_BATCHES = 6144;
 constexpr uint64_t UNIFORM_KEY_XOR = 0x5555555555555555ull;
 
 int ffail(Flow *f, int code, const std::string &msg) { return fail(f ? f->c : nullptr, code, msg); }

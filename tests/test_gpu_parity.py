@@ -108,6 +108,27 @@ def test_run_per_batch_matches_oracle(engine, cfg, oracle, valeurs, rng, jump, n
 
 
 @pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
+def test_batches_past_the_first_chunk_match_oracle(engine, cfg, oracle, valeurs, rng):
+    """N = 7e7 (7000 batches): RANF runs are expanded and looped in chunks of
+    6144 batches, and the xoshiro grid (3072 wavefronts) strides past its
+    first two passes; the batches at the chunk edge and the last one against
+    the oracle started at those batches."""
+    n_events = 70_000_000
+    print(f"chunked run rng={rng} N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c)
+    per, _, _ = engine.run(params, RNG[rng], tp.SEED_SKIPAHEAD, tp.rng_state(RNG[rng]), n_events)
+    ocfg, opts = oracle.config(valeurs, n_events), oracle.opts(rng)  # the weight carries 1/N
+    for b in (0, 3071, 3072, 6143, 6144, 6999):
+        g = oracle.rng(rng)
+        oracle.skip(g, 120000 * b)
+        o = oracle.simulate_batch(ocfg, opts, g, 10000).as_array()
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]], np.float64)
+        acc_close(a, o, 10000)
+
+
+@pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
 def test_simulate_batch_is_the_closure(engine, cfg, oracle, valeurs, rng):
     """Successive calls consume the stream like main.rs:103-128 with &mut rng."""
     params = tp.params_from_config(cfg)
