@@ -57,6 +57,7 @@ struct FlowKernelArgs {
     // mixture bookkeeping for the weighted event loop: q_mix = alpha q + (1 - alpha)
     double alpha;
     double *invq_out;    // [n] 1 / q_mix
+    uint32_t max_blocks; // grid cap of the forward and inverse kernels (the CU count), 0: none
 };
 
 struct FlowBackArgs {

@@ -12,6 +12,8 @@
 #include "tp_adjoint.h"
 #include "tp_xoshiro.h"
 
+#include <algorithm>
+
 // The network code below is not bound to any reference operation order:
 // let the compiler fuse multiply-adds.  (The physics included above keeps
 // the build's contraction-free setting; it is switched back off before the
@@ -63,27 +65,35 @@ struct LdsLayer {
     float b[H + H + O];
 };
 
-__device__ __forceinline__ void load_layer(LdsLayer &s, const uint16_t *wl, bool transposed)
+// Forward weights only (W1, W2, W3 and the biases, same offsets as LdsLayer):
+// the forward and inverse kernels keep all L layers of them resident.
+struct LdsFwd {
+    uint16_t w[L_FWD];
+    float b[H + H + O];
+};
+
+template <class Lds> __device__ __forceinline__ void load_layer(Lds &s, const uint16_t *wl, bool transposed)
 {
-    const float *unused = nullptr;
-    (void)unused;
     for (int i = threadIdx.x; i < H * DP; i += blockDim.x) {
         const int r = i / DP, c = i % DP;
         s.w[L_W1 + r * S_W1 + c] = wl[OFF_W1 + i];
-        if (transposed)
-            s.w[L_W1T + c * S_W1T + r] = wl[OFF_W1 + i];
+        if constexpr (sizeof(s.w) > sizeof(uint16_t) * L_FWD)
+            if (transposed)
+                s.w[L_W1T + c * S_W1T + r] = wl[OFF_W1 + i];
     }
     for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
         const int r = i / H, c = i % H;
         s.w[L_W2 + r * S_W2 + c] = wl[OFF_W2 + i];
-        if (transposed)
-            s.w[L_W2T + c * S_W2T + r] = wl[OFF_W2 + i];
+        if constexpr (sizeof(s.w) > sizeof(uint16_t) * L_FWD)
+            if (transposed)
+                s.w[L_W2T + c * S_W2T + r] = wl[OFF_W2 + i];
     }
     for (int i = threadIdx.x; i < O * H; i += blockDim.x) {
         const int r = i / H, c = i % H;
         s.w[L_W3 + r * S_W3 + c] = wl[OFF_W3 + i];
-        if (transposed)
-            s.w[L_W3T + c * S_W3T + r] = wl[OFF_W3 + i];
+        if constexpr (sizeof(s.w) > sizeof(uint16_t) * L_FWD)
+            if (transposed)
+                s.w[L_W3T + c * S_W3T + r] = wl[OFF_W3 + i];
     }
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
         s.b[i] = bf2f(wl[OFF_B1 + i]);
@@ -102,7 +112,10 @@ __device__ __forceinline__ s4 atile(const uint16_t *m, int stride, int mb, int k
 }
 
 // Bias values of rows r0 .. r0 + 3 (the lane's rows of an output tile).
-__device__ __forceinline__ f4 bias4(const LdsLayer &s, int r0) { return *reinterpret_cast<const f4 *>(s.b + r0); }
+template <class Lds> __device__ __forceinline__ f4 bias4(const Lds &s, int r0)
+{
+    return *reinterpret_cast<const f4 *>(s.b + r0);
+}
 
 struct MlpFwd {
     s4 a;          // bf16 input (features 4g..4g+3)
@@ -113,7 +126,7 @@ struct MlpFwd {
     f4 o0, o1;     // s raw (rows 0..15), t raw (rows 16..31)
 };
 
-__device__ __forceinline__ void mlp_forward(const LdsLayer &s, f4 a_in, MlpFwd &r)
+template <class Lds> __device__ __forceinline__ void mlp_forward(const Lds &s, f4 a_in, MlpFwd &r)
 {
     const int g = (threadIdx.x & 63) >> 4;
     r.a = pack4(a_in);
@@ -211,18 +224,28 @@ __device__ __forceinline__ double sigmoid_u(float z)
 }
 
 // ---- forward ----------------------------------------------------------------
-// One 256-thread block takes 4 waves x TILES tiles x 16 samples per pass.
-// Tiles per wavefront: 4 for long runs (two wavefronts per SIMD fit; 8 tiles
-// leave one); batches below FWD_SMALL_N samples take FWD_TILES_SMALL so that
-// the grid still covers every CU.
-constexpr int FWD_TILES = 4, FWD_TILES_SMALL = 2, BWD_TILES = 4;
+// The forward and inverse kernels load the forward weights of all L layers
+// into LDS once (97.5 KB, one block per CU) and then stride over the samples
+// with no further barrier: one 512-thread block (8 wavefronts, two per SIMD)
+// takes 8 waves x TILES tiles x 16 samples per pass.  Tiles per wavefront: 4
+// for long runs; batches below FWD_SMALL_N samples take FWD_TILES_SMALL so
+// that the grid still covers every CU.
+constexpr int FWD_WAVES = 8, FWD_TILES = 4, FWD_TILES_SMALL = 2, BWD_TILES = 4;
 constexpr uint64_t FWD_SMALL_N = 1u << 20;
 
-template <int TILES>
-__global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
+__device__ __forceinline__ void load_resident(LdsFwd (&s)[L], const uint16_t *wbf)
 {
-    constexpr int CHUNK = 4 * TILES * 16;
-    __shared__ LdsLayer s;
+    for (int l = 0; l < L; ++l)
+        load_layer(s[l], wbf + (size_t)l * LP, false);
+    __syncthreads();
+}
+
+template <int TILES>
+__global__ __launch_bounds__(64 * FWD_WAVES) void flow_forward_kernel(FlowKernelArgs a)
+{
+    constexpr int CHUNK = FWD_WAVES * TILES * 16;
+    __shared__ LdsFwd s[L];
+    load_resident(s, a.wbf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
         f4 z[TILES];
@@ -256,16 +279,13 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
             ld[t] = l;
         }
         for (int l = 0; l < L; ++l) {
-            __syncthreads();
-            load_layer(s, a.wbf + (size_t)l * LP, false);
-            __syncthreads();
             const uint32_t cond = COND_MASK[l];
 #pragma unroll
             for (int t = 0; t < TILES; ++t) {
                 if (a.zsave && valid[t])
                     *reinterpret_cast<f4 *>(a.zsave + ((size_t)l * a.n + idx[t]) * 16 + 4 * g) = z[t];
                 MlpFwd r;
-                mlp_forward(s, masked(z[t], cond), r);
+                mlp_forward(s[l], masked(z[t], cond), r);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) { // branch-free: exp(0) = 1 exactly
                     const bool tr = transformed(cond, 4 * g + i);
@@ -308,10 +328,11 @@ __global__ __launch_bounds__(256) void flow_forward_kernel(FlowKernelArgs a)
 
 // ---- inverse: density of given or uniformly drawn points --------------------
 template <int TILES>
-__global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
+__global__ __launch_bounds__(64 * FWD_WAVES) void flow_inverse_kernel(FlowKernelArgs a)
 {
-    constexpr int CHUNK = 4 * TILES * 16;
-    __shared__ LdsLayer s;
+    constexpr int CHUNK = FWD_WAVES * TILES * 16;
+    __shared__ LdsFwd s[L];
+    load_resident(s, a.wbf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     for (uint64_t base = (uint64_t)blockIdx.x * CHUNK; base < a.n; base += (uint64_t)gridDim.x * CHUNK) {
         f4 z[TILES];
@@ -345,14 +366,11 @@ __global__ __launch_bounds__(256) void flow_inverse_kernel(FlowKernelArgs a)
             ld[t] = l;
         }
         for (int l = L - 1; l >= 0; --l) {
-            __syncthreads();
-            load_layer(s, a.wbf + (size_t)l * LP, false);
-            __syncthreads();
             const uint32_t cond = COND_MASK[l];
 #pragma unroll
             for (int t = 0; t < TILES; ++t) {
                 MlpFwd r;
-                mlp_forward(s, masked(z[t], cond), r);
+                mlp_forward(s[l], masked(z[t], cond), r);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) { // branch-free: exp(0) = 1 exactly
                     const bool tr = transformed(cond, 4 * g + i);
@@ -663,22 +681,33 @@ __global__ void flow_to_bf16_kernel(const float *w, uint16_t *wbf)
 // ---- launchers ---------------------------------------------------------------
 template <int TILES> static unsigned grid_for(uint64_t n) { return (unsigned)((n + 64 * TILES - 1) / (64 * TILES)); }
 
+// Resident-weight kernels: one pass per block covers FWD_WAVES * TILES * 16
+// samples; at most max_blocks blocks (one per CU), striding over the rest.
+template <int TILES> static unsigned grid_resident(const FlowKernelArgs &a)
+{
+    const uint64_t chunk = (uint64_t)FWD_WAVES * TILES * 16, need = (a.n + chunk - 1) / chunk;
+    const uint64_t cap = a.max_blocks ? a.max_blocks : need;
+    return (unsigned)std::max<uint64_t>(1, std::min(need, cap));
+}
+
 void launch_flow_forward(const FlowKernelArgs &a, hipStream_t st)
 {
     if (a.n < FWD_SMALL_N)
-        hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES_SMALL>, dim3(grid_for<FWD_TILES_SMALL>(a.n)), dim3(256), 0,
-                           st, a);
+        hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES_SMALL>, dim3(grid_resident<FWD_TILES_SMALL>(a)),
+                           dim3(64 * FWD_WAVES), 0, st, a);
     else
-        hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(flow_forward_kernel<FWD_TILES>, dim3(grid_resident<FWD_TILES>(a)), dim3(64 * FWD_WAVES), 0,
+                           st, a);
 }
 
 void launch_flow_inverse(const FlowKernelArgs &a, hipStream_t st)
 {
     if (a.n < FWD_SMALL_N)
-        hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES_SMALL>, dim3(grid_for<FWD_TILES_SMALL>(a.n)), dim3(256), 0,
-                           st, a);
+        hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES_SMALL>, dim3(grid_resident<FWD_TILES_SMALL>(a)),
+                           dim3(64 * FWD_WAVES), 0, st, a);
     else
-        hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES>, dim3(grid_for<FWD_TILES>(a.n)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(flow_inverse_kernel<FWD_TILES>, dim3(grid_resident<FWD_TILES>(a)), dim3(64 * FWD_WAVES), 0,
+                           st, a);
 }
 
 void launch_flow_backward(const FlowBackArgs &a, hipStream_t st)

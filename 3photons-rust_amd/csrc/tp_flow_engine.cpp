@@ -128,6 +128,7 @@ tpf::FlowKernelArgs base_args(const Flow *f, uint64_t n)
     a.wbf = static_cast<const uint16_t *>(f->wbf.p);
     a.n = n;
     a.alpha = 1.;
+    a.max_blocks = (uint32_t)f->c->cus;
     return a;
 }
 
