@@ -47,7 +47,8 @@ struct FlowKernelArgs {
     uint64_t key;
     uint64_t sample0;    // stream offset of sample 0, in samples
     // outputs (each may be null)
-    double *u_out;       // [n][12] f64
+    double *u_out;       // [n][12] f64, or [12][u_stride] when u_stride > 0
+    uint64_t u_stride;
     float *z_out;        // [n][12] f32 flow output z = logit(u)
     float *logq_out;     // [n]
     float *zsave;        // [L][n][16] layer inputs (training)

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void flow_forward_kernel(FlowKernel
                 if (j < D) {
                     const double u = sigmoid_u(z[t][i]);
                     if (a.u_out)
-                        a.u_out[12 * idx[t] + j] = u;
+                        a.u_out[a.u_stride ? j * a.u_stride + idx[t] : 12 * idx[t] + j] = u;
                     if (a.z_out)
                         a.z_out[12 * idx[t] + j] = z[t][i];
                 }
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(64 * FWD_WAVES) void flow_inverse_kernel(FlowKernel
                     } else {
                         u = stream_u(a.key, 12ull * (a.sample0 + idx[t]) + j);
                         if (a.u_out)
-                            a.u_out[12 * idx[t] + j] = u;
+                            a.u_out[a.u_stride ? j * a.u_stride + idx[t] : 12 * idx[t] + j] = u;
                     }
                 }
                 // u and 1 - u formed in f64, logs in f32

@@ -583,7 +583,11 @@ extern "C" int tp_flow_run_device(void *flow, const tp_params *params, double al
         fa.num_events = num_events;
         fa.batch_size = bs;
         fa.precision = TP_PREC_F64;
+        // u as 12 rows of the chunk's events (flow samples first, then the
+        // uniform points), the layout the event loop loads contiguously
+        const uint64_t ustride = chunk_b * bs;
         fa.u = static_cast<const double *>(f->ru.p);
+        fa.u_stride = ustride;
         fa.invq = static_cast<const double *>(f->rinvq.p);
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -598,6 +602,7 @@ extern "C" int tp_flow_run_device(void *flow, const tp_params *params, double al
                 a.key = key;
                 a.sample0 = flow_done;
                 a.u_out = static_cast<double *>(f->ru.p);
+                a.u_stride = ustride;
                 a.alpha = alpha;
                 a.invq_out = static_cast<double *>(f->rinvq.p);
                 tpf::launch_flow_forward(a, c->stream);
@@ -607,7 +612,8 @@ extern "C" int tp_flow_run_device(void *flow, const tp_params *params, double al
                 tpf::FlowKernelArgs a = base_args(f, ne - nf);
                 a.key = key ^ UNIFORM_KEY_XOR;
                 a.sample0 = unif_done;
-                a.u_out = static_cast<double *>(f->ru.p) + 12 * nf;
+                a.u_out = static_cast<double *>(f->ru.p) + nf;
+                a.u_stride = ustride;
                 a.alpha = alpha;
                 a.invq_out = static_cast<double *>(f->rinvq.p) + nf;
                 tpf::launch_flow_inverse(a, c->stream);

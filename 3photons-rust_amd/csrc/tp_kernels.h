@@ -29,11 +29,14 @@ struct FusedArgs {
     // buffer source: RANF values (int32 in [0, 1e9)) of this launch's batches,
     // batch (b - b_first) at offset 12*batch_size*(b - b_first).
     const int32_t *draws;
-    // weighted source (importance sampling): uniforms u[12] (f64) and weight
-    // 1/q(u) per event, event e of batch (b - b_first) at index
-    // batch_size*(b - b_first) + e; every |M|^2 is multiplied by its weight.
+    // weighted source (importance sampling): uniforms (f64) and weight 1/q(u)
+    // per event; event i = batch_size*(b - b_first) + e has its uniform j at
+    // u[j * u_stride + i] (one row per uniform, so a wavefront's loads are
+    // contiguous) and its weight at invq[i]; every |M|^2 is multiplied by
+    // its weight.
     const double *u;
     const double *invq;
+    uint64_t u_stride;
     // faster-evgen source (RANF, f64): stream values from position qbase in
     // draws[], event e of batch (b - b_first) starting at stream position
     // evq[batch_size*(b - b_first) + e].

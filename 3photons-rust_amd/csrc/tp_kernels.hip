@@ -238,7 +238,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                     const size_t e = wbase + (size_t)(k * 64 + lane);
 #pragma unroll
                     for (int j = 0; j < 12; ++j)
-                        u[j] = a.u[12 * e + j];
+                        u[j] = a.u[j * a.u_stride + e];
                     wt = a.invq[e];
                 }
                 pass = generate_f64<SOURCE == 0>(fresh_params(), u, p, d);
