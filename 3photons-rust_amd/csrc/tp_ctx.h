@@ -53,6 +53,13 @@ struct Ctx {
     // faster-evgen: event start positions of the last run and the state after it
     std::vector<uint64_t> fe_starts;
     tp::RanfState fe_end{};
+    // device walk: walkers' start lists, their counts and join indices, the
+    // joined pieces (begin, length, output offset), and host staging of them
+    DevBuf fe_lists, fe_meta, fe_piece;
+    std::vector<uint32_t> fe_counts, fe_beg, fe_len;
+    std::vector<int32_t> fe_jprev, fe_jnext;
+    std::vector<uint64_t> fe_off;
+    uint64_t fe_qnext = 0;
 };
 
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws

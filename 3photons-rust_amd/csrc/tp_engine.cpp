@@ -310,72 +310,193 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
     return TP_OK;
 }
 
-// RANF with `faster-evgen`: the host walks the stream for the event starts
-// (the reference's simulate_event_batch walk), then each chunk of batches is
-// expanded into HBM and simulated from those starts.
-constexpr uint64_t FE_CHUNK_BATCHES = 1024;
+// RANF with `faster-evgen`: the event starts of the reference's
+// simulate_event_batch walk, then each chunk of batches is simulated from
+// them with its stream expanded into HBM.  Runs of FE_DEVICE_WALK_MIN events
+// or more walk the stream on the device (fe_walk_kernel: segment walkers
+// joined where they meet); shorter runs, and any chunk whose device walkers
+// do not all join, take the host walk.
+constexpr uint64_t FE_DEVICE_WALK_MIN = 1u << 20;
+constexpr uint64_t FE_WALK_BATCHES = 6144; // batches per chunk: two strides of the event-loop grid
+constexpr double FE_POSITIONS_PER_EVENT = 18.5; // stream positions allowed for per event (mean 17.9)
+constexpr uint64_t FE_EVENTS_PER_WALKER = 1024;
+constexpr uint64_t FE_OVERLAP = 64 * 55; // positions a walker goes past its segment's end
+
+namespace {
+
+// RANF values of stream positions [q_first, q_last) into c->draws (position
+// q_first at index 0), from the master origin uploaded by the caller.
+int fe_expand(Ctx *c, uint64_t p0_master, int K, uint64_t q_first, uint64_t q_last)
+{
+    int rc = ensure(c, c->draws, (q_last - q_first) * sizeof(int32_t));
+    if (rc)
+        return rc;
+    tpk::RanfArgs r;
+    std::memset(&r, 0, sizeof r);
+    r.origin = static_cast<const int32_t *>(c->origin.p);
+    r.p0 = static_cast<const int32_t *>(c->p0.p);
+    r.polys = static_cast<const uint32_t *>(c->polys.p);
+    r.K = K;
+    r.R = RANF_R;
+    r.out = static_cast<int32_t *>(c->draws.p);
+    r.d_begin = q_first - p0_master;
+    r.d_end = q_last - p0_master;
+    r.seg_first = q_first / 55 / RANF_R;
+    r.nsegs = (q_last - 1) / 55 / RANF_R - r.seg_first + 1;
+    r.piece_stride = 0;
+    tpk::launch_ranf_expand(r, 1, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return TP_OK;
+}
+
+// Device walk of n + 1 event starts from the true start q_start into c->evq,
+// with the stream of the walked range expanded into c->draws (qbase q_start).
+// *ok is false when some walker did not meet its predecessor or the joined
+// walk is too short; the caller then walks on the host.
+int fe_device_walk(Ctx *c, uint64_t p0_master, int K, uint64_t q_start, uint64_t n, bool *ok)
+{
+    *ok = false;
+    const uint64_t span = (uint64_t)std::ceil((double)(n + 1) * FE_POSITIONS_PER_EVENT) + 55;
+    const uint32_t T = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(64, n / FE_EVENTS_PER_WALKER));
+    tpk::FeWalkArgs w;
+    std::memset(&w, 0, sizeof w);
+    w.q_start = q_start;
+    w.seg_origin = q_start / 55 * 55;
+    w.seg = ((q_start - w.seg_origin + span) / T / 55 + 1) * 55;
+    w.overlap = FE_OVERLAP;
+    w.nwalkers = T;
+    // every event takes at least 15 positions: no walk holds more starts
+    w.cap = (uint32_t)((w.seg + w.overlap + 63) / 15 + 2);
+    w.qbase = q_start;
+    w.qlimit = w.seg_origin + (uint64_t)T * w.seg + w.overlap + 128;
+    int rc = fe_expand(c, p0_master, K, w.qbase, w.qlimit);
+    if (!rc)
+        rc = ensure(c, c->fe_lists, (size_t)T * w.cap * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->fe_meta, (size_t)T * 3 * sizeof(uint32_t));
+    if (!rc)
+        rc = ensure(c, c->fe_piece, (size_t)T * (2 * sizeof(uint32_t) + sizeof(uint64_t)));
+    if (!rc)
+        rc = ensure(c, c->evq, (n + 1) * sizeof(uint64_t));
+    if (rc)
+        return rc;
+    w.buf = static_cast<const int32_t *>(c->draws.p);
+    w.lists = static_cast<uint64_t *>(c->fe_lists.p);
+    w.counts = static_cast<uint32_t *>(c->fe_meta.p);
+    w.join_prev = reinterpret_cast<int32_t *>(w.counts + T);
+    w.join_next = w.join_prev + T;
+    tpk::launch_fe_walk(w, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->fe_counts.resize(T);
+    c->fe_jprev.resize(T);
+    c->fe_jnext.resize(T);
+    HIPCHK(c, hipMemcpyAsync(c->fe_counts.data(), w.counts, T * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->fe_jprev.data(), w.join_prev, T * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->fe_jnext.data(), w.join_next, T * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // The true walk: walker 0's list from its start; walker k's from the index
+    // where walker k-1 met it, which must come after walker k-1 itself joined.
+    c->fe_beg.assign(T, 0);
+    c->fe_len.assign(T, 0);
+    c->fe_off.assign(T, 0);
+    uint32_t begin = 0;
+    for (uint32_t k = 1; k < T; ++k) {
+        if (c->fe_jprev[k] < 0 || (uint32_t)c->fe_jprev[k] < begin)
+            return TP_OK;
+        c->fe_beg[k - 1] = begin;
+        c->fe_len[k - 1] = (uint32_t)c->fe_jprev[k] - begin;
+        begin = (uint32_t)c->fe_jnext[k];
+    }
+    if (c->fe_counts[T - 1] < begin)
+        return TP_OK;
+    c->fe_beg[T - 1] = begin;
+    c->fe_len[T - 1] = c->fe_counts[T - 1] - begin;
+    uint64_t have = 0;
+    for (uint32_t k = 0; k < T; ++k) { // keep exactly n + 1 starts
+        c->fe_off[k] = have;
+        c->fe_len[k] = (uint32_t)std::min<uint64_t>(c->fe_len[k], n + 1 - have);
+        have += c->fe_len[k];
+    }
+    if (have < n + 1)
+        return TP_OK;
+    uint32_t *dbeg = static_cast<uint32_t *>(c->fe_piece.p), *dlen = dbeg + T;
+    uint64_t *doff = reinterpret_cast<uint64_t *>(dlen + T);
+    HIPCHK(c, hipMemcpyAsync(dbeg, c->fe_beg.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dlen, c->fe_len.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(doff, c->fe_off.data(), T * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    tpk::launch_fe_scatter(w, dbeg, dlen, doff, static_cast<uint64_t *>(c->evq.p), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&c->fe_qnext, static_cast<uint64_t *>(c->evq.p) + n, sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors are reused
+    *ok = true;
+    return TP_OK;
+}
+
+} // namespace
 
 int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs, uint64_t num_events,
                               uint32_t batch_size, void *per_batch_dev)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors below are reused
-    tp::RanfState g;
-    std::memcpy(g.numbers, rs->numbers, sizeof g.numbers);
-    g.index = rs->index;
-    g.seed = rs->seed;
-    const uint64_t p0_master = 55u - rs->index;
-    c->fe_starts.resize(num_events);
-    uint64_t q_end = 0;
-    tp::ranf_walk_faster_evgen(g, num_events, c->fe_starts.data(), &q_end);
-    c->fe_end = g;
-    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
-    const uint64_t chunk = std::min<uint64_t>(nb, FE_CHUNK_BATCHES);
-    int rc = ensure(c, c->evq, chunk * batch_size * sizeof(uint64_t));
+    tp::RanfState g0;
+    std::memcpy(g0.numbers, rs->numbers, sizeof g0.numbers);
+    g0.index = rs->index;
+    g0.seed = rs->seed;
+    const uint64_t p0_master = 55u - rs->index; // position of the next value
+    const bool device_walk = num_events >= FE_DEVICE_WALK_MIN;
+    // segment jumps cover positions up to 30 per event (the mean is 17.9)
+    const int K = std::max(1, bits_for((p0_master + 30 * (num_events + 1) + 4 * FE_OVERLAP) / 55 / RANF_R + 2));
+    int rc = ensure_polys(c, RANF_R, K);
     if (!rc)
         rc = ensure(c, c->origin, 56 * sizeof(int32_t));
     if (!rc)
         rc = ensure(c, c->p0, sizeof(int32_t));
     if (rc)
         return rc;
-    const int K = std::max(1, bits_for(q_end / 55 / RANF_R + 2));
-    rc = ensure_polys(c, RANF_R, K);
-    if (rc)
-        return rc;
     std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
     c->p00 = static_cast<int32_t>(p0_master);
     HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    const uint64_t chunk = std::min<uint64_t>(nb, FE_WALK_BATCHES);
+    uint64_t q_start = p0_master;
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
         const uint64_t b1 = std::min(nb, b0 + chunk);
         const uint64_t e0 = b0 * batch_size, e1 = std::min<uint64_t>(b1 * batch_size, num_events);
-        const uint64_t q_first = c->fe_starts[e0], q_last = e1 < num_events ? c->fe_starts[e1] : q_end;
-        rc = ensure(c, c->draws, (q_last - q_first) * sizeof(int32_t));
-        if (rc)
-            return rc;
-        HIPCHK(c, hipMemcpyAsync(c->evq.p, c->fe_starts.data() + e0, (e1 - e0) * sizeof(uint64_t),
-                                 hipMemcpyHostToDevice, c->stream));
-        tpk::RanfArgs r;
-        std::memset(&r, 0, sizeof r);
-        r.origin = static_cast<const int32_t *>(c->origin.p);
-        r.p0 = static_cast<const int32_t *>(c->p0.p);
-        r.polys = static_cast<const uint32_t *>(c->polys.p);
-        r.K = K;
-        r.R = RANF_R;
-        r.out = static_cast<int32_t *>(c->draws.p);
-        r.d_begin = q_first - p0_master;
-        r.d_end = q_last - p0_master;
-        r.seg_first = q_first / 55 / RANF_R;
-        r.nsegs = (q_last - 1) / 55 / RANF_R - r.seg_first + 1;
-        r.piece_stride = 0;
+        const uint64_t n = e1 - e0;
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-        tpk::launch_ranf_expand(r, 1, c->stream);
-        HIPCHK(c, hipGetLastError());
+        bool ok = false;
+        uint64_t q_next = 0;
+        if (device_walk) {
+            rc = fe_device_walk(c, p0_master, K, q_start, n, &ok);
+            if (rc)
+                return rc;
+            q_next = c->fe_qnext;
+        }
+        if (!ok) { // host walk of this chunk, from the state at q_start
+            tp::RanfState g = g0;
+            tp::ranf_advance(g, q_start - p0_master);
+            const uint64_t rel = q_start - (55u - g.index); // absolute position of g's round start
+            c->fe_starts.resize(n);
+            tp::ranf_walk_faster_evgen(g, n, c->fe_starts.data(), &q_next);
+            for (uint64_t &q : c->fe_starts)
+                q += rel;
+            q_next += rel;
+            rc = ensure(c, c->evq, n * sizeof(uint64_t));
+            if (!rc)
+                rc = fe_expand(c, p0_master, K, q_start, q_next);
+            if (rc)
+                return rc;
+            HIPCHK(c, hipMemcpyAsync(c->evq.p, c->fe_starts.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                     c->stream));
+        }
         a.nb = b1 - b0;
         a.b_first = b0;
         a.draws = static_cast<const int32_t *>(c->draws.p);
-        a.qbase = q_first;
+        a.qbase = q_start;
         a.evq = static_cast<const uint64_t *>(c->evq.p);
         a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
         if (first) {
@@ -384,8 +505,13 @@ int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs
         }
         tpk::launch_fused(a, 3, grid_for(c, 3, 0, a.nb), c->stream);
         HIPCHK(c, hipGetLastError());
+        if (!ok) // fe_starts is staged for the copy above
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        q_start = q_next;
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    c->fe_end = g0;
+    tp::ranf_advance(c->fe_end, q_start - p0_master);
     return TP_OK;
 }
 

@@ -101,6 +101,31 @@ struct F32Args {
 void launch_f32_events(const F32Args &a, int grid, hipStream_t st);
 void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st);
 
+// ---- faster-evgen stream walk on the device ----------------------------------
+// Event starts of the faster-evgen draws (evgen.rs:209-255, random_array's
+// discarded round leftovers ranf.rs:89-92), split like the host walk: walker
+// k starts at seg_origin + k*seg (walker 0 at the true start q_start), as if
+// an event started there, and records its event starts until it passes the
+// next segment's start by `overlap` positions.  Positions are counted from
+// the start of the master state's round; buf holds positions [qbase, qlimit).
+struct FeWalkArgs {
+    const int32_t *buf;
+    uint64_t qbase, qlimit;
+    uint64_t q_start;
+    uint64_t seg_origin, seg, overlap;
+    uint32_t nwalkers;
+    uint32_t cap;          // list capacity per walker (>= its longest possible walk)
+    uint64_t *lists;       // [nwalkers][cap] event starts, ascending
+    uint32_t *counts;      // [nwalkers]
+    int32_t *join_prev;    // [nwalkers] for k >= 1: index in list k-1 of the first start >= the
+    int32_t *join_next;    //   start of segment k that list k also holds, and its index there (-1: none)
+};
+void launch_fe_walk(const FeWalkArgs &a, hipStream_t st);
+// Copies the pieces of the joined walk into out: piece k is lists[k][begin[k] ..
+// begin[k] + len[k]) at out[off[k] ..].
+void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_t *len, const uint64_t *off,
+                       uint64_t *out, hipStream_t st);
+
 } // namespace tpk
 
 namespace tpk {

@@ -532,6 +532,132 @@ void launch_ranf_expand(const RanfArgs &a, uint64_t npieces, hipStream_t st)
     hipLaunchKernelGGL(ranf_expand_kernel, dim3((unsigned)blocks), dim3(64), 0, st, a);
 }
 
+// ---- faster-evgen stream walk --------------------------------------------------
+// take(n) of random_array::<N> on the stored stream: a request that does not
+// fit in the current round skips its leftovers (ranf.rs:89-92).  False (and no
+// read) when the values would lie past the stored range.
+__device__ __forceinline__ bool fe_take(const FeWalkArgs &a, uint64_t &q, int n, double *out)
+{
+    const uint32_t rem = 55u - (uint32_t)(q % 55u);
+    if (rem < (uint32_t)n)
+        q += rem;
+    if (q + (uint64_t)n > a.qlimit)
+        return false;
+    if (out) {
+        const int32_t *v = a.buf + (q - a.qbase);
+        for (int k = 0; k < n; ++k)
+            out[k] = (double)v[n - 1 - k] * 1e-9;
+    }
+    q += (uint64_t)n;
+    return true;
+}
+
+// Consumption of one event (raw_photons_faster_f64's draws: 9, 6, then 2 per
+// re-rolled point of the unit-disc test).  False when the stored range ends
+// inside the event.
+__device__ __forceinline__ bool fe_walk_event(const FeWalkArgs &a, uint64_t &q)
+{
+    double r6[6];
+    if (!fe_take(a, q, 9, nullptr) || !fe_take(a, q, 6, r6))
+        return false;
+    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    for (int i = 0; i < 3; ++i) {
+        const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
+        double r2 = (0. + x * x) + y * y;
+        while (r2 > 1. || r2 < min2) {
+            double r[2];
+            if (!fe_take(a, q, 2, r))
+                return false;
+            const double xa = 2. * r[0] - 1., ya = 2. * r[1] - 1.;
+            r2 = 0. + (xa * xa + ya * ya);
+        }
+    }
+    return true;
+}
+
+// One thread per walker.  Every step records a start and then consumes at
+// least 15 positions, and the loop stops at the segment end, the list
+// capacity or the stored range, so it always ends.
+__global__ __launch_bounds__(256) void fe_walk_kernel(FeWalkArgs a)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nwalkers)
+        return;
+    uint64_t q = k == 0 ? a.q_start : a.seg_origin + (uint64_t)k * a.seg;
+    const uint64_t stop = a.seg_origin + (uint64_t)(k + 1) * a.seg + a.overlap;
+    uint64_t *list = a.lists + (size_t)k * a.cap;
+    uint32_t n = 0;
+    while (q < stop && n < a.cap) {
+        list[n++] = q;
+        if (!fe_walk_event(a, q))
+            break;
+    }
+    a.counts[k] = n;
+}
+
+// Walker k's first start that walker k-1 also reached past segment k's start:
+// from there on both walks agree.
+__global__ __launch_bounds__(256) void fe_join_kernel(FeWalkArgs a)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nwalkers)
+        return;
+    if (k == 0) {
+        a.join_prev[0] = a.join_next[0] = 0;
+        return;
+    }
+    const uint64_t *pa = a.lists + (size_t)(k - 1) * a.cap, *pb = a.lists + (size_t)k * a.cap;
+    const uint32_t na = a.counts[k - 1], nb = a.counts[k];
+    const uint64_t qs = a.seg_origin + (uint64_t)k * a.seg;
+    uint32_t lo = 0, hi = na; // first index of list k-1 with a start >= qs
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pa[mid] < qs)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    uint32_t i = lo, j = 0;
+    int32_t ji = -1, jj = -1;
+    while (i < na && j < nb) {
+        if (pa[i] == pb[j]) {
+            ji = (int32_t)i;
+            jj = (int32_t)j;
+            break;
+        }
+        if (pa[i] < pb[j])
+            ++i;
+        else
+            ++j;
+    }
+    a.join_prev[k] = ji;
+    a.join_next[k] = jj;
+}
+
+void launch_fe_walk(const FeWalkArgs &a, hipStream_t st)
+{
+    const unsigned blocks = (a.nwalkers + 255) / 256;
+    hipLaunchKernelGGL(fe_walk_kernel, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fe_join_kernel, dim3(blocks), dim3(256), 0, st, a);
+}
+
+// One block per piece.
+__global__ __launch_bounds__(256) void fe_scatter_kernel(FeWalkArgs a, const uint32_t *begin, const uint32_t *len,
+                                                         const uint64_t *off, uint64_t *out)
+{
+    const uint32_t k = blockIdx.x;
+    const uint64_t *src = a.lists + (size_t)k * a.cap + begin[k];
+    uint64_t *dst = out + off[k];
+    for (uint32_t i = threadIdx.x; i < len[k]; i += blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_t *len, const uint64_t *off,
+                       uint64_t *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(fe_scatter_kernel, dim3(a.nwalkers), dim3(256), 0, st, a, begin, len, off, out);
+}
+
 // seeded_new (ranf.rs:50-82) for jump mode, one thread per batch.
 __global__ void ranf_seed_kernel(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0)
 {

@@ -288,8 +288,9 @@ def main() -> int:
         sums = per_batch[:nb * 104].view(torch.float64).reshape(nb, 13)[:, 1:]
         nan_batches = int((~torch.isfinite(sums).all(dim=1)).sum().item())
         # faster-evgen: event starts in the RANF stream depend on every earlier
-        # draw (rejection re-rolls, round discards), so the host walks the
-        # stream first; timed with that walk, on a bounded sample
+        # draw (rejection re-rolls, round discards), so the stream is walked
+        # first (on the device, segment walkers joined where they meet); timed
+        # with that walk, on a bounded sample
         fe_events = min(args.events, args.fe_events)
         pfe = tp.params_from_config(cfg.with_events(fe_events), faster_evgen=True)
         vfe, dtfe, kfe, accfe = measure(tp, torch, eng, pfe, fe_events, 1, 1, per_batch, merged, tp.RNG_RANF)
@@ -303,7 +304,7 @@ def main() -> int:
                                    "sigma_pb_f64_xoshiro256": f64r.sigma},
             "faster_evgen_f64": {"value": vfe, "unit": "events/s", "events": fe_events, "ms_per_step": dtfe * 1e3,
                                  "kernels_ms": kfe, "selected_events": int(accfe.selected_events),
-                                 "includes": "host walk of the RANF stream (event start positions)"},
+                                 "includes": "walk of the RANF stream for the event starts (device)"},
         }
     if not args.no_adjoint:
         out["adjoint"] = measure_adjoint(tp, torch, eng, p64, args.adjoint_points)

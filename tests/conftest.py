@@ -35,6 +35,24 @@ def golden_path(name: str) -> str:
 
 
 @pytest.fixture(scope="session")
+def walklib(tmp_path_factory):
+    """tests/walk_check.cpp with the host faster-evgen walk
+    (3photons-rust_amd/csrc/tp_rng_host.cpp), built for the host."""
+    import ctypes as C
+    out = tmp_path_factory.mktemp("walk") / "libwalk.so"
+    csrc = os.path.join(PKG, "csrc")
+    here = os.path.dirname(os.path.abspath(__file__))
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-ffp-contract=off", "-shared", "-fPIC", "-I" + csrc,
+                    os.path.join(here, "walk_check.cpp"), os.path.join(csrc, "tp_rng_host.cpp"), "-o", str(out)],
+                   check=True)
+    lib = C.CDLL(str(out))
+    lib.tp_walk_mismatch.argtypes = [C.c_uint64, C.c_int, C.c_uint32, C.c_int]
+    lib.tp_walk_end.argtypes = [C.POINTER(C.c_int32), C.c_uint32, C.c_uint64, C.c_int]
+    lib.tp_walk_end.restype = C.c_uint64
+    return lib
+
+
+@pytest.fixture(scope="session")
 def oracle():
     from oracle_ffi import Oracle
     return Oracle()

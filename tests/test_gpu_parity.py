@@ -165,6 +165,33 @@ def test_faster_evgen_per_batch_matches_oracle(engine, cfg, oracle, valeurs, n_e
     acc_close(merged.as_array(), om.as_array(), n_events)
 
 
+def test_faster_evgen_device_walk_across_chunks(engine, cfg, oracle, valeurs, walklib):
+    """N = 6.2e7 (6200 batches): the event starts come from the device walk,
+    in two chunks (6144 + 56 batches), the second from the first one's last
+    start.  Batches at the chunk edge and the last one against the oracle
+    started where the host walk (checked against the sequential walk in
+    test_walk.py) puts those batches."""
+    import ctypes as C
+    n_events = 62_000_000
+    print(f"faster-evgen device walk N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, faster_evgen=True)
+    st0 = tp.rng_state(tp.RNG_RANF)
+    per, _, _ = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, st0, n_events)
+    ocfg = oracle.config(valeurs, n_events)
+    opts = oracle.opts("ranf", faster_evgen=True)
+    numbers = (C.c_int32 * 56)(*list(st0.numbers))
+    q0 = 55 - st0.index
+    for b in (0, 1, 6143, 6144, 6199):
+        q = walklib.tp_walk_end(numbers, st0.index, 10000 * b, 8)
+        g = oracle.rng("ranf")
+        oracle.skip(g, q - q0)
+        o = oracle.simulate_batch(ocfg, opts, g, 10000).as_array()
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]], np.float64)
+        acc_close(a, o, 10000)
+
+
 def test_faster_evgen_simulate_batch_advances_like_the_closure(engine, cfg, oracle, valeurs):
     params = tp.params_from_config(cfg, faster_evgen=True)
     st = tp.rng_state(tp.RNG_RANF)

@@ -26,3 +26,16 @@ extern "C" int tp_walk_mismatch(uint64_t n, int seed, uint32_t index, int thread
     bad += a.index != b.index || std::memcmp(a.numbers, b.numbers, sizeof a.numbers) != 0;
     return bad;
 }
+
+// Stream position, counted from the start of the round of the state
+// (numbers[1..55], index), where event n of the faster-evgen walk from that
+// state starts: the position a batch beginning with event n starts from.
+extern "C" uint64_t tp_walk_end(const int32_t *numbers, uint32_t index, uint64_t n, int threads)
+{
+    tp::RanfState g;
+    std::memcpy(g.numbers, numbers, sizeof g.numbers);
+    g.index = index;
+    uint64_t q = 0;
+    tp::ranf_walk_faster_evgen(g, n, nullptr, &q, threads);
+    return q;
+}
