@@ -319,8 +319,10 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
 constexpr uint64_t FE_DEVICE_WALK_MIN = 1u << 20;
 constexpr uint64_t FE_WALK_BATCHES = 6144; // batches per chunk: two strides of the event-loop grid
 constexpr double FE_POSITIONS_PER_EVENT = 18.5; // stream positions allowed for per event (mean 17.9)
-constexpr uint64_t FE_EVENTS_PER_WALKER = 1024;
-constexpr uint64_t FE_OVERLAP = 64 * 55; // positions a walker goes past its segment's end
+constexpr uint64_t FE_EVENTS_PER_WALKER = 512, FE_MAX_WALKERS = 1u << 17;
+// Positions a walker goes past its segment's end: 24 rounds, ~74 events; a
+// walker meets the true walk within 40 events in the runs checked.
+constexpr uint64_t FE_OVERLAP = 24 * 55;
 
 namespace {
 
@@ -357,7 +359,7 @@ int fe_device_walk(Ctx *c, uint64_t p0_master, int K, uint64_t q_start, uint64_t
 {
     *ok = false;
     const uint64_t span = (uint64_t)std::ceil((double)(n + 1) * FE_POSITIONS_PER_EVENT) + 55;
-    const uint32_t T = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(64, n / FE_EVENTS_PER_WALKER));
+    const uint32_t T = (uint32_t)std::min<uint64_t>(FE_MAX_WALKERS, std::max<uint64_t>(64, n / FE_EVENTS_PER_WALKER));
     tpk::FeWalkArgs w;
     std::memset(&w, 0, sizeof w);
     w.q_start = q_start;
