@@ -829,9 +829,31 @@ void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t 
 // writes each event's m[5] and cut decision; the batch sums are then formed
 // in event order, as the reference's sequential loop does in f32
 // (resacc.rs:121-129), where a tree sum would round differently.
+// Events that pass the cuts are queued in LDS (momenta and event index) and
+// their helicity sums computed 64 at a time on dense lanes, as in the f64
+// event loop; each event's m[5] and cut flag go to its own slot of a.ev.
 template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F32Args a)
 {
+    __shared__ float queue[12][QCAP];
+    __shared__ uint32_t qidx[QCAP];
     const uint32_t lane = threadIdx.x;
+    const uint64_t lanemask_lt = (1ull << lane) - 1ull;
+    // helicity sums of the queued event in `slot`, stored to its output slot
+    auto flush_one = [&](uint32_t slot) {
+        float pq[3][4], m[5];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                pq[i][j] = queue[4 * i + j][slot];
+        const uint32_t idx = qidx[slot];
+        matrix_elements_f32(a.P, pq, m);
+        float *o = a.ev + 6 * (size_t)idx;
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            o[c] = m[c];
+        o[5] = 1.f;
+    };
     uint32_t st[4] = {0, 0, 0, 0}; // xoshiro128+ state at this lane's chunk of the previous batch
     bool have = false;
     for (uint64_t lb = blockIdx.x; lb < a.nb; lb += gridDim.x) {
@@ -867,26 +889,56 @@ template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F3
             s[2] = st[2];
             s[3] = st[3];
         }
-        for (uint32_t i = 0; i < len; ++i) {
+        const uint32_t iters = q + (r ? 1u : 0u);
+        uint32_t qn = 0; // queued events (uniform)
+        for (uint32_t i = 0; i < iters; ++i) {
             const uint64_t e = lb * a.batch_size + e0 + i;
-            float u[12], m[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-            if (SOURCE == 0) {
+            bool pass = false;
+            float p[3][4];
+            if (i < len) {
+                float u[12];
+                if (SOURCE == 0) {
 #pragma unroll
-                for (int j = 0; j < 12; ++j)
-                    u[j] = xo128_uniform(s);
-            } else {
-                const int32_t *d = a.draws + 12 * e;
+                    for (int j = 0; j < 12; ++j)
+                        u[j] = xo128_uniform(s);
+                } else {
+                    const int32_t *d = a.draws + 12 * e;
 #pragma unroll
-                for (int j = 0; j < 12; ++j)
-                    u[j] = (float)d[j] * 1e-9f; // (n as f32) * INV_MODULO (ranf.rs:99)
+                    for (int j = 0; j < 12; ++j)
+                        u[j] = (float)d[j] * 1e-9f; // (n as f32) * INV_MODULO (ranf.rs:99)
+                }
+                pass = generate_f32(a.P, u, p);
+                if (!pass) {
+                    float *o = a.ev + 6 * e;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        o[c] = 0.f;
+                }
             }
-            const bool keep = event_f32(a.P, u, m);
-            float *o = a.ev + 6 * e;
+            const uint64_t mask = __ballot(pass);
+            if (pass) {
+                const uint32_t slot = qn + (uint32_t)__popcll(mask & lanemask_lt);
 #pragma unroll
-            for (int c = 0; c < 5; ++c)
-                o[c] = m[c];
-            o[5] = keep ? 1.f : 0.f;
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        queue[4 * k + j][slot] = p[k][j];
+                qidx[slot] = (uint32_t)e;
+            }
+            qn += (uint32_t)__popcll(mask);
+            if (qn >= 64) {
+                __syncthreads();
+                flush_one(qn - 64 + lane);
+                __syncthreads();
+                qn -= 64;
+            }
         }
+        if (qn > 0) { // partial flush
+            __syncthreads();
+            if (lane < qn)
+                flush_one(lane);
+        }
+        __syncthreads();
     }
 }
 

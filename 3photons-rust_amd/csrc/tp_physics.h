@@ -686,21 +686,35 @@ __device__ __forceinline__ bool keep_f32(const FParams &P, const float p[3][4])
     return ok;
 }
 
-// One f32 event from 12 uniforms: cut decision and helicity sums m[5].
-__device__ __forceinline__ bool event_f32(const FParams &P, const float u[12], float m[5])
+// One f32 event from 12 uniforms: photon momenta (sorted when P.sort) and
+// the cut decision.
+__device__ __forceinline__ bool generate_f32(const FParams &P, const float u[12], float p[3][4])
 {
-    float q[3][4], p[3][4];
+    float q[3][4];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         raw_photon_f32(u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3], q[i]);
     conformal_f32(P, q, p);
     if (P.sort)
         sort3(p);
-    if (!keep_f32(P, p))
-        return false;
+    return keep_f32(P, p);
+}
+
+// Helicity sums m[5] of an f32 event that passed the cuts.
+__device__ __forceinline__ void matrix_elements_f32(const FParams &P, const float p[3][4], float m[5])
+{
     Spin<float> S;
     spinor_products<float>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
     helicity_sums<float, false>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+}
+
+// One f32 event from 12 uniforms: cut decision and helicity sums m[5].
+__device__ __forceinline__ bool event_f32(const FParams &P, const float u[12], float m[5])
+{
+    float p[3][4];
+    if (!generate_f32(P, u, p))
+        return false;
+    matrix_elements_f32(P, p, m);
     return true;
 }
 
