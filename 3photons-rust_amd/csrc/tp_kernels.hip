@@ -175,8 +175,8 @@ __device__ __forceinline__ const Params &fresh_params()
 template <int SOURCE, int PREC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void fused_kernel(FusedArgs a)
 {
-    constexpr int QROWS = SOURCE == 2 ? 13 : 12; // momenta (+ event weight)
-    __shared__ double queue[QROWS][QCAP];       // structure of arrays
+    __shared__ double queue[12][QCAP];      // momenta, structure of arrays
+    __shared__ uint32_t qev[SOURCE == 2 ? QCAP : 1]; // weighted source: the event's index in the chunk
     __shared__ double red[1][13];
     const uint32_t lane = threadIdx.x;
     const uint64_t lanemask_lt = (1ull << lane) - 1ull;
@@ -219,7 +219,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
             bool pass = false;
             double p[3][4];
             int d[3]; // sort slots of the three photons
-            double wt = 1.;
             if (k < len && SOURCE == 3) {
                 RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
                 pass = generate_faster_f64(fresh_params(), w, p, d);
@@ -239,7 +238,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
 #pragma unroll
                     for (int j = 0; j < 12; ++j)
                         u[j] = a.u[j * a.u_stride + e];
-                    wt = a.invq[e];
                 }
                 pass = generate_f64<SOURCE == 0>(fresh_params(), u, p, d);
             }
@@ -252,7 +250,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                     for (int j = 0; j < 4; ++j)
                         queue[4 * d[i] + j][slot] = p[i][j];
                 if (SOURCE == 2)
-                    queue[QROWS - 1][slot] = wt;
+                    qev[slot] = (uint32_t)(wbase + (size_t)(k * 64 + lane));
             }
             qn += (uint32_t)__popcll(mask);
             if (qn >= 64) {
@@ -264,7 +262,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         pq[i][j] = queue[4 * i + j][slot];
-                const double wq = SOURCE == 2 ? queue[QROWS - 1][slot] : 1.;
+                const double wq = SOURCE == 2 ? a.invq[qev[slot]] : 1.;
                 __syncthreads();
                 qn -= 64;
                 matrix_elements<PREC>(fresh_params(), pq, m);
@@ -286,7 +284,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
                         pq[i][j] = queue[4 * i + j][lane];
                 matrix_elements<PREC>(fresh_params(), pq, m);
                 if (SOURCE == 2) {
-                    const double wq = queue[QROWS - 1][lane];
+                    const double wq = a.invq[qev[lane]];
 #pragma unroll
                     for (int c = 0; c < 5; ++c)
                         m[c] = m[c] * wq;
