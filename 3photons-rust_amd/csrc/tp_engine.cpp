@@ -479,14 +479,8 @@ int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs
             q_next = c->fe_qnext;
         }
         if (!ok) { // host walk of this chunk, from the state at q_start
-            tp::RanfState g = g0;
-            tp::ranf_advance(g, q_start - p0_master);
-            const uint64_t rel = q_start - (55u - g.index); // absolute position of g's round start
             c->fe_starts.resize(n);
-            tp::ranf_walk_faster_evgen(g, n, c->fe_starts.data(), &q_next);
-            for (uint64_t &q : c->fe_starts)
-                q += rel;
-            q_next += rel;
+            tp::ranf_walk_faster_evgen_from(g0, q_start, n, c->fe_starts.data(), &q_next);
             rc = ensure(c, c->evq, n * sizeof(uint64_t));
             if (!rc)
                 rc = fe_expand(c, p0_master, K, q_start, q_next);

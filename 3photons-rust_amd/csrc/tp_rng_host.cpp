@@ -586,4 +586,19 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
         *q_end = qe;
 }
 
+void ranf_walk_faster_evgen_from(const RanfState &g0, uint64_t q, uint64_t num_events, uint64_t *starts,
+                                 uint64_t *q_end, int max_threads)
+{
+    RanfState g = g0;
+    ranf_advance(g, q - (55u - g0.index));
+    const uint64_t rel = q - (55u - g.index); // g0-frame position of g's round start
+    uint64_t qe = 0;
+    ranf_walk_faster_evgen(g, num_events, starts, &qe, max_threads);
+    if (starts)
+        for (uint64_t e = 0; e < num_events; ++e)
+            starts[e] += rel;
+    if (q_end)
+        *q_end = qe + rel;
+}
+
 } // namespace tp

@@ -55,6 +55,10 @@ void ranf_advance(RanfState &g, uint64_t n);
 // max_threads threads (1: sequential); the result does not depend on it.
 void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end,
                             int max_threads = 16);
+// The same walk from stream position q of g0's frame (q >= 55 - g0.index, the
+// start of an event of the walk from g0): starts[] and *q_end in g0's frame.
+void ranf_walk_faster_evgen_from(const RanfState &g0, uint64_t q, uint64_t num_events, uint64_t *starts,
+                                 uint64_t *q_end, int max_threads = 16);
 
 // ---- xoshiro256+ -------------------------------------------------------------
 void xoshiro_seed(uint64_t seed, uint64_t s[4]); // seed_from_u64 (SplitMix64 expansion)

@@ -16,3 +16,11 @@ def lib(walklib):
 def test_threaded_walk_equals_sequential(lib, n, seed, index):
     # 8 threads, whatever the host has: the join logic is what is tested
     assert lib.tp_walk_mismatch(n, seed, index, 8) == 0
+
+
+@pytest.mark.parametrize("e0,n,seed,index", [(1, 1000, 1234, 55), (777, 50_000, 98765, 17), (123_457, 4_100_000, 7, 0)])
+def test_walk_from_an_event_start(lib, e0, n, seed, index):
+    """A long faster-evgen run is walked chunk by chunk, each chunk from the
+    last start of the previous one (mid-round in general): the same starts
+    and end position as the walk from the state itself."""
+    assert lib.tp_walk_from_mismatch(e0, n, seed, index, 8) == 0

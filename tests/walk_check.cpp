@@ -27,6 +27,26 @@ extern "C" int tp_walk_mismatch(uint64_t n, int seed, uint32_t index, int thread
     return bad;
 }
 
+// 0 when the walk of n events from the start of event e0 (the way a chunk of
+// a long run is walked) equals events e0 .. e0 + n of the walk from the
+// state itself: starts and end position.
+extern "C" int tp_walk_from_mismatch(uint64_t e0, uint64_t n, int seed, uint32_t index, int threads)
+{
+    tp::RanfState g0;
+    tp::ranf_seeded(g0, seed);
+    g0.index = index;
+    tp::RanfState a = g0;
+    std::vector<uint64_t> sa(e0 + n), sb(n);
+    uint64_t qa = 0, qb = 0;
+    tp::ranf_walk_faster_evgen(a, e0 + n, sa.data(), &qa, 1);
+    const uint64_t q = e0 < e0 + n ? sa[e0] : qa;
+    tp::ranf_walk_faster_evgen_from(g0, q, n, sb.data(), &qb, threads);
+    int bad = qa != qb;
+    for (uint64_t e = 0; e < n; ++e)
+        bad += sa[e0 + e] != sb[e];
+    return bad;
+}
+
 // Stream position, counted from the start of the round of the state
 // (numbers[1..55], index), where event n of the faster-evgen walk from that
 // state starts: the position a batch beginning with event n starts from.
