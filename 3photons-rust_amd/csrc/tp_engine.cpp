@@ -319,9 +319,9 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
 constexpr uint64_t FE_DEVICE_WALK_MIN = 1u << 20;
 constexpr uint64_t FE_WALK_BATCHES = 6144; // batches per chunk: two strides of the event-loop grid
 constexpr double FE_POSITIONS_PER_EVENT = 18.5; // stream positions allowed for per event (mean 17.9)
-constexpr uint64_t FE_EVENTS_PER_WALKER = 512, FE_MAX_WALKERS = 1u << 17;
+constexpr uint64_t FE_EVENTS_PER_WALKER = 256, FE_MAX_WALKERS = 1u << 18;
 // Positions a walker goes past its segment's end: 24 rounds, ~74 events; a
-// walker meets the true walk within 40 events in the runs checked.
+// walker meets the true walk within 49 events in the runs checked.
 constexpr uint64_t FE_OVERLAP = 24 * 55;
 
 namespace {
