@@ -163,3 +163,26 @@ def test_f32_grid_stride_batches_match_oracle(engine, cfg, oracle32, valeurs):
         row = per[b]
         a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
         close_f32(a, o, 10000)
+
+
+def test_f32_ranf_batches_past_the_first_chunk_match_oracle(engine, cfg, oracle32, valeurs):
+    """RANF in the all-f32 mode at N = 7e7 (7000 batches): the stream is
+    expanded and the events and sums run per chunk of 6144 batches; the
+    batches at the chunk edge and the last one against the f32 oracle
+    started at those batches."""
+    n_events = 70_000_000
+    print(f"f32 RANF run N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, precision=tp.PREC_F32)
+    per, _, _ = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), n_events)
+    ocfg, opts = oracle32.config(valeurs, n_events), oracle32.opts("ranf")  # the weight carries 1/N
+    for b in (0, 6143, 6144, 6999):
+        g = oracle32.rng("ranf")
+        oracle32.skip(g, 120000 * b)
+        o = oracle32.simulate_batch(ocfg, opts, g, 10000).as_array()
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        if np.isfinite(o).all():
+            close_f32(a, o, 10000)
+        else:  # a batch whose f32 spinor products overflow: the same NaN sums on both sides
+            assert a[0] == o[0] and np.array_equal(np.isnan(a), np.isnan(o)), (b, a, o)
