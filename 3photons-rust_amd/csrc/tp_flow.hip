@@ -72,28 +72,28 @@ struct LdsFwd {
     float b[H + H + O];
 };
 
-template <class Lds> __device__ __forceinline__ void load_layer(Lds &s, const uint16_t *wl, bool transposed)
+// One layer's weights and biases into LDS; TRANSPOSED (backward pass, an
+// LdsLayer) also writes W1^T, W2^T, W3^T.
+template <bool TRANSPOSED, class Lds> __device__ __forceinline__ void load_layer(Lds &s, const uint16_t *wl)
 {
+    static_assert(!TRANSPOSED || sizeof(s.w) == sizeof(uint16_t) * L_ALL, "the transposed copies need an LdsLayer");
     for (int i = threadIdx.x; i < H * DP; i += blockDim.x) {
         const int r = i / DP, c = i % DP;
         s.w[L_W1 + r * S_W1 + c] = wl[OFF_W1 + i];
-        if constexpr (sizeof(s.w) > sizeof(uint16_t) * L_FWD)
-            if (transposed)
-                s.w[L_W1T + c * S_W1T + r] = wl[OFF_W1 + i];
+        if constexpr (TRANSPOSED)
+            s.w[L_W1T + c * S_W1T + r] = wl[OFF_W1 + i];
     }
     for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
         const int r = i / H, c = i % H;
         s.w[L_W2 + r * S_W2 + c] = wl[OFF_W2 + i];
-        if constexpr (sizeof(s.w) > sizeof(uint16_t) * L_FWD)
-            if (transposed)
-                s.w[L_W2T + c * S_W2T + r] = wl[OFF_W2 + i];
+        if constexpr (TRANSPOSED)
+            s.w[L_W2T + c * S_W2T + r] = wl[OFF_W2 + i];
     }
     for (int i = threadIdx.x; i < O * H; i += blockDim.x) {
         const int r = i / H, c = i % H;
         s.w[L_W3 + r * S_W3 + c] = wl[OFF_W3 + i];
-        if constexpr (sizeof(s.w) > sizeof(uint16_t) * L_FWD)
-            if (transposed)
-                s.w[L_W3T + c * S_W3T + r] = wl[OFF_W3 + i];
+        if constexpr (TRANSPOSED)
+            s.w[L_W3T + c * S_W3T + r] = wl[OFF_W3 + i];
     }
     for (int i = threadIdx.x; i < H; i += blockDim.x) {
         s.b[i] = bf2f(wl[OFF_B1 + i]);
@@ -236,7 +236,7 @@ constexpr uint64_t FWD_SMALL_N = 1u << 20;
 __device__ __forceinline__ void load_resident(LdsFwd (&s)[L], const uint16_t *wbf)
 {
     for (int l = 0; l < L; ++l)
-        load_layer(s[l], wbf + (size_t)l * LP, false);
+        load_layer<false>(s[l], wbf + (size_t)l * LP);
     __syncthreads();
 }
 
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(256) void flow_backward_kernel(FlowBackArgs a)
         }
         for (int l = L - 1; l >= 0; --l) {
             __syncthreads();
-            load_layer(s, a.wbf + (size_t)l * LP, true);
+            load_layer<true>(s, a.wbf + (size_t)l * LP);
             __syncthreads();
             const uint32_t cond = COND_MASK[l];
             uint16_t *sv = a.save + (size_t)l * SAVE_F * a.n;
