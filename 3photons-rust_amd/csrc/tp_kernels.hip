@@ -423,41 +423,41 @@ __device__ __forceinline__ int32_t norm_mod(int32_t v)
     return v;
 }
 
-// One IRN55 round (ranf.rs:120-135) with lane L holding X[L+1].  The three
-// dependency phases of the round are folded into closed forms in the old
-// values, so all 55 lanes update at once:
-//   i <= 24:       X[i] - X[i+31]
-//   25 <= i <= 48: X[i] - X[i-24] + X[i+7]
-//   i >= 49:       X[i] - X[i-24] + X[i-48] - X[i-17]        (all mod 1e9)
 // One RANF round in closed form (lane i - 1 holds X[i], i = 1..55):
 //   i <= 24:  X'[i] = X[i] - X[i+31]
 //   25..48:   X'[i] = X[i] - X'[i-24] = X[i] - X[i-24] + X[i+7]
 //   49..55:   X'[i] = X[i] - X[i-24] + X[i-48] - X[i-17]
-// (ranf.rs:86-97 applies the first two lines as in-place updates; the third
-// is the second with X'[i-24] expanded once more).  Every lane evaluates
-// (w[a] - w[b]) + (w[c] - w[d]) mod 1e9 with its own slots, slot RANF_ZERO
-// holding 0 where a case has fewer terms; each value is the unique
-// representative in [0, 1e9), so the sum order does not matter.
-constexpr int RANF_ZERO = 111;
+// (ranf.rs:120-135 applies the first two lines as in-place updates; the third
+// is the second with X'[i-24] expanded once more), all mod 1e9.  Every lane
+// evaluates (X[i] - b) + (c - d) with b, c, d read from other lanes of the
+// same wave by ds_bpermute (no LDS storage, no barrier).  Lanes 55..63 hold 0
+// and keep it, so a term a case does not have reads lane 63; each value is the
+// unique representative in [0, 1e9), so the sum order does not matter.  The
+// whole wave must be active: one wave holds one window.
 __device__ __forceinline__ uint32_t mod_sub(uint32_t a, uint32_t b) // (a - b) mod 1e9, a, b in [0, 1e9)
 {
     const uint32_t t = a - b;
     return min(t, t + (uint32_t)MODULO);
 }
-__device__ __forceinline__ int32_t ranf_next_round(int32_t x, int32_t *win)
+struct RanfLanes { // byte addresses for ds_bpermute of the lanes a lane reads
+    int b, c, d;
+};
+__device__ __forceinline__ RanfLanes ranf_lanes()
 {
-    const int lane = threadIdx.x;
-    const int i = lane + 1;
-    const int sa = lane < 55 ? i - 1 : RANF_ZERO;
-    const int sb = lane < 55 ? (i <= 24 ? i + 30 : i - 25) : RANF_ZERO;
-    const int sc = (lane < 55 && i > 24) ? (i <= 48 ? i + 6 : i - 49) : RANF_ZERO;
-    const int sd = (lane < 55 && i > 48) ? i - 18 : RANF_ZERO;
-    if (lane < 55)
-        win[lane] = x;
-    __syncthreads();
-    const uint32_t wa = (uint32_t)win[sa], wb = (uint32_t)win[sb], wc = (uint32_t)win[sc], wd = (uint32_t)win[sd];
-    __syncthreads();
-    const uint32_t v = mod_sub(wa, wb) + mod_sub(wc, wd); // [0, 2e9)
+    const int lane = (int)(threadIdx.x & 63u), i = lane + 1;
+    const bool live = lane < 55;
+    RanfLanes l;
+    l.b = (live ? (i <= 24 ? i + 30 : i - 25) : 63) << 2;          // X[i+31] or X[i-24]
+    l.c = (live && i > 24 ? (i <= 48 ? i + 6 : i - 49) : 63) << 2; // X[i+7] or X[i-48]
+    l.d = (live && i > 48 ? i - 18 : 63) << 2;                     // X[i-17]
+    return l;
+}
+__device__ __forceinline__ int32_t ranf_next_round(int32_t x, const RanfLanes &l)
+{
+    const uint32_t wb = (uint32_t)__builtin_amdgcn_ds_bpermute(l.b, x);
+    const uint32_t wc = (uint32_t)__builtin_amdgcn_ds_bpermute(l.c, x);
+    const uint32_t wd = (uint32_t)__builtin_amdgcn_ds_bpermute(l.d, x);
+    const uint32_t v = mod_sub((uint32_t)x, wb) + mod_sub(wc, wd); // [0, 2e9)
     return (int32_t)min(v, v - (uint32_t)MODULO);
 }
 
@@ -466,7 +466,7 @@ __device__ __forceinline__ int32_t ranf_next_round(int32_t x, int32_t *win)
 __device__ __forceinline__ int32_t ranf_jump(int32_t x, const uint32_t *__restrict__ c, int32_t *win)
 {
     const int lane = threadIdx.x;
-    int32_t nx = ranf_next_round(x, win);
+    int32_t nx = ranf_next_round(x, ranf_lanes());
     if (lane < 55) {
         win[lane] = x;
         win[55 + lane] = nx;
@@ -497,30 +497,41 @@ __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
     const uint64_t r_begin = seg * (uint64_t)a.R;
     if (r_begin * 55 >= q_end)
         return; // whole workgroup
-    if (lane == 0)
-        win[RANF_ZERO] = 0;
     int32_t x = lane < 55 ? a.origin[56 * piece + lane + 1] : 0;
     for (int k = 0; k < a.K; ++k)
         if ((seg >> k) & 1ull)
             x = ranf_jump(x, a.polys + 55 * k, win);
     int32_t *out = a.out + piece * a.piece_stride;
-    // X[lane+1] of round r is emitted at stream position 55 r + 54 - lane
-    const int64_t off = (int64_t)(54 - lane) - (int64_t)q_begin;
-    for (int j = 0; j < a.R; ++j) {
-        const uint64_t q0 = 55 * (r_begin + j);
-        if (q0 >= q_end)
-            break;
-        if (lane < 55) {
-            if (q0 >= q_begin && q0 + 55 <= q_end) { // the whole round is in range (uniform)
-                out[(int64_t)q0 + off] = x;
-            } else {
-                const uint64_t qq = q0 + (uint64_t)(54 - lane);
-                if (qq >= q_begin && qq < q_end)
-                    out[qq - q_begin] = x;
-            }
-        }
-        if (q0 + 55 < q_end)
-            x = ranf_next_round(x, win);
+    const RanfLanes l = ranf_lanes();
+    // X[lane+1] of round r is emitted at stream position 55 r + 54 - lane.
+    // Rounds [r_begin, r_end) reach into [q_begin, q_end); of those,
+    // [f_lo, f_hi) lie wholly inside and are stored without range checks.
+    const uint64_t r_end = min(r_begin + (uint64_t)a.R, (q_end + 54) / 55);
+    const uint64_t f_lo = min(max((q_begin + 54) / 55, r_begin), r_end);
+    const uint64_t f_hi = max(f_lo, min(r_end, q_end / 55));
+    const uint64_t qq0 = (uint64_t)(54 - lane);
+    uint64_t r = r_begin;
+    for (; r < f_lo; ++r) {
+        const uint64_t qq = 55 * r + qq0;
+        if (lane < 55 && qq >= q_begin && qq < q_end)
+            out[qq - q_begin] = x;
+        x = ranf_next_round(x, l);
+    }
+    int32_t *row = out + (55 * f_lo - q_begin); // uniform
+    const int off = 54 - lane;
+    const uint32_t nfull = (uint32_t)(f_hi - f_lo); // at most R
+    for (uint32_t k = 0; k < nfull; ++k) {
+        if (lane < 55)
+            row[off] = x;
+        x = ranf_next_round(x, l);
+        row += 55;
+    }
+    r = f_hi;
+    for (; r < r_end; ++r) {
+        const uint64_t qq = 55 * r + qq0;
+        if (lane < 55 && qq >= q_begin && qq < q_end)
+            out[qq - q_begin] = x;
+        x = ranf_next_round(x, l);
     }
 }
 
