@@ -152,12 +152,13 @@ int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int 
     return TP_OK;
 }
 
-// 16384 * 10000 events * 24 B = 3.9 GB of per-event values; with a grid of
-// 16 workgroups per CU each lane steps through 4 batches of a launch by the
+// 24576 * 10000 events * 24 B = 5.9 GB of per-event values.  The grid is 24
+// workgroups per CU (6 waves per SIMD, the occupancy the kernel's register
+// budget allows), so each lane steps through 4 batches of a full chunk by the
 // grid-stride matrix instead of a full skip-ahead per batch.
-constexpr uint64_t F32_CHUNK_BATCHES = 16384;
+constexpr uint64_t F32_CHUNK_BATCHES = 24576;
 
-int f32_grid(const Ctx *c, uint64_t nb) { return (int)std::max<uint64_t>(1, std::min<uint64_t>(nb, 16ull * c->cus)); }
+int f32_grid(const Ctx *c, uint64_t nb) { return (int)std::max<uint64_t>(1, std::min<uint64_t>(nb, 24ull * c->cus)); }
 
 // All-f32 run with the xoshiro128+ stream.
 int enqueue_xo128_f32(Ctx *c, const tp_params *params, int seeding, const tp_xoshiro128_state *x,

@@ -841,7 +841,8 @@ void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t 
 // Events that pass the cuts are queued in LDS (momenta and event index) and
 // their helicity sums computed 64 at a time on dense lanes, as in the f64
 // event loop; each event's m[5] and cut flag go to its own slot of a.ev.
-template <int SOURCE> __global__ __launch_bounds__(64) void f32_events_kernel(F32Args a)
+template <int SOURCE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void f32_events_kernel(F32Args a)
 {
     __shared__ float queue[12][QCAP];
     __shared__ uint32_t qidx[QCAP];
