@@ -23,6 +23,21 @@ struct DevBuf {
     size_t n = 0;
 };
 
+// Temporary device buffers of one call, freed on every return path.
+template <int N> struct ScopedBufs {
+    DevBuf b[N];
+    ScopedBufs() = default;
+    ScopedBufs(const ScopedBufs &) = delete;
+    ScopedBufs &operator=(const ScopedBufs &) = delete;
+    DevBuf &operator[](int i) { return b[i]; }
+    ~ScopedBufs()
+    {
+        for (DevBuf &x : b)
+            if (x.p)
+                (void)hipFree(x.p);
+    }
+};
+
 struct Tables { // device copies of GF(2) jump tables for one (seeding, batch size, grid)
     DevBuf bt, stride;
     int kb = 0;

@@ -101,10 +101,12 @@ using namespace tpe;
 
 int bits_for(uint64_t n);
 
-// xoshiro128+ skip-ahead tables for the all-f32 mode: M^(2^k) (k < 20: draw
-// offsets inside a batch) and B^(2^k) with B = M^(12 batch_size) or the 2^64
-// jump.
-constexpr int KM128 = 20;
+// xoshiro128+ skip-ahead tables for the all-f32 mode: M^(2^k) (k < KM128:
+// a lane's draw offset inside its batch, below 12 * batch_size) and B^(2^k)
+// with B = M^(12 batch_size) or the 2^64 jump.  The largest batch is one
+// tp_simulate_batch call of 640 000 events: offsets below 7 680 000 < 2^23.
+constexpr int KM128 = 24;
+static_assert(12ull * 64 * TP_BATCH_SIZE < (1ull << KM128), "xoshiro128+ offset table too short");
 
 int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int grid, DevBuf **bt_out, int *kb_out)
 {
@@ -165,6 +167,8 @@ int enqueue_xo128_f32(Ctx *c, const tp_params *params, int seeding, const tp_xos
                       uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
 {
     const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    if (12ull * batch_size >= (1ull << KM128))
+        return fail(c, TP_ERR_ARG, "all-f32 batch exceeds the xoshiro128+ offset range");
     DevBuf *bt = nullptr;
     int kb = 0;
     const int sgrid = f32_grid(c, F32_CHUNK_BATCHES);
@@ -904,7 +908,8 @@ extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const
             a.draws = static_cast<const int32_t *>(c->draws.p);
             source = 1;
         }
-        DevBuf bu, bp, bk, bm;
+        ScopedBufs<4> bufs; // freed on every return below
+        DevBuf &bu = bufs[0], &bp = bufs[1], &bk = bufs[2], &bm = bufs[3];
         tpk::EventOut o{nullptr, nullptr, nullptr, nullptr};
         struct Item {
             void *host;
@@ -936,9 +941,6 @@ extern "C" int tp_events(void *ctx, const tp_params *params, int rng_kind, const
             if (rc == TP_OK && e != hipSuccess)
                 rc = fail(c, TP_ERR_HIP, std::string("events sync: ") + hipGetErrorString(e));
         }
-        for (Item &it : items)
-            if (it.buf->p)
-                (void)hipFree(it.buf->p);
         return rc;
     });
 }
@@ -979,7 +981,7 @@ extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, cons
         const size_t es = precision == TP_PREC_F32 ? sizeof(float) : sizeof(double);
         const size_t bytes[3] = {5 * n * es, 60 * n * es, 60 * n * es};
         void *outs[3] = {F, dF_du, dF_dp};
-        DevBuf tmp[4];
+        ScopedBufs<4> tmp; // freed on every return below
         if (on_device) {
             a.u_in = static_cast<const double *>(u_in);
             a.F = F;
@@ -1024,9 +1026,6 @@ extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, cons
             c->last.events = n;
             c->last.batches = 0;
         }
-        for (DevBuf &b : tmp)
-            if (b.p)
-                (void)hipFree(b.p);
         return rc;
     });
 }

@@ -627,6 +627,25 @@ void OR_FN(or_events_from_uniforms)(const or_config_t *cfg, const or_opts_t *opt
     }
 }
 
+/* Spinor products s_ij of given momenta (spinor.rs:31-54): the full 5x5
+ * matrix of each event, s[n][5][5][2] as (re, im). */
+void OR_FN(or_spinors_at)(uint64_t n, const or_scalar *p, or_scalar *s)
+{
+    uint64_t e;
+    for (e = 0; e < n; ++e) {
+        or_real pp[5][4];
+        or_spinor sp;
+        int i, j;
+        memcpy(pp, p + 20 * e, sizeof pp);
+        or_spinor_new(&sp, pp);
+        for (i = 0; i < 5; ++i)
+            for (j = 0; j < 5; ++j) {
+                s[50 * e + 10 * i + 2 * j] = sp.s[i][j].re;
+                s[50 * e + 10 * i + 2 * j + 1] = sp.s[i][j].im;
+            }
+    }
+}
+
 /* Multi-threaded reproducible driver (multi_threading.rs:16-127): the
  * scheduling thread hands each batch a clone of the master stream and then
  * advances the master by the batch's draws (or jumps it in faster-threading

@@ -111,6 +111,10 @@ void OR_FN(or_events_from_uniforms)(const or_config_t *cfg, const or_opts_t *opt
                                     const or_scalar *u, or_scalar *p, int32_t *keep,
                                     or_scalar *m2, or_scalar *m, or_scalar *w);
 
+/* Spinor products of given momenta p[n][5][4] (spinor.rs:31-54):
+ * s[n][5][5][2], (re, im) of s_ij. */
+void OR_FN(or_spinors_at)(uint64_t n, const or_scalar *p, or_scalar *s);
+
 /* One batch: the closure of main.rs:103-128 (advances *g like the reference). */
 void OR_FN(or_simulate_batch)(const or_config_t *cfg, const or_opts_t *opt, or_rng_t *g,
                               uint64_t n, or_acc_t *out);

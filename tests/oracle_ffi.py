@@ -121,6 +121,14 @@ class Oracle:
                                             ptr(m2), ptr(m), ptr(w))
         return p, keep.astype(bool), m2, m, w
 
+    def spinors(self, p: np.ndarray) -> np.ndarray:
+        """s_ij of given momenta p[n][5][4] as complex [n][5][5]."""
+        dt = np.float32 if self.f32 else np.float64
+        p = np.ascontiguousarray(p, dtype=dt).reshape(-1, 5, 4)
+        s = np.zeros((p.shape[0], 5, 5, 2), dt)
+        self._fn("or_spinors_at")(C.c_uint64(p.shape[0]), p.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p))
+        return s[..., 0] + 1j * s[..., 1]
+
     def simulate_batch(self, cfg, opts, g: OrRng, n: int):
         acc = self.Acc()
         self._fn("or_simulate_batch")(C.byref(cfg), C.byref(opts), C.byref(g), C.c_uint64(n), C.byref(acc))

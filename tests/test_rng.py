@@ -116,3 +116,82 @@ def test_xoshiro128_seed_skip_jump_match_oracle(oracle32):
         oracle32.jump(g)
         tp.rng_jump(tp.RNG_XOSHIRO128, st)
         assert list(st.s) == list(g.xs32)
+
+
+# ---- jumps pinned by the step function alone --------------------------------
+# The published jump() of xoshiro256+ advances the state as 2^128 calls of
+# next(), and xoshiro128+'s as 2^64 (Blackman & Vigna, "Scrambled linear
+# pseudorandom number generators", 2018).  Both generators are linear over
+# GF(2), so the jump is the step matrix raised to that power: built here
+# from the state update alone (no jump constants) by repeated squaring.
+
+def _gf2_step_matrix(words: int, bits: int, a: int, rot: int) -> np.ndarray:
+    """Column j = one state update of the unit state with bit j set; the
+    update is t = s1 << a; s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t;
+    s3 = rotl(s3, rot)."""
+    mask = (1 << bits) - 1
+    n = words * bits
+    m = np.zeros((n, n), np.uint8)
+    for j in range(n):
+        s = [0] * 4
+        s[j // bits] = 1 << (j % bits)
+        t = (s[1] << a) & mask
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << rot) | (s[3] >> (bits - rot))) & mask
+        for w in range(4):
+            for b in range(bits):
+                m[w * bits + b, j] = (s[w] >> b) & 1
+    return m
+
+
+def _gf2_pow2(m: np.ndarray, k: int) -> np.ndarray:
+    f = m.astype(np.float64)
+    for _ in range(k):
+        f = np.mod(f @ f, 2.0)  # entries are sums of at most 256 ones: exact
+    return f.astype(np.uint8)
+
+
+def _apply(m: np.ndarray, state, bits: int):
+    v = np.array([(state[w] >> b) & 1 for w in range(4) for b in range(bits)], np.uint8)
+    r = (m.astype(np.int64) @ v.astype(np.int64)) % 2
+    return [sum(int(r[w * bits + b]) << b for b in range(bits)) for w in range(4)]
+
+
+def test_step_matrix_is_the_generator(oracle, oracle32):
+    """The matrices above step the oracle's generators (so the jump checks
+    below rest on the published state update, not on a table)."""
+    for bits, a, rot, orc, kind, field in ((64, 17, 45, oracle, tp.RNG_XOSHIRO, "xs"),
+                                            (32, 9, 11, oracle32, tp.RNG_XOSHIRO128, "xs32")):
+        m = _gf2_step_matrix(4, bits, a, rot)
+        g = orc.rng("xoshiro")
+        s0 = list(getattr(g, field))
+        orc.skip(g, 1)
+        assert _apply(m, s0, bits) == list(getattr(g, field))
+
+
+def test_xoshiro256_jump_is_two_to_the_128_steps(oracle):
+    m = _gf2_pow2(_gf2_step_matrix(4, 64, 17, 45), 128)
+    st = tp.rng_state(tp.RNG_XOSHIRO)
+    g = oracle.rng("xoshiro")
+    expect = _apply(m, list(st.s), 64)
+    tp.rng_jump(tp.RNG_XOSHIRO, st)
+    oracle.jump(g)
+    assert list(st.s) == expect and list(g.xs) == expect
+
+
+def test_xoshiro128_jump_is_two_to_the_64_steps(oracle32):
+    """Also two GF(2) skips of 2^63 through tp_rng_skip."""
+    m = _gf2_pow2(_gf2_step_matrix(4, 32, 9, 11), 64)
+    st = tp.rng_state(tp.RNG_XOSHIRO128)
+    expect = _apply(m, list(st.s), 32)
+    sk = tp.rng_state(tp.RNG_XOSHIRO128)
+    tp.rng_skip(tp.RNG_XOSHIRO128, sk, 2 ** 63)
+    tp.rng_skip(tp.RNG_XOSHIRO128, sk, 2 ** 63)
+    g = oracle32.rng("xoshiro")
+    oracle32.jump(g)
+    tp.rng_jump(tp.RNG_XOSHIRO128, st)
+    assert list(st.s) == expect and list(sk.s) == expect and list(g.xs32) == expect
