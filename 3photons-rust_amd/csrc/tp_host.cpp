@@ -116,7 +116,7 @@ std::string pad_right(const std::string &s, size_t w)
     return s.size() >= w ? s : s + std::string(w - s.size(), ' ');
 }
 
-// write_engineering (output.rs:226-269), %g-like with SIG_DIGITS significant
+// write_engineering (output.rs:230-269), %g-like with SIG_DIGITS significant
 // digits, the logarithm taken in the value's own type.
 template <class T> std::string write_engineering_t(T x, int sig_digits)
 {
@@ -142,13 +142,13 @@ template <class T> std::string write_engineering_t(T x, int sig_digits)
     return rust_exp((double)x, precision);
 }
 
-// f64::DIGITS - 1 or f32::DIGITS - 1 (output.rs:217-222)
+// f64::DIGITS - 1 or f32::DIGITS - 1 (output.rs:26)
 template <class T> constexpr int sig_digits() { return sizeof(T) == 4 ? 6 - 1 : 15 - 1; }
 
 struct Out {
     std::string s;
     void line(const std::string &l) { s += l + "\n"; }
-    // writeln_3p with a (key, value) pair (output.rs:376-416)
+    // writeln_3p with a (key, value) pair (output.rs:180-227)
     void kv(const char *key, const std::string &v) { s += " " + pad_right(key, 31) + ": " + v + "\n"; }
 };
 
@@ -594,7 +594,7 @@ namespace {
 // and rounds to the value's type.
 template <class T> T cpow(T x, int n) { return (T)std::pow((double)x, (double)n); }
 
-// Text of res.data (output.rs:258-338) in the run's float type.
+// Text of res.data (output.rs:64-142) in the run's float type.
 template <class T> std::string res_data_text(const tp_config *cfg, const tp_final *res)
 {
     Out o;
@@ -782,7 +782,7 @@ template <class T> int dump_results(const tp_config *cfg, const tp_final *res, d
     const std::string report = report_text<T>(cfg, res);
     std::fputs(report.c_str(), stdout);
     std::fflush(stdout);
-    // "[day]-[month repr:short]-[year repr:last_two] [hour]:[minute]:[second]" (output.rs:232-236), UTC.
+    // "[day]-[month repr:short]-[year repr:last_two] [hour]:[minute]:[second]" (output.rs:37-39), UTC.
     std::time_t now = std::time(nullptr);
     std::tm tmv;
     gmtime_r(&now, &tmv);

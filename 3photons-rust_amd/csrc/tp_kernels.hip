@@ -427,7 +427,7 @@ __device__ __forceinline__ int32_t norm_mod(int32_t v)
 //   i <= 24:  X'[i] = X[i] - X[i+31]
 //   25..48:   X'[i] = X[i] - X'[i-24] = X[i] - X[i-24] + X[i+7]
 //   49..55:   X'[i] = X[i] - X[i-24] + X[i-48] - X[i-17]
-// (ranf.rs:120-135 applies the first two lines as in-place updates; the third
+// (ranf.rs:106-119 applies the first two lines as in-place updates; the third
 // is the second with X'[i-24] expanded once more), all mod 1e9.  Every lane
 // evaluates (X[i] - b) + (c - d) with b, c, d read from other lanes of the
 // same wave by ds_bpermute (no LDS storage, no barrier).  Lanes 55..63 hold 0
@@ -667,7 +667,7 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
     hipLaunchKernelGGL(fe_scatter_kernel, dim3(a.nwalkers), dim3(256), 0, st, a, begin, len, off, out);
 }
 
-// seeded_new (ranf.rs:50-82) for jump mode, one thread per batch.
+// seeded_new (ranf.rs:36-66) for jump mode, one thread per batch.
 __global__ void ranf_seed_kernel(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

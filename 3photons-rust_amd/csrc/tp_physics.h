@@ -486,7 +486,7 @@ __device__ __forceinline__ bool generate_f64(const Params &P, const double u[12]
 // Reads the RANF stream expanded in HBM.  Stream position q holds the value
 // at index 55 - q % 55 of round q / 55; random_array::<N> first skips the
 // round's leftovers when fewer than N remain, then returns its N values in
-// ascending array index, i.e. descending stream position (ranf.rs:77-104).
+// ascending array index, i.e. descending stream position (ranf.rs:78-102).
 struct RanfWalk {
     const int32_t *buf; // stream values from position qbase on
     uint64_t qbase;

@@ -28,8 +28,8 @@ struct RanfState {
     int32_t seed;
 };
 
-void ranf_seeded(RanfState &g, int32_t seed);     // IN55 + 10 warm-up rounds (ranf.rs:50-82)
-void ranf_round(int32_t *numbers /*[56]*/);       // IRN55 (ranf.rs:120-135)
+void ranf_seeded(RanfState &g, int32_t seed);     // IN55 + 10 warm-up rounds (ranf.rs:36-66)
+void ranf_round(int32_t *numbers /*[56]*/);       // IRN55 (ranf.rs:106-119)
 
 // Polynomial modulo x^55 + x^31 - 1 with coefficients mod 1e9.
 using RanfPoly = std::vector<uint32_t>; // 55 coefficients, c[k] of x^k
@@ -42,7 +42,7 @@ void ranf_advance(RanfState &g, uint64_t n);
 
 // `faster-evgen` consumes a variable number of draws per event (9 + 6, plus
 // 2 per rejected unit-disc point, each random_array::<N> call first
-// discarding a round's leftovers when fewer than N remain: ranf.rs:77-104,
+// discarding a round's leftovers when fewer than N remain: ranf.rs:78-102,
 // evgen.rs:143-173,221-249), so event starts are found by walking the stream
 // as EventGenerator::simulate_event_batch does (evgen.rs:257-267).  Stream
 // positions q count values from the start of the state's current round

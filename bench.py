@@ -40,22 +40,84 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 
 
-def cpu_baseline(sample_events: int) -> dict:
-    """The oracle (C restatement of the reference CPU path) on the host's
-    cores, multi-threaded over batches like the reference's reproducible
-    multi-threading mode, on a bounded sample of the same workload."""
+# Host cores the measurement box allots to one GPU: the worker-pool size the
+# baseline uses there, whatever the machine's own core count is.
+HOST_CORE_SHARE = 16
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle() -> tuple[str, str]:
+    """The oracle built for this host (make -C oracle native: -O3
+    -march=native -ffp-contract=off), or the portable build if that fails."""
+    import subprocess
+    odir = os.path.join(ROOT, "oracle")
+    try:
+        subprocess.run(["make", "-s", "-C", odir, "native"], check=True, capture_output=True, timeout=300)
+        return os.path.join(odir, "build", "native", "liboracle.so"), "gcc -O3 -march=native -ffp-contract=off"
+    except (OSError, subprocess.SubprocessError) as e:
+        return (os.path.join(odir, "build", "liboracle.so"),
+                f"gcc -O3 -march=x86-64-v2 -ffp-contract=off (native build failed: {type(e).__name__})")
+
+
+def cpu_baseline(sample_events: int, sample_1thread: int):
+    """The oracle (C restatement of the reference CPU path) multi-threaded
+    over batches like the reference's reproducible multi-threading mode
+    (multi_threading.rs:16-75), on a bounded sample of the same workload,
+    on the host's core share and on one thread.  Returns the JSON object and
+    the oracle's accumulator of the sample (for the GPU's error against it)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ffi import Oracle
-    oracle = Oracle()
-    threads = min(16, os.cpu_count() or 1)
+    path, build = native_oracle()
+    oracle = Oracle(path=path)
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(HOST_CORE_SHARE, affinity))
     cfg = oracle.config(VALEURS, sample_events)
-    opts = oracle.opts("xoshiro", threads=threads)
     t0 = time.perf_counter()
-    oracle.run(cfg, opts)
+    merged, _ = oracle.run(cfg, oracle.opts("xoshiro", threads=threads))
     dt = time.perf_counter() - t0
-    return {"value": sample_events / dt, "unit": "events/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample_events:.3g} events of the same xoshiro256+ f64 stream "
-                      f"({(sample_events + 9999) // 10000} batches), oracle/liboracle.so, {dt:.2f} s wall"}
+    cfg1 = oracle.config(VALEURS, sample_1thread)
+    t0 = time.perf_counter()
+    oracle.run(cfg1, oracle.opts("xoshiro", threads=1))
+    dt1 = time.perf_counter() - t0
+    out = {"value": sample_events / dt, "unit": "events/s", "cores": threads, "kind": "port",
+           "sample": f"first {sample_events:.3g} events of the same xoshiro256+ f64 stream "
+                     f"({(sample_events + 9999) // 10000} batches), {dt:.2f} s wall on {threads} threads",
+           "value_1thread": sample_1thread / dt1,
+           "sample_1thread": f"first {sample_1thread:.3g} events, {dt1:.2f} s wall on 1 thread",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "affinity_cpus": affinity,
+           "core_share_note": f"the box allots {HOST_CORE_SHARE} host cores per GPU; the baseline uses "
+                              f"min({HOST_CORE_SHARE}, affinity) threads",
+           "build": build}
+    return out, merged
+
+
+def rel_errors(tp, cfg, acc, ref) -> dict:
+    """Per-quantity relative error of run accumulator acc against ref (both
+    finalised with the same host code): the res.data quantities, the |M|^2
+    sums and second moments; the cancelling interference sums R_MX, I_MX
+    against their statistical scale sqrt(N sum x^2)."""
+    fa, fr = tp.finalize(cfg, acc), tp.finalize(cfg, ref)
+    e = {q: abs(getattr(fa, q) - getattr(fr, q)) / abs(getattr(fr, q))
+         for q in ("sigma", "prec", "variance", "beta_min", "ss_p", "inc_ss_p", "ss_m", "inc_ss_m")}
+    n = cfg.num_events
+    for k, name in enumerate(["A", "B+", "B-", "R_MX", "I_MX"]):
+        if k < 3:
+            e[f"spm2[{name}]"] = abs(acc.spm2[k] - ref.spm2[k]) / abs(ref.spm2[k])
+        else:
+            e[f"spm2[{name}]/stat"] = abs(acc.spm2[k] - ref.spm2[k]) / math.sqrt(n * ref.vars[k])
+        e[f"vars[{name}]"] = abs(acc.vars[k] - ref.vars[k]) / abs(ref.vars[k])
+    e["selected_events_diff"] = int(acc.selected_events) - int(ref.selected_events)
+    return e
 
 
 def traffic_from_profiles(name: str):
@@ -194,6 +256,7 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-events", type=int, default=10_000_000_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
+    ap.add_argument("--cpu-sample-1thread", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
@@ -311,7 +374,20 @@ def main() -> int:
     if not args.no_flow:
         out["flow"] = measure_flow(tp, eng, cfg, value, acc64, args.events, args.flow_steps, args.flow_batch)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+        out["cpu_baseline"], oacc = cpu_baseline(args.cpu_sample, args.cpu_sample_1thread)
+        # the FP64 oracle's result on the sample against the GPU's f64 and
+        # mixed runs of the same events
+        ns = args.cpu_sample
+        cs = cfg.with_events(ns)
+        ref = tp.TpAcc.from_buffer_copy(bytes(oacc))
+        accs = {}
+        for name, prec in (("f64", tp.PREC_F64), ("mixed", tp.PREC_MIXED)):
+            _, _, _, accs[name] = measure(tp, torch, eng, tp.params_from_config(cs, precision=prec), ns, 1, 0,
+                                          per_batch, merged)
+        out["rel_err_vs_oracle"] = {"events": ns, "f64": rel_errors(tp, cs, accs["f64"], ref)}
+        if "fp32_mixed" in out:
+            out["fp32_mixed"]["rel_err_vs_oracle"] = rel_errors(tp, cs, accs["mixed"], ref)
+            out["fp32_mixed"]["rel_err_vs_oracle_events"] = ns
     print(json.dumps(json_safe(out), allow_nan=False))
     eng.close()
     return 0

@@ -95,9 +95,11 @@ typedef struct {
     double variance;
 } tp_acc;
 
-/* Generator states, field for field the reference structs.
- * RANF: ranf.rs:27-38 (numbers[0] unused, index = values left in the round,
- * seed = current seed for jump()).  xoshiro256+: rand_xoshiro's [u64; 4]. */
+/* Generator states: the values of the reference's generators, not their
+ * memory layout (a caller copies in and out).
+ * RANF: ranf.rs:12-22 (numbers[0] unused, index = values left in the round
+ * (usize there), seed = current seed for jump(), a field the reference has
+ * only with faster-threading).  xoshiro256+: rand_xoshiro's four words. */
 typedef struct {
     int32_t numbers[56];
     uint32_t index;
@@ -127,18 +129,18 @@ int tp_destroy(void *ctx);
 const char *tp_last_error(const void *ctx);
 
 /* Initial stream of a run: RandomGenerator::new() (scheduling/mod.rs:40 ->
- * ranf.rs:44-48 seed 234612947, standard.rs:21-24 seed_from_u64(12345)).
+ * ranf.rs:28-32 seed 234612947, standard.rs:21-24 seed_from_u64(12345)).
  * state points to tp_ranf_state or tp_xoshiro_state. */
 int tp_rng_init(int rng_kind, void *state);
 /* Same with an explicit seed (RANF seeded_new / xoshiro seed_from_u64). */
 int tp_rng_seed(int rng_kind, uint64_t seed, void *state);
 
-/* RandomGenerator::skip (ranf.rs:137-143, standard.rs:194-200): advance the
+/* RandomGenerator::skip (ranf.rs:122-127, standard.rs:37-43): advance the
  * state as n calls of random() would, in O(log n) (polynomial / GF(2)
  * skip-ahead instead of stepping). */
 int tp_rng_skip(int rng_kind, void *state, uint64_t n);
-/* RandomGenerator::jump (ranf.rs:151-156 reseed with seed+123456,
- * standard.rs:208-212 the 2^128 jump). */
+/* RandomGenerator::jump (ranf.rs:135-140 reseed with seed+123456,
+ * standard.rs:51-55 the 2^128 jump). */
 int tp_rng_jump(int rng_kind, void *state);
 
 /* Drop-in for one call of the per-batch closure (main.rs:103-128):

@@ -64,13 +64,13 @@ int tp_finalize_f32(const tp_config *cfg, const tp_acc *merged, tp_final *out);
 
 /* Outputs of an f32 run: the values printed as f32 (Display as the shortest
  * f32 round trip, write_engineering with f32::DIGITS - 1 = 5 significant
- * digits, output.rs:26,217-269) and the report computed in f32. */
+ * digits, output.rs:26,180-269) and the report computed in f32. */
 int tp_config_format_f32(const tp_config *cfg, char *buf, int buf_len);
 int tp_format_res_data_f32(const tp_config *cfg, const tp_final *res, char *buf, int buf_len);
 int tp_format_report_f32(const tp_config *cfg, const tp_final *res, char *buf, int buf_len);
 int tp_dump_results_f32(const tp_config *cfg, const tp_final *res, double elapsed_s, const char *dir);
 
-/* Text of res.data (output.rs:258-338) into buf; returns bytes needed. */
+/* Text of res.data (output.rs:64-142) into buf; returns bytes needed. */
 int tp_format_res_data(const tp_config *cfg, const tp_final *res, char *buf, int buf_len);
 
 /* Text of FinalResults::eric + fawzi (resfin.rs:66-194) into buf. */

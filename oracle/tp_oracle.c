@@ -29,10 +29,10 @@
 /* ======================================================================== */
 
 #define RANF_MODULO 1000000000
-#define RANF_DEFAULT_SEED 234612947 /* ranf.rs:44-48 */
+#define RANF_DEFAULT_SEED 234612947 /* ranf.rs:28-32 */
 #define XOSHIRO_DEFAULT_SEED 12345u /* standard.rs:21-24 */
 
-/* IRN55 round (ranf.rs:120-135). */
+/* IRN55 round (ranf.rs:106-119). */
 static void ranf_reset(or_rng_t *g)
 {
     int i;
@@ -48,7 +48,7 @@ static void ranf_reset(or_rng_t *g)
     }
 }
 
-/* IN55 initialisation + 10 warm-up rounds (ranf.rs:50-82).  Rust release
+/* IN55 initialisation + 10 warm-up rounds (ranf.rs:36-66).  Rust release
  * builds wrap on i32 overflow; the arithmetic is done in uint32 to mirror that
  * for the (unused by the default configuration) negative-seed cases. */
 static void ranf_seeded(or_rng_t *g, int32_t seed)
@@ -71,7 +71,7 @@ static void ranf_seeded(or_rng_t *g, int32_t seed)
         ranf_reset(g);
 }
 
-/* random_array::<N> (ranf.rs:92-118): N values taken in ascending index order
+/* random_array::<N> (ranf.rs:78-102): N values taken in ascending index order
  * from the top of the unconsumed part of the round; a round with fewer than N
  * values left is discarded. */
 static void ranf_array(or_rng_t *g, int n, or_scalar *out)
@@ -213,7 +213,7 @@ uint64_t OR_FN(or_rng_next_raw)(or_rng_t *g)
 #endif
 }
 
-/* One uniform: RANF int * 1e-9 (ranf.rs:115), or rand's Standard
+/* One uniform: RANF int * 1e-9 (ranf.rs:99), or rand's Standard
  * distribution: (u64 >> 11) * 2^-53 for f64, (u32 >> 8) * 2^-24 for f32. */
 or_scalar OR_FN(or_rng_random)(or_rng_t *g)
 {
@@ -251,7 +251,7 @@ void OR_FN(or_rng_skip)(or_rng_t *g, uint64_t n)
         (void)OR_FN(or_rng_next_raw)(g);
 }
 
-/* jump(): RANF reseeds with seed + 123456 (ranf.rs:151-156, i32 wrapping);
+/* jump(): RANF reseeds with seed + 123456 (ranf.rs:135-140, i32 wrapping);
  * xoshiro applies its published jump polynomial (standard.rs:52-55). */
 void OR_FN(or_rng_jump)(or_rng_t *g)
 {

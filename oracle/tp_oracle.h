@@ -38,13 +38,13 @@ typedef double or_scalar;
 /* Random stream kinds. */
 enum { OR_RNG_RANF = 0, OR_RNG_XOSHIRO = 1 };
 
-/* One generator: RANF (ranf.rs:27-38) or xoshiro (xoshiro256+ in f64 builds,
+/* One generator: RANF (ranf.rs:12-22) or xoshiro (xoshiro256+ in f64 builds,
  * xoshiro128+ in f32 builds, standard.rs:8-11). */
 typedef struct {
     int32_t kind;
-    int32_t ranf_seed;      /* current seed, used by jump() (ranf.rs:151-156) */
-    int32_t numbers[56];    /* ranf.rs:33-34 */
-    uint32_t index;         /* ranf.rs:36-37 */
+    int32_t ranf_seed;      /* current seed, used by jump() (ranf.rs:14-15,135-140) */
+    int32_t numbers[56];    /* ranf.rs:18 */
+    uint32_t index;         /* ranf.rs:21 */
     uint32_t pad;
     uint64_t xs[4];         /* xoshiro256+ state */
     uint32_t xs32[4];       /* xoshiro128+ state */

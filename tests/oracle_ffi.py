@@ -55,9 +55,9 @@ def _final_type(f32):
 
 
 class Oracle:
-    def __init__(self, f32: bool = False):
+    def __init__(self, f32: bool = False, path: str | None = None):
         self.f32 = f32
-        path = os.path.join(ORACLE_DIR, "liboracle_f32.so" if f32 else "liboracle.so")
+        path = path or os.path.join(ORACLE_DIR, "liboracle_f32.so" if f32 else "liboracle.so")
         if not os.path.exists(path):
             raise FileNotFoundError(f"oracle not built: {path} (run make -C oracle)")
         self.lib = C.CDLL(path)

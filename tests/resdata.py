@@ -1,4 +1,4 @@
-"""The numbers res.data prints (output.rs:258-338), recomputed from a final
+"""The numbers res.data prints (output.rs:64-142), recomputed from a final
 result in its own precision, in file order -- for comparing unrounded values
 against the golden files' printed digits."""
 from __future__ import annotations

@@ -192,3 +192,26 @@ def test_f32_ranf_batches_past_the_first_chunk_match_oracle(engine, cfg, oracle3
             close_f32(a, o, 10000)
         else:  # a batch whose f32 spinor products overflow: the same NaN sums on both sides
             assert a[0] == o[0] and np.array_equal(np.isnan(a), np.isnan(o)), (b, a, o)
+
+
+@pytest.mark.parametrize("prec", [tp.PREC_F32, tp.PREC_F64])
+def test_device_fold_equals_host_fold(engine, cfg, prec):
+    """merge_kernel<float> (all-f32 runs) and merge_kernel<double>: the
+    device's in-order fold of 2 345 batches (ten LDS tiles, the last partial)
+    is the host's left fold, bit for bit."""
+    import torch
+    n = 23_450_000
+    kind = tp.RNG_XOSHIRO128 if prec == tp.PREC_F32 else tp.RNG_XOSHIRO
+    print(f"device fold precision={prec} N={n}", flush=True)
+    nb = (n + tp.BATCH_SIZE - 1) // tp.BATCH_SIZE
+    dev = torch.device("cuda:0")
+    per_dev = torch.zeros(nb * 104, dtype=torch.uint8, device=dev)
+    merged_dev = torch.zeros(104, dtype=torch.uint8, device=dev)
+    params = tp.params_from_config(cfg.with_events(n), precision=prec)
+    engine.run_device(params, kind, tp.SEED_SKIPAHEAD, tp.rng_state(kind), n, per_dev.data_ptr(),
+                      merged_dev.data_ptr())
+    engine.sync()
+    per = np.frombuffer(per_dev.cpu().numpy().tobytes(), dtype=tp.ACC_DTYPE)
+    dfold = tp.TpAcc.from_buffer_copy(merged_dev.cpu().numpy().tobytes())
+    hfold = tp.merge(per, f32=prec == tp.PREC_F32)
+    np.testing.assert_array_equal(dfold.as_array(), hfold.as_array())

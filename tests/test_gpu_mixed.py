@@ -74,3 +74,103 @@ def test_run_mixed_vs_f64(engine, cfg):
     # a negligible fraction of it
     stat = np.sqrt(m64.vars[4])
     assert abs(m32.spm2[4] - m64.spm2[4]) < 1e-3 * stat
+
+
+# ---- against the FP64 oracle (config 3: every mixed result carries its error
+# against the oracle's f64 result on the same stream) -------------------------
+NAMES = ["A", "B+", "B-", "R_MX", "I_MX"]
+
+
+def mixed_errors(a, b, n_events):
+    """Per-quantity relative errors of accumulator a (mixed) against b (f64
+    oracle): the |M|^2 sums A, B+, B- and sigma relative to themselves, the
+    cancelling interference sums R_MX, I_MX relative to their statistical
+    scale sqrt(n sum x^2), the second moments relative to themselves."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    e = {}
+    for k in range(5):
+        if k < 3:
+            e[f"spm2[{NAMES[k]}]"] = abs(a[1 + k] - b[1 + k]) / abs(b[1 + k])
+        else:
+            e[f"spm2[{NAMES[k]}]/stat"] = abs(a[1 + k] - b[1 + k]) / np.sqrt(n_events * b[6 + k])
+        e[f"vars[{NAMES[k]}]"] = abs(a[6 + k] - b[6 + k]) / abs(b[6 + k])
+    e["sigma"] = abs(a[11] - b[11]) / abs(b[11])
+    e["variance"] = abs(a[12] - b[12]) / abs(b[12])
+    return e
+
+
+# Budget per quantity for a batch (or run) of >= 1e4 events: the f32 spinor
+# and amplitude algebra leaves per-event |M|^2 errors of ~2e-6 (median), which
+# average down in the sums.
+BUDGET = {"spm2": 1e-5, "stat": 1e-6, "vars": 5e-5, "sigma": 1e-6, "variance": 5e-5}
+
+
+def assert_budget(e):
+    for k, v in e.items():
+        lim = BUDGET["stat"] if k.endswith("/stat") else BUDGET[k.split("[")[0]]
+        assert v <= lim, (k, v, lim)
+
+
+@pytest.mark.parametrize("rng", ["xoshiro", "ranf"])
+@pytest.mark.parametrize("n_events", [9999, 35017])
+def test_mixed_per_batch_vs_oracle(engine, cfg, oracle, valeurs, rng, n_events):
+    kind = {"ranf": tp.RNG_RANF, "xoshiro": tp.RNG_XOSHIRO}[rng]
+    print(f"mixed vs oracle per batch rng={rng} N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    per, merged, _ = engine.run(tp.params_from_config(c, precision=tp.PREC_MIXED), kind, tp.SEED_SKIPAHEAD,
+                                tp.rng_state(kind), n_events)
+    om, oper = oracle.run(oracle.config(valeurs, n_events), oracle.opts(rng, threads=4), per_batch=True)
+    worst = {}
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        o = oper[b].as_array()
+        assert a[0] == o[0]
+        e = mixed_errors(a, o, min(10000, n_events - 10000 * b))
+        assert_budget(e)
+        worst = {k: max(v, worst.get(k, 0.0)) for k, v in e.items()}
+    print("  worst per-batch errors: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+
+
+def test_mixed_finals_vs_oracle(engine, cfg, oracle_run):
+    """N = 1e7 (xoshiro256+): every res.data quantity of the mixed run against
+    the oracle's f64 run."""
+    n = 10_000_000
+    print(f"mixed finals vs oracle N={n}", flush=True)
+    c = cfg.with_events(n)
+    per, merged, _ = engine.run(tp.params_from_config(c, precision=tp.PREC_MIXED), tp.RNG_XOSHIRO,
+                                tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), n)
+    ocfg, omerged, _, ofin = oracle_run("xoshiro", False, True, False)
+    assert merged.selected_events == omerged.selected
+    e = mixed_errors(merged.as_array(), omerged.as_array(), n)
+    assert_budget(e)
+    fin = tp.finalize(c, merged)
+    for q in QUANTITIES:
+        e[q] = abs(getattr(fin, q) - getattr(ofin, q)) / abs(getattr(ofin, q))
+    print("  " + ", ".join(f"{k} {v:.1e}" for k, v in e.items()))
+    for q in ("sigma", "prec", "variance", "beta_min", "ss_p", "ss_m"):
+        assert e[q] <= 1e-6, (q, e[q])
+    for q in ("inc_ss_p", "inc_ss_m"):
+        assert e[q] <= 1e-5, (q, e[q])
+
+
+def test_mixed_full_size_1e10(engine, cfg):
+    """Config 3 at full size: 1e10 events (1e6 batches) of the xoshiro256+
+    stream, mixed against f64 on the GPU (the oracle would need hours):
+    identical selection, sigma within 1e-6, no non-finite batch."""
+    n = 10_000_000_000
+    c = cfg.with_events(n)
+    out = {}
+    for prec in (tp.PREC_MIXED, tp.PREC_F64):
+        print(f"full-size run precision={prec} N={n}", flush=True)
+        per, merged, _ = engine.run(tp.params_from_config(c, precision=prec), tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD,
+                                    tp.rng_state(tp.RNG_XOSHIRO), n)
+        finite = (np.isfinite(per["spm2"]).all() and np.isfinite(per["vars"]).all()
+                  and np.isfinite(per["sigma"]).all() and np.isfinite(per["variance"]).all())
+        assert finite, f"non-finite batch at precision {prec}"
+        assert int(per["selected_events"].sum()) == merged.selected_events
+        out[prec] = (merged, tp.finalize(c, merged))
+    (mm, fm), (m64, f64) = out[tp.PREC_MIXED], out[tp.PREC_F64]
+    print(f"  selected {mm.selected_events} (f64 {m64.selected_events}); sigma rel {abs(fm.sigma / f64.sigma - 1):.1e}")
+    assert mm.selected_events == m64.selected_events
+    assert abs(fm.sigma / f64.sigma - 1) <= 1e-6

@@ -56,20 +56,24 @@ def test_events_match_oracle(engine, cfg, oracle, valeurs, rng, sort):
     ph = op[:, 2:, :]
     escale = np.abs(ph[:, :, 3]).max(axis=1)[:, None, None]
     dp = np.abs(p - ph) / np.spacing(escale)
+    edges = [0, 0.5, 1.5, 2.5, 4.5, 8.5, 16.5, np.inf]
+    hist, _ = np.histogram(dp, bins=edges)
     print(f"momenta ulp(E): max {dp.max():.1f} p99 {np.percentile(dp, 99):.1f} p99.9 {np.percentile(dp, 99.9):.1f} "
-          f"exact {np.mean(dp == 0):.3f}")
+          f"exact {np.mean(dp == 0):.4f}; histogram [0, 1, 2, 3-4, 5-8, 9-16, >16] = {hist.tolist()}")
     # sin/cos/log are correctly rounded on both sides but for the rare
     # arguments next to a rounding boundary (about 1 in 700 calls), where the
     # device and glibc can differ by 1 ulp; RAMBO's conformal map
-    # (M^2 = R_E^2 - |R|^2) can amplify such a difference.
-    assert np.mean(dp == 0) > 0.99 and np.percentile(dp, 99) <= 1 and dp.max() <= 256
+    # (M^2 = R_E^2 - |R|^2) amplifies such a difference by up to
+    # R_E^2 / M^2 (measured: at most 14 ulp(E) over these events).
+    assert np.mean(dp == 0) > 0.99 and np.percentile(dp, 99) <= 1 and dp.max() <= 16
     k = okeep
     mk, omk = m[k], om[k]
     rel = np.abs(mk[:, :3] - omk[:, :3]) / np.abs(omk[:, :3])
     scale_x = 2 * np.sqrt(omk[:, 0] * omk[:, 1])
     relx = np.abs(mk[:, 3:] - omk[:, 3:]) / scale_x[:, None]
-    print(f"m rel: max {rel.max():.2e} mixed {relx.max():.2e}; exact fraction {np.mean(mk == omk):.3f}")
-    assert rel.max() < 1e-11 and relx.max() < 1e-11
+    print(f"m rel: max {rel.max():.2e} mixed {relx.max():.2e}; exact fraction {np.mean(mk == omk):.4f}")
+    # measured: 3.3e-13 (the momenta's few-ulp differences, propagated)
+    assert rel.max() < 5e-13 and relx.max() < 5e-13
     assert np.mean(mk == omk) > 0.98
     assert np.all(m[~k] == 0)
 
@@ -136,7 +140,7 @@ def test_simulate_batch_is_the_closure(engine, cfg, oracle, valeurs, rng):
     g = oracle.rng(rng)
     ocfg = oracle.config(valeurs)
     opts = oracle.opts(rng)
-    for n in (10000, 7, 0, 1, 23456):
+    for n in (10000, 7, 0, 1, 23456, 640000):
         print(f"simulate_batch rng={rng} n={n}", flush=True)
         acc = engine.simulate_batch(params, RNG[rng], st, n)
         oacc = oracle.simulate_batch(ocfg, opts, g, n)
@@ -221,6 +225,13 @@ GOLDEN = [("features_", "ranf", "skipahead", True, False),
           ("features_faster-evgen", "ranf", "skipahead", True, True)]
 
 
+# Golden files whose GPU res.data differs from the reference's in the last
+# printed digit of some field (the GPU sums each batch in another order, a
+# relative difference of ~1e-16 that can cross a rounding boundary of the
+# 14-digit print); all others are reproduced character for character.
+LAST_DIGIT: set = set()
+
+
 @pytest.mark.parametrize("suffix,rng,seeding,sort,fe", GOLDEN)
 def test_full_run_reproduces_golden(engine, cfg, oracle_run, suffix, rng, seeding, sort, fe):
     """N = 1e7 with the shipped valeurs: res.data numerically equal to the
@@ -235,16 +246,26 @@ def test_full_run_reproduces_golden(engine, cfg, oracle_run, suffix, rng, seedin
         assert abs(a - b) <= 1e-12 * abs(b), (f, a, b)
     sp, osp = np.array(res.final.spm2), np.array(ofin.spm2)
     ok = np.abs(sp - osp) <= 1e-12 * np.abs(osp)
-    ok[:, 4] = np.abs(sp[:, 4] - osp[:, 4]) <= 1e-9 * np.abs(osp[:, 4])  # I_MX cancels (printed rel. error ~3)
+    # I_MX cancels (its printed relative error is ~3): bound it by its
+    # statistical scale, 1e-12 sqrt(N) |I_MX| vars[I_MX], as acc_close bounds
+    # the batch sums (1e-12 sqrt(N sum x^2))
+    n = cfg.num_events
+    stat = np.sqrt(n) * np.abs(osp[:, 4]) * np.array(ofin.vars)[:, 4]
+    dimx = np.abs(sp[:, 4] - osp[:, 4])
+    print(f"  I_MX: |GPU - oracle| = {dimx.max():.2e} ({(dimx / np.abs(osp[:, 4])).max():.1e} relative, "
+          f"{(dimx / stat).max():.1e} x 1e-12 of the statistical scale)")
+    ok[:, 4] = dimx <= 1e-12 * stat
     assert ok.all(), (sp, osp)
     text = tp.format_res_data(cfg, res.final)
     golden = open(golden_path("res.data-" + suffix)).read()
-    # one unit in the last printed digit; the jump-mode xoshiro file was written
-    # with a completion-order merge and CI compares it with -a 1.1e-16
-    # (ci.yml:184-186)
+    # the jump-mode xoshiro file was written with a completion-order merge and
+    # CI compares it with -a 1.1e-16 (ci.yml:184-186); every other file
+    # compares with zero tolerance, as the reference's CI does
     abs_ = 1.1e-16 if seeding == "jump" and rng == "xoshiro" else 0.0
-    errs = compare_texts(golden, text, abs_=abs_, digits_slack=1.5)
-    assert errs == [], errs
+    exact = compare_texts(golden, text, abs_=abs_)
+    print(f"  res.data vs golden, zero slack: {len(exact)} fields differ {exact}")
+    assert exact == [] or suffix in LAST_DIGIT, exact
+    assert compare_texts(golden, text, abs_=abs_, digits_slack=1.0) == []
 
 
 def test_cli_end_to_end(tmp_path, valeurs):

@@ -25,7 +25,7 @@ def test_init_matches_oracle(oracle):
 
 def test_ranf_is_knuth_recurrence(oracle):
     """Generation order: X_g = X_{g-55} - X_{g-24} mod 1e9; each round of 55 is
-    emitted from index 55 down to 1 (ranf.rs:92-135)."""
+    emitted from index 55 down to 1 (ranf.rs:95-96,106-119)."""
     g = oracle.rng("ranf")
     draws = oracle.raw(g, 55 * 40).astype(np.int64)
     rounds = draws.reshape(40, 55)[:, ::-1].reshape(-1)  # generation order

@@ -1,6 +1,6 @@
 """faster-evgen: event start positions in the RANF stream depend on every
 earlier draw (evgen.rs:143-173, 221-249: unit-disc re-rolls, and
-random_array::<N> discarding a round's leftovers, ranf.rs:77-104).  Long
+random_array::<N> discarding a round's leftovers, ranf.rs:78-102).  Long
 walks split the stream over host threads that start speculatively at round
 boundaries and are joined where the true walk meets them; the result must
 be the sequential walk's, bit for bit."""
