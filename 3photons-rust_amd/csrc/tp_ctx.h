@@ -46,7 +46,7 @@ struct Tables { // device copies of GF(2) jump tables for one (seeding, batch si
 struct Ctx {
     int device = 0;
     int cus = 1;                    // compute units of the device
-    int blocks_per_cu[4][2] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
+    int blocks_per_cu[5][2] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
@@ -71,6 +71,10 @@ struct Ctx {
     // device walk: walkers' start lists, their counts and join indices, the
     // joined pieces (begin, length, output offset), and host staging of them
     DevBuf fe_lists, fe_meta, fe_piece;
+    DevBuf fe_flag;  // jump seeding: set by a batch walk that left its piece
+    DevBuf xo_states, xo_draws; // faster-evgen on xoshiro256+: piece states and uniforms
+    DevBuf adj_u;               // tp_adjoint: the stream's points, expanded
+    uint32_t fe_flag_host = 0;
     std::vector<uint32_t> fe_counts, fe_beg, fe_len;
     std::vector<int32_t> fe_jprev, fe_jnext;
     std::vector<uint64_t> fe_off;

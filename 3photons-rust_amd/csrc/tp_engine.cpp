@@ -308,8 +308,9 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
         return fail(c, TP_ERR_UNSUPPORTED, "precision " + std::to_string(params->precision) + " not available");
     if ((params->precision == TP_PREC_F32) != (rng_kind == TP_RNG_XOSHIRO128) && rng_kind != TP_RNG_RANF)
         return fail(c, TP_ERR_ARG, "xoshiro128+ is the stream of the all-f32 mode, xoshiro256+ of the others");
-    if (params->faster_evgen && (rng_kind != TP_RNG_RANF || params->precision != TP_PREC_F64))
-        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available with the RANF stream in f64");
+    if (params->faster_evgen && (rng_kind == TP_RNG_XOSHIRO128 || params->precision == TP_PREC_MIXED))
+        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available in f64 (RANF, or xoshiro256+ with jump "
+                                           "seeding) and in f32 (RANF with jump seeding)");
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
         return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
     return TP_OK;
@@ -516,6 +517,223 @@ int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs
     return TP_OK;
 }
 
+// RANF with `faster-evgen` and jump seeding (faster-threading): batch b
+// reseeds (seeded_new(seed + 123456 b), multi_threading.rs:64-69 and
+// ranf.rs:135-140; batch 0 runs on the caller's state), so its stream is a
+// piece of its own: each chunk of batches is expanded piece by piece, walked
+// one thread per batch, and simulated from those starts.  A batch consumes
+// about 179 000 positions (17.9 per event, at most a few hundred more than
+// that in any batch of 10 000 events); a piece holds 220 000.
+constexpr uint64_t FE_JUMP_PIECE = 55 * 4000;
+static_assert(FE_JUMP_PIECE % 55 == 0, "pieces keep every position's place in its round");
+
+int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedArgs &a, const tp_ranf_state *rs,
+                                   uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
+{
+    const bool f32 = params->precision == TP_PREC_F32;
+    if (batch_size > 10u * TP_BATCH_SIZE)
+        return fail(c, TP_ERR_ARG, "faster-evgen with jump seeding runs batches of at most 100 000 events");
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    const uint64_t chunk = std::min<uint64_t>(nb, FE_WALK_BATCHES);
+    const uint64_t PS = FE_JUMP_PIECE * ((batch_size + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE);
+    const uint64_t max_rounds = PS / 55 + 2;
+    const int K = std::max(1, bits_for(max_rounds / RANF_R + 1));
+    int rc = ensure_polys(c, RANF_R, K);
+    if (!rc)
+        rc = ensure(c, c->draws, chunk * PS * sizeof(int32_t));
+    if (!rc)
+        rc = ensure(c, c->origin, chunk * 56 * sizeof(int32_t));
+    if (!rc)
+        rc = ensure(c, c->p0, chunk * sizeof(int32_t));
+    if (!rc)
+        rc = ensure(c, c->evq, chunk * batch_size * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->fe_flag, sizeof(uint32_t));
+    if (!rc && f32)
+        rc = ensure(c, c->f32ev, chunk * batch_size * 6 * sizeof(float));
+    if (rc)
+        return rc;
+    tpk::F32Args fa;
+    std::memset(&fa, 0, sizeof fa);
+    if (f32) {
+        fa.P = make_fparams(params);
+        fa.num_events = num_events;
+        fa.batch_size = batch_size;
+        fa.source = 2;
+        fa.ev = static_cast<float *>(c->f32ev.p);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // previous users of the staging copies are done
+    std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
+    c->p00 = static_cast<int32_t>(55u - rs->index);
+    c->fe_flag_host = 0;
+    HIPCHK(c, hipMemcpyAsync(c->fe_flag.p, &c->fe_flag_host, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    auto expand = [&](uint64_t first_piece, uint64_t npieces, uint64_t d_begin, uint64_t d_end) -> int {
+        tpk::RanfArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.origin = static_cast<const int32_t *>(c->origin.p) + 56 * first_piece;
+        r.p0 = static_cast<const int32_t *>(c->p0.p) + first_piece;
+        r.polys = static_cast<const uint32_t *>(c->polys.p);
+        r.K = K;
+        r.R = RANF_R;
+        r.out = static_cast<int32_t *>(c->draws.p) + first_piece * PS;
+        r.d_begin = d_begin;
+        r.d_end = d_end;
+        r.seg_first = 0;
+        r.nsegs = (max_rounds + RANF_R - 1) / RANF_R + 1;
+        r.piece_stride = PS;
+        tpk::launch_ranf_expand(r, npieces, c->stream);
+        HIPCHK(c, hipGetLastError());
+        return TP_OK;
+    };
+    bool first = true;
+    for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
+        if (first)
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        tpk::launch_ranf_seed(rs->seed, b0, cnb, static_cast<int32_t *>(c->origin.p), static_cast<int32_t *>(c->p0.p),
+                              c->stream);
+        HIPCHK(c, hipGetLastError());
+        if (b0 == 0) {
+            // batch 0 runs on the caller's state, which may be inside its
+            // round: its piece is expanded from the round's start (d_begin
+            // = -p0, modulo 2^64), so that buffer index = frame position
+            HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
+            const uint64_t p00 = (uint64_t)c->p00;
+            rc = expand(0, 1, 0ull - p00, PS - p00);
+            if (!rc && cnb > 1)
+                rc = expand(1, cnb - 1, 0, PS);
+        } else {
+            rc = expand(0, cnb, 0, PS);
+        }
+        if (rc)
+            return rc;
+        tpk::FeBatchWalkArgs w;
+        std::memset(&w, 0, sizeof w);
+        w.buf = static_cast<const int32_t *>(c->draws.p);
+        w.piece_stride = PS;
+        w.p0 = static_cast<const int32_t *>(c->p0.p);
+        w.nb = cnb;
+        w.b_first = b0;
+        w.num_events = num_events;
+        w.batch_size = batch_size;
+        w.evq = static_cast<uint64_t *>(c->evq.p);
+        w.overflow = static_cast<uint32_t *>(c->fe_flag.p);
+        w.f32 = f32 ? 1 : 0;
+        tpk::launch_fe_walk_batches(w, c->stream);
+        HIPCHK(c, hipGetLastError());
+        if (first) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            first = false;
+        }
+        if (f32) {
+            fa.nb = cnb;
+            fa.b_first = b0;
+            fa.draws = static_cast<const int32_t *>(c->draws.p);
+            fa.qbase = 0;
+            fa.evq = static_cast<const uint64_t *>(c->evq.p);
+            fa.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+            tpk::launch_f32_events(fa, f32_grid(c, cnb), c->stream);
+            HIPCHK(c, hipGetLastError());
+            tpk::launch_f32_accumulate(fa, f32_grid(c, cnb), c->stream);
+            HIPCHK(c, hipGetLastError());
+            continue;
+        }
+        a.nb = cnb;
+        a.b_first = b0;
+        a.draws = static_cast<const int32_t *>(c->draws.p);
+        a.qbase = 0;
+        a.evq = static_cast<const uint64_t *>(c->evq.p);
+        a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+        tpk::launch_fused(a, 3, grid_for(c, 3, 0, cnb), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->fe_flag_host, c->fe_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->fe_flag_host)
+        return fail(c, TP_ERR_INTERNAL, "faster-evgen: a batch consumed more of its stream than was expanded");
+    return TP_OK;
+}
+
+// xoshiro256+ with `faster-evgen` and jump seeding (faster-threading with
+// standard-random): batch b's stream starts at J^b s0 (J the 2^128 jump,
+// standard.rs:51-55) and random_array::<N> takes N consecutive draws with no
+// round to discard (standard.rs:33-35).  Each chunk's batches are expanded
+// into HBM as f64 uniforms, one piece per batch, walked one thread per batch,
+// and simulated from those starts.  A batch consumes about 166 000 draws
+// (16.6 per event); a piece holds 200 000.
+constexpr uint64_t FE_XO_PIECE = 200000;
+constexpr uint32_t FE_XO_PER_THREAD = 3125; // one wavefront expands a piece
+
+int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_state *x, uint64_t num_events,
+                                 uint32_t batch_size, void *per_batch_dev)
+{
+    if (batch_size > TP_BATCH_SIZE)
+        return fail(c, TP_ERR_ARG, "faster-evgen with jump seeding runs batches of at most 10 000 events");
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    const uint64_t chunk = std::min<uint64_t>(nb, FE_WALK_BATCHES);
+    Tables *t = nullptr;
+    int rc = ensure_tables(c, TP_SEED_JUMP, batch_size, nb, grid_for(c, 4, 0, chunk), &t);
+    if (!rc)
+        rc = ensure(c, c->xo_states, chunk * 4 * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->xo_draws, chunk * FE_XO_PIECE * sizeof(double));
+    if (!rc)
+        rc = ensure(c, c->evq, chunk * batch_size * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->fe_flag, sizeof(uint32_t));
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->fe_flag_host = 0;
+    HIPCHK(c, hipMemcpyAsync(c->fe_flag.p, &c->fe_flag_host, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    tpk::XoPieceArgs p;
+    std::memset(&p, 0, sizeof p);
+    std::memcpy(p.s0, x->s, sizeof p.s0);
+    p.bt = static_cast<const uint64_t *>(t->bt.p);
+    p.kb = t->kb;
+    p.mt = static_cast<const uint64_t *>(c->mt.p);
+    p.km = c->km;
+    p.states = static_cast<uint64_t *>(c->xo_states.p);
+    p.piece_len = FE_XO_PIECE;
+    p.per_thread = FE_XO_PER_THREAD;
+    p.xbuf = static_cast<double *>(c->xo_draws.p);
+    p.num_events = num_events;
+    p.batch_size = batch_size;
+    p.evq = static_cast<uint64_t *>(c->evq.p);
+    p.overflow = static_cast<uint32_t *>(c->fe_flag.p);
+    static_assert(FE_XO_PIECE % FE_XO_PER_THREAD == 0, "whole threads per piece");
+    bool first = true;
+    for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
+        if (first)
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        p.nb = cnb;
+        p.b_first = b0;
+        tpk::launch_xo_pieces(p, c->stream);
+        HIPCHK(c, hipGetLastError());
+        a.nb = cnb;
+        a.b_first = b0;
+        a.xdraws = p.xbuf;
+        a.qbase = 0;
+        a.evq = p.evq;
+        a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+        if (first) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            first = false;
+        }
+        tpk::launch_fused(a, 4, grid_for(c, 4, 0, cnb), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->fe_flag_host, c->fe_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->fe_flag_host)
+        return fail(c, TP_ERR_INTERNAL, "faster-evgen: a batch consumed more of its stream than was expanded");
+    return TP_OK;
+}
+
 // Enqueue a whole run (or one closure-sized batch when batch_size == n).
 int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, const void *state0,
                 uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
@@ -548,6 +766,13 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
             return rc;
         fa.ev = static_cast<float *>(c->f32ev.p);
     }
+    if (rng_kind == TP_RNG_XOSHIRO && params->faster_evgen) {
+        if (seeding != TP_SEED_JUMP)
+            return fail(c, TP_ERR_UNSUPPORTED,
+                        "faster-evgen with the xoshiro256+ stream runs with jump seeding (faster-threading) only");
+        return enqueue_xo_faster_evgen_jump(c, a, static_cast<const tp_xoshiro_state *>(state0), num_events,
+                                            batch_size, per_batch_dev);
+    }
     if (rng_kind == TP_RNG_XOSHIRO) {
         const tp_xoshiro_state *x = static_cast<const tp_xoshiro_state *>(state0);
         std::memcpy(a.s0, x->s, sizeof a.s0);
@@ -574,8 +799,10 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
     if (rs->index > 55)
         return fail(c, TP_ERR_ARG, "RANF state index out of range");
     if (params->faster_evgen) {
-        if (seeding != TP_SEED_SKIPAHEAD)
-            return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen runs use skip-ahead seeding");
+        if (seeding == TP_SEED_JUMP)
+            return enqueue_ranf_faster_evgen_jump(c, params, a, rs, num_events, batch_size, per_batch_dev);
+        if (f32)
+            return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen in f32 runs with jump seeding (faster-threading) only");
         return enqueue_ranf_faster_evgen(c, a, rs, num_events, batch_size, per_batch_dev);
     }
     const uint64_t chunk = std::min<uint64_t>(nb, RANF_CHUNK_BATCHES);
@@ -711,7 +938,7 @@ extern "C" int tp_create(int device, void **ctx_out)
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
             c->cus = prop.multiProcessorCount;
-        for (int src = 0; src < 4; ++src)
+        for (int src = 0; src < 5; ++src)
             for (int prec = 0; prec < 2; ++prec)
                 c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
         *ctx_out = c;
@@ -728,7 +955,8 @@ extern "C" int tp_destroy(void *ctx)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
         for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
-                          &c->mt128, &c->bt128, &c->st128, &c->f32ev})
+                          &c->mt128, &c->bt128, &c->st128, &c->f32ev, &c->fe_lists, &c->fe_meta, &c->fe_piece,
+                          &c->fe_flag, &c->xo_states, &c->xo_draws, &c->adj_u})
             if (b->p)
                 (void)hipFree(b->p);
         for (auto &kv : c->tables) {
@@ -973,11 +1201,15 @@ extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, cons
         a.P = make_params(params);
         a.w = params->ev_weight;
         a.n = n;
-        a.per_thread = n >= (1u << 20) ? 16 : 1;
         a.mt = static_cast<const uint64_t *>(c->mt.p);
         a.km = c->km;
-        if (!u_in)
+        if (!u_in) {
             std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state)->s, sizeof a.s0);
+            rc = ensure(c, c->adj_u, 12 * n * sizeof(double)); // the stream's points, expanded first
+            if (rc)
+                return rc;
+            a.u_gen = static_cast<double *>(c->adj_u.p);
+        }
         const size_t es = precision == TP_PREC_F32 ? sizeof(float) : sizeof(double);
         const size_t bytes[3] = {5 * n * es, 60 * n * es, 60 * n * es};
         void *outs[3] = {F, dF_du, dF_dp};
@@ -1044,6 +1276,9 @@ extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kin
             return fail(c, TP_ERR_ARG, "null output");
         if (n > 64u * TP_BATCH_SIZE)
             return fail(c, TP_ERR_ARG, "n exceeds 640000 events per call");
+        if (params->faster_evgen && (rng_kind == TP_RNG_XOSHIRO || params->precision == TP_PREC_F32))
+            return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen with the xoshiro256+ stream or in f32 runs with jump "
+                                               "seeding (faster-threading) only; the closure consumes the stream in order");
         HIPCHK(c, hipSetDevice(c->device));
         if (n == 0) { // the reference's closure returns an empty accumulator
             std::memset(out, 0, sizeof *out);

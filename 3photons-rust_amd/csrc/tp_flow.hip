@@ -23,8 +23,10 @@
 namespace tpf {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr float LOG2E = 1.4426950408889634f, LN2 = 0.6931471805599453f;
 
@@ -39,6 +41,15 @@ __device__ __forceinline__ s4 pack4(f4 v)
     return __builtin_bit_cast(s4, b);
 }
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// gfx950's 16x16x32 product (twice the K of the one above per instruction):
+// lane l supplies row / column l % 16 and eight K values of group l / 16.
+__device__ __forceinline__ f4 mfma32(s8 a, s8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+// The B operand of a K = 32 product from two 16-row activation tiles: lane
+// group g holds rows 4g..4g+3 of each, the eight K values of its group.
+__device__ __forceinline__ s8 cat8(s4 lo, s4 hi) { return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7); }
 // Hardware exp2 / log2 / reciprocal (about 1 ulp): the network's activations
 // are rounded to bf16 anyway.
 __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
@@ -56,7 +67,17 @@ __device__ __forceinline__ float ftanh(float x) { return 1.f - 2.f * frcp(fexp(2
 __device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + flog(1.f + fexp(-fabsf(x))); }
 
 // ---- LDS image of one layer (bf16, padded row strides against bank conflicts)
-constexpr int S_W1 = 20, S_W2 = 68, S_W3 = 68, S_W1T = 68, S_W2T = 68, S_W3T = 36;
+// W2 and W3 (K = 64) feed K = 32 products: their columns are stored in the
+// order those products read them.  Within each 32-column half c, lane group g
+// reads the eight stored columns 32c + 8g .. 32c + 8g + 7, which hold the
+// original columns 32c + 4g + (0..3) and 32c + 16 + 4g + (0..3): the rows
+// 4g..4g+3 of the two 16-row activation tiles 2c and 2c+1 that the lane
+// holds (cat8).  Row strides of 72 keep each lane's 16-byte read aligned.
+constexpr int S_W1 = 20, S_W2 = 72, S_W3 = 72, S_W1T = 68, S_W2T = 68, S_W3T = 36;
+__host__ __device__ constexpr int k32_pos(int k) // stored position of original column k
+{
+    return 32 * (k / 32) + 8 * ((k % 16) / 4) + (k % 4) + 4 * ((k % 32) / 16);
+}
 constexpr int L_W1 = 0, L_W2 = L_W1 + H * S_W1, L_W3 = L_W2 + H * S_W2, L_FWD = L_W3 + O * S_W3;
 constexpr int L_W1T = L_FWD, L_W2T = L_W1T + DP * S_W1T, L_W3T = L_W2T + H * S_W2T, L_ALL = L_W3T + H * S_W3T;
 
@@ -85,13 +106,13 @@ template <bool TRANSPOSED, class Lds> __device__ __forceinline__ void load_layer
     }
     for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
         const int r = i / H, c = i % H;
-        s.w[L_W2 + r * S_W2 + c] = wl[OFF_W2 + i];
+        s.w[L_W2 + r * S_W2 + k32_pos(c)] = wl[OFF_W2 + i];
         if constexpr (TRANSPOSED)
             s.w[L_W2T + c * S_W2T + r] = wl[OFF_W2 + i];
     }
     for (int i = threadIdx.x; i < O * H; i += blockDim.x) {
         const int r = i / H, c = i % H;
-        s.w[L_W3 + r * S_W3 + c] = wl[OFF_W3 + i];
+        s.w[L_W3 + r * S_W3 + k32_pos(c)] = wl[OFF_W3 + i];
         if constexpr (TRANSPOSED)
             s.w[L_W3T + c * S_W3T + r] = wl[OFF_W3 + i];
     }
@@ -109,6 +130,14 @@ __device__ __forceinline__ s4 atile(const uint16_t *m, int stride, int mb, int k
     const int lane = threadIdx.x & 63;
     const int r = 16 * mb + (lane & 15), c = 16 * kb + 4 * (lane >> 4);
     return *reinterpret_cast<const s4 *>(m + r * stride + c);
+}
+
+// A operand of a K = 32 product: rows 16 mb .., stored columns 32 c + 8 g ..
+__device__ __forceinline__ s8 atile8(const uint16_t *m, int stride, int mb, int c)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = 16 * mb + (lane & 15), col = 32 * c + 8 * (lane >> 4);
+    return *reinterpret_cast<const s8 *>(m + r * stride + col);
 }
 
 // Bias values of rows r0 .. r0 + 3 (the lane's rows of an output tile).
@@ -148,8 +177,8 @@ template <class Lds> __device__ __forceinline__ void mlp_forward(const Lds &s, f
     for (int mb = 0; mb < 4; ++mb) {
         f4 c = bias4(s, H + 16 * mb + 4 * g);
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb)
-            c = mfma(atile(s.w + L_W2, S_W2, mb, kb), r.h1[kb], c);
+        for (int kc = 0; kc < 2; ++kc)
+            c = mfma32(atile8(s.w + L_W2, S_W2, mb, kc), cat8(r.h1[2 * kc], r.h1[2 * kc + 1]), c);
         f4 h, d;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -163,9 +192,10 @@ template <class Lds> __device__ __forceinline__ void mlp_forward(const Lds &s, f
     }
     f4 o0 = bias4(s, 2 * H + 4 * g), o1 = bias4(s, 2 * H + 16 + 4 * g);
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-        o0 = mfma(atile(s.w + L_W3, S_W3, 0, kb), r.h2[kb], o0);
-        o1 = mfma(atile(s.w + L_W3, S_W3, 1, kb), r.h2[kb], o1);
+    for (int kc = 0; kc < 2; ++kc) {
+        const s8 b = cat8(r.h2[2 * kc], r.h2[2 * kc + 1]);
+        o0 = mfma32(atile8(s.w + L_W3, S_W3, 0, kc), b, o0);
+        o1 = mfma32(atile8(s.w + L_W3, S_W3, 1, kc), b, o1);
     }
     r.o0 = o0;
     r.o1 = o1;

@@ -42,6 +42,9 @@ struct FusedArgs {
     // evq[batch_size*(b - b_first) + e].
     const uint64_t *evq;
     uint64_t qbase;
+    // faster-evgen from the xoshiro256+ stream (source 4): its uniforms, the
+    // starts evq and qbase as above
+    const double *xdraws;
     // output: one accumulator per batch, index b - b_first
     void *out;             // tp_acc*
 };
@@ -62,8 +65,9 @@ struct RanfArgs {
     int32_t *out;
 };
 
-void launch_fused(const FusedArgs &a, int source /*0 xoshiro, 1 buffer, 2 weighted, 3 faster-evgen (f64 only)*/, int grid,
-                  hipStream_t st);
+void launch_fused(const FusedArgs &a,
+                  int source /*0 xoshiro, 1 buffer, 2 weighted, 3 faster-evgen RANF, 4 faster-evgen xoshiro (f64 only)*/,
+                  int grid, hipStream_t st);
 int fused_blocks_per_cu(int source, int precision); // resident workgroups per CU (occupancy)
 // Per-event diagnostics: one thread per event, same device code as the loop.
 // Event e draws from offset 12*e (xoshiro: a.s0 with a.mt; buffer: a.draws).
@@ -95,6 +99,10 @@ struct F32Args {
     int32_t km, kb;
     int32_t stride_grid;
     const int32_t *draws;  // RANF values of this launch's batches, 12 per event, batch-major
+    // faster-evgen (source 2, RANF): event starts evq[batch_size*(b - b_first) + e]
+    // in draws from position qbase, as FusedArgs
+    const uint64_t *evq;
+    uint64_t qbase;
     float *ev;             // per event m[5] and keep (0/1), at batch_size*(b - b_first) + e
     void *out;             // tp_acc per batch, index b - b_first (f32 values)
 };
@@ -126,23 +134,74 @@ void launch_fe_walk(const FeWalkArgs &a, hipStream_t st);
 void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_t *len, const uint64_t *off,
                        uint64_t *out, hipStream_t st);
 
+// faster-evgen with jump seeding (faster-threading): every batch reseeds, so
+// each batch's stream is its own piece and is walked by one thread from the
+// piece's first value.  Piece k (batch b_first + k) holds the frame positions
+// q of its state (q counted from the start of the state's round) at
+// buf[k * piece_stride + q], piece_stride a multiple of 55, so that the
+// global position G = k * piece_stride + q keeps q's place in its round
+// (G % 55 = q % 55).  evq[k * batch_size + e] receives event e's start as G.
+// A walk that would leave its piece sets *overflow and fills the rest of its
+// batch with the piece's first position (every later read stays in bounds).
+struct FeBatchWalkArgs {
+    const int32_t *buf;
+    uint64_t piece_stride;
+    const int32_t *p0;     // [nb] frame position of each piece's first value
+    uint64_t nb, b_first, num_events;
+    uint32_t batch_size;
+    uint64_t *evq;
+    uint32_t *overflow;
+    int32_t f32;           // 1: the unit-disc re-roll test in f32 (all-f32 mode)
+};
+void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st);
+
+// The same for the xoshiro256+ stream with jump seeding: piece k is batch
+// b_first + k's stream, started from J^(b_first + k) s0 (J the 2^128 jump),
+// its first piece_len uniforms at xbuf[k * piece_len ..]; positions are
+// G = k * piece_len + draw index.
+struct XoPieceArgs {
+    uint64_t s0[4];
+    const uint64_t *bt;    // J^(2^k), k < kb
+    int32_t kb;
+    const uint64_t *mt;    // M^(2^k), k < km
+    int32_t km;
+    uint64_t nb, b_first;
+    uint64_t *states;      // [nb][4] scratch: each piece's first state
+    uint64_t piece_len;    // uniforms expanded per piece
+    uint32_t per_thread;   // uniforms per expanding thread (divides piece_len)
+    double *xbuf;          // [nb][piece_len]
+    // walk
+    uint64_t num_events;
+    uint32_t batch_size;
+    uint64_t *evq;
+    uint32_t *overflow;
+};
+void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st); // piece states, expansion, walk
+
 } // namespace tpk
 
 namespace tpk {
 
 // Arguments of the adjoint kernel (tp_adjoint).
+// Thread t of the adjoint kernel takes point t, so that a wavefront's stores
+// cover 64 consecutive rows of each output and complete their cache lines
+// within one pass.  Points of the xoshiro stream (u_in null) are first
+// expanded into u_gen by a kernel whose threads each step through GEN_POINTS
+// consecutive points from one skip-ahead.
 struct AdjArgs {
     tpd::Params P;
     double w;              // RAMBO event weight
     uint64_t n;            // points
-    uint32_t per_thread;   // consecutive points per thread
-    // uniforms: u_in (n x 12 doubles) if non-null, else xoshiro from s0 via mt
+    // uniforms: u_in (n x 12 doubles) if non-null, else xoshiro from s0 via mt,
+    // expanded into u_gen (n x 12 doubles)
     const double *u_in;
+    double *u_gen;
     uint64_t s0[4];
     const uint64_t *mt;
     int32_t km;
     void *F, *dF_du, *dF_dp; // [n][5], [n][5][12], [n][5][12] of double or float
 };
+constexpr uint32_t ADJ_GEN_POINTS = 256;
 
 void launch_adjoint(const AdjArgs &a, int precision /*TP_PREC_F64 or TP_PREC_F32*/, hipStream_t st);
 

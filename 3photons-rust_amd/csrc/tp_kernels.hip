@@ -222,6 +222,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
             if (k < len && SOURCE == 3) {
                 RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
                 pass = generate_faster_f64(fresh_params(), w, p, d);
+            } else if (k < len && SOURCE == 4) {
+                XoWalk w{a.xdraws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
+                pass = generate_faster_f64(fresh_params(), w, p, d);
             } else if (k < len) {
                 double u[12];
                 if (SOURCE == 0) {
@@ -314,6 +317,8 @@ int fused_blocks_per_cu(int source, int precision)
         return blocks_per_cu<2, 0>();
     if (source == 3)
         return blocks_per_cu<3, 0>();
+    if (source == 4)
+        return blocks_per_cu<4, 0>();
     return precision == 0 ? blocks_per_cu<1, 0>() : blocks_per_cu<1, 1>();
 }
 
@@ -332,8 +337,10 @@ void launch_fused(const FusedArgs &a, int source, int grid, hipStream_t st)
             hipLaunchKernelGGL((fused_kernel<1, 1>), g, b, 0, st, a);
     } else if (source == 2) {
         hipLaunchKernelGGL((fused_kernel<2, 0>), g, b, 0, st, a);
-    } else {
+    } else if (source == 3) {
         hipLaunchKernelGGL((fused_kernel<3, 0>), g, b, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((fused_kernel<4, 0>), g, b, 0, st, a);
     }
 }
 
@@ -544,8 +551,9 @@ void launch_ranf_expand(const RanfArgs &a, uint64_t npieces, hipStream_t st)
 // ---- faster-evgen stream walk --------------------------------------------------
 // take(n) of random_array::<N> on the stored stream: a request that does not
 // fit in the current round skips its leftovers (ranf.rs:89-92).  False (and no
-// read) when the values would lie past the stored range.
-__device__ __forceinline__ bool fe_take(const FeWalkArgs &a, uint64_t &q, int n, double *out)
+// read) when the values would lie past the stored range.  T is the run's
+// float type: (n as T) * 1e-9 (ranf.rs:99).
+template <class T> __device__ __forceinline__ bool fe_take(const FeWalkArgs &a, uint64_t &q, int n, T *out)
 {
     const uint32_t rem = 55u - (uint32_t)(q % 55u);
     if (rem < (uint32_t)n)
@@ -555,30 +563,31 @@ __device__ __forceinline__ bool fe_take(const FeWalkArgs &a, uint64_t &q, int n,
     if (out) {
         const int32_t *v = a.buf + (q - a.qbase);
         for (int k = 0; k < n; ++k)
-            out[k] = (double)v[n - 1 - k] * 1e-9;
+            out[k] = (T)v[n - 1 - k] * (T)1e-9;
     }
     q += (uint64_t)n;
     return true;
 }
 
 // Consumption of one event (raw_photons_faster_f64's draws: 9, 6, then 2 per
-// re-rolled point of the unit-disc test).  False when the stored range ends
-// inside the event.
-__device__ __forceinline__ bool fe_walk_event(const FeWalkArgs &a, uint64_t &q)
+// re-rolled point of the unit-disc test, decided in the run's float type T).
+// False when the stored range ends inside the event.
+template <class T = double> __device__ __forceinline__ bool fe_walk_event(const FeWalkArgs &a, uint64_t &q)
 {
-    double r6[6];
-    if (!fe_take(a, q, 9, nullptr) || !fe_take(a, q, 6, r6))
+    T r6[6];
+    if (!fe_take<T>(a, q, 9, nullptr) || !fe_take<T>(a, q, 6, r6))
         return false;
-    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    const T minpos = sizeof(T) == 8 ? (T)2.2250738585072014e-308 : (T)1.17549435e-38f;
+    const T min2 = minpos * minpos; // 0 in f32, as in the reference's f32 build
     for (int i = 0; i < 3; ++i) {
-        const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
-        double r2 = (0. + x * x) + y * y;
-        while (r2 > 1. || r2 < min2) {
-            double r[2];
-            if (!fe_take(a, q, 2, r))
+        const T x = (T)2. * r6[i] - (T)1., y = (T)2. * r6[3 + i] - (T)1.;
+        T r2 = ((T)0. + x * x) + y * y;
+        while (r2 > (T)1. || r2 < min2) {
+            T r[2];
+            if (!fe_take<T>(a, q, 2, r))
                 return false;
-            const double xa = 2. * r[0] - 1., ya = 2. * r[1] - 1.;
-            r2 = 0. + (xa * xa + ya * ya);
+            const T xa = (T)2. * r[0] - (T)1., ya = (T)2. * r[1] - (T)1.;
+            r2 = (T)0. + (xa * xa + ya * ya);
         }
     }
     return true;
@@ -665,6 +674,141 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
                        uint64_t *out, hipStream_t st)
 {
     hipLaunchKernelGGL(fe_scatter_kernel, dim3(a.nwalkers), dim3(256), 0, st, a, begin, len, off, out);
+}
+
+// One thread per batch: the batch's walk from its piece's first value.  Each
+// event consumes at least 15 positions and the walk stops at the piece's end,
+// so the loop ends.
+__global__ __launch_bounds__(64) void fe_walk_batches_kernel(FeBatchWalkArgs a)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nb)
+        return;
+    const uint64_t b = a.b_first + k;
+    const uint64_t left = a.num_events - b * a.batch_size;
+    const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
+    const uint64_t base = k * a.piece_stride;
+    FeWalkArgs w;
+    w.buf = a.buf;
+    w.qbase = 0;
+    w.qlimit = base + a.piece_stride;
+    uint64_t q = base + (uint64_t)a.p0[k];
+    const uint64_t q0 = q;
+    uint64_t *out = a.evq + k * a.batch_size;
+    uint32_t e = 0;
+    for (; e < n; ++e) {
+        out[e] = q;
+        if (!(a.f32 ? fe_walk_event<float>(w, q) : fe_walk_event<double>(w, q)))
+            break;
+    }
+    if (e < n) {
+        *a.overflow = 1u;
+        for (; e < n; ++e)
+            out[e] = q0;
+    }
+}
+
+void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(fe_walk_batches_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
+}
+
+// ---- faster-evgen on the xoshiro256+ stream, jump seeding --------------------
+// Each piece's first state J^b s0, one thread per piece (divergent bit tests;
+// the matrix columns still come in with uniform scalar loads).
+__global__ __launch_bounds__(64) void xo_piece_state_kernel(XoPieceArgs a)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nb)
+        return;
+    const uint64_t b = a.b_first + k;
+    uint64_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
+    for (int j = 0; j < a.kb; ++j)
+        if ((b >> j) & 1ull)
+            gf2_apply(a.bt + 1024 * j, s);
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        a.states[4 * k + w] = s[w];
+}
+
+// Each thread expands per_thread consecutive uniforms of one piece from the
+// piece's state skipped to its first draw.
+__global__ __launch_bounds__(64) void xo_expand_kernel(XoPieceArgs a)
+{
+    const uint64_t per_piece = a.piece_len / a.per_thread;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nb * per_piece)
+        return;
+    const uint64_t k = t / per_piece, j = t % per_piece;
+    uint64_t s[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        s[w] = a.states[4 * k + w];
+    const uint64_t off = j * a.per_thread;
+    for (int i = 0; i < a.km; ++i)
+        if ((off >> i) & 1ull)
+            gf2_apply(a.mt + 1024 * i, s);
+    double *out = a.xbuf + k * a.piece_len + off;
+    for (uint32_t i = 0; i < a.per_thread; ++i)
+        out[i] = xo_uniform(s);
+}
+
+// One event's consumption of the xoshiro stream: 9, 6, then 2 per re-rolled
+// point of the unit-disc test, no discards.  False when the piece ends inside
+// the event.
+__device__ __forceinline__ bool xo_walk_event(const double *buf, uint64_t qlimit, uint64_t &q)
+{
+    q += 9;
+    if (q + 6 > qlimit)
+        return false;
+    const double *r6 = buf + q;
+    q += 6;
+    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    for (int i = 0; i < 3; ++i) {
+        const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
+        double r2 = (0. + x * x) + y * y;
+        while (r2 > 1. || r2 < min2) {
+            if (q + 2 > qlimit)
+                return false;
+            const double xa = 2. * buf[q] - 1., ya = 2. * buf[q + 1] - 1.;
+            q += 2;
+            r2 = 0. + (xa * xa + ya * ya);
+        }
+    }
+    return true;
+}
+
+// One thread per batch, as fe_walk_batches_kernel.
+__global__ __launch_bounds__(64) void xo_walk_batches_kernel(XoPieceArgs a)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nb)
+        return;
+    const uint64_t b = a.b_first + k;
+    const uint64_t left = a.num_events - b * a.batch_size;
+    const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
+    const uint64_t base = k * a.piece_len;
+    uint64_t q = base;
+    uint64_t *out = a.evq + k * a.batch_size;
+    uint32_t e = 0;
+    for (; e < n; ++e) {
+        out[e] = q;
+        if (!xo_walk_event(a.xbuf, base + a.piece_len, q))
+            break;
+    }
+    if (e < n) {
+        *a.overflow = 1u;
+        for (; e < n; ++e)
+            out[e] = base;
+    }
+}
+
+void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(xo_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
+    const uint64_t threads = a.nb * (a.piece_len / a.per_thread);
+    hipLaunchKernelGGL(xo_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(xo_walk_batches_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
 }
 
 // seeded_new (ranf.rs:36-66) for jump mode, one thread per batch.
@@ -905,7 +1049,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
             const uint64_t e = lb * a.batch_size + e0 + i;
             bool pass = false;
             float p[3][4];
-            if (i < len) {
+            if (i < len && SOURCE == 2) {
+                RanfWalkF32 w{a.draws, a.qbase, a.evq[e]};
+                pass = generate_faster_f32(a.P, w, p);
+                if (!pass) {
+                    float *o = a.ev + 6 * e;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        o[c] = 0.f;
+                }
+            } else if (i < len) {
                 float u[12];
                 if (SOURCE == 0) {
 #pragma unroll
@@ -1021,8 +1174,10 @@ void launch_f32_events(const F32Args &a, int grid, hipStream_t st)
 {
     if (a.source == 0)
         hipLaunchKernelGGL(f32_events_kernel<0>, dim3(grid), dim3(64), 0, st, a);
-    else
+    else if (a.source == 1)
         hipLaunchKernelGGL(f32_events_kernel<1>, dim3(grid), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(f32_events_kernel<2>, dim3(grid), dim3(64), 0, st, a);
 }
 
 void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st)
@@ -1039,41 +1194,49 @@ namespace tpk {
 
 template <class T> __device__ __forceinline__ void adjoint_body(const AdjArgs &a)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t first = t * a.per_thread;
-    if (first >= a.n)
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n)
         return;
     const AdjParams<T> P = adj_params<T>(a.P, a.w);
-    uint64_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
-    if (!a.u_in) {
-        const uint64_t off = 12ull * first;
-        for (int k = 0; k < a.km; ++k)
-            if ((off >> k) & 1ull)
-                gf2_apply(a.mt + 1024 * k, s);
-    }
     T *F = reinterpret_cast<T *>(a.F), *Gu = reinterpret_cast<T *>(a.dF_du), *Gp = reinterpret_cast<T *>(a.dF_dp);
-    const uint64_t last = first + a.per_thread < a.n ? first + a.per_thread : a.n;
-    for (uint64_t i = first; i < last; ++i) {
-        T u[12];
+    T u[12];
 #pragma unroll
-        for (int j = 0; j < 12; ++j)
-            u[j] = a.u_in ? (T)a.u_in[12 * i + j] : (T)xo_uniform(s);
-        // one forward pass, then one reverse sweep per coupling component
-        T m[5], du[12], dp[12];
-        event_adjoint_sweeps<T, 0, 5>(P, u, nullptr, m, du, dp, (T)0, (T)INFINITY,
-                                      [&](int c, const T *mc, const T *duc, const T *dpc) {
-                                          if (F)
-                                              F[5 * i + c] = P.w * (c == 0 ? mc[0] : c == 1 ? mc[1] : c == 2 ? mc[2] : c == 3 ? mc[3] : mc[4]);
-                                          if (Gu)
+    for (int j = 0; j < 12; ++j)
+        u[j] = (T)a.u_in[12 * i + j];
+    // one forward pass, then one reverse sweep per coupling component
+    T m[5], du[12], dp[12];
+    event_adjoint_sweeps<T, 0, 5>(P, u, nullptr, m, du, dp, (T)0, (T)INFINITY,
+                                  [&](int c, const T *mc, const T *duc, const T *dpc) {
+                                      if (F)
+                                          F[5 * i + c] = P.w * (c == 0 ? mc[0] : c == 1 ? mc[1] : c == 2 ? mc[2] : c == 3 ? mc[3] : mc[4]);
+                                      if (Gu)
 #pragma unroll
-                                              for (int j = 0; j < 12; ++j)
-                                                  Gu[60 * i + 12 * c + j] = duc[j];
-                                          if (Gp)
+                                          for (int j = 0; j < 12; ++j)
+                                              Gu[60 * i + 12 * c + j] = duc[j];
+                                      if (Gp)
 #pragma unroll
-                                              for (int j = 0; j < 12; ++j)
-                                                  Gp[60 * i + 12 * c + j] = dpc[j];
-                                      });
-    }
+                                          for (int j = 0; j < 12; ++j)
+                                              Gp[60 * i + 12 * c + j] = dpc[j];
+                                  });
+}
+
+// The stream's points, ADJ_GEN_POINTS consecutive ones per thread from one
+// skip-ahead (12 draws per point, rand's Standard f64).
+__global__ __launch_bounds__(64) void adjoint_points_kernel(AdjArgs a)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t first = t * ADJ_GEN_POINTS;
+    if (first >= a.n)
+        return;
+    uint64_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
+    const uint64_t off = 12ull * first;
+    for (int k = 0; k < a.km; ++k)
+        if ((off >> k) & 1ull)
+            gf2_apply(a.mt + 1024 * k, s);
+    const uint64_t last = first + ADJ_GEN_POINTS < a.n ? first + ADJ_GEN_POINTS : a.n;
+    double *out = a.u_gen + 12 * first;
+    for (uint64_t k = 0; k < 12 * (last - first); ++k)
+        out[k] = xo_uniform(s);
 }
 
 // f64: 255 VGPRs + 240 AGPRs, one wavefront per SIMD (two would spill
@@ -1085,10 +1248,15 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     adjoint_body<float>(a);
 }
 
-void launch_adjoint(const AdjArgs &a, int precision, hipStream_t st)
+void launch_adjoint(const AdjArgs &a0, int precision, hipStream_t st)
 {
-    const uint64_t threads = (a.n + a.per_thread - 1) / a.per_thread;
-    const dim3 g((unsigned)((threads + 127) / 128)), b(128);
+    AdjArgs a = a0;
+    if (!a.u_in) {
+        const uint64_t threads = (a.n + ADJ_GEN_POINTS - 1) / ADJ_GEN_POINTS;
+        hipLaunchKernelGGL(adjoint_points_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
+        a.u_in = a.u_gen;
+    }
+    const dim3 g((unsigned)((a.n + 127) / 128)), b(128);
     if (precision == 2)
         hipLaunchKernelGGL(adjoint_kernel_f32, g, b, 0, st, a);
     else

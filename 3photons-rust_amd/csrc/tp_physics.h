@@ -503,9 +503,25 @@ struct RanfWalk {
     }
 };
 
+// The xoshiro256+ stream (standard.rs) expanded into HBM as its f64
+// uniforms: random_array::<N> is N consecutive draws in array order
+// (standard.rs:33-35, no round to discard).
+struct XoWalk {
+    const double *buf; // uniforms from position qbase on
+    uint64_t qbase;
+    uint64_t q;        // next position
+    __device__ __forceinline__ void take(int n, double *out)
+    {
+        const double *v = buf + (q - qbase);
+        for (int k = 0; k < n; ++k)
+            out[k] = v[k];
+        q += (uint64_t)n;
+    }
+};
+
 // Massless photons from 9 + 6 draws and unit-disc rejection re-rolls, in the
 // reference's operation order (matrix fills column-major, nalgebra sums).
-__device__ __forceinline__ void raw_photons_faster_f64(RanfWalk &w, double q[3][4])
+template <class Walk> __device__ __forceinline__ void raw_photons_faster_f64(Walk &w, double q[3][4])
 {
     double r9[9], r6[6], x[3], y[3], r2[3];
     w.take(9, r9);
@@ -540,7 +556,7 @@ __device__ __forceinline__ void raw_photons_faster_f64(RanfWalk &w, double q[3][
     }
 }
 
-__device__ __forceinline__ bool generate_faster_f64(const Params &P, RanfWalk &w, double p[3][4], int d[3])
+template <class Walk> __device__ __forceinline__ bool generate_faster_f64(const Params &P, Walk &w, double p[3][4], int d[3])
 {
     double q[3][4];
     raw_photons_faster_f64(w, q);
@@ -706,6 +722,73 @@ __device__ __forceinline__ void matrix_elements_f32(const FParams &P, const floa
     Spin<float> S;
     spinor_products<float>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
     helicity_sums<float, false>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+}
+
+// ---- faster-evgen in f32 (evgen.rs:143-173, 221-249 with Float = f32) -------
+// The RANF values as (n as f32) * 1e-9 (ranf.rs:99), random_array's round
+// discards as in RanfWalk.
+struct RanfWalkF32 {
+    const int32_t *buf;
+    uint64_t qbase;
+    uint64_t q;
+    __device__ __forceinline__ void take(int n, float *out)
+    {
+        const uint32_t rem = 55u - (uint32_t)(q % 55u);
+        if (rem < (uint32_t)n)
+            q += rem;
+        const int32_t *v = buf + (q - qbase);
+        for (int k = 0; k < n; ++k)
+            out[k] = (float)v[n - 1 - k] * 1e-9f;
+        q += (uint64_t)n;
+    }
+};
+
+// f32::MIN_POSITIVE squared underflows to 0 in f32, so the re-roll test
+// r2 < MIN_POSITIVE^2 never fires, as in the reference's f32 build.
+template <class Walk> __device__ __forceinline__ void raw_photons_faster_f32(Walk &w, float q[3][4])
+{
+    float r9[9], r6[6], x[3], y[3], r2[3];
+    w.take(9, r9);
+    w.take(6, r6);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        x[i] = 2.f * r6[i] - 1.f;
+        y[i] = 2.f * r6[3 + i] - 1.f;
+        r2[i] = (0.f + x[i] * x[i]) + y[i] * y[i];
+    }
+    const float minpos = 1.17549435e-38f;
+    const float min2 = minpos * minpos;
+    for (int i = 0; i < 3; ++i)
+        while (r2[i] > 1.f || r2[i] < min2) {
+            float r[2];
+            w.take(2, r);
+            x[i] = 2.f * r[0] - 1.f;
+            y[i] = 2.f * r[1] - 1.f;
+            r2[i] = 0.f + (x[i] * x[i] + y[i] * y[i]);
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float nrm = 1.f / sqrtf(r2[i]);
+        const float cx = x[i] * nrm, cy = y[i] * nrm;
+        const float cos_t = 2.f * r9[i] - 1.f;
+        const float em = r9[3 + i] * r9[6 + i];
+        const float st = sqrtf(1.f - cos_t * cos_t);
+        const float en = -(float)tpm::log_pos((double)(em + minpos)); // correctly rounded f32 ln
+        q[i][0] = en * (st * cx);
+        q[i][1] = en * (st * cy);
+        q[i][2] = en * cos_t;
+        q[i][3] = en * 1.f;
+    }
+}
+
+template <class Walk> __device__ __forceinline__ bool generate_faster_f32(const FParams &P, Walk &w, float p[3][4])
+{
+    float q[3][4];
+    raw_photons_faster_f32(w, q);
+    conformal_f32(P, q, p);
+    if (P.sort)
+        sort3(p);
+    return keep_f32(P, p);
 }
 
 // One f32 event from 12 uniforms: cut decision and helicity sums m[5].

@@ -101,6 +101,15 @@ class TpFinal(C.Structure):
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libtp.so not built at {LIB_PATH}; run __graft_entry__.build() or make -C 3photons-rust_amd")
+    # PyTorch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7) as the system one libtp.so links: whichever is loaded
+    # first serves the whole process, and torch cannot use the system one.  So
+    # when torch is installed it is loaded first, and libtp.so binds to its
+    # runtime, whichever of the two modules the caller imports first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp, P = C.c_void_p, C.POINTER
     sig = {

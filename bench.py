@@ -58,12 +58,14 @@ def cpu_model() -> str:
 
 def native_oracle() -> tuple[str, str]:
     """The oracle built for this host (make -C oracle native: -O3
-    -march=native -ffp-contract=off), or the portable build if that fails."""
+    -march=native -fno-tree-slp-vectorize -ffp-contract=off), or the
+    portable build if that fails."""
     import subprocess
     odir = os.path.join(ROOT, "oracle")
     try:
-        subprocess.run(["make", "-s", "-C", odir, "native"], check=True, capture_output=True, timeout=300)
-        return os.path.join(odir, "build", "native", "liboracle.so"), "gcc -O3 -march=native -ffp-contract=off"
+        # -B: always compile here, for this host's CPU
+        subprocess.run(["make", "-B", "-s", "-C", odir, "native"], check=True, capture_output=True, timeout=300)
+        return os.path.join(odir, "build", "native", "liboracle.so"), "gcc -O3 -march=native -fno-tree-slp-vectorize -ffp-contract=off"
     except (OSError, subprocess.SubprocessError) as e:
         return (os.path.join(odir, "build", "liboracle.so"),
                 f"gcc -O3 -march=x86-64-v2 -ffp-contract=off (native build failed: {type(e).__name__})")
@@ -150,12 +152,13 @@ def json_safe(o):
     return o
 
 
-def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged, kind=None):
+def measure(tp, torch, eng, params, events, steps, warmup, per_batch, merged, kind=None, seeding=None):
     kind = tp.RNG_XOSHIRO if kind is None else kind
+    seeding = tp.SEED_SKIPAHEAD if seeding is None else seeding
 
     def step():
         st = tp.rng_state(kind)
-        eng.run_device(params, kind, tp.SEED_SKIPAHEAD, st, events, per_batch.data_ptr(), merged.data_ptr())
+        eng.run_device(params, kind, seeding, st, events, per_batch.data_ptr(), merged.data_ptr())
         return eng.sync()
 
     for _ in range(warmup):
@@ -357,6 +360,17 @@ def main() -> int:
         fe_events = min(args.events, args.fe_events)
         pfe = tp.params_from_config(cfg.with_events(fe_events), faster_evgen=True)
         vfe, dtfe, kfe, accfe = measure(tp, torch, eng, pfe, fe_events, 1, 1, per_batch, merged, tp.RNG_RANF)
+        # faster-evgen with faster-threading seeding, the reference's fastest
+        # builds (VALIDATION.md:50-60: 1e7 events on a 4-core i7-4720HQ)
+        fe_jump = {}
+        for name, kind, prec, ref_s in (("ranf_f64", tp.RNG_RANF, tp.PREC_F64, 0.57),
+                                        ("xoshiro_f64", tp.RNG_XOSHIRO, tp.PREC_F64, 0.59),
+                                        ("ranf_f32", tp.RNG_RANF, tp.PREC_F32, 0.48)):
+            pj = tp.params_from_config(cfg.with_events(fe_events), precision=prec, faster_evgen=True)
+            vj, dtj, kj, accj = measure(tp, torch, eng, pj, fe_events, 1, 1, per_batch, merged, kind, tp.SEED_JUMP)
+            fe_jump[name] = {"value": vj, "unit": "events/s", "events": fe_events, "ms_per_step": dtj * 1e3,
+                             "kernels_ms": kj, "selected_events": int(accj.selected_events),
+                             "reference_published_events_per_s": 1e7 / ref_s}
         out["variants"] = {
             "ranf_f64": {"value": vr, "unit": "events/s", "ms_per_step": dtr * 1e3 / 2,
                          "kernels_ms": kr, "selected_events": int(accr.selected_events)},
@@ -368,6 +382,7 @@ def main() -> int:
             "faster_evgen_f64": {"value": vfe, "unit": "events/s", "events": fe_events, "ms_per_step": dtfe * 1e3,
                                  "kernels_ms": kfe, "selected_events": int(accfe.selected_events),
                                  "includes": "walk of the RANF stream for the event starts (device)"},
+            "faster_evgen_faster_threading": fe_jump,
         }
     if not args.no_adjoint:
         out["adjoint"] = measure_adjoint(tp, torch, eng, p64, args.adjoint_points)

@@ -215,3 +215,47 @@ def test_device_fold_equals_host_fold(engine, cfg, prec):
     dfold = tp.TpAcc.from_buffer_copy(merged_dev.cpu().numpy().tobytes())
     hfold = tp.merge(per, f32=prec == tp.PREC_F32)
     np.testing.assert_array_equal(dfold.as_array(), hfold.as_array())
+
+
+@pytest.mark.parametrize("n_events", [1, 9999, 35017, 123457])
+def test_f32_faster_evgen_jump_per_batch_matches_oracle(engine, cfg, oracle32, valeurs, n_events):
+    """faster-evgen in f32 with faster-threading seeding (the reference's
+    fastest build, VALIDATION.md:58-60): RANF values as f32, the unit-disc
+    re-roll decided in f32, every batch reseeded; against the f32 oracle."""
+    print(f"f32 faster-evgen jump N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, precision=tp.PREC_F32, faster_evgen=True)
+    per, merged, _ = engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, tp.rng_state(tp.RNG_RANF), n_events)
+    ocfg = oracle32.config(valeurs, n_events)
+    om, oper = oracle32.run(ocfg, oracle32.opts("ranf", faster_threading=True, threads=4, faster_evgen=True),
+                            per_batch=True)
+    worst = 0.0
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        worst = max(worst, close_f32(a, oper[b].as_array(), min(10000, n_events - 10000 * b)))
+    close_f32(merged.as_array(), om.as_array(), n_events)
+    print(f"  worst relative difference of a batch sum: {worst:.2e}")
+
+
+def test_f32_faster_evgen_jump_across_chunks(engine, cfg, oracle32, valeurs):
+    n_events = 62_000_000
+    print(f"f32 faster-evgen jump N={n_events}", flush=True)
+    params = tp.params_from_config(cfg.with_events(n_events), precision=tp.PREC_F32, faster_evgen=True)
+    per, _, _ = engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, tp.rng_state(tp.RNG_RANF), n_events)
+    ocfg = oracle32.config(valeurs, n_events)
+    opts = oracle32.opts("ranf", faster_evgen=True)
+    for b in (0, 6143, 6144, 6199):
+        g = oracle32.rng("ranf")
+        for _ in range(b):
+            oracle32.jump(g)
+        o = oracle32.simulate_batch(ocfg, opts, g, 10000).as_array()
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        close_f32(a, o, 10000)
+
+
+def test_f32_faster_evgen_needs_jump_seeding(engine, cfg):
+    params = tp.params_from_config(cfg.with_events(100), precision=tp.PREC_F32, faster_evgen=True)
+    with pytest.raises(tp.TpError, match="faster-evgen"):
+        engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), 100)

@@ -76,7 +76,10 @@ def berends_kleiss(p, alpha):
 
 
 def fawzi_sigma(c):
-    """Cut cross-sections sigma+, sigma- (pb) of resfin.rs:101-194 (fawzi)."""
+    """Cut cross-sections sigma+, sigma- (pb) of resfin.rs:101-194 (fawzi),
+    from the oracle's or the package's configuration record."""
+    br = c.br_ep_em if hasattr(c, "br_ep_em") else c.branching_ep_em
+    conv = c.conv if hasattr(c, "conv") else c.gev2_to_picobarn
     mre = c.m_z0 / c.e_total
     gre = c.g_z0 * c.m_z0 / c.e_total ** 2
     x = 1 - mre ** 2
@@ -84,8 +87,8 @@ def fawzi_sigma(c):
     dl = (1 - c.cut_pp) / 2
     e4 = (2 * c.e_min / c.e_total) ** 4
     bra = c.m_z0 / (3 * 6 * math.pi ** 3 * 16 * 120)
-    sig = (12 * math.pi / c.m_z0 ** 2 * c.br_ep_em * c.g_z0 * bra / c.e_total ** 2 * (c.e_total / c.m_z0) ** 8
-           * sdz2 * c.conv)
+    sig = (12 * math.pi / c.m_z0 ** 2 * br * c.g_z0 * bra / c.e_total ** 2 * (c.e_total / c.m_z0) ** 8
+           * sdz2 * conv)
     d2, d3 = dl * dl, dl ** 3
     f1 = 1 - 15 * e4 - 9 / 7 * (1 - 70 * e4) * d2 + 6 / 7 * (1 + 70 * e4) * d3
     g1 = 1 - 30 * e4 - 9 / 7 * (1 - 70 * e4) * dl - 90 * e4 * d2 - 1 / 7 * (1 - 420 * e4) * d3
@@ -241,7 +244,7 @@ def test_gpu_anomalous_sigma_against_fawzi(valeurs):
     """1e7 events on the GPU without the beam cut against fawzi(), as the
     oracle test above."""
     cfg = tp.Configuration.load(valeurs).with_events(10_000_000)
-    cfg.cut_beam = 1.0
+    cfg.raw.cut_beam = 1.0
     print("gpu fawzi run N=1e7", flush=True)
     with tp.Engine(0) as eng:
         res = tp.run_simulation(cfg, rng="xoshiro", engine=eng)
