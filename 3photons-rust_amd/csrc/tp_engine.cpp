@@ -911,6 +911,72 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 
 } // namespace
 
+// ---- tables of the no-allocation entry point ---------------------------------
+// tp_run_device_noalloc runs the xoshiro256+ event loop without allocating
+// device memory: the GF(2) tables its kernels read live in `scratch`, a
+// file-scope device array, filled once per device at set-up (tp_create) and
+// only read after that, so calls on one device that overlap do not race on
+// it.  Each device holds its own copy at its own address, which set-up takes
+// with hipGetSymbolAddress while that device is current; host code uses that
+// address, never the array's name.
+//   matrices [0, KM)                M^(2^k), the in-batch draw offsets
+//   [KM, KM + NS_KB)                B^(2^k), B = M^(12 * 10 000) (skip-ahead)
+//   [KM + NS_KB, KM + 2 NS_KB)      B^(2^k), B = the 2^128 jump (jump seeding)
+//   KM + 2 NS_KB + 2 seeding + p    B^grid of the full persistent grid, p = 0 f64, 1 mixed
+constexpr int NS_KB = 32;                            // runs of fewer than 2^32 batches
+constexpr int NS_BT = KM, NS_STRIDE = KM + 2 * NS_KB, NS_MATS = KM + 2 * NS_KB + 4;
+constexpr size_t NS_WORDS = (size_t)NS_MATS * 1024; // 100 matrices of 8 KiB: 800 KiB
+static_assert(sizeof(tp::Gf2Mat) == 1024 * sizeof(uint64_t), "matrix layout");
+static __device__ __align__(256) uint64_t scratch[NS_WORDS];
+
+namespace {
+constexpr int NS_MAX_DEVICES = 64;
+struct NoallocSetup {
+    std::once_flag once;
+    int rc = TP_ERR_INTERNAL;
+    std::string err;
+    uint64_t *ptr = nullptr; // this device's address of `scratch`
+    int full_grid[2] = {0, 0};
+};
+NoallocSetup g_noalloc[NS_MAX_DEVICES];
+
+// Once per device, with the device current: the array's address and its tables.
+void noalloc_setup(NoallocSetup &S, int cus, const int blocks[2])
+{
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        S.err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        S.rc = TP_ERR_HIP;
+        return;
+    }
+    std::vector<tp::Gf2Mat> mats(NS_MATS);
+    const std::vector<tp::Gf2Mat> &mt = tp::xoshiro_step_pow2();
+    for (int k = 0; k < KM; ++k)
+        mats[k] = mt[k];
+    tp::Gf2Mat B[2];
+    tp::gf2_pow_from_table(mt, 12ull * TP_BATCH_SIZE, B[0]);
+    tp::gf2_jump128(B[1]);
+    for (int sd = 0; sd < 2; ++sd) {
+        const std::vector<tp::Gf2Mat> bt = tp::gf2_pow2_table(B[sd], NS_KB);
+        for (int k = 0; k < NS_KB; ++k)
+            mats[NS_BT + sd * NS_KB + k] = bt[k];
+        for (int p = 0; p < 2; ++p) {
+            S.full_grid[p] = std::max(1, cus * blocks[p]);
+            tp::gf2_pow_from_table(bt, (uint64_t)S.full_grid[p], mats[NS_STRIDE + 2 * sd + p]);
+        }
+    }
+    e = hipMemcpy(ptr, mats.data(), NS_WORDS * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        S.err = std::string("scratch upload: ") + hipGetErrorString(e);
+        S.rc = TP_ERR_HIP;
+        return;
+    }
+    S.ptr = static_cast<uint64_t *>(ptr);
+    S.rc = TP_OK;
+}
+} // namespace
+
 extern "C" int tp_create(int device, void **ctx_out)
 {
     return tp_guard(nullptr, [&]() -> int {
@@ -941,6 +1007,10 @@ extern "C" int tp_create(int device, void **ctx_out)
         for (int src = 0; src < 5; ++src)
             for (int prec = 0; prec < 2; ++prec)
                 c->blocks_per_cu[src][prec] = tpk::fused_blocks_per_cu(src, prec);
+        if (device < NS_MAX_DEVICES) { // tables of tp_run_device_noalloc (device current)
+            NoallocSetup &S = g_noalloc[device];
+            std::call_once(S.once, [&] { noalloc_setup(S, c->cus, c->blocks_per_cu[0]); });
+        }
         *ctx_out = c;
         return TP_OK;
     });
@@ -1341,6 +1411,74 @@ extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, i
             return rc;
         if (merged_dev) {
             tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream, params->precision == TP_PREC_F32);
+            HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        c->last.events = num_events;
+        c->last.batches = nb;
+        c->timed = true;
+        if (stats)
+            *stats = c->last;
+        return TP_OK;
+    });
+}
+
+extern "C" int tp_run_device_noalloc(void *ctx, const tp_params *params, int rng_kind, int seeding,
+                                     const void *rng_state0, uint64_t num_events, tp_acc *per_batch_dev,
+                                     tp_acc *merged_dev, tp_stats *stats)
+{
+    return tp_guard(ctx_err(ctx), [&]() -> int {
+        Ctx *c = static_cast<Ctx *>(ctx);
+        if (!c)
+            return TP_ERR_ARG;
+        int rc = check_common(c, params, rng_kind, rng_state0);
+        if (rc)
+            return rc;
+        if (rng_kind != TP_RNG_XOSHIRO || params->faster_evgen ||
+            (params->precision != TP_PREC_F64 && params->precision != TP_PREC_MIXED))
+            return fail(c, TP_ERR_UNSUPPORTED,
+                        "tp_run_device_noalloc runs the xoshiro256+ stream in f64 or mixed precision only");
+        if (seeding != TP_SEED_SKIPAHEAD && seeding != TP_SEED_JUMP)
+            return fail(c, TP_ERR_ARG, "unknown seeding");
+        if (num_events == 0)
+            return fail(c, TP_ERR_ARG, "Must simulate at least one event");
+        if (!per_batch_dev)
+            return fail(c, TP_ERR_ARG, "null per-batch output");
+        const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
+        if (nb >= (1ull << NS_KB))
+            return fail(c, TP_ERR_ARG, "tp_run_device_noalloc: at most 2^32 - 1 batches of 10 000 events");
+        if (c->device >= NS_MAX_DEVICES)
+            return fail(c, TP_ERR_UNSUPPORTED, "tp_run_device_noalloc: device ordinal above 63");
+        const NoallocSetup &S = g_noalloc[c->device];
+        if (S.rc != TP_OK)
+            return fail(c, S.rc, "tp_run_device_noalloc set-up failed: " + S.err);
+        HIPCHK(c, hipSetDevice(c->device));
+        const int p = params->precision == TP_PREC_MIXED ? 1 : 0;
+        const int grid = grid_for(c, 0, params->precision, nb);
+        if ((uint64_t)grid < nb && grid != S.full_grid[p])
+            return fail(c, TP_ERR_INTERNAL, "tp_run_device_noalloc: grid differs from the set-up's");
+        tpk::FusedArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.P = make_params(params);
+        a.num_events = num_events;
+        a.batch_size = TP_BATCH_SIZE;
+        a.precision = params->precision;
+        std::memcpy(a.s0, static_cast<const tp_xoshiro_state *>(rng_state0)->s, sizeof a.s0);
+        a.mt = S.ptr;
+        a.km = KM;
+        a.bt = S.ptr + (size_t)(NS_BT + seeding * NS_KB) * 1024;
+        a.kb = std::max(1, bits_for(nb));
+        a.stride = S.ptr + (size_t)(NS_STRIDE + 2 * seeding + p) * 1024;
+        a.nb = nb;
+        a.b_first = 0;
+        a.out = per_batch_dev;
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        tpk::launch_fused(a, 0, grid, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        if (merged_dev) {
+            tpk::launch_merge(per_batch_dev, nb, merged_dev, c->stream, 0);
             HIPCHK(c, hipGetLastError());
         }
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));

@@ -123,6 +123,8 @@ def _load() -> C.CDLL:
         "tp_simulate_batch": (C.c_int, [vp, P(TpParams), C.c_int, vp, C.c_uint32, P(TpAcc)]),
         "tp_run": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, P(TpAcc), P(TpStats)]),
         "tp_run_device": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, vp, P(TpStats)]),
+        "tp_run_device_noalloc": (C.c_int, [vp, P(TpParams), C.c_int, C.c_int, vp, C.c_uint64, vp, vp,
+                                            P(TpStats)]),
         "tp_events": (C.c_int, [vp, P(TpParams), C.c_int, vp, C.c_uint32, vp, vp, vp, vp]),
         "tp_adjoint": (C.c_int, [vp, P(TpParams), C.c_int, vp, vp, C.c_uint64, C.c_int, C.c_int, vp, vp, vp]),
         "tp_sync": (C.c_int, [vp, P(TpStats)]),
@@ -323,6 +325,13 @@ class Engine:
         self._check(lib.tp_run_device(self.h, C.byref(params), kind, seeding, C.byref(st), num_events,
                                       C.c_void_p(per_batch_ptr), C.c_void_p(merged_ptr or 0), None),
                     "tp_run_device")
+
+    def run_device_noalloc(self, params: TpParams, kind: int, seeding: int, st, num_events: int,
+                           per_batch_ptr: int, merged_ptr: int | None) -> None:
+        """tp_run_device_noalloc: the same run, tables from the static device array."""
+        self._check(lib.tp_run_device_noalloc(self.h, C.byref(params), kind, seeding, C.byref(st), num_events,
+                                              C.c_void_p(per_batch_ptr), C.c_void_p(merged_ptr or 0), None),
+                    "tp_run_device_noalloc")
 
     def sync(self) -> TpStats:
         s = TpStats()

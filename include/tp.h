@@ -167,6 +167,21 @@ int tp_run_device(void *ctx, const tp_params *params, int rng_kind, int seeding,
                   const void *rng_state0, uint64_t num_events, tp_acc *per_batch_dev,
                   tp_acc *merged_dev, tp_stats *stats);
 
+/* tp_run_device without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * runs the xoshiro256+ stream only (rng_kind TP_RNG_XOSHIRO, precision
+ * TP_PREC_F64 or TP_PREC_MIXED, both seedings, no faster-evgen; anything else
+ * returns TP_ERR_UNSUPPORTED) and at most 2^32 - 1 batches (TP_ERR_ARG
+ * beyond).  Its skip-ahead tables sit in a static device array of 800 KiB
+ * per GPU, filled once by tp_create and only read afterwards, so calls that
+ * overlap on one GPU share it without racing; per_batch_dev and merged_dev
+ * are the caller's, as for tp_run_device.  Returns when enqueued; tp_sync
+ * waits. */
+int tp_run_device_noalloc(void *ctx, const tp_params *params, int rng_kind, int seeding,
+                          const void *rng_state0, uint64_t num_events, tp_acc *per_batch_dev,
+                          tp_acc *merged_dev, tp_stats *stats);
+
 /* Per-event diagnostics (the reference's debug-only Display of Event and
  * MEsContributions, event.rs:113-126, matelems.rs:89-105): runs the same
  * device code as the event loop on n events drawn from *rng_state (not
