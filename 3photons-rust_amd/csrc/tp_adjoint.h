@@ -129,8 +129,12 @@ template <class T> __device__ __forceinline__ T sigmoid(T x) { return (T)1 / ((T
 // sweeps over the stored forward state: NLAM 1 with the given lam, NLAM 5
 // (SOFT 0 only) with lam = e_0 .. e_4, the five coupling components.  After
 // each sweep it calls sink(c, m, du, dp); a cut event calls it NLAM times
-// with zeros.
-template <class T, int SOFT, int NLAM, class Sink>
+// with zeros.  With CONVERGE a cut event does not leave early: it runs the
+// same passes on its (possibly non-finite) values and hands the sink zeros,
+// so that every lane of a wavefront reaches each sink call at the same
+// point of the program (a sink that synchronises the workgroup needs that;
+// the lanes of a wavefront execute both paths anyway).
+template <class T, int SOFT, int NLAM, bool CONVERGE = false, class Sink>
 __device__ __forceinline__ bool event_adjoint_sweeps(const AdjParams<T> &P, const T u[12], const T *lam_in, T m[5],
                                                      T du[12], T dp[12], T tau, T fcap, Sink &&sink)
 {
@@ -241,7 +245,7 @@ __device__ __forceinline__ bool event_adjoint_sweeps(const AdjParams<T> &P, cons
             keep = keep && !(fabs(P.el_x * n0) < P.cut_plane * (P.el_e * sqrt((n0 * n0 + n1 * n1) + n2 * n2)));
         }
     }
-    if (!keep) {
+    if (!CONVERGE && !keep) {
 #pragma unroll
         for (int c = 0; c < 5; ++c)
             m[c] = (T)0;
@@ -583,9 +587,24 @@ __device__ __forceinline__ bool event_adjoint_sweeps(const AdjParams<T> &P, cons
             du[4 * i + 2] = P.w * (emb * u[4 * i + 3]);
             du[4 * i + 3] = P.w * (emb * u[4 * i + 2]);
         }
-        sink(sw, m, du, dp);
+        if constexpr (CONVERGE) {
+            // du and dp are rewritten by every sweep, so they are zeroed in
+            // place; m feeds the later sweeps and goes out as a copy
+            T mz[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                mz[c] = keep ? m[c] : (T)0;
+#pragma unroll
+            for (int j = 0; j < 12; ++j) {
+                du[j] = keep ? du[j] : (T)0;
+                dp[j] = keep ? dp[j] : (T)0;
+            }
+            sink(sw, mz, du, dp);
+        } else {
+            sink(sw, m, du, dp);
+        }
     }
-    return true;
+    return keep;
 }
 
 template <class T, int SOFT = 0>

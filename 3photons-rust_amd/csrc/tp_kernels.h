@@ -183,11 +183,12 @@ void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st); // piece states, ex
 namespace tpk {
 
 // Arguments of the adjoint kernel (tp_adjoint).
-// Thread t of the adjoint kernel takes point t, so that a wavefront's stores
-// cover 64 consecutive rows of each output and complete their cache lines
-// within one pass.  Points of the xoshiro stream (u_in null) are first
-// expanded into u_gen by a kernel whose threads each step through GEN_POINTS
-// consecutive points from one skip-ahead.
+// The adjoint kernels run one 64-lane workgroup per 64 consecutive points,
+// one point per lane; the f32 kernel stores the rows through LDS
+// (tp_adjstore.h), the f64 kernel straight from registers.  Points of the
+// xoshiro stream (u_in null) are first expanded into u_gen by a kernel whose
+// threads each step through ADJ_GEN_POINTS consecutive points from one
+// skip-ahead.
 struct AdjArgs {
     tpd::Params P;
     double w;              // RAMBO event weight
