@@ -81,8 +81,10 @@ typedef struct {
     double ev_weight;                     /* RAMBO event weight (evgen.rs:59-62) */
     int32_t sort_photons;                 /* 1 = sort by energy (default), 0 = no-photon-sorting */
     int32_t precision;                    /* tp_precision */
-    int32_t faster_evgen;                 /* 1 = `faster-evgen` generation (evgen.rs:143-173, 221-249;
-                                             RANF stream, f64, skip-ahead seeding), 0 = default */
+    int32_t faster_evgen;                 /* 1 = `faster-evgen` generation (evgen.rs:143-173, 221-249),
+                                             0 = default.  Available for RANF in f64 (either seeding),
+                                             RANF in f32 and xoshiro256+ in f64 (jump seeding only);
+                                             not in mixed precision or with xoshiro128+ */
     int32_t reserved;                     /* 0 */
 } tp_params;
 
@@ -162,7 +164,9 @@ int tp_run(void *ctx, const tp_params *params, int rng_kind, int seeding, const 
 /* tp_run with per_batch_dev (device memory, ceil(N/10 000) tp_acc) written on
  * the context stream and merged_dev (device memory, one tp_acc, may be null)
  * holding the in-order fold computed on the device.  Returns when the work is
- * enqueued; tp_sync() waits for it. */
+ * enqueued; tp_sync() waits for it.  A faster-evgen run returns when its
+ * batches are done: its device stream walk reports through a flag read back
+ * before the call returns. */
 int tp_run_device(void *ctx, const tp_params *params, int rng_kind, int seeding,
                   const void *rng_state0, uint64_t num_events, tp_acc *per_batch_dev,
                   tp_acc *merged_dev, tp_stats *stats);
