@@ -79,6 +79,8 @@ struct Ctx {
     std::vector<int32_t> fe_jprev, fe_jnext;
     std::vector<uint64_t> fe_off;
     uint64_t fe_qnext = 0;
+    uint64_t fe_xo_end = 0; // faster-evgen on xoshiro256+ with skip-ahead seeding: draws the last run consumed
+    uint64_t fe_host_chunks = 0; // faster-evgen: chunks of the last run walked on the host
 };
 
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -309,8 +310,8 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
     if ((params->precision == TP_PREC_F32) != (rng_kind == TP_RNG_XOSHIRO128) && rng_kind != TP_RNG_RANF)
         return fail(c, TP_ERR_ARG, "xoshiro128+ is the stream of the all-f32 mode, xoshiro256+ of the others");
     if (params->faster_evgen && (rng_kind == TP_RNG_XOSHIRO128 || params->precision == TP_PREC_MIXED))
-        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available in f64 (RANF, or xoshiro256+ with jump "
-                                           "seeding) and in f32 (RANF with jump seeding)");
+        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available in f64 (RANF or xoshiro256+) and in f32 "
+                                           "(RANF), not in mixed precision or with xoshiro128+");
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
         return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
     return TP_OK;
@@ -329,6 +330,21 @@ constexpr uint64_t FE_EVENTS_PER_WALKER = 256, FE_MAX_WALKERS = 1u << 18;
 // Positions a walker goes past its segment's end: 24 rounds, ~74 events; a
 // walker meets the true walk within 49 events in the runs checked.
 constexpr uint64_t FE_OVERLAP = 24 * 55;
+// The xoshiro256+ stream has no rounds whose discards bring walkers together:
+// a walker meets the true walk only where both happen to start an event, after
+// a median of 22 events but a long tail (at most 832 over 176 447 walkers of a
+// simulated 1.5e9-draw stream, 12 of them past 512).  Its walkers therefore
+// take 1024 events each and go 1024 events (17 500 positions) past their
+// segment; 16.64 positions per event on average, 17.0 allowed for.
+constexpr double FE_XO_POSITIONS_PER_EVENT = 17.0;
+constexpr uint64_t FE_XO_EVENTS_PER_WALKER = 1024, FE_XO_OVERLAP = 17500;
+// xoshiro256+ streams are expanded in pieces of FE_XO_PIECE uniforms, one
+// wavefront per piece and FE_XO_PER_THREAD consecutive draws per lane from one
+// skip-ahead.  With jump seeding a piece is a batch's stream; with skip-ahead
+// seeding the pieces tile the run's one stream (piece p starts FE_XO_PIECE p
+// draws after the run's state).
+constexpr uint64_t FE_XO_PIECE = 200000;
+constexpr uint32_t FE_XO_PER_THREAD = 3125;
 
 namespace {
 
@@ -357,27 +373,108 @@ int fe_expand(Ctx *c, uint64_t p0_master, int K, uint64_t q_first, uint64_t q_la
     return TP_OK;
 }
 
+// The stream a faster-evgen walk runs on.
+struct FeStream {
+    int kind = 0;                 // tpk::FeWalkArgs kind: 0 RANF f64, 1 RANF f32, 2 xoshiro256+
+    uint64_t p0_master = 0;       // RANF: frame position of the master origin's next value
+    int K = 0;                    // RANF: segment-jump depth of c->polys
+    const uint64_t *s0 = nullptr; // xoshiro256+: the run's state (position 0)
+};
+
+// (M^FE_XO_PIECE)^(2^k), k < kb, on the device: the states of the pieces.
+int ensure_xo_piece_table(Ctx *c, uint64_t max_piece, Tables **out)
+{
+    const int kb = std::max(1, bits_for(max_piece + 1));
+    const std::string key = "xo-piece:" + std::to_string(FE_XO_PIECE);
+    auto it = c->tables.find(key);
+    if (it != c->tables.end() && it->second->kb >= kb) {
+        *out = it->second.get();
+        return TP_OK;
+    }
+    tp::Gf2Mat B;
+    tp::gf2_pow_from_table(tp::xoshiro_step_pow2(), FE_XO_PIECE, B);
+    std::unique_ptr<Tables> t(new Tables);
+    int rc = upload_mats(c, t->bt, tp::gf2_pow2_table(B, kb));
+    if (rc)
+        return rc;
+    t->kb = kb;
+    *out = t.get();
+    c->tables[key] = std::move(t);
+    return TP_OK;
+}
+
+// The stream of positions [q_first, q_last) into HBM: RANF values into
+// c->draws from q_first on, or xoshiro256+ uniforms into c->xo_draws from the
+// first piece boundary at or before q_first.  *qbase is the position held at
+// index 0.
+int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_last, uint64_t *qbase)
+{
+    if (fs.kind != 2) {
+        *qbase = q_first;
+        return fe_expand(c, fs.p0_master, fs.K, q_first, q_last);
+    }
+    const uint64_t p_first = q_first / FE_XO_PIECE, p_end = (q_last + FE_XO_PIECE - 1) / FE_XO_PIECE;
+    const uint64_t np = p_end - p_first;
+    Tables *t = nullptr;
+    int rc = ensure_xo_piece_table(c, p_end, &t);
+    if (!rc)
+        rc = ensure_mt(c);
+    if (!rc)
+        rc = ensure(c, c->xo_states, np * 4 * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->xo_draws, np * FE_XO_PIECE * sizeof(double));
+    if (rc)
+        return rc;
+    tpk::XoPieceArgs p;
+    std::memset(&p, 0, sizeof p);
+    std::memcpy(p.s0, fs.s0, sizeof p.s0);
+    p.bt = static_cast<const uint64_t *>(t->bt.p);
+    p.kb = t->kb;
+    p.mt = static_cast<const uint64_t *>(c->mt.p);
+    p.km = c->km;
+    p.nb = np;
+    p.b_first = p_first;
+    p.states = static_cast<uint64_t *>(c->xo_states.p);
+    p.piece_len = FE_XO_PIECE;
+    p.per_thread = FE_XO_PER_THREAD;
+    p.xbuf = static_cast<double *>(c->xo_draws.p);
+    tpk::launch_xo_expand(p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    *qbase = p_first * FE_XO_PIECE;
+    return TP_OK;
+}
+
 // Device walk of n + 1 event starts from the true start q_start into c->evq,
-// with the stream of the walked range expanded into c->draws (qbase q_start).
-// *ok is false when some walker did not meet its predecessor or the joined
-// walk is too short; the caller then walks on the host.
-int fe_device_walk(Ctx *c, uint64_t p0_master, int K, uint64_t q_start, uint64_t n, bool *ok)
+// with the stream of the walked range expanded into HBM (c->draws or
+// c->xo_draws, position *qbase at index 0).  *ok is false when some walker
+// did not meet its predecessor or the joined walk is too short; the caller
+// then walks on the host.
+int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, bool *ok, uint64_t *qbase)
 {
     *ok = false;
-    const uint64_t span = (uint64_t)std::ceil((double)(n + 1) * FE_POSITIONS_PER_EVENT) + 55;
-    const uint32_t T = (uint32_t)std::min<uint64_t>(FE_MAX_WALKERS, std::max<uint64_t>(64, n / FE_EVENTS_PER_WALKER));
+    const bool xo = fs.kind == 2;
     tpk::FeWalkArgs w;
     std::memset(&w, 0, sizeof w);
+    w.kind = fs.kind;
     w.q_start = q_start;
-    w.seg_origin = q_start / 55 * 55;
-    w.seg = ((q_start - w.seg_origin + span) / T / 55 + 1) * 55;
-    w.overlap = FE_OVERLAP;
-    w.nwalkers = T;
+    if (xo) { // no rounds: segments start anywhere
+        const uint64_t span = (uint64_t)std::ceil((double)(n + 1) * FE_XO_POSITIONS_PER_EVENT) + 128;
+        w.nwalkers = (uint32_t)std::min<uint64_t>(FE_MAX_WALKERS, std::max<uint64_t>(1, n / FE_XO_EVENTS_PER_WALKER));
+        w.seg_origin = q_start;
+        w.seg = span / w.nwalkers + 1;
+        w.overlap = FE_XO_OVERLAP;
+    } else {
+        const uint64_t span = (uint64_t)std::ceil((double)(n + 1) * FE_POSITIONS_PER_EVENT) + 55;
+        w.nwalkers = (uint32_t)std::min<uint64_t>(FE_MAX_WALKERS, std::max<uint64_t>(64, n / FE_EVENTS_PER_WALKER));
+        w.seg_origin = q_start / 55 * 55;
+        w.seg = ((q_start - w.seg_origin + span) / w.nwalkers / 55 + 1) * 55;
+        w.overlap = FE_OVERLAP;
+    }
+    const uint32_t T = w.nwalkers;
     // every event takes at least 15 positions: no walk holds more starts
     w.cap = (uint32_t)((w.seg + w.overlap + 63) / 15 + 2);
-    w.qbase = q_start;
     w.qlimit = w.seg_origin + (uint64_t)T * w.seg + w.overlap + 128;
-    int rc = fe_expand(c, p0_master, K, w.qbase, w.qlimit);
+    int rc = fe_stream_expand(c, fs, q_start, w.qlimit, &w.qbase);
     if (!rc)
         rc = ensure(c, c->fe_lists, (size_t)T * w.cap * sizeof(uint64_t));
     if (!rc)
@@ -389,6 +486,7 @@ int fe_device_walk(Ctx *c, uint64_t p0_master, int K, uint64_t q_start, uint64_t
     if (rc)
         return rc;
     w.buf = static_cast<const int32_t *>(c->draws.p);
+    w.xbuf = static_cast<const double *>(c->xo_draws.p);
     w.lists = static_cast<uint64_t *>(c->fe_lists.p);
     w.counts = static_cast<uint32_t *>(c->fe_meta.p);
     w.join_prev = reinterpret_cast<int32_t *>(w.counts + T);
@@ -437,38 +535,34 @@ int fe_device_walk(Ctx *c, uint64_t p0_master, int K, uint64_t q_start, uint64_t
     HIPCHK(c, hipMemcpyAsync(&c->fe_qnext, static_cast<uint64_t *>(c->evq.p) + n, sizeof(uint64_t),
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors are reused
+    *qbase = w.qbase;
     *ok = true;
     return TP_OK;
 }
 
 } // namespace
 
-int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs, uint64_t num_events,
-                              uint32_t batch_size, void *per_batch_dev)
+// Chunks of FE_WALK_BATCHES batches on one faster-evgen stream, from the event
+// start q_start: each chunk's event starts (device walk for runs of
+// FE_DEVICE_WALK_MIN events or more, else -- or when the device walkers do not
+// all join -- host_walk), then its events: fused_kernel<3> (RANF) or <4>
+// (xoshiro256+), or in f32 (fa non-null) f32_events_kernel<2> and the
+// in-order sums.  *q_end receives the position after the last event.
+using FeHostWalk = std::function<void(uint64_t q, uint64_t n, uint64_t *starts, uint64_t *q_next)>;
+int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, tpk::FusedArgs &a, tpk::F32Args *fa,
+                      uint64_t q_start, uint64_t num_events, uint32_t batch_size, void *per_batch_dev, uint64_t *q_end)
 {
-    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors below are reused
-    tp::RanfState g0;
-    std::memcpy(g0.numbers, rs->numbers, sizeof g0.numbers);
-    g0.index = rs->index;
-    g0.seed = rs->seed;
-    const uint64_t p0_master = 55u - rs->index; // position of the next value
     const bool device_walk = num_events >= FE_DEVICE_WALK_MIN;
-    // segment jumps cover positions up to 30 per event (the mean is 17.9)
-    const int K = std::max(1, bits_for((p0_master + 30 * (num_events + 1) + 4 * FE_OVERLAP) / 55 / RANF_R + 2));
-    int rc = ensure_polys(c, RANF_R, K);
-    if (!rc)
-        rc = ensure(c, c->origin, 56 * sizeof(int32_t));
-    if (!rc)
-        rc = ensure(c, c->p0, sizeof(int32_t));
-    if (rc)
-        return rc;
-    std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
-    c->p00 = static_cast<int32_t>(p0_master);
-    HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
     const uint64_t nb = (num_events + batch_size - 1) / batch_size;
     const uint64_t chunk = std::min<uint64_t>(nb, FE_WALK_BATCHES);
-    uint64_t q_start = p0_master;
+    c->fe_host_chunks = 0;
+    if (fa) {
+        int rc = ensure(c, c->f32ev, chunk * batch_size * 6 * sizeof(float));
+        if (rc)
+            return rc;
+        fa->ev = static_cast<float *>(c->f32ev.p);
+    }
+    const int source = fs.kind == 2 ? 4 : 3;
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
         const uint64_t b1 = std::min(nb, b0 + chunk);
@@ -477,43 +571,119 @@ int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_ranf_state *rs
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         bool ok = false;
-        uint64_t q_next = 0;
+        uint64_t q_next = 0, qbase = 0;
         if (device_walk) {
-            rc = fe_device_walk(c, p0_master, K, q_start, n, &ok);
+            int rc = fe_device_walk(c, fs, q_start, n, &ok, &qbase);
             if (rc)
                 return rc;
             q_next = c->fe_qnext;
         }
-        if (!ok) { // host walk of this chunk, from the state at q_start
+        if (!ok) { // host walk of this chunk from q_start
+            ++c->fe_host_chunks;
             c->fe_starts.resize(n);
-            tp::ranf_walk_faster_evgen_from(g0, q_start, n, c->fe_starts.data(), &q_next);
-            rc = ensure(c, c->evq, n * sizeof(uint64_t));
+            host_walk(q_start, n, c->fe_starts.data(), &q_next);
+            int rc = ensure(c, c->evq, n * sizeof(uint64_t));
             if (!rc)
-                rc = fe_expand(c, p0_master, K, q_start, q_next);
+                rc = fe_stream_expand(c, fs, q_start, q_next, &qbase);
             if (rc)
                 return rc;
             HIPCHK(c, hipMemcpyAsync(c->evq.p, c->fe_starts.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice,
                                      c->stream));
         }
-        a.nb = b1 - b0;
-        a.b_first = b0;
-        a.draws = static_cast<const int32_t *>(c->draws.p);
-        a.qbase = q_start;
-        a.evq = static_cast<const uint64_t *>(c->evq.p);
-        a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
         if (first) {
             HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
             first = false;
         }
-        tpk::launch_fused(a, 3, grid_for(c, 3, 0, a.nb), c->stream);
-        HIPCHK(c, hipGetLastError());
+        if (fa) {
+            fa->nb = b1 - b0;
+            fa->b_first = b0;
+            fa->draws = static_cast<const int32_t *>(c->draws.p);
+            fa->qbase = qbase;
+            fa->evq = static_cast<const uint64_t *>(c->evq.p);
+            fa->out = static_cast<tp_acc *>(per_batch_dev) + b0;
+            tpk::launch_f32_events(*fa, f32_grid(c, fa->nb), c->stream);
+            HIPCHK(c, hipGetLastError());
+            tpk::launch_f32_accumulate(*fa, f32_grid(c, fa->nb), c->stream);
+            HIPCHK(c, hipGetLastError());
+        } else {
+            a.nb = b1 - b0;
+            a.b_first = b0;
+            a.draws = static_cast<const int32_t *>(c->draws.p);
+            a.xdraws = static_cast<const double *>(c->xo_draws.p);
+            a.qbase = qbase;
+            a.evq = static_cast<const uint64_t *>(c->evq.p);
+            a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+            tpk::launch_fused(a, source, grid_for(c, source, 0, a.nb), c->stream);
+            HIPCHK(c, hipGetLastError());
+        }
         if (!ok) // fe_starts is staged for the copy above
             HIPCHK(c, hipStreamSynchronize(c->stream));
         q_start = q_next;
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    *q_end = q_start;
+    return TP_OK;
+}
+
+int enqueue_ranf_faster_evgen(Ctx *c, tpk::FusedArgs &a, tpk::F32Args *fa, const tp_ranf_state *rs,
+                              uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors below are reused
+    tp::RanfState g0;
+    std::memcpy(g0.numbers, rs->numbers, sizeof g0.numbers);
+    g0.index = rs->index;
+    g0.seed = rs->seed;
+    FeStream fs;
+    fs.kind = fa ? 1 : 0;
+    fs.p0_master = 55u - rs->index; // position of the next value
+    // segment jumps cover positions up to 30 per event (the mean is 17.9)
+    fs.K = std::max(1, bits_for((fs.p0_master + 30 * (num_events + 1) + 4 * FE_OVERLAP) / 55 / RANF_R + 2));
+    int rc = ensure_polys(c, RANF_R, fs.K);
+    if (!rc)
+        rc = ensure(c, c->origin, 56 * sizeof(int32_t));
+    if (!rc)
+        rc = ensure(c, c->p0, sizeof(int32_t));
+    if (rc)
+        return rc;
+    std::memcpy(c->origin0, rs->numbers, sizeof c->origin0);
+    c->p00 = static_cast<int32_t>(fs.p0_master);
+    HIPCHK(c, hipMemcpyAsync(c->origin.p, c->origin0, sizeof c->origin0, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->p0.p, &c->p00, sizeof c->p00, hipMemcpyHostToDevice, c->stream));
+    const bool f32 = fa != nullptr;
+    const FeHostWalk walk = [&](uint64_t q, uint64_t n, uint64_t *starts, uint64_t *q_next) {
+        tp::ranf_walk_faster_evgen_from(g0, q, n, starts, q_next, 16, f32);
+    };
+    uint64_t q_end = 0;
+    rc = enqueue_fe_chunks(c, fs, walk, a, fa, fs.p0_master, num_events, batch_size, per_batch_dev, &q_end);
+    if (rc)
+        return rc;
     c->fe_end = g0;
-    tp::ranf_advance(c->fe_end, q_start - p0_master);
+    tp::ranf_advance(c->fe_end, q_end - fs.p0_master);
+    return TP_OK;
+}
+
+// xoshiro256+ with `faster-evgen` and skip-ahead seeding (standard-random,
+// faster-evgen): one stream from the run's state, random_array::<N> taking N
+// consecutive draws (standard.rs:33-35), walked like the RANF stream above
+// with walkers of FE_XO_EVENTS_PER_WALKER events.  c->fe_xo_end receives the
+// number of draws the run consumed.
+int enqueue_xo_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_state *x, uint64_t num_events,
+                            uint32_t batch_size, void *per_batch_dev)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors below are reused
+    uint64_t s0[4];
+    std::memcpy(s0, x->s, sizeof s0);
+    FeStream fs;
+    fs.kind = 2;
+    fs.s0 = s0;
+    const FeHostWalk walk = [&](uint64_t q, uint64_t n, uint64_t *starts, uint64_t *q_next) {
+        tp::xoshiro_walk_faster_evgen(s0, q, n, starts, q_next);
+    };
+    uint64_t q_end = 0;
+    int rc = enqueue_fe_chunks(c, fs, walk, a, nullptr, 0, num_events, batch_size, per_batch_dev, &q_end);
+    if (rc)
+        return rc;
+    c->fe_xo_end = q_end;
     return TP_OK;
 }
 
@@ -663,8 +833,7 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
 // into HBM as f64 uniforms, one piece per batch, walked one thread per batch,
 // and simulated from those starts.  A batch consumes about 166 000 draws
 // (16.6 per event); a piece holds 200 000.
-constexpr uint64_t FE_XO_PIECE = 200000;
-constexpr uint32_t FE_XO_PER_THREAD = 3125; // one wavefront expands a piece
+
 
 int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_state *x, uint64_t num_events,
                                  uint32_t batch_size, void *per_batch_dev)
@@ -767,11 +936,11 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
         fa.ev = static_cast<float *>(c->f32ev.p);
     }
     if (rng_kind == TP_RNG_XOSHIRO && params->faster_evgen) {
-        if (seeding != TP_SEED_JUMP)
-            return fail(c, TP_ERR_UNSUPPORTED,
-                        "faster-evgen with the xoshiro256+ stream runs with jump seeding (faster-threading) only");
-        return enqueue_xo_faster_evgen_jump(c, a, static_cast<const tp_xoshiro_state *>(state0), num_events,
-                                            batch_size, per_batch_dev);
+        if (seeding == TP_SEED_JUMP)
+            return enqueue_xo_faster_evgen_jump(c, a, static_cast<const tp_xoshiro_state *>(state0), num_events,
+                                                batch_size, per_batch_dev);
+        return enqueue_xo_faster_evgen(c, a, static_cast<const tp_xoshiro_state *>(state0), num_events, batch_size,
+                                       per_batch_dev);
     }
     if (rng_kind == TP_RNG_XOSHIRO) {
         const tp_xoshiro_state *x = static_cast<const tp_xoshiro_state *>(state0);
@@ -802,8 +971,8 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
         if (seeding == TP_SEED_JUMP)
             return enqueue_ranf_faster_evgen_jump(c, params, a, rs, num_events, batch_size, per_batch_dev);
         if (f32)
-            return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen in f32 runs with jump seeding (faster-threading) only");
-        return enqueue_ranf_faster_evgen(c, a, rs, num_events, batch_size, per_batch_dev);
+            fa.source = 2;
+        return enqueue_ranf_faster_evgen(c, a, f32 ? &fa : nullptr, rs, num_events, batch_size, per_batch_dev);
     }
     const uint64_t chunk = std::min<uint64_t>(nb, RANF_CHUNK_BATCHES);
     const size_t per_batch_draws = 12ull * batch_size;
@@ -1327,6 +1496,7 @@ extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, cons
             c->last.device_ms = c->last.kernel_ms;
             c->last.events = n;
             c->last.batches = 0;
+            c->last.host_walk_chunks = 0;
         }
         return rc;
     });
@@ -1346,9 +1516,6 @@ extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kin
             return fail(c, TP_ERR_ARG, "null output");
         if (n > 64u * TP_BATCH_SIZE)
             return fail(c, TP_ERR_ARG, "n exceeds 640000 events per call");
-        if (params->faster_evgen && (rng_kind == TP_RNG_XOSHIRO || params->precision == TP_PREC_F32))
-            return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen with the xoshiro256+ stream or in f32 runs with jump "
-                                               "seeding (faster-threading) only; the closure consumes the stream in order");
         HIPCHK(c, hipSetDevice(c->device));
         if (n == 0) { // the reference's closure returns an empty accumulator
             std::memset(out, 0, sizeof *out);
@@ -1364,7 +1531,9 @@ extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kin
         HIPCHK(c, hipStreamSynchronize(c->stream));
         // `&mut rng` side effect: 12 draws per event (evgen.rs:182-187), or the
         // walk of the faster-evgen draws.
-        if (params->faster_evgen) {
+        if (params->faster_evgen && rng_kind == TP_RNG_XOSHIRO) {
+            tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, c->fe_xo_end);
+        } else if (params->faster_evgen) {
             tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
             std::memcpy(s->numbers, c->fe_end.numbers, sizeof s->numbers);
             s->index = c->fe_end.index;
@@ -1406,6 +1575,7 @@ extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, i
         const uint64_t nb = (num_events + TP_BATCH_SIZE - 1) / TP_BATCH_SIZE;
         // Event 0 and 1 brackets the first kernel; 2 follows the event loop.
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        c->fe_host_chunks = 0;
         rc = enqueue_run(c, params, rng_kind, seeding, rng_state0, num_events, TP_BATCH_SIZE, per_batch_dev);
         if (rc)
             return rc;
@@ -1416,6 +1586,7 @@ extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, i
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         c->last.events = num_events;
         c->last.batches = nb;
+        c->last.host_walk_chunks = c->fe_host_chunks;
         c->timed = true;
         if (stats)
             *stats = c->last;
@@ -1484,6 +1655,7 @@ extern "C" int tp_run_device_noalloc(void *ctx, const tp_params *params, int rng
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         c->last.events = num_events;
         c->last.batches = nb;
+        c->last.host_walk_chunks = 0;
         c->timed = true;
         if (stats)
             *stats = c->last;

@@ -117,7 +117,9 @@ void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st);
 // next segment's start by `overlap` positions.  Positions are counted from
 // the start of the master state's round; buf holds positions [qbase, qlimit).
 struct FeWalkArgs {
-    const int32_t *buf;
+    const int32_t *buf;    // RANF values of positions qbase .. qlimit - 1 (kinds 0, 1)
+    const double *xbuf;    // xoshiro256+ uniforms of the same positions (kind 2)
+    int32_t kind;          // 0: RANF, unit-disc test in f64; 1: RANF, in f32; 2: xoshiro256+
     uint64_t qbase, qlimit;
     uint64_t q_start;
     uint64_t seg_origin, seg, overlap;
@@ -177,6 +179,7 @@ struct XoPieceArgs {
     uint32_t *overflow;
 };
 void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st); // piece states, expansion, walk
+void launch_xo_expand(const XoPieceArgs &a, hipStream_t st); // piece states and expansion only
 
 } // namespace tpk
 

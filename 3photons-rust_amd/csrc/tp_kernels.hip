@@ -593,6 +593,42 @@ template <class T = double> __device__ __forceinline__ bool fe_walk_event(const 
     return true;
 }
 
+// One event's consumption of the xoshiro stream: 9, 6, then 2 per re-rolled
+// point of the unit-disc test, no discards.  buf holds the uniforms of
+// positions qbase .. qlimit - 1.  False when the stored range ends inside the
+// event.
+__device__ __forceinline__ bool xo_walk_event(const double *buf, uint64_t qbase, uint64_t qlimit, uint64_t &q)
+{
+    q += 9;
+    if (q + 6 > qlimit)
+        return false;
+    const double *r6 = buf + (q - qbase);
+    q += 6;
+    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    for (int i = 0; i < 3; ++i) {
+        const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
+        double r2 = (0. + x * x) + y * y;
+        while (r2 > 1. || r2 < min2) {
+            if (q + 2 > qlimit)
+                return false;
+            const double *v = buf + (q - qbase);
+            const double xa = 2. * v[0] - 1., ya = 2. * v[1] - 1.;
+            q += 2;
+            r2 = 0. + (xa * xa + ya * ya);
+        }
+    }
+    return true;
+}
+
+// One event of the walk of kind a.kind: RANF with the unit-disc test in f64
+// (0) or f32 (1), or the xoshiro256+ stream (2).
+__device__ __forceinline__ bool fe_walk_any(const FeWalkArgs &a, uint64_t &q)
+{
+    if (a.kind == 2)
+        return xo_walk_event(a.xbuf, a.qbase, a.qlimit, q);
+    return a.kind == 1 ? fe_walk_event<float>(a, q) : fe_walk_event<double>(a, q);
+}
+
 // One thread per walker.  Every step records a start and then consumes at
 // least 15 positions, and the loop stops at the segment end, the list
 // capacity or the stored range, so it always ends.
@@ -607,7 +643,7 @@ __global__ __launch_bounds__(256) void fe_walk_kernel(FeWalkArgs a)
     uint32_t n = 0;
     while (q < stop && n < a.cap) {
         list[n++] = q;
-        if (!fe_walk_event(a, q))
+        if (!fe_walk_any(a, q))
             break;
     }
     a.counts[k] = n;
@@ -753,31 +789,6 @@ __global__ __launch_bounds__(64) void xo_expand_kernel(XoPieceArgs a)
         out[i] = xo_uniform(s);
 }
 
-// One event's consumption of the xoshiro stream: 9, 6, then 2 per re-rolled
-// point of the unit-disc test, no discards.  False when the piece ends inside
-// the event.
-__device__ __forceinline__ bool xo_walk_event(const double *buf, uint64_t qlimit, uint64_t &q)
-{
-    q += 9;
-    if (q + 6 > qlimit)
-        return false;
-    const double *r6 = buf + q;
-    q += 6;
-    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
-    for (int i = 0; i < 3; ++i) {
-        const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
-        double r2 = (0. + x * x) + y * y;
-        while (r2 > 1. || r2 < min2) {
-            if (q + 2 > qlimit)
-                return false;
-            const double xa = 2. * buf[q] - 1., ya = 2. * buf[q + 1] - 1.;
-            q += 2;
-            r2 = 0. + (xa * xa + ya * ya);
-        }
-    }
-    return true;
-}
-
 // One thread per batch, as fe_walk_batches_kernel.
 __global__ __launch_bounds__(64) void xo_walk_batches_kernel(XoPieceArgs a)
 {
@@ -793,7 +804,7 @@ __global__ __launch_bounds__(64) void xo_walk_batches_kernel(XoPieceArgs a)
     uint32_t e = 0;
     for (; e < n; ++e) {
         out[e] = q;
-        if (!xo_walk_event(a.xbuf, base + a.piece_len, q))
+        if (!xo_walk_event(a.xbuf, 0, base + a.piece_len, q))
             break;
     }
     if (e < n) {
@@ -801,6 +812,13 @@ __global__ __launch_bounds__(64) void xo_walk_batches_kernel(XoPieceArgs a)
         for (; e < n; ++e)
             out[e] = base;
     }
+}
+
+void launch_xo_expand(const XoPieceArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(xo_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
+    const uint64_t threads = a.nb * (a.piece_len / a.per_thread);
+    hipLaunchKernelGGL(xo_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
 }
 
 void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st)

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <system_error>
 #include <thread>
+#include <limits>
 #include <vector>
 #include <mutex>
 
@@ -424,33 +425,43 @@ struct RanfWalker {
 
 namespace {
 // Event starts from the walker's position onward, until n events are recorded
-// or the position reaches q_stop (when q_stop is not 0).
-void walk_events(RanfWalker &w, uint64_t n, uint64_t q_stop, std::vector<uint64_t> &out)
+// or the position reaches q_stop (when q_stop is not 0).  T is the run's float
+// type: the unit-disc test runs on (n as T) * 1e-9 values (ranf.rs:99), and
+// f32's MIN_POSITIVE squared underflows to 0, as in the reference's f32 build.
+template <class T> void walk_events_t(RanfWalker &w, uint64_t n, uint64_t q_stop, std::vector<uint64_t> &out)
 {
+    const T minpos = std::numeric_limits<T>::min();
+    const T min2 = minpos * minpos;
     for (uint64_t e = 0; e < n; ++e) {
         if (q_stop && w.q >= q_stop)
             return;
         out.push_back(w.q);
         w.take(9); // only their positions matter here
         const int32_t *r6 = w.take(6);
-        double px[3], py[3];
+        T px[3], py[3];
         for (int i = 0; i < 3; ++i) {
-            px[i] = (double)r6[i] * 1e-9;
-            py[i] = (double)r6[3 + i] * 1e-9;
+            px[i] = (T)r6[i] * (T)1e-9;
+            py[i] = (T)r6[3 + i] * (T)1e-9;
         }
         // radii of the three points (evgen.rs:228-246); only the accept test
         // matters here, with the reference's float operations
         for (int i = 0; i < 3; ++i) {
-            const double x = 2. * px[i] - 1., y = 2. * py[i] - 1.;
-            double r2 = (0. + x * x) + y * y;
-            const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
-            while (r2 > 1. || r2 < min2) {
+            const T x = (T)2 * px[i] - (T)1, y = (T)2 * py[i] - (T)1;
+            T r2 = ((T)0 + x * x) + y * y;
+            while (r2 > (T)1 || r2 < min2) {
                 const int32_t *r2v = w.take(2);
-                const double a = 2. * ((double)r2v[0] * 1e-9) - 1., b = 2. * ((double)r2v[1] * 1e-9) - 1.;
-                r2 = 0. + (a * a + b * b);
+                const T a = (T)2 * ((T)r2v[0] * (T)1e-9) - (T)1, b = (T)2 * ((T)r2v[1] * (T)1e-9) - (T)1;
+                r2 = (T)0 + (a * a + b * b);
             }
         }
     }
+}
+void walk_events(RanfWalker &w, uint64_t n, uint64_t q_stop, std::vector<uint64_t> &out, bool f32)
+{
+    if (f32)
+        walk_events_t<float>(w, n, q_stop, out);
+    else
+        walk_events_t<double>(w, n, q_stop, out);
 }
 
 // A walker whose next value is at stream position q (q >= q0, the position
@@ -470,7 +481,8 @@ RanfWalker walker_at(const RanfState &g0, uint64_t q0, uint64_t q)
 // true walk from the previous segment reaches, within a few events, an event
 // start the next thread also recorded; from there on both agree, so the
 // segments are joined at that point (a sequential walk covers any gap).
-void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end, int max_threads)
+void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end, int max_threads,
+                            bool f32)
 {
     const uint64_t q0 = 55u - g.index;
     unsigned hw = std::thread::hardware_concurrency();
@@ -485,7 +497,7 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
         lists.assign(1, {});
         RanfWalker w{g, q0};
         lists[0].reserve(num_events + 1);
-        walk_events(w, num_events + 1, 0, lists[0]);
+        walk_events(w, num_events + 1, 0, lists[0], f32);
         pieces.assign(1, Piece{0, 0, lists[0].size()});
     };
     // ~17.9 draws per event; T segments of whole rounds, each walked 64 rounds past its end
@@ -505,7 +517,7 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
                     RanfWalker w = k == 0 ? RanfWalker{g, q0} : walker_at(g, q0, qs);
                     lists[(size_t)k].reserve((size_t)(seg / 17));
                     walk_events(w, ~0ull, k + 1 < T ? (uint64_t)(k + 1) * seg + overlap : est + overlap,
-                                lists[(size_t)k]);
+                                lists[(size_t)k], f32);
                 });
         } catch (const std::system_error &) { // no thread to be had: walk alone
             started = false;
@@ -546,7 +558,7 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
                 while (!joined && a.back() <= b.back()) {
                     stop += overlap;
                     std::vector<uint64_t> ext;
-                    walk_events(w, ~0ull, stop, ext);
+                    walk_events(w, ~0ull, stop, ext, f32);
                     a.insert(a.end(), ext.begin() + (first ? 1 : 0), ext.end()); // the first repeats a.back()
                     first = false;
                     joined = meet();
@@ -568,7 +580,7 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
         const Piece &p = pieces.back();
         RanfWalker w = walker_at(g, q0, lists[p.list][p.end - 1]);
         lists.emplace_back();
-        walk_events(w, num_events + 1 - total + 1, 0, lists.back());
+        walk_events(w, num_events + 1 - total + 1, 0, lists.back(), f32);
         pieces.push_back({lists.size() - 1, 1, lists.back().size()}); // [0] repeats the previous start
     }
     uint64_t qe = 0, off = 0; // the start of event num_events is where the last one ended
@@ -587,18 +599,48 @@ void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts,
 }
 
 void ranf_walk_faster_evgen_from(const RanfState &g0, uint64_t q, uint64_t num_events, uint64_t *starts,
-                                 uint64_t *q_end, int max_threads)
+                                 uint64_t *q_end, int max_threads, bool f32)
 {
     RanfState g = g0;
     ranf_advance(g, q - (55u - g0.index));
     const uint64_t rel = q - (55u - g.index); // g0-frame position of g's round start
     uint64_t qe = 0;
-    ranf_walk_faster_evgen(g, num_events, starts, &qe, max_threads);
+    ranf_walk_faster_evgen(g, num_events, starts, &qe, max_threads, f32);
     if (starts)
         for (uint64_t e = 0; e < num_events; ++e)
             starts[e] += rel;
     if (q_end)
         *q_end = qe + rel;
+}
+
+void xoshiro_walk_faster_evgen(const uint64_t s0[4], uint64_t q, uint64_t num_events, uint64_t *starts,
+                               uint64_t *q_end)
+{
+    uint64_t s[4] = {s0[0], s0[1], s0[2], s0[3]};
+    xoshiro_advance(s, q);
+    auto uniform = [&]() { return (double)(xoshiro_next(s) >> 11) * 0x1.0p-53; };
+    const double min2 = std::numeric_limits<double>::min() * std::numeric_limits<double>::min();
+    for (uint64_t e = 0; e < num_events; ++e) {
+        if (starts)
+            starts[e] = q;
+        for (int k = 0; k < 9; ++k) // random_array::<9>: only their positions matter here
+            (void)xoshiro_next(s);
+        double r6[6];
+        for (int k = 0; k < 6; ++k)
+            r6[k] = uniform();
+        q += 15;
+        for (int i = 0; i < 3; ++i) { // evgen.rs:228-246, the reference's float operations
+            const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
+            double r2 = (0. + x * x) + y * y;
+            while (r2 > 1. || r2 < min2) {
+                const double a = 2. * uniform() - 1., b = 2. * uniform() - 1.;
+                q += 2;
+                r2 = 0. + (a * a + b * b);
+            }
+        }
+    }
+    if (q_end)
+        *q_end = q;
 }
 
 } // namespace tp

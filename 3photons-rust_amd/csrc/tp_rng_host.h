@@ -53,12 +53,13 @@ void ranf_advance(RanfState &g, uint64_t n);
 // faster-evgen stream walk: the start position of each event; g ends at the
 // position after the last event.  Walks of 4e6 events or more use up to
 // max_threads threads (1: sequential); the result does not depend on it.
+// With f32 the unit-disc test runs in f32, as in the reference's f32 build.
 void ranf_walk_faster_evgen(RanfState &g, uint64_t num_events, uint64_t *starts, uint64_t *q_end,
-                            int max_threads = 16);
+                            int max_threads = 16, bool f32 = false);
 // The same walk from stream position q of g0's frame (q >= 55 - g0.index, the
 // start of an event of the walk from g0): starts[] and *q_end in g0's frame.
 void ranf_walk_faster_evgen_from(const RanfState &g0, uint64_t q, uint64_t num_events, uint64_t *starts,
-                                 uint64_t *q_end, int max_threads = 16);
+                                 uint64_t *q_end, int max_threads = 16, bool f32 = false);
 
 // ---- xoshiro256+ -------------------------------------------------------------
 void xoshiro_seed(uint64_t seed, uint64_t s[4]); // seed_from_u64 (SplitMix64 expansion)
@@ -80,6 +81,12 @@ void gf2_pow_from_table(const std::vector<Gf2Mat> &table, uint64_t n, Gf2Mat &ou
 const std::vector<Gf2Mat> &xoshiro_step_pow2();
 // Advance a state by n steps with the cached tables.
 void xoshiro_advance(uint64_t s[4], uint64_t n);
+// faster-evgen walk of the xoshiro256+ stream (standard.rs:33-35: each
+// random_array::<N> is N consecutive draws, no discards): num_events event
+// starts from position q of s0's frame (q = draws after s0, the start of an
+// event), and *q_end the position after the last event.  Sequential.
+void xoshiro_walk_faster_evgen(const uint64_t s0[4], uint64_t q, uint64_t num_events, uint64_t *starts,
+                               uint64_t *q_end);
 
 // ---- xoshiro128+ (the generator of the all-f32 mode, standard.rs with the
 // f32 feature; rand_xoshiro 0.6.0): same structure on 4 x 32-bit words ------

@@ -356,7 +356,8 @@ def main() -> int:
         # faster-evgen: event starts in the RANF stream depend on every earlier
         # draw (rejection re-rolls, round discards), so the stream is walked
         # first (on the device, segment walkers joined where they meet); timed
-        # with that walk, on a bounded sample
+        # with that walk, on a bounded sample.  The reference's faster-evgen +
+        # multi-threading build: 1e7 events in 0.87 s (VALIDATION.md:45-47)
         fe_events = min(args.events, args.fe_events)
         pfe = tp.params_from_config(cfg.with_events(fe_events), faster_evgen=True)
         vfe, dtfe, kfe, accfe = measure(tp, torch, eng, pfe, fe_events, 1, 1, per_batch, merged, tp.RNG_RANF)
@@ -371,6 +372,17 @@ def main() -> int:
             fe_jump[name] = {"value": vj, "unit": "events/s", "events": fe_events, "ms_per_step": dtj * 1e3,
                              "kernels_ms": kj, "selected_events": int(accj.selected_events),
                              "reference_published_events_per_s": 1e7 / ref_s}
+        # faster-evgen with skip-ahead seeding on the other streams: xoshiro256+
+        # (standard-random, no published figure) and RANF in f32 (VALIDATION.md:41-43,
+        # single-threaded), each walked on the device as above
+        fe_more = {}
+        for name, kind, prec, ref_s in (("xoshiro_f64", tp.RNG_XOSHIRO, tp.PREC_F64, None),
+                                        ("ranf_f32", tp.RNG_RANF, tp.PREC_F32, 2.11)):
+            pj = tp.params_from_config(cfg.with_events(fe_events), precision=prec, faster_evgen=True)
+            vj, dtj, kj, accj = measure(tp, torch, eng, pj, fe_events, 1, 1, per_batch, merged, kind)
+            fe_more[name] = {"value": vj, "unit": "events/s", "events": fe_events, "ms_per_step": dtj * 1e3,
+                             "kernels_ms": kj, "selected_events": int(accj.selected_events),
+                             "reference_published_events_per_s": None if ref_s is None else 1e7 / ref_s}
         out["variants"] = {
             "ranf_f64": {"value": vr, "unit": "events/s", "ms_per_step": dtr * 1e3 / 2,
                          "kernels_ms": kr, "selected_events": int(accr.selected_events)},
@@ -381,7 +393,9 @@ def main() -> int:
                                    "sigma_pb_f64_xoshiro256": f64r.sigma},
             "faster_evgen_f64": {"value": vfe, "unit": "events/s", "events": fe_events, "ms_per_step": dtfe * 1e3,
                                  "kernels_ms": kfe, "selected_events": int(accfe.selected_events),
-                                 "includes": "walk of the RANF stream for the event starts (device)"},
+                                 "includes": "walk of the RANF stream for the event starts (device)",
+                                 "reference_published_events_per_s": 1e7 / 0.87},
+            "faster_evgen_skipahead": fe_more,
             "faster_evgen_faster_threading": fe_jump,
         }
     if not args.no_adjoint:

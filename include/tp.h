@@ -82,9 +82,9 @@ typedef struct {
     int32_t sort_photons;                 /* 1 = sort by energy (default), 0 = no-photon-sorting */
     int32_t precision;                    /* tp_precision */
     int32_t faster_evgen;                 /* 1 = `faster-evgen` generation (evgen.rs:143-173, 221-249),
-                                             0 = default.  Available for RANF in f64 (either seeding),
-                                             RANF in f32 and xoshiro256+ in f64 (jump seeding only);
-                                             not in mixed precision or with xoshiro128+ */
+                                             0 = default.  Available for RANF in f64 and f32 and for
+                                             xoshiro256+ in f64, with either seeding; not in mixed
+                                             precision or with xoshiro128+ */
     int32_t reserved;                     /* 0 */
 } tp_params;
 
@@ -123,6 +123,9 @@ typedef struct {
     double kernel_ms;   /* the fused event-loop kernel(s) only */
     uint64_t events;
     uint64_t batches;
+    uint64_t host_walk_chunks; /* faster-evgen with skip-ahead seeding: chunks of batches whose event
+                                  starts came from the host walk (runs below 2^20 events, the
+                                  closure, or device walkers that did not all meet) */
 } tp_stats;
 
 /* Context: binds device `device` (HIP ordinal), creates its stream. */

@@ -46,8 +46,10 @@ def walklib(tmp_path_factory):
                     os.path.join(here, "walk_check.cpp"), os.path.join(csrc, "tp_rng_host.cpp"), "-o", str(out)],
                    check=True)
     lib = C.CDLL(str(out))
-    lib.tp_walk_mismatch.argtypes = [C.c_uint64, C.c_int, C.c_uint32, C.c_int]
-    lib.tp_walk_end.argtypes = [C.POINTER(C.c_int32), C.c_uint32, C.c_uint64, C.c_int]
+    lib.tp_walk_mismatch.argtypes = [C.c_uint64, C.c_int, C.c_uint32, C.c_int, C.c_int]
+    lib.tp_walk_end.argtypes = [C.POINTER(C.c_int32), C.c_uint32, C.c_uint64, C.c_int, C.c_int]
+    lib.tp_xo_walk_end.argtypes = [C.POINTER(C.c_uint64), C.c_uint64]
+    lib.tp_xo_walk_end.restype = C.c_uint64
     lib.tp_walk_end.restype = C.c_uint64
     lib.tp_walk_from_mismatch.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_int]
     return lib

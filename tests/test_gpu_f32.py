@@ -255,7 +255,57 @@ def test_f32_faster_evgen_jump_across_chunks(engine, cfg, oracle32, valeurs):
         close_f32(a, o, 10000)
 
 
-def test_f32_faster_evgen_needs_jump_seeding(engine, cfg):
+@pytest.mark.parametrize("n_events", [1, 9999, 35017, 123457])
+def test_f32_faster_evgen_per_batch_matches_oracle(engine, cfg, oracle32, valeurs, n_events):
+    """faster-evgen in f32 with skip-ahead seeding (the `f32,faster-evgen`
+    build): one RANF stream walked with the unit-disc test in f32 (host walk
+    at these sizes), batch by batch against the f32 oracle."""
+    print(f"f32 faster-evgen N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, precision=tp.PREC_F32, faster_evgen=True)
+    per, merged, _ = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), n_events)
+    ocfg = oracle32.config(valeurs, n_events)
+    om, oper = oracle32.run(ocfg, oracle32.opts("ranf", threads=4, faster_evgen=True), per_batch=True)
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        close_f32(a, oper[b].as_array(), min(10000, n_events - 10000 * b))
+    close_f32(merged.as_array(), om.as_array(), n_events)
+
+
+def test_f32_faster_evgen_device_walk_across_chunks(engine, cfg, oracle32, valeurs):
+    """N = 6.2e7: the device walk with the unit-disc test in f32, two chunks;
+    every batch against the f32 oracle's run."""
+    n_events = 62_000_000
+    print(f"f32 faster-evgen device walk N={n_events}", flush=True)
+    params = tp.params_from_config(cfg.with_events(n_events), precision=tp.PREC_F32, faster_evgen=True)
+    per, merged, stats = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), n_events)
+    assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    print("oracle run", flush=True)
+    om, oper = oracle32.run(oracle32.config(valeurs, n_events), oracle32.opts("ranf", threads=16, faster_evgen=True),
+                            per_batch=True)
+    for b in range(len(per)):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        close_f32(a, oper[b].as_array(), 10000)
+
+
+def test_f32_faster_evgen_simulate_batch_advances_like_the_closure(engine, cfg, oracle32, valeurs):
+    params = tp.params_from_config(cfg, precision=tp.PREC_F32, faster_evgen=True)
+    st = tp.rng_state(tp.RNG_RANF)
+    g = oracle32.rng("ranf")
+    ocfg = oracle32.config(valeurs)
+    opts = oracle32.opts("ranf", faster_evgen=True)
+    for n in (10000, 7, 1, 23456):
+        acc = engine.simulate_batch(params, tp.RNG_RANF, st, n)
+        oacc = oracle32.simulate_batch(ocfg, opts, g, n)
+        close_f32(acc.as_array(), oacc.as_array(), n)
+        assert st.index == g.index and list(st.numbers)[1:] == list(g.numbers)[1:]
+
+
+def test_f32_faster_evgen_rejects_xoshiro128(engine, cfg):
+    """faster-evgen with the xoshiro128+ stream (standard-random,f32,faster-evgen)
+    is not built."""
     params = tp.params_from_config(cfg.with_events(100), precision=tp.PREC_F32, faster_evgen=True)
     with pytest.raises(tp.TpError, match="faster-evgen"):
-        engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), 100)
+        engine.run(params, tp.RNG_XOSHIRO128, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO128), 100)

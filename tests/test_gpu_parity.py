@@ -181,13 +181,14 @@ def test_faster_evgen_device_walk_across_chunks(engine, cfg, oracle, valeurs, wa
     c = cfg.with_events(n_events)
     params = tp.params_from_config(c, faster_evgen=True)
     st0 = tp.rng_state(tp.RNG_RANF)
-    per, _, _ = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, st0, n_events)
+    per, _, stats = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, st0, n_events)
+    assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
     ocfg = oracle.config(valeurs, n_events)
     opts = oracle.opts("ranf", faster_evgen=True)
     numbers = (C.c_int32 * 56)(*list(st0.numbers))
     q0 = 55 - st0.index
     for b in (0, 1, 6143, 6144, 6199):
-        q = walklib.tp_walk_end(numbers, st0.index, 10000 * b, 8)
+        q = walklib.tp_walk_end(numbers, st0.index, 10000 * b, 8, 0)
         g = oracle.rng("ranf")
         oracle.skip(g, q - q0)
         o = oracle.simulate_batch(ocfg, opts, g, 10000).as_array()
@@ -295,12 +296,67 @@ def test_faster_evgen_xoshiro_jump_across_chunks(engine, cfg, oracle, valeurs):
         acc_close(a, o, 10000)
 
 
+@pytest.mark.parametrize("n_events", [1, 9999, 35017, 123457])
+def test_faster_evgen_xoshiro_per_batch_matches_oracle(engine, cfg, oracle, valeurs, n_events):
+    """faster-evgen on the xoshiro256+ stream with skip-ahead seeding (the
+    `standard-random,faster-evgen` build): one stream, random_array::<N> as N
+    consecutive draws; the host walk's starts, batch by batch against the
+    oracle."""
+    print(f"faster-evgen xoshiro run N={n_events}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, faster_evgen=True)
+    per, merged, _ = engine.run(params, tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), n_events)
+    ocfg = oracle.config(valeurs, n_events)
+    om, oper = oracle.run(ocfg, oracle.opts("xoshiro", threads=4, faster_evgen=True), per_batch=True)
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]], np.float64)
+        acc_close(a, oper[b].as_array(), min(10000, n_events - 10000 * b))
+    acc_close(merged.as_array(), om.as_array(), n_events)
+
+
+def test_faster_evgen_xoshiro_device_walk_across_chunks(engine, cfg, oracle, valeurs):
+    """N = 6.2e7 (6200 batches): the event starts come from the device walk
+    (segment walkers of 1024 events on the xoshiro256+ stream, joined where
+    they meet), in two chunks, the second from the first one's end.  Every
+    batch against the oracle's run."""
+    n_events = 62_000_000
+    print(f"faster-evgen xoshiro device walk N={n_events}", flush=True)
+    params = tp.params_from_config(cfg.with_events(n_events), faster_evgen=True)
+    per, merged, stats = engine.run(params, tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), n_events)
+    assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    print("oracle run", flush=True)
+    om, oper = oracle.run(oracle.config(valeurs, n_events), oracle.opts("xoshiro", threads=16, faster_evgen=True),
+                          per_batch=True)
+    for b in range(len(per)):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]], np.float64)
+        acc_close(a, oper[b].as_array(), 10000)
+    acc_close(merged.as_array(), om.as_array(), n_events)
+
+
+def test_faster_evgen_xoshiro_simulate_batch_advances_like_the_closure(engine, cfg, oracle, valeurs):
+    params = tp.params_from_config(cfg, faster_evgen=True)
+    st = tp.rng_state(tp.RNG_XOSHIRO)
+    g = oracle.rng("xoshiro")
+    ocfg = oracle.config(valeurs)
+    opts = oracle.opts("xoshiro", faster_evgen=True)
+    for n in (10000, 7, 1, 23456):
+        acc = engine.simulate_batch(params, tp.RNG_XOSHIRO, st, n)
+        oacc = oracle.simulate_batch(ocfg, opts, g, n)
+        acc_close(acc.as_array(), oacc.as_array(), n)
+        assert list(st.s) == list(g.xs)
+
+
 def test_faster_evgen_unsupported_combinations_fail_cleanly(engine, cfg):
-    params = tp.params_from_config(cfg.with_events(100), faster_evgen=True)
-    with pytest.raises(tp.TpError, match="faster-evgen"):
-        engine.run(params, tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), 100)
-    with pytest.raises(tp.TpError, match="faster-evgen"):
-        engine.simulate_batch(params, tp.RNG_XOSHIRO, tp.rng_state(tp.RNG_XOSHIRO), 100)
+    """faster-evgen is not built in mixed precision (an addition of this
+    build, not a reference feature)."""
+    params = tp.params_from_config(cfg.with_events(100), precision=tp.PREC_MIXED, faster_evgen=True)
+    for kind in (tp.RNG_RANF, tp.RNG_XOSHIRO):
+        with pytest.raises(tp.TpError, match="faster-evgen"):
+            engine.run(params, kind, tp.SEED_SKIPAHEAD, tp.rng_state(kind), 100)
+        with pytest.raises(tp.TpError, match="faster-evgen"):
+            engine.simulate_batch(params, kind, tp.rng_state(kind), 100)
 
 
 GOLDEN = [("features_", "ranf", "skipahead", True, False),

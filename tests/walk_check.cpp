@@ -1,6 +1,8 @@
 // walk_check.cpp -- TEST INFRASTRUCTURE ONLY: the faster-evgen stream walk
 // (3photons-rust_amd/csrc/tp_rng_host.cpp) split over threads against the
-// same walk done sequentially: event starts, end position and final state.
+// same walk done sequentially: event starts, end position and final state;
+// and the walks' end positions, which tests/test_walk.py checks against the
+// oracle's closure.
 //
 //   g++ -O2 -std=c++17 -pthread -shared -fPIC -I<csrc> walk_check.cpp <csrc>/tp_rng_host.cpp
 #include "tp_rng_host.h"
@@ -10,7 +12,7 @@
 
 // 0 when the threaded walk of n events from seed / index equals the
 // sequential one.
-extern "C" int tp_walk_mismatch(uint64_t n, int seed, uint32_t index, int threads)
+extern "C" int tp_walk_mismatch(uint64_t n, int seed, uint32_t index, int threads, int f32)
 {
     tp::RanfState g0;
     tp::ranf_seeded(g0, seed);
@@ -18,8 +20,8 @@ extern "C" int tp_walk_mismatch(uint64_t n, int seed, uint32_t index, int thread
     tp::RanfState a = g0, b = g0;
     std::vector<uint64_t> sa(n), sb(n);
     uint64_t qa = 0, qb = 0;
-    tp::ranf_walk_faster_evgen(a, n, sa.data(), &qa, 1);
-    tp::ranf_walk_faster_evgen(b, n, sb.data(), &qb, threads);
+    tp::ranf_walk_faster_evgen(a, n, sa.data(), &qa, 1, f32 != 0);
+    tp::ranf_walk_faster_evgen(b, n, sb.data(), &qb, threads, f32 != 0);
     int bad = 0;
     bad += qa != qb;
     bad += sa != sb;
@@ -50,12 +52,21 @@ extern "C" int tp_walk_from_mismatch(uint64_t e0, uint64_t n, int seed, uint32_t
 // Stream position, counted from the start of the round of the state
 // (numbers[1..55], index), where event n of the faster-evgen walk from that
 // state starts: the position a batch beginning with event n starts from.
-extern "C" uint64_t tp_walk_end(const int32_t *numbers, uint32_t index, uint64_t n, int threads)
+extern "C" uint64_t tp_walk_end(const int32_t *numbers, uint32_t index, uint64_t n, int threads, int f32)
 {
     tp::RanfState g;
     std::memcpy(g.numbers, numbers, sizeof g.numbers);
     g.index = index;
     uint64_t q = 0;
-    tp::ranf_walk_faster_evgen(g, n, nullptr, &q, threads);
+    tp::ranf_walk_faster_evgen(g, n, nullptr, &q, threads, f32 != 0);
+    return q;
+}
+
+// Draws after s0 where event n of the faster-evgen walk of the xoshiro256+
+// stream from s0 starts.
+extern "C" uint64_t tp_xo_walk_end(const uint64_t *s0, uint64_t n)
+{
+    uint64_t q = 0;
+    tp::xoshiro_walk_faster_evgen(s0, 0, n, nullptr, &q);
     return q;
 }
