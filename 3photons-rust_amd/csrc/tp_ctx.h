@@ -72,6 +72,7 @@ struct Ctx {
     // joined pieces (begin, length, output offset), and host staging of them
     DevBuf fe_lists, fe_meta, fe_piece;
     DevBuf fe_flag;  // jump seeding: set by a batch walk that left its piece
+    DevBuf fe_seq;   // jump seeding: batches left to the one-thread walk
     DevBuf xo_states, xo_draws; // faster-evgen on xoshiro256+: piece states and uniforms
     DevBuf adj_u;               // tp_adjoint: the stream's points, expanded
     uint32_t fe_flag_host = 0;

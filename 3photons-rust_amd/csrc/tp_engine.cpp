@@ -697,6 +697,37 @@ int enqueue_xo_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_state *x
 constexpr uint64_t FE_JUMP_PIECE = 55 * 4000;
 static_assert(FE_JUMP_PIECE % 55 == 0, "pieces keep every position's place in its round");
 
+// Segment walkers for the batches of a chunk (jump seeding): S walkers per
+// batch of the stream kind's length and overlap, their lists and join
+// indices, and the per-batch flags of the one-thread walk.
+int fe_batch_walk_setup(Ctx *c, tpk::FeBatchWalkArgs &w, int kind, uint64_t max_batches, uint32_t batch_size)
+{
+    const bool xo = kind == 2;
+    const uint64_t per_walker = xo ? FE_XO_EVENTS_PER_WALKER : FE_EVENTS_PER_WALKER;
+    w.kind = kind;
+    w.S = (uint32_t)std::max<uint64_t>(1, (batch_size + per_walker - 1) / per_walker);
+    const uint64_t span =
+        (uint64_t)std::ceil((double)batch_size * (xo ? FE_XO_POSITIONS_PER_EVENT : FE_POSITIONS_PER_EVENT)) + 128;
+    w.seg = xo ? span / w.S + 1 : (span / w.S / 55 + 1) * 55; // > 55: past every piece's first position
+    w.overlap = xo ? FE_XO_OVERLAP : FE_OVERLAP;
+    // every event takes at least 15 positions: no walk holds more starts
+    w.cap = (uint32_t)((w.seg + w.overlap + 63) / 15 + 2);
+    const uint64_t walkers = max_batches * w.S;
+    int rc = ensure(c, c->fe_lists, walkers * w.cap * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->fe_meta, walkers * 3 * sizeof(uint32_t));
+    if (!rc)
+        rc = ensure(c, c->fe_seq, max_batches * sizeof(uint32_t));
+    if (rc)
+        return rc;
+    w.lists = static_cast<uint64_t *>(c->fe_lists.p);
+    w.counts = static_cast<uint32_t *>(c->fe_meta.p);
+    w.join_prev = reinterpret_cast<int32_t *>(w.counts + walkers);
+    w.join_next = w.join_prev + walkers;
+    w.seq = static_cast<uint32_t *>(c->fe_seq.p);
+    return TP_OK;
+}
+
 int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedArgs &a, const tp_ranf_state *rs,
                                    uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
 {
@@ -780,6 +811,9 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
             return rc;
         tpk::FeBatchWalkArgs w;
         std::memset(&w, 0, sizeof w);
+        rc = fe_batch_walk_setup(c, w, f32 ? 1 : 0, chunk, batch_size);
+        if (rc)
+            return rc;
         w.buf = static_cast<const int32_t *>(c->draws.p);
         w.piece_stride = PS;
         w.p0 = static_cast<const int32_t *>(c->p0.p);
@@ -789,7 +823,6 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
         w.batch_size = batch_size;
         w.evq = static_cast<uint64_t *>(c->evq.p);
         w.overflow = static_cast<uint32_t *>(c->fe_flag.p);
-        w.f32 = f32 ? 1 : 0;
         tpk::launch_fe_walk_batches(w, c->stream);
         HIPCHK(c, hipGetLastError());
         if (first) {
@@ -868,11 +901,18 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
     p.piece_len = FE_XO_PIECE;
     p.per_thread = FE_XO_PER_THREAD;
     p.xbuf = static_cast<double *>(c->xo_draws.p);
-    p.num_events = num_events;
-    p.batch_size = batch_size;
-    p.evq = static_cast<uint64_t *>(c->evq.p);
-    p.overflow = static_cast<uint32_t *>(c->fe_flag.p);
     static_assert(FE_XO_PIECE % FE_XO_PER_THREAD == 0, "whole threads per piece");
+    tpk::FeBatchWalkArgs w;
+    std::memset(&w, 0, sizeof w);
+    rc = fe_batch_walk_setup(c, w, 2, chunk, batch_size);
+    if (rc)
+        return rc;
+    w.xbuf = p.xbuf;
+    w.piece_stride = FE_XO_PIECE;
+    w.num_events = num_events;
+    w.batch_size = batch_size;
+    w.evq = static_cast<uint64_t *>(c->evq.p);
+    w.overflow = static_cast<uint32_t *>(c->fe_flag.p);
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
         const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
@@ -880,13 +920,17 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         p.nb = cnb;
         p.b_first = b0;
-        tpk::launch_xo_pieces(p, c->stream);
+        tpk::launch_xo_expand(p, c->stream);
+        HIPCHK(c, hipGetLastError());
+        w.nb = cnb;
+        w.b_first = b0;
+        tpk::launch_fe_walk_batches(w, c->stream);
         HIPCHK(c, hipGetLastError());
         a.nb = cnb;
         a.b_first = b0;
         a.xdraws = p.xbuf;
         a.qbase = 0;
-        a.evq = p.evq;
+        a.evq = w.evq;
         a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
         if (first) {
             HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -1195,7 +1239,7 @@ extern "C" int tp_destroy(void *ctx)
         (void)hipStreamSynchronize(c->stream);
         for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
                           &c->mt128, &c->bt128, &c->st128, &c->f32ev, &c->fe_lists, &c->fe_meta, &c->fe_piece,
-                          &c->fe_flag, &c->xo_states, &c->xo_draws, &c->adj_u})
+                          &c->fe_flag, &c->fe_seq, &c->xo_states, &c->xo_draws, &c->adj_u})
             if (b->p)
                 (void)hipFree(b->p);
         for (auto &kv : c->tables) {

@@ -145,25 +145,41 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
 // (G % 55 = q % 55).  evq[k * batch_size + e] receives event e's start as G.
 // A walk that would leave its piece sets *overflow and fills the rest of its
 // batch with the piece's first position (every later read stays in bounds).
+// The xoshiro256+ pieces (kind 2) hold uniforms from position 0 of each
+// batch's own stream (p0 null).
+//
+// Each batch is walked by S segment walkers, as the one-stream walk is
+// (walker s of batch k starts at k * piece_stride + s * seg, walker 0 at the
+// batch's first position, and walks overlap positions past its segment),
+// joined where they meet; a batch whose walkers do not all meet before its
+// events are covered is walked by one thread from its first position.
 struct FeBatchWalkArgs {
-    const int32_t *buf;
+    const int32_t *buf;    // RANF pieces (kinds 0, 1)
+    const double *xbuf;    // xoshiro256+ pieces (kind 2)
+    int32_t kind;          // 0: RANF, unit-disc test in f64; 1: in f32 (all-f32 mode); 2: xoshiro256+
     uint64_t piece_stride;
-    const int32_t *p0;     // [nb] frame position of each piece's first value
+    const int32_t *p0;     // [nb] frame position of each piece's first value (RANF)
     uint64_t nb, b_first, num_events;
     uint32_t batch_size;
     uint64_t *evq;
     uint32_t *overflow;
-    int32_t f32;           // 1: the unit-disc re-roll test in f32 (all-f32 mode)
+    uint32_t S;            // walkers per batch
+    uint64_t seg, overlap; // segment length (a multiple of 55 for RANF) and overlap
+    uint32_t cap;          // list capacity per walker
+    uint64_t *lists;       // [nb * S][cap]
+    uint32_t *counts;      // [nb * S]
+    int32_t *join_prev, *join_next; // [nb * S] as FeWalkArgs, within each batch
+    uint32_t *seq;         // [nb] 1: walk the batch with one thread
 };
 void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st);
 
-// The same for the xoshiro256+ stream with jump seeding: piece k is batch
-// b_first + k's stream, started from J^(b_first + k) s0 (J the 2^128 jump),
-// its first piece_len uniforms at xbuf[k * piece_len ..]; positions are
-// G = k * piece_len + draw index.
+// Pieces of the xoshiro256+ stream: piece k starts from B^(b_first + k) s0
+// (B the 2^128 jump with jump seeding, each piece a batch's stream; or
+// M^piece_len, the pieces tiling one stream) and holds its first piece_len
+// uniforms at xbuf[k * piece_len ..].
 struct XoPieceArgs {
     uint64_t s0[4];
-    const uint64_t *bt;    // J^(2^k), k < kb
+    const uint64_t *bt;    // B^(2^k), k < kb
     int32_t kb;
     const uint64_t *mt;    // M^(2^k), k < km
     int32_t km;
@@ -172,14 +188,8 @@ struct XoPieceArgs {
     uint64_t piece_len;    // uniforms expanded per piece
     uint32_t per_thread;   // uniforms per expanding thread (divides piece_len)
     double *xbuf;          // [nb][piece_len]
-    // walk
-    uint64_t num_events;
-    uint32_t batch_size;
-    uint64_t *evq;
-    uint32_t *overflow;
 };
-void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st); // piece states, expansion, walk
-void launch_xo_expand(const XoPieceArgs &a, hipStream_t st); // piece states and expansion only
+void launch_xo_expand(const XoPieceArgs &a, hipStream_t st); // piece states and expansion
 
 } // namespace tpk
 

@@ -712,29 +712,156 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
     hipLaunchKernelGGL(fe_scatter_kernel, dim3(a.nwalkers), dim3(256), 0, st, a, begin, len, off, out);
 }
 
-// One thread per batch: the batch's walk from its piece's first value.  Each
-// event consumes at least 15 positions and the walk stops at the piece's end,
-// so the loop ends.
-__global__ __launch_bounds__(64) void fe_walk_batches_kernel(FeBatchWalkArgs a)
+// Events of batch b_first + k.
+__device__ __forceinline__ uint32_t fe_batch_events(const FeBatchWalkArgs &a, uint64_t k)
+{
+    const uint64_t left = a.num_events - (a.b_first + k) * a.batch_size;
+    return left < a.batch_size ? (uint32_t)left : a.batch_size;
+}
+
+// The batch's stored stream as a walk's view of it.
+__device__ __forceinline__ FeWalkArgs fe_batch_view(const FeBatchWalkArgs &a, uint64_t k)
+{
+    FeWalkArgs w;
+    w.buf = a.buf;
+    w.xbuf = a.xbuf;
+    w.kind = a.kind;
+    w.qbase = 0;
+    w.qlimit = (k + 1) * a.piece_stride;
+    return w;
+}
+
+// Walker s of batch k (thread k * S + s).  Every step records a start and
+// then consumes at least 15 positions, and the loop stops at the segment end
+// plus the overlap, the piece's end or the list capacity, so it always ends.
+__global__ __launch_bounds__(256) void fe_seg_walk_kernel(FeBatchWalkArgs a)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nb * a.S)
+        return;
+    const uint64_t k = t / a.S;
+    const uint32_t s = (uint32_t)(t % a.S);
+    const uint64_t base = k * a.piece_stride;
+    uint64_t q = s == 0 ? base + (a.p0 ? (uint64_t)a.p0[k] : 0) : base + (uint64_t)s * a.seg;
+    const uint64_t stop = min(base + (uint64_t)(s + 1) * a.seg + a.overlap, base + a.piece_stride);
+    const FeWalkArgs w = fe_batch_view(a, k);
+    uint64_t *list = a.lists + t * a.cap;
+    uint32_t n = 0;
+    while (q < stop && n < a.cap) {
+        list[n++] = q;
+        if (!fe_walk_any(w, q))
+            break;
+    }
+    a.counts[t] = n;
+}
+
+// Walker s's first start that walker s - 1 of the same batch also reached
+// past segment s's start, as fe_join_kernel.
+__global__ __launch_bounds__(256) void fe_seg_join_kernel(FeBatchWalkArgs a)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nb * a.S)
+        return;
+    const uint64_t k = t / a.S;
+    const uint32_t s = (uint32_t)(t % a.S);
+    if (s == 0) {
+        a.join_prev[t] = a.join_next[t] = 0;
+        return;
+    }
+    const uint64_t *pa = a.lists + (t - 1) * a.cap, *pb = a.lists + t * a.cap;
+    const uint32_t na = a.counts[t - 1], nb = a.counts[t];
+    const uint64_t qs = k * a.piece_stride + (uint64_t)s * a.seg;
+    uint32_t lo = 0, hi = na; // first index of list s - 1 with a start >= qs
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pa[mid] < qs)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    uint32_t i = lo, j = 0;
+    int32_t ji = -1, jj = -1;
+    while (i < na && j < nb) {
+        if (pa[i] == pb[j]) {
+            ji = (int32_t)i;
+            jj = (int32_t)j;
+            break;
+        }
+        if (pa[i] < pb[j])
+            ++i;
+        else
+            ++j;
+    }
+    a.join_prev[t] = ji;
+    a.join_next[t] = jj;
+}
+
+// One thread per batch: the true walk is walker 0's list from its first
+// start, then each walker's from where its predecessor met it.  Its first n
+// starts go to the batch's slots of evq; seq[k] = 1 (nothing written) when
+// the true walk leaves a list before meeting the next one, or the lists end,
+// before n starts.
+__global__ __launch_bounds__(64) void fe_seg_assemble_kernel(FeBatchWalkArgs a)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nb)
         return;
-    const uint64_t b = a.b_first + k;
-    const uint64_t left = a.num_events - b * a.batch_size;
-    const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
+    const uint32_t n = fe_batch_events(a, k);
+    // check first, so that a batch left to the sequential walk gets no partial starts
+    uint32_t begin = 0;
+    uint64_t have = 0;
+    bool ok = false;
+    for (uint32_t s = 0; s < a.S; ++s) {
+        const uint64_t w = k * a.S + s;
+        const int32_t jp = s + 1 < a.S ? a.join_prev[w + 1] : -1;
+        const bool meets = jp >= 0 && (uint32_t)jp >= begin;
+        const uint32_t end = meets ? (uint32_t)jp : a.counts[w];
+        have += end > begin ? end - begin : 0;
+        if (have >= n) {
+            ok = true;
+            break;
+        }
+        if (!meets)
+            break;
+        begin = (uint32_t)a.join_next[w + 1];
+    }
+    a.seq[k] = ok ? 0u : 1u;
+    if (!ok)
+        return;
+    uint64_t *out = a.evq + k * a.batch_size;
+    uint64_t off = 0;
+    begin = 0;
+    for (uint32_t s = 0; s < a.S && off < n; ++s) {
+        const uint64_t w = k * a.S + s;
+        const int32_t jp = s + 1 < a.S ? a.join_prev[w + 1] : -1;
+        const bool meets = jp >= 0 && (uint32_t)jp >= begin;
+        const uint32_t end = meets ? (uint32_t)jp : a.counts[w];
+        const uint64_t *list = a.lists + w * a.cap;
+        for (uint32_t i = begin; i < end && off < n; ++i)
+            out[off++] = list[i];
+        if (meets)
+            begin = (uint32_t)a.join_next[w + 1];
+    }
+}
+
+// One thread per batch that the joined walk did not cover: its walk from its
+// piece's first position.  Each event consumes at least 15 positions and the
+// walk stops at the piece's end, so the loop ends.
+__global__ __launch_bounds__(64) void fe_seq_walk_kernel(FeBatchWalkArgs a)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nb || !a.seq[k])
+        return;
+    const uint32_t n = fe_batch_events(a, k);
     const uint64_t base = k * a.piece_stride;
-    FeWalkArgs w;
-    w.buf = a.buf;
-    w.qbase = 0;
-    w.qlimit = base + a.piece_stride;
-    uint64_t q = base + (uint64_t)a.p0[k];
+    const FeWalkArgs w = fe_batch_view(a, k);
+    uint64_t q = base + (a.p0 ? (uint64_t)a.p0[k] : 0);
     const uint64_t q0 = q;
     uint64_t *out = a.evq + k * a.batch_size;
     uint32_t e = 0;
     for (; e < n; ++e) {
         out[e] = q;
-        if (!(a.f32 ? fe_walk_event<float>(w, q) : fe_walk_event<double>(w, q)))
+        if (!fe_walk_any(w, q))
             break;
     }
     if (e < n) {
@@ -746,7 +873,12 @@ __global__ __launch_bounds__(64) void fe_walk_batches_kernel(FeBatchWalkArgs a)
 
 void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(fe_walk_batches_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
+    const uint64_t walkers = a.nb * a.S;
+    const unsigned wb = (unsigned)((walkers + 255) / 256), bb = (unsigned)((a.nb + 63) / 64);
+    hipLaunchKernelGGL(fe_seg_walk_kernel, dim3(wb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fe_seg_join_kernel, dim3(wb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fe_seg_assemble_kernel, dim3(bb), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(fe_seq_walk_kernel, dim3(bb), dim3(64), 0, st, a);
 }
 
 // ---- faster-evgen on the xoshiro256+ stream, jump seeding --------------------
@@ -789,31 +921,6 @@ __global__ __launch_bounds__(64) void xo_expand_kernel(XoPieceArgs a)
         out[i] = xo_uniform(s);
 }
 
-// One thread per batch, as fe_walk_batches_kernel.
-__global__ __launch_bounds__(64) void xo_walk_batches_kernel(XoPieceArgs a)
-{
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nb)
-        return;
-    const uint64_t b = a.b_first + k;
-    const uint64_t left = a.num_events - b * a.batch_size;
-    const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
-    const uint64_t base = k * a.piece_len;
-    uint64_t q = base;
-    uint64_t *out = a.evq + k * a.batch_size;
-    uint32_t e = 0;
-    for (; e < n; ++e) {
-        out[e] = q;
-        if (!xo_walk_event(a.xbuf, 0, base + a.piece_len, q))
-            break;
-    }
-    if (e < n) {
-        *a.overflow = 1u;
-        for (; e < n; ++e)
-            out[e] = base;
-    }
-}
-
 void launch_xo_expand(const XoPieceArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(xo_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
@@ -821,13 +928,6 @@ void launch_xo_expand(const XoPieceArgs &a, hipStream_t st)
     hipLaunchKernelGGL(xo_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
 }
 
-void launch_xo_pieces(const XoPieceArgs &a, hipStream_t st)
-{
-    hipLaunchKernelGGL(xo_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
-    const uint64_t threads = a.nb * (a.piece_len / a.per_thread);
-    hipLaunchKernelGGL(xo_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(xo_walk_batches_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
-}
 
 // seeded_new (ranf.rs:36-66) for jump mode, one thread per batch.
 __global__ void ranf_seed_kernel(int32_t seed0, uint64_t b_first, uint64_t nb, int32_t *origin, int32_t *p0)
