@@ -74,6 +74,9 @@ struct Ctx {
     DevBuf fe_flag;  // jump seeding: set by a batch walk that left its piece
     DevBuf fe_seq;   // jump seeding: batches left to the one-thread walk
     DevBuf xo_states, xo_draws; // faster-evgen on xoshiro256+: piece states and uniforms
+    DevBuf xo128_states, xo128_draws, xo128_pt; // the same on xoshiro128+ (f32), and the piece matrix powers
+    std::string xo128_pt_key;
+    int xo128_pt_kb = 0;
     DevBuf adj_u;               // tp_adjoint: the stream's points, expanded
     uint32_t fe_flag_host = 0;
     std::vector<uint32_t> fe_counts, fe_beg, fe_len;

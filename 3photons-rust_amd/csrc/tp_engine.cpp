@@ -109,7 +109,7 @@ int bits_for(uint64_t n);
 constexpr int KM128 = 24;
 static_assert(12ull * 64 * TP_BATCH_SIZE < (1ull << KM128), "xoshiro128+ offset table too short");
 
-int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int grid, DevBuf **bt_out, int *kb_out)
+int ensure_mt128(Ctx *c)
 {
     if (!c->mt128.p) {
         const std::vector<tp::Gf2Mat128> &t = tp::xoshiro128_step_pow2();
@@ -118,6 +118,38 @@ int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int 
             return rc;
         HIPCHK(c, hipMemcpy(c->mt128.p, t.data(), KM128 * sizeof(tp::Gf2Mat128), hipMemcpyHostToDevice));
     }
+    return TP_OK;
+}
+
+// B^(2^k), k < kb, of the xoshiro128+ piece matrix B (the 2^64 jump, or
+// M^piece_len for pieces tiling one stream), into c->xo128_pt.
+int ensure_xo128_piece_table(Ctx *c, bool jump, uint64_t piece_len, uint64_t max_piece)
+{
+    const int kb = std::max(1, bits_for(max_piece + 1));
+    const std::string key = jump ? std::string("jump") : "piece:" + std::to_string(piece_len);
+    if (c->xo128_pt_key == key && c->xo128_pt_kb >= kb)
+        return TP_OK;
+    tp::Gf2Mat128 B;
+    if (jump)
+        tp::gf2_128_jump64(B);
+    else
+        tp::gf2_128_pow_from_table(tp::xoshiro128_step_pow2(), piece_len, B);
+    const std::vector<tp::Gf2Mat128> bt = tp::gf2_128_pow2_table(B, kb);
+    int rc = ensure(c, c->xo128_pt, bt.size() * sizeof(tp::Gf2Mat128));
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->xo128_pt.p, bt.data(), bt.size() * sizeof(tp::Gf2Mat128), hipMemcpyHostToDevice));
+    c->xo128_pt_key = key;
+    c->xo128_pt_kb = kb;
+    return TP_OK;
+}
+
+int ensure_tables128(Ctx *c, int seeding, uint32_t batch_size, uint64_t nb, int grid, DevBuf **bt_out, int *kb_out)
+{
+    int rc0 = ensure_mt128(c);
+    if (rc0)
+        return rc0;
     const int kb = std::max(1, bits_for(nb));
     const std::string key = std::to_string(seeding) + ":" + std::to_string(batch_size);
     if (c->bt128_key != key || c->bt128_kb < kb) {
@@ -309,9 +341,8 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
         return fail(c, TP_ERR_UNSUPPORTED, "precision " + std::to_string(params->precision) + " not available");
     if ((params->precision == TP_PREC_F32) != (rng_kind == TP_RNG_XOSHIRO128) && rng_kind != TP_RNG_RANF)
         return fail(c, TP_ERR_ARG, "xoshiro128+ is the stream of the all-f32 mode, xoshiro256+ of the others");
-    if (params->faster_evgen && (rng_kind == TP_RNG_XOSHIRO128 || params->precision == TP_PREC_MIXED))
-        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available in f64 (RANF or xoshiro256+) and in f32 "
-                                           "(RANF), not in mixed precision or with xoshiro128+");
+    if (params->faster_evgen && params->precision == TP_PREC_MIXED)
+        return fail(c, TP_ERR_UNSUPPORTED, "faster-evgen is available in f64 and f32, not in mixed precision");
     if (!(params->e_total > 0.) || !std::isfinite(params->e_total))
         return fail(c, TP_ERR_ARG, "e_total must be positive and finite");
     return TP_OK;
@@ -379,7 +410,39 @@ struct FeStream {
     uint64_t p0_master = 0;       // RANF: frame position of the master origin's next value
     int K = 0;                    // RANF: segment-jump depth of c->polys
     const uint64_t *s0 = nullptr; // xoshiro256+: the run's state (position 0)
+    const uint32_t *s0_128 = nullptr; // xoshiro128+ (kind 3): the run's state (position 0)
 };
+
+// Expands xoshiro128+ pieces b_first .. b_first + np - 1 of piece_len f32
+// uniforms into c->xo128_draws (piece k from B^(b_first + k) s0, B as
+// ensure_xo128_piece_table made it).
+int xo128_expand(Ctx *c, const uint32_t s0[4], uint64_t b_first, uint64_t np)
+{
+    int rc = ensure_mt128(c);
+    if (!rc)
+        rc = ensure(c, c->xo128_states, np * 4 * sizeof(uint32_t));
+    if (!rc)
+        rc = ensure(c, c->xo128_draws, np * FE_XO_PIECE * sizeof(float));
+    if (rc)
+        return rc;
+    tpk::Xo128PieceArgs p;
+    std::memset(&p, 0, sizeof p);
+    std::memcpy(p.s0, s0, sizeof p.s0);
+    p.bt = static_cast<const uint32_t *>(c->xo128_pt.p);
+    p.kb = c->xo128_pt_kb;
+    p.mt = static_cast<const uint32_t *>(c->mt128.p);
+    p.km = KM128;
+    p.nb = np;
+    p.b_first = b_first;
+    p.states = static_cast<uint32_t *>(c->xo128_states.p);
+    p.piece_len = FE_XO_PIECE;
+    p.per_thread = FE_XO_PER_THREAD;
+    p.xbuf = static_cast<float *>(c->xo128_draws.p);
+    static_assert(FE_XO_PIECE < (1ull << KM128), "in-piece offsets within the step table");
+    tpk::launch_xo128_expand(p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return TP_OK;
+}
 
 // (M^FE_XO_PIECE)^(2^k), k < kb, on the device: the states of the pieces.
 int ensure_xo_piece_table(Ctx *c, uint64_t max_piece, Tables **out)
@@ -409,12 +472,21 @@ int ensure_xo_piece_table(Ctx *c, uint64_t max_piece, Tables **out)
 // index 0.
 int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_last, uint64_t *qbase)
 {
-    if (fs.kind != 2) {
+    if (fs.kind < 2) {
         *qbase = q_first;
         return fe_expand(c, fs.p0_master, fs.K, q_first, q_last);
     }
     const uint64_t p_first = q_first / FE_XO_PIECE, p_end = (q_last + FE_XO_PIECE - 1) / FE_XO_PIECE;
     const uint64_t np = p_end - p_first;
+    if (fs.kind == 3) {
+        int rc = ensure_xo128_piece_table(c, false, FE_XO_PIECE, p_end);
+        if (!rc)
+            rc = xo128_expand(c, fs.s0_128, p_first, np);
+        if (rc)
+            return rc;
+        *qbase = p_first * FE_XO_PIECE;
+        return TP_OK;
+    }
     Tables *t = nullptr;
     int rc = ensure_xo_piece_table(c, p_end, &t);
     if (!rc)
@@ -452,7 +524,7 @@ int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_la
 int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, bool *ok, uint64_t *qbase)
 {
     *ok = false;
-    const bool xo = fs.kind == 2;
+    const bool xo = fs.kind >= 2; // the xoshiro streams: no rounds
     tpk::FeWalkArgs w;
     std::memset(&w, 0, sizeof w);
     w.kind = fs.kind;
@@ -487,6 +559,7 @@ int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, boo
         return rc;
     w.buf = static_cast<const int32_t *>(c->draws.p);
     w.xbuf = static_cast<const double *>(c->xo_draws.p);
+    w.x32buf = static_cast<const float *>(c->xo128_draws.p);
     w.lists = static_cast<uint64_t *>(c->fe_lists.p);
     w.counts = static_cast<uint32_t *>(c->fe_meta.p);
     w.join_prev = reinterpret_cast<int32_t *>(w.counts + T);
@@ -598,6 +671,7 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
             fa->nb = b1 - b0;
             fa->b_first = b0;
             fa->draws = static_cast<const int32_t *>(c->draws.p);
+            fa->x32draws = static_cast<const float *>(c->xo128_draws.p);
             fa->qbase = qbase;
             fa->evq = static_cast<const uint64_t *>(c->evq.p);
             fa->out = static_cast<tp_acc *>(per_batch_dev) + b0;
@@ -702,7 +776,7 @@ static_assert(FE_JUMP_PIECE % 55 == 0, "pieces keep every position's place in it
 // indices, and the per-batch flags of the one-thread walk.
 int fe_batch_walk_setup(Ctx *c, tpk::FeBatchWalkArgs &w, int kind, uint64_t max_batches, uint32_t batch_size)
 {
-    const bool xo = kind == 2;
+    const bool xo = kind >= 2; // the xoshiro streams: no rounds
     const uint64_t per_walker = xo ? FE_XO_EVENTS_PER_WALKER : FE_EVENTS_PER_WALKER;
     w.kind = kind;
     w.S = (uint32_t)std::max<uint64_t>(1, (batch_size + per_walker - 1) / per_walker);
@@ -947,6 +1021,101 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
     return TP_OK;
 }
 
+// xoshiro128+ with `faster-evgen` (standard-random,f32,faster-evgen), the
+// all-f32 mode: skip-ahead seeding walks one stream as the xoshiro256+ path
+// does (f32 uniforms, the unit-disc test in f32); jump seeding (the 2^64
+// jump per batch, standard.rs:51-55) gives each batch a piece of its own.
+// Events go through f32_events_kernel<3> and the in-order f32 sums.
+int enqueue_xo128_faster_evgen(Ctx *c, const tp_params *params, int seeding, const tp_xoshiro128_state *x,
+                               uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors below are reused
+    uint32_t s0[4];
+    std::memcpy(s0, x->s, sizeof s0);
+    tpk::FusedArgs unused;
+    std::memset(&unused, 0, sizeof unused);
+    tpk::F32Args fa;
+    std::memset(&fa, 0, sizeof fa);
+    fa.P = make_fparams(params);
+    fa.num_events = num_events;
+    fa.batch_size = batch_size;
+    fa.source = 3;
+    if (seeding != TP_SEED_JUMP) {
+        FeStream fs;
+        fs.kind = 3;
+        fs.s0_128 = s0;
+        const FeHostWalk walk = [&](uint64_t q, uint64_t n, uint64_t *starts, uint64_t *q_next) {
+            tp::xoshiro128_walk_faster_evgen(s0, q, n, starts, q_next);
+        };
+        uint64_t q_end = 0;
+        int rc = enqueue_fe_chunks(c, fs, walk, unused, &fa, 0, num_events, batch_size, per_batch_dev, &q_end);
+        if (rc)
+            return rc;
+        c->fe_xo_end = q_end;
+        return TP_OK;
+    }
+    if (batch_size > TP_BATCH_SIZE)
+        return fail(c, TP_ERR_ARG, "faster-evgen with jump seeding runs batches of at most 10 000 events");
+    const uint64_t nb = (num_events + batch_size - 1) / batch_size;
+    const uint64_t chunk = std::min<uint64_t>(nb, FE_WALK_BATCHES);
+    int rc = ensure_xo128_piece_table(c, true, FE_XO_PIECE, nb);
+    if (!rc)
+        rc = ensure(c, c->evq, chunk * batch_size * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->fe_flag, sizeof(uint32_t));
+    if (!rc)
+        rc = ensure(c, c->f32ev, chunk * batch_size * 6 * sizeof(float));
+    if (rc)
+        return rc;
+    fa.ev = static_cast<float *>(c->f32ev.p);
+    c->fe_flag_host = 0;
+    HIPCHK(c, hipMemcpyAsync(c->fe_flag.p, &c->fe_flag_host, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    tpk::FeBatchWalkArgs w;
+    std::memset(&w, 0, sizeof w);
+    rc = fe_batch_walk_setup(c, w, 3, chunk, batch_size);
+    if (rc)
+        return rc;
+    w.piece_stride = FE_XO_PIECE;
+    w.num_events = num_events;
+    w.batch_size = batch_size;
+    w.evq = static_cast<uint64_t *>(c->evq.p);
+    w.overflow = static_cast<uint32_t *>(c->fe_flag.p);
+    bool first = true;
+    for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
+        if (first)
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        rc = xo128_expand(c, s0, b0, cnb);
+        if (rc)
+            return rc;
+        w.x32buf = static_cast<const float *>(c->xo128_draws.p);
+        w.nb = cnb;
+        w.b_first = b0;
+        tpk::launch_fe_walk_batches(w, c->stream);
+        HIPCHK(c, hipGetLastError());
+        if (first) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            first = false;
+        }
+        fa.nb = cnb;
+        fa.b_first = b0;
+        fa.x32draws = w.x32buf;
+        fa.qbase = 0;
+        fa.evq = w.evq;
+        fa.out = static_cast<tp_acc *>(per_batch_dev) + b0;
+        tpk::launch_f32_events(fa, f32_grid(c, cnb), c->stream);
+        HIPCHK(c, hipGetLastError());
+        tpk::launch_f32_accumulate(fa, f32_grid(c, cnb), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->fe_flag_host, c->fe_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->fe_flag_host)
+        return fail(c, TP_ERR_INTERNAL, "faster-evgen: a batch consumed more of its stream than was expanded");
+    return TP_OK;
+}
+
 // Enqueue a whole run (or one closure-sized batch when batch_size == n).
 int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, const void *state0,
                 uint64_t num_events, uint32_t batch_size, void *per_batch_dev)
@@ -963,6 +1132,9 @@ int enqueue_run(Ctx *c, const tp_params *params, int rng_kind, int seeding, cons
         return rc;
     a.mt = static_cast<const uint64_t *>(c->mt.p);
     a.km = c->km;
+    if (rng_kind == TP_RNG_XOSHIRO128 && params->faster_evgen)
+        return enqueue_xo128_faster_evgen(c, params, seeding, static_cast<const tp_xoshiro128_state *>(state0),
+                                          num_events, batch_size, per_batch_dev);
     if (rng_kind == TP_RNG_XOSHIRO128)
         return enqueue_xo128_f32(c, params, seeding, static_cast<const tp_xoshiro128_state *>(state0), num_events,
                                  batch_size, per_batch_dev);
@@ -1239,7 +1411,8 @@ extern "C" int tp_destroy(void *ctx)
         (void)hipStreamSynchronize(c->stream);
         for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
                           &c->mt128, &c->bt128, &c->st128, &c->f32ev, &c->fe_lists, &c->fe_meta, &c->fe_piece,
-                          &c->fe_flag, &c->fe_seq, &c->xo_states, &c->xo_draws, &c->adj_u})
+                          &c->fe_flag, &c->fe_seq, &c->xo_states, &c->xo_draws, &c->xo128_states,
+                          &c->xo128_draws, &c->xo128_pt, &c->adj_u})
             if (b->p)
                 (void)hipFree(b->p);
         for (auto &kv : c->tables) {
@@ -1577,6 +1750,8 @@ extern "C" int tp_simulate_batch(void *ctx, const tp_params *params, int rng_kin
         // walk of the faster-evgen draws.
         if (params->faster_evgen && rng_kind == TP_RNG_XOSHIRO) {
             tp::xoshiro_advance(static_cast<tp_xoshiro_state *>(rng_state)->s, c->fe_xo_end);
+        } else if (params->faster_evgen && rng_kind == TP_RNG_XOSHIRO128) {
+            tp::xoshiro128_advance(static_cast<tp_xoshiro128_state *>(rng_state)->s, c->fe_xo_end);
         } else if (params->faster_evgen) {
             tp_ranf_state *s = static_cast<tp_ranf_state *>(rng_state);
             std::memcpy(s->numbers, c->fe_end.numbers, sizeof s->numbers);

@@ -91,7 +91,8 @@ struct F32Args {
     tpd::FParams P;
     uint64_t nb, b_first, num_events;
     uint32_t batch_size;
-    int32_t source;        // 0 xoshiro128+ in registers, 1 RANF values from draws
+    int32_t source;        // 0 xoshiro128+ in registers, 1 RANF values from draws, 2 faster-evgen on
+                           // RANF draws, 3 faster-evgen on xoshiro128+ uniforms (x32draws)
     uint32_t s0[4];        // xoshiro128+ state of the run
     const uint32_t *mt;    // M^(2^k) of xoshiro128+, k < km (128 columns x 4 words)
     const uint32_t *bt;    // B^(2^k), k < kb: B = M^(12 batch_size) or the 2^64 jump
@@ -103,6 +104,7 @@ struct F32Args {
     // in draws from position qbase, as FusedArgs
     const uint64_t *evq;
     uint64_t qbase;
+    const float *x32draws; // faster-evgen (source 3): xoshiro128+ uniforms from position qbase
     float *ev;             // per event m[5] and keep (0/1), at batch_size*(b - b_first) + e
     void *out;             // tp_acc per batch, index b - b_first (f32 values)
 };
@@ -119,7 +121,8 @@ void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st);
 struct FeWalkArgs {
     const int32_t *buf;    // RANF values of positions qbase .. qlimit - 1 (kinds 0, 1)
     const double *xbuf;    // xoshiro256+ uniforms of the same positions (kind 2)
-    int32_t kind;          // 0: RANF, unit-disc test in f64; 1: RANF, in f32; 2: xoshiro256+
+    const float *x32buf;   // xoshiro128+ uniforms of the same positions (kind 3)
+    int32_t kind;          // 0: RANF, unit-disc test in f64; 1: RANF, in f32; 2: xoshiro256+; 3: xoshiro128+ (f32)
     uint64_t qbase, qlimit;
     uint64_t q_start;
     uint64_t seg_origin, seg, overlap;
@@ -156,7 +159,9 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
 struct FeBatchWalkArgs {
     const int32_t *buf;    // RANF pieces (kinds 0, 1)
     const double *xbuf;    // xoshiro256+ pieces (kind 2)
-    int32_t kind;          // 0: RANF, unit-disc test in f64; 1: in f32 (all-f32 mode); 2: xoshiro256+
+    const float *x32buf;   // xoshiro128+ pieces (kind 3)
+    int32_t kind;          // 0: RANF, unit-disc test in f64; 1: in f32 (all-f32 mode); 2: xoshiro256+;
+                           // 3: xoshiro128+ (all-f32 mode)
     uint64_t piece_stride;
     const int32_t *p0;     // [nb] frame position of each piece's first value (RANF)
     uint64_t nb, b_first, num_events;
@@ -190,6 +195,22 @@ struct XoPieceArgs {
     double *xbuf;          // [nb][piece_len]
 };
 void launch_xo_expand(const XoPieceArgs &a, hipStream_t st); // piece states and expansion
+
+// The same for the xoshiro128+ stream of the all-f32 mode (B the 2^64 jump
+// or M^piece_len; f32 uniforms).
+struct Xo128PieceArgs {
+    uint32_t s0[4];
+    const uint32_t *bt;    // B^(2^k), k < kb (128 columns x 4 words each)
+    int32_t kb;
+    const uint32_t *mt;    // M^(2^k), k < km
+    int32_t km;
+    uint64_t nb, b_first;
+    uint32_t *states;      // [nb][4] scratch: each piece's first state
+    uint64_t piece_len;
+    uint32_t per_thread;
+    float *xbuf;           // [nb][piece_len]
+};
+void launch_xo128_expand(const Xo128PieceArgs &a, hipStream_t st);
 
 } // namespace tpk
 

@@ -620,12 +620,38 @@ __device__ __forceinline__ bool xo_walk_event(const double *buf, uint64_t qbase,
     return true;
 }
 
+// The same on the xoshiro128+ stream of the all-f32 mode: the unit-disc test
+// in f32, whose MIN_POSITIVE squared underflows to 0 (only r2 > 1 re-rolls).
+__device__ __forceinline__ bool xo128_walk_event(const float *buf, uint64_t qbase, uint64_t qlimit, uint64_t &q)
+{
+    q += 9;
+    if (q + 6 > qlimit)
+        return false;
+    const float *r6 = buf + (q - qbase);
+    q += 6;
+    for (int i = 0; i < 3; ++i) {
+        const float x = 2.f * r6[i] - 1.f, y = 2.f * r6[3 + i] - 1.f;
+        float r2 = (0.f + x * x) + y * y;
+        while (r2 > 1.f) {
+            if (q + 2 > qlimit)
+                return false;
+            const float *v = buf + (q - qbase);
+            const float xa = 2.f * v[0] - 1.f, ya = 2.f * v[1] - 1.f;
+            q += 2;
+            r2 = 0.f + (xa * xa + ya * ya);
+        }
+    }
+    return true;
+}
+
 // One event of the walk of kind a.kind: RANF with the unit-disc test in f64
-// (0) or f32 (1), or the xoshiro256+ stream (2).
+// (0) or f32 (1), the xoshiro256+ stream (2) or the xoshiro128+ one (3).
 __device__ __forceinline__ bool fe_walk_any(const FeWalkArgs &a, uint64_t &q)
 {
     if (a.kind == 2)
         return xo_walk_event(a.xbuf, a.qbase, a.qlimit, q);
+    if (a.kind == 3)
+        return xo128_walk_event(a.x32buf, a.qbase, a.qlimit, q);
     return a.kind == 1 ? fe_walk_event<float>(a, q) : fe_walk_event<double>(a, q);
 }
 
@@ -725,6 +751,7 @@ __device__ __forceinline__ FeWalkArgs fe_batch_view(const FeBatchWalkArgs &a, ui
     FeWalkArgs w;
     w.buf = a.buf;
     w.xbuf = a.xbuf;
+    w.x32buf = a.x32buf;
     w.kind = a.kind;
     w.qbase = 0;
     w.qlimit = (k + 1) * a.piece_stride;
@@ -919,6 +946,49 @@ __global__ __launch_bounds__(64) void xo_expand_kernel(XoPieceArgs a)
     double *out = a.xbuf + k * a.piece_len + off;
     for (uint32_t i = 0; i < a.per_thread; ++i)
         out[i] = xo_uniform(s);
+}
+
+// xoshiro128+ pieces, as the xoshiro256+ ones above.
+__global__ __launch_bounds__(64) void xo128_piece_state_kernel(Xo128PieceArgs a)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nb)
+        return;
+    const uint64_t b = a.b_first + k;
+    uint32_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
+    for (int j = 0; j < a.kb; ++j)
+        if ((b >> j) & 1ull)
+            gf2_apply128(a.bt + 512 * j, s);
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        a.states[4 * k + w] = s[w];
+}
+
+__global__ __launch_bounds__(64) void xo128_expand_kernel(Xo128PieceArgs a)
+{
+    const uint64_t per_piece = a.piece_len / a.per_thread;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nb * per_piece)
+        return;
+    const uint64_t k = t / per_piece, j = t % per_piece;
+    uint32_t s[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        s[w] = a.states[4 * k + w];
+    const uint64_t off = j * a.per_thread;
+    for (int i = 0; i < a.km; ++i)
+        if ((off >> i) & 1ull)
+            gf2_apply128(a.mt + 512 * i, s);
+    float *out = a.xbuf + k * a.piece_len + off;
+    for (uint32_t i = 0; i < a.per_thread; ++i)
+        out[i] = xo128_uniform(s);
+}
+
+void launch_xo128_expand(const Xo128PieceArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(xo128_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
+    const uint64_t threads = a.nb * (a.piece_len / a.per_thread);
+    hipLaunchKernelGGL(xo128_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
 }
 
 void launch_xo_expand(const XoPieceArgs &a, hipStream_t st)
@@ -1167,9 +1237,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
             const uint64_t e = lb * a.batch_size + e0 + i;
             bool pass = false;
             float p[3][4];
-            if (i < len && SOURCE == 2) {
-                RanfWalkF32 w{a.draws, a.qbase, a.evq[e]};
-                pass = generate_faster_f32(a.P, w, p);
+            if (i < len && (SOURCE == 2 || SOURCE == 3)) {
+                if constexpr (SOURCE == 2) {
+                    RanfWalkF32 w{a.draws, a.qbase, a.evq[e]};
+                    pass = generate_faster_f32(a.P, w, p);
+                } else {
+                    Xo128WalkF32 w{a.x32draws, a.qbase, a.evq[e]};
+                    pass = generate_faster_f32(a.P, w, p);
+                }
                 if (!pass) {
                     float *o = a.ev + 6 * e;
 #pragma unroll
@@ -1294,8 +1369,10 @@ void launch_f32_events(const F32Args &a, int grid, hipStream_t st)
         hipLaunchKernelGGL(f32_events_kernel<0>, dim3(grid), dim3(64), 0, st, a);
     else if (a.source == 1)
         hipLaunchKernelGGL(f32_events_kernel<1>, dim3(grid), dim3(64), 0, st, a);
-    else
+    else if (a.source == 2)
         hipLaunchKernelGGL(f32_events_kernel<2>, dim3(grid), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(f32_events_kernel<3>, dim3(grid), dim3(64), 0, st, a);
 }
 
 void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st)

@@ -743,6 +743,22 @@ struct RanfWalkF32 {
     }
 };
 
+// The xoshiro128+ stream (standard.rs with the f32 feature) expanded into HBM
+// as its f32 uniforms: random_array::<N> is N consecutive draws in array
+// order (standard.rs:33-35, no round to discard).
+struct Xo128WalkF32 {
+    const float *buf; // uniforms from position qbase on
+    uint64_t qbase;
+    uint64_t q;       // next position
+    __device__ __forceinline__ void take(int n, float *out)
+    {
+        const float *v = buf + (q - qbase);
+        for (int k = 0; k < n; ++k)
+            out[k] = v[k];
+        q += (uint64_t)n;
+    }
+};
+
 // f32::MIN_POSITIVE squared underflows to 0 in f32, so the re-roll test
 // r2 < MIN_POSITIVE^2 never fires, as in the reference's f32 build.
 template <class Walk> __device__ __forceinline__ void raw_photons_faster_f32(Walk &w, float q[3][4])

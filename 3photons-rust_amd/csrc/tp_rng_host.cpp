@@ -643,4 +643,33 @@ void xoshiro_walk_faster_evgen(const uint64_t s0[4], uint64_t q, uint64_t num_ev
         *q_end = q;
 }
 
+void xoshiro128_walk_faster_evgen(const uint32_t s0[4], uint64_t q, uint64_t num_events, uint64_t *starts,
+                                  uint64_t *q_end)
+{
+    uint32_t s[4] = {s0[0], s0[1], s0[2], s0[3]};
+    xoshiro128_advance(s, q);
+    auto uniform = [&]() { return (float)(xoshiro128_next(s) >> 8) * 0x1.0p-24f; };
+    for (uint64_t e = 0; e < num_events; ++e) {
+        if (starts)
+            starts[e] = q;
+        for (int k = 0; k < 9; ++k) // random_array::<9>: only their positions matter here
+            (void)xoshiro128_next(s);
+        float r6[6];
+        for (int k = 0; k < 6; ++k)
+            r6[k] = uniform();
+        q += 15;
+        for (int i = 0; i < 3; ++i) { // evgen.rs:228-246 in f32: MIN_POSITIVE^2 is 0, only r2 > 1 re-rolls
+            const float x = 2.f * r6[i] - 1.f, y = 2.f * r6[3 + i] - 1.f;
+            float r2 = (0.f + x * x) + y * y;
+            while (r2 > 1.f) {
+                const float a = 2.f * uniform() - 1.f, b = 2.f * uniform() - 1.f;
+                q += 2;
+                r2 = 0.f + (a * a + b * b);
+            }
+        }
+    }
+    if (q_end)
+        *q_end = q;
+}
+
 } // namespace tp

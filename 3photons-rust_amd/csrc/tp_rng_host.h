@@ -106,5 +106,9 @@ std::vector<Gf2Mat128> gf2_128_pow2_table(const Gf2Mat128 &base, int count);
 void gf2_128_pow_from_table(const std::vector<Gf2Mat128> &table, uint64_t n, Gf2Mat128 &out);
 const std::vector<Gf2Mat128> &xoshiro128_step_pow2(); // M^(2^k), k = 0..63 (cached)
 void xoshiro128_advance(uint32_t s[4], uint64_t n);
+// faster-evgen walk of the xoshiro128+ stream of the all-f32 mode, as
+// xoshiro_walk_faster_evgen (the unit-disc test in f32).
+void xoshiro128_walk_faster_evgen(const uint32_t s0[4], uint64_t q, uint64_t num_events, uint64_t *starts,
+                                  uint64_t *q_end);
 
 } // namespace tp

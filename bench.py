@@ -366,18 +366,21 @@ def main() -> int:
         fe_jump = {}
         for name, kind, prec, ref_s in (("ranf_f64", tp.RNG_RANF, tp.PREC_F64, 0.57),
                                         ("xoshiro_f64", tp.RNG_XOSHIRO, tp.PREC_F64, 0.59),
-                                        ("ranf_f32", tp.RNG_RANF, tp.PREC_F32, 0.48)):
+                                        ("ranf_f32", tp.RNG_RANF, tp.PREC_F32, 0.48),
+                                        ("xoshiro128_f32", tp.RNG_XOSHIRO128, tp.PREC_F32, None)):
             pj = tp.params_from_config(cfg.with_events(fe_events), precision=prec, faster_evgen=True)
             vj, dtj, kj, accj = measure(tp, torch, eng, pj, fe_events, 1, 1, per_batch, merged, kind, tp.SEED_JUMP)
             fe_jump[name] = {"value": vj, "unit": "events/s", "events": fe_events, "ms_per_step": dtj * 1e3,
                              "kernels_ms": kj, "selected_events": int(accj.selected_events),
-                             "reference_published_events_per_s": 1e7 / ref_s}
+                             "reference_published_events_per_s": None if ref_s is None else 1e7 / ref_s}
         # faster-evgen with skip-ahead seeding on the other streams: xoshiro256+
-        # (standard-random, no published figure) and RANF in f32 (VALIDATION.md:41-43,
-        # single-threaded), each walked on the device as above
+        # (standard-random, no published figure), RANF in f32 (VALIDATION.md:41-43,
+        # single-threaded) and xoshiro128+ in f32 (standard-random,f32), each
+        # walked on the device as above
         fe_more = {}
         for name, kind, prec, ref_s in (("xoshiro_f64", tp.RNG_XOSHIRO, tp.PREC_F64, None),
-                                        ("ranf_f32", tp.RNG_RANF, tp.PREC_F32, 2.11)):
+                                        ("ranf_f32", tp.RNG_RANF, tp.PREC_F32, 2.11),
+                                        ("xoshiro128_f32", tp.RNG_XOSHIRO128, tp.PREC_F32, None)):
             pj = tp.params_from_config(cfg.with_events(fe_events), precision=prec, faster_evgen=True)
             vj, dtj, kj, accj = measure(tp, torch, eng, pj, fe_events, 1, 1, per_batch, merged, kind)
             fe_more[name] = {"value": vj, "unit": "events/s", "events": fe_events, "ms_per_step": dtj * 1e3,

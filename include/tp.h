@@ -82,9 +82,9 @@ typedef struct {
     int32_t sort_photons;                 /* 1 = sort by energy (default), 0 = no-photon-sorting */
     int32_t precision;                    /* tp_precision */
     int32_t faster_evgen;                 /* 1 = `faster-evgen` generation (evgen.rs:143-173, 221-249),
-                                             0 = default.  Available for RANF in f64 and f32 and for
-                                             xoshiro256+ in f64, with either seeding; not in mixed
-                                             precision or with xoshiro128+ */
+                                             0 = default.  Available for every stream (RANF,
+                                             xoshiro256+, xoshiro128+ in f32) in f64 and f32, with
+                                             either seeding; not in mixed precision */
     int32_t reserved;                     /* 0 */
 } tp_params;
 

@@ -50,6 +50,8 @@ def walklib(tmp_path_factory):
     lib.tp_walk_end.argtypes = [C.POINTER(C.c_int32), C.c_uint32, C.c_uint64, C.c_int, C.c_int]
     lib.tp_xo_walk_end.argtypes = [C.POINTER(C.c_uint64), C.c_uint64]
     lib.tp_xo_walk_end.restype = C.c_uint64
+    lib.tp_xo128_walk_end.argtypes = [C.POINTER(C.c_uint32), C.c_uint64]
+    lib.tp_xo128_walk_end.restype = C.c_uint64
     lib.tp_walk_end.restype = C.c_uint64
     lib.tp_walk_from_mismatch.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_uint32, C.c_int]
     return lib

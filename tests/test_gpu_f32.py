@@ -303,9 +303,57 @@ def test_f32_faster_evgen_simulate_batch_advances_like_the_closure(engine, cfg, 
         assert st.index == g.index and list(st.numbers)[1:] == list(g.numbers)[1:]
 
 
-def test_f32_faster_evgen_rejects_xoshiro128(engine, cfg):
-    """faster-evgen with the xoshiro128+ stream (standard-random,f32,faster-evgen)
-    is not built."""
-    params = tp.params_from_config(cfg.with_events(100), precision=tp.PREC_F32, faster_evgen=True)
-    with pytest.raises(tp.TpError, match="faster-evgen"):
-        engine.run(params, tp.RNG_XOSHIRO128, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO128), 100)
+@pytest.mark.parametrize("jump", [False, True])
+@pytest.mark.parametrize("n_events", [1, 9999, 35017, 123457])
+def test_f32_faster_evgen_xoshiro128_per_batch_matches_oracle(engine, cfg, oracle32, valeurs, n_events, jump):
+    """faster-evgen on the xoshiro128+ stream of the all-f32 mode
+    (standard-random,f32,faster-evgen, with or without faster-threading):
+    random_array::<N> as N consecutive f32 draws, the unit-disc test in f32;
+    batch by batch against the f32 oracle."""
+    print(f"f32 faster-evgen xoshiro128 N={n_events} jump={jump}", flush=True)
+    c = cfg.with_events(n_events)
+    params = tp.params_from_config(c, precision=tp.PREC_F32, faster_evgen=True)
+    seeding = tp.SEED_JUMP if jump else tp.SEED_SKIPAHEAD
+    per, merged, _ = engine.run(params, tp.RNG_XOSHIRO128, seeding, tp.rng_state(tp.RNG_XOSHIRO128), n_events)
+    ocfg = oracle32.config(valeurs, n_events)
+    om, oper = oracle32.run(ocfg, oracle32.opts("xoshiro", faster_threading=jump, threads=4, faster_evgen=True),
+                            per_batch=True)
+    for b in range((n_events + 9999) // 10000):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        close_f32(a, oper[b].as_array(), min(10000, n_events - 10000 * b))
+    close_f32(merged.as_array(), om.as_array(), n_events)
+
+
+@pytest.mark.parametrize("jump", [False, True])
+def test_f32_faster_evgen_xoshiro128_across_chunks(engine, cfg, oracle32, valeurs, jump):
+    """N = 6.2e7 (two chunks): with skip-ahead seeding the device walk of the
+    one stream, with jump seeding the per-batch walks; every batch against the
+    f32 oracle's run."""
+    n_events = 62_000_000
+    print(f"f32 faster-evgen xoshiro128 N={n_events} jump={jump}", flush=True)
+    params = tp.params_from_config(cfg.with_events(n_events), precision=tp.PREC_F32, faster_evgen=True)
+    seeding = tp.SEED_JUMP if jump else tp.SEED_SKIPAHEAD
+    per, merged, stats = engine.run(params, tp.RNG_XOSHIRO128, seeding, tp.rng_state(tp.RNG_XOSHIRO128), n_events)
+    assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    print("oracle run", flush=True)
+    om, oper = oracle32.run(oracle32.config(valeurs, n_events),
+                            oracle32.opts("xoshiro", faster_threading=jump, threads=16, faster_evgen=True),
+                            per_batch=True)
+    for b in range(len(per)):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        close_f32(a, oper[b].as_array(), 10000)
+
+
+def test_f32_faster_evgen_xoshiro128_simulate_batch_advances_like_the_closure(engine, cfg, oracle32, valeurs):
+    params = tp.params_from_config(cfg, precision=tp.PREC_F32, faster_evgen=True)
+    st = tp.rng_state(tp.RNG_XOSHIRO128)
+    g = oracle32.rng("xoshiro")
+    ocfg = oracle32.config(valeurs)
+    opts = oracle32.opts("xoshiro", faster_evgen=True)
+    for n in (10000, 7, 1, 23456):
+        acc = engine.simulate_batch(params, tp.RNG_XOSHIRO128, st, n)
+        oacc = oracle32.simulate_batch(ocfg, opts, g, n)
+        close_f32(acc.as_array(), oacc.as_array(), n)
+        assert list(st.s) == list(g.xs32)

@@ -60,3 +60,19 @@ def test_walk_from_an_event_start(lib, e0, n, seed, index):
     last start of the previous one (mid-round in general): the same starts
     and end position as the walk from the state itself."""
     assert lib.tp_walk_from_mismatch(e0, n, seed, index, 8) == 0
+
+
+def test_xoshiro128_walk_end_is_the_f32_oracle_closure_advance(lib, oracle32, valeurs):
+    """The same on the xoshiro128+ stream of the all-f32 mode (the unit-disc
+    test in f32)."""
+    import ctypes as C
+    ocfg = oracle32.config(valeurs)
+    opts = oracle32.opts("xoshiro", faster_evgen=True)
+    for n in (1, 7, 10000, 23456):
+        g = oracle32.rng("xoshiro")
+        oracle32.simulate_batch(ocfg, opts, g, n)
+        h = oracle32.rng("xoshiro")
+        q = lib.tp_xo128_walk_end((C.c_uint32 * 4)(*list(h.xs32)), n)
+        assert 15 * n <= q <= 40 * n
+        oracle32.skip(h, q)
+        assert list(h.xs32) == list(g.xs32), n

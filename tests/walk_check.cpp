@@ -70,3 +70,11 @@ extern "C" uint64_t tp_xo_walk_end(const uint64_t *s0, uint64_t n)
     tp::xoshiro_walk_faster_evgen(s0, 0, n, nullptr, &q);
     return q;
 }
+
+// The same on the xoshiro128+ stream of the all-f32 mode.
+extern "C" uint64_t tp_xo128_walk_end(const uint32_t *s0, uint64_t n)
+{
+    uint64_t q = 0;
+    tp::xoshiro128_walk_faster_evgen(s0, 0, n, nullptr, &q);
+    return q;
+}
