@@ -6,7 +6,7 @@
 //   --standard-random     xoshiro256+ stream instead of RANF
 //   --faster-threading    per-batch jump()/reseed instead of skip-ahead
 //   --no-photon-sorting   keep RAMBO's photon order
-//   --faster-evgen        the reference's faster-evgen generation (RANF, f64)
+//   --faster-evgen        the reference's faster-evgen generation (any stream, f64 or f32)
 //   --mixed               f32 spinor/amplitude arithmetic (tp_precision MIXED)
 //   --f32                 the reference's f32 feature: everything in f32,
 //                         xoshiro128+ with --standard-random, f32 outputs

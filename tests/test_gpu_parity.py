@@ -9,7 +9,7 @@ import pytest
 
 import trois_photons as tp
 from conftest import PKG, golden_path
-from numdiff import compare_texts
+from numdiff import compare_texts, tokens
 from test_host import to_tp_acc
 
 pytestmark = pytest.mark.gpu
@@ -408,6 +408,57 @@ def test_full_run_reproduces_golden(engine, cfg, oracle_run, suffix, rng, seedin
     print(f"  res.data vs golden, zero slack: {len(exact)} fields differ {exact}")
     assert exact == [] or suffix in LAST_DIGIT, exact
     assert compare_texts(golden, text, abs_=abs_, digits_slack=1.0) == []
+
+
+# The reference's CI matrix (ci.yml:130-140) as the driver's flags: the
+# multi-threading feature changes no result (the driver always runs the
+# reproducible schedule), so its files equal the single-threaded ones.
+CLI_GOLDEN = [("features_", []), ("features_f32", ["--f32"]), ("features_faster-evgen", ["--faster-evgen"]),
+              ("features_multi-threading", []), ("features_multi-threading,faster-threading", ["--faster-threading"]),
+              ("features_no-photon-sorting", ["--no-photon-sorting"]),
+              ("features_standard-random", ["--standard-random"]),
+              ("features_standard-random,f32", ["--standard-random", "--f32"]),
+              ("features_standard-random,multi-threading", ["--standard-random"]),
+              ("features_standard-random,multi-threading,faster-threading", ["--standard-random", "--faster-threading"])]
+
+
+@pytest.mark.parametrize("suffix,flags", CLI_GOLDEN)
+def test_cli_reproduces_every_ci_golden_file(tmp_path, valeurs, suffix, flags):
+    """The command-line driver, run with the flags that stand for each feature
+    combination of the reference's CI, writes res.data as that CI accepts it
+    (ci.yml:178-191): zero tolerance, -a 1.1e-16 for the xoshiro jump file
+    (written with a completion-order merge), -a 1.1e-8 for the f32 files
+    (where the f32 print of a value one rounding away from the reference's may
+    also differ in its last digit)."""
+    exe = os.path.join(PKG, "build", "trois_photons")
+    shutil.copy(valeurs, tmp_path / "valeurs")
+    print(f"cli {' '.join(flags)} for {suffix}", flush=True)
+    r = subprocess.run([exe, *flags], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stderr
+    golden = open(golden_path("res.data-" + suffix)).read()
+    text = (tmp_path / "res.data").read_text()
+    if "f32" in suffix:
+        errs = compare_texts(golden, text, abs_=1.1e-8, digits_slack=1.0)
+    else:
+        abs_ = 1.1e-16 if suffix == "features_standard-random,multi-threading,faster-threading" else 0.0
+        errs = compare_texts(golden, text, abs_=abs_)
+    assert errs == [], errs
+
+
+def test_cli_runs_the_faster_evgen_combinations(tmp_path, valeurs):
+    """faster-evgen with every stream, precision and seeding (no golden files
+    exist for these; the library runs are checked batch by batch against the
+    oracle elsewhere): the driver completes and writes a finite sigma."""
+    exe = os.path.join(PKG, "build", "trois_photons")
+    shutil.copy(valeurs, tmp_path / "valeurs")
+    for flags in (["--standard-random"], ["--f32"], ["--standard-random", "--f32"], ["--faster-threading"],
+                  ["--standard-random", "--faster-threading"], ["--f32", "--faster-threading"],
+                  ["--standard-random", "--f32", "--faster-threading"]):
+        print(f"cli --faster-evgen {' '.join(flags)}", flush=True)
+        r = subprocess.run([exe, "--faster-evgen", *flags], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+        assert r.returncode == 0, (flags, r.stderr)
+        nums = [float(t) for k, t in tokens((tmp_path / "res.data").read_text()) if k == "num" and t != "NaN"]
+        assert len(nums) > 20 and all(np.isfinite(nums)), flags
 
 
 def test_cli_end_to_end(tmp_path, valeurs):
