@@ -56,12 +56,30 @@ __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x
 __device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * LN2; }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sigm(float x) { return frcp(1.f + fexp(-x)); }
-// SiLU and its derivative from one sigmoid (the forward-only kernels drop d).
-__device__ __forceinline__ void silu_d(float x, float &y, float &d)
+
+// The bf16 copy of the parameters (what the kernels multiply with) is scaled
+// so that the hidden layers work in base 2: W1, b1 and b2 times -log2(e), W3
+// times -ln 2, W2 and b3 as they are.  A hidden pre-activation then comes out
+// as c' = -log2(e) c, and SiLU(c) = c sigmoid(c) becomes
+// c' / (1 + 2^c') = -log2(e) SiLU(c): one exp2 and one reciprocal with no
+// scaling multiply.  W2 sees -log2(e) SiLU and yields -log2(e) times the next
+// pre-activation (so b2 carries the factor), and W3's -ln 2 takes it out
+// again: the network is the same function, up to the bf16 rounding of the
+// scaled values.  The master f32 parameters are the unscaled ones, and their
+// gradient is the scaled one's times the same factor.
+__host__ __device__ constexpr float param_scale(int i)
 {
-    const float s = sigm(x);
-    y = x * s;
-    d = s * (1.f + x * (1.f - s));
+    const int o = i % LP;
+    return o < OFF_W2 ? -LOG2E : o < OFF_B2 ? 1.f : o < OFF_W3 ? -LOG2E : o < OFF_B3 ? -LN2 : 1.f;
+}
+// SiLU in base 2 and its derivative: y = c' / (1 + 2^c') and
+// dy/dc' = s (1 - ln 2 c' (1 - s)), s = 1 / (1 + 2^c') (the forward-only
+// kernels drop d).
+__device__ __forceinline__ void silu_d(float c, float &y, float &d)
+{
+    const float s = frcp(1.f + __builtin_amdgcn_exp2f(c));
+    y = c * s;
+    d = s * (1.f - LN2 * c * (1.f - s));
 }
 __device__ __forceinline__ float ftanh(float x) { return 1.f - 2.f * frcp(fexp(2.f * x) + 1.f); }
 __device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + flog(1.f + fexp(-fabsf(x))); }
@@ -681,7 +699,7 @@ __global__ __launch_bounds__(256) void flow_grad_reduce_kernel(const float *part
         for (int j = 0; j < 16; ++j)
             sum += v[j];
     }
-    grad[i] = sum * inv_n;
+    grad[i] = sum * (inv_n * param_scale(i));
 }
 
 __global__ void flow_adam_kernel(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr,
@@ -698,14 +716,14 @@ __global__ void flow_adam_kernel(float *w, uint16_t *wbf, float *m, float *v, co
     const float mh = mi / (1.f - powf(b1, (float)step)), vh = vi / (1.f - powf(b2, (float)step));
     const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
     w[i] = wi;
-    wbf[i] = f2bf(wi);
+    wbf[i] = f2bf(param_scale(i) * wi);
 }
 
 __global__ void flow_to_bf16_kernel(const float *w, uint16_t *wbf)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < NPARAM)
-        wbf[i] = f2bf(w[i]);
+        wbf[i] = f2bf(param_scale(i) * w[i]);
 }
 
 // ---- launchers ---------------------------------------------------------------

@@ -10,7 +10,11 @@ log q(u) = sum log v(1 - v) - sum s - sum log u(1 - u).
 With ``emulate=True`` every operand of a matrix product (weights, biases and
 activations) is rounded to bf16 as the kernels round them, with f32
 arithmetic elsewhere; the rounding passes gradients straight through, so
-autograd gives the gradient the kernels' backward pass approximates.
+autograd gives the gradient the kernels' backward pass approximates.  The
+kernels' hidden layers work in base 2: they multiply bf16 copies of W1, b1
+and b2 scaled by -log2(e) and of W3 scaled by -ln 2, and evaluate
+c' / (1 + 2^c') = -log2(e) SiLU(c) -- the same network in exact arithmetic,
+so only the emulation follows that form.
 """
 from __future__ import annotations
 
@@ -22,6 +26,8 @@ LP = 7328
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 1024, 1088, 5184, 5248, 7296
 MASKS = (0x03F, 0xFC0, 0x555, 0xAAA, 0x1C7, 0xE38)
 SCALE = 1.5
+NEG_LOG2E = torch.tensor(-1.4426950408889634, dtype=torch.float32)  # the kernels' f32 constants
+NEG_LN2 = torch.tensor(-0.6931471805599453, dtype=torch.float32)
 
 
 def bf(x: torch.Tensor) -> torch.Tensor:
@@ -41,10 +47,17 @@ def cond_mask(l: int) -> torch.Tensor:
 
 def mlp(a16: torch.Tensor, prm, emulate: bool):
     W1, b1, W2, b2, W3, b3 = prm
-    r = bf if emulate else (lambda x: x)
-    h1 = torch.nn.functional.silu(r(a16) @ r(W1).T + r(b1))
-    h2 = torch.nn.functional.silu(r(h1) @ r(W2).T + r(b2))
-    o = r(h2) @ r(W3).T + r(b3)
+    if not emulate:
+        h1 = torch.nn.functional.silu(a16 @ W1.T + b1)
+        h2 = torch.nn.functional.silu(h1 @ W2.T + b2)
+        o = h2 @ W3.T + b3
+        return o[:, :D], o[:, 16:16 + D]
+    # base-2 hidden layers: c' = -log2(e) c, and c' / (1 + 2^c') = c' sigmoid(-ln 2 c')
+    c1 = bf(a16) @ bf(NEG_LOG2E * W1).T + bf(NEG_LOG2E * b1)
+    h1 = c1 * torch.sigmoid(NEG_LN2 * c1)
+    c2 = bf(h1) @ bf(W2).T + bf(NEG_LOG2E * b2)
+    h2 = c2 * torch.sigmoid(NEG_LN2 * c2)
+    o = bf(h2) @ bf(NEG_LN2 * W3).T + bf(b3)
     return o[:, :D], o[:, 16:16 + D]
 
 
