@@ -823,18 +823,20 @@ __global__ __launch_bounds__(256) void fe_seg_join_kernel(FeBatchWalkArgs a)
     a.join_next[t] = jj;
 }
 
-// One thread per batch: the true walk is walker 0's list from its first
+// One wavefront per batch: the true walk is walker 0's list from its first
 // start, then each walker's from where its predecessor met it.  Its first n
-// starts go to the batch's slots of evq; seq[k] = 1 (nothing written) when
-// the true walk leaves a list before meeting the next one, or the lists end,
-// before n starts.
+// starts go to the batch's slots of evq, each piece copied by the 64 lanes
+// together; seq[k] = 1 (nothing written) when the true walk leaves a list
+// before meeting the next one, or the lists end, before n starts.  The piece
+// bounds are the same in every lane (uniform loads), so the loops are too.
 __global__ __launch_bounds__(64) void fe_seg_assemble_kernel(FeBatchWalkArgs a)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t k = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
     if (k >= a.nb)
         return;
     const uint32_t n = fe_batch_events(a, k);
-    // check first, so that a batch left to the sequential walk gets no partial starts
+    // check first, so that a batch left to the one-thread walk gets no partial starts
     uint32_t begin = 0;
     uint64_t have = 0;
     bool ok = false;
@@ -852,7 +854,8 @@ __global__ __launch_bounds__(64) void fe_seg_assemble_kernel(FeBatchWalkArgs a)
             break;
         begin = (uint32_t)a.join_next[w + 1];
     }
-    a.seq[k] = ok ? 0u : 1u;
+    if (lane == 0)
+        a.seq[k] = ok ? 0u : 1u;
     if (!ok)
         return;
     uint64_t *out = a.evq + k * a.batch_size;
@@ -864,8 +867,10 @@ __global__ __launch_bounds__(64) void fe_seg_assemble_kernel(FeBatchWalkArgs a)
         const bool meets = jp >= 0 && (uint32_t)jp >= begin;
         const uint32_t end = meets ? (uint32_t)jp : a.counts[w];
         const uint64_t *list = a.lists + w * a.cap;
-        for (uint32_t i = begin; i < end && off < n; ++i)
-            out[off++] = list[i];
+        const uint32_t cnt = end > begin ? (uint32_t)min<uint64_t>(end - begin, n - off) : 0u;
+        for (uint32_t i = lane; i < cnt; i += 64)
+            out[off + i] = list[begin + i];
+        off += cnt;
         if (meets)
             begin = (uint32_t)a.join_next[w + 1];
     }
@@ -904,7 +909,7 @@ void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st)
     const unsigned wb = (unsigned)((walkers + 255) / 256), bb = (unsigned)((a.nb + 63) / 64);
     hipLaunchKernelGGL(fe_seg_walk_kernel, dim3(wb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(fe_seg_join_kernel, dim3(wb), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(fe_seg_assemble_kernel, dim3(bb), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(fe_seg_assemble_kernel, dim3((unsigned)a.nb), dim3(64), 0, st, a);
     hipLaunchKernelGGL(fe_seq_walk_kernel, dim3(bb), dim3(64), 0, st, a);
 }
 
