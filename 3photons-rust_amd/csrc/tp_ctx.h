@@ -75,6 +75,9 @@ struct Ctx {
     DevBuf fe_seq;   // jump seeding: batches left to the one-thread walk
     DevBuf xo_states, xo_draws; // faster-evgen on xoshiro256+: piece states and uniforms
     DevBuf xo128_states, xo128_draws, xo128_pt; // the same on xoshiro128+ (f32), and the piece matrix powers
+    DevBuf xo_acc;              // accept bits of the expanded xoshiro pieces: acc1, then acc3
+    DevBuf xo_lt, xo128_lt;     // the expanding lanes' skip matrices (xoshiro256+, xoshiro128+)
+    uint64_t xo_acc_words = 0;  // words per array in xo_acc
     std::string xo128_pt_key;
     int xo128_pt_kb = 0;
     DevBuf adj_u;               // tp_adjoint: the stream's points, expanded

@@ -369,13 +369,16 @@ constexpr uint64_t FE_OVERLAP = 24 * 55;
 // segment; 16.64 positions per event on average, 17.0 allowed for.
 constexpr double FE_XO_POSITIONS_PER_EVENT = 17.0;
 constexpr uint64_t FE_XO_EVENTS_PER_WALKER = 1024, FE_XO_OVERLAP = 17500;
-// xoshiro256+ streams are expanded in pieces of FE_XO_PIECE uniforms, one
+// xoshiro streams are expanded in pieces of FE_XO_PIECE uniforms, one
 // wavefront per piece and FE_XO_PER_THREAD consecutive draws per lane from one
-// skip-ahead.  With jump seeding a piece is a batch's stream; with skip-ahead
+// skip-ahead (a multiple of 128: the lanes' accept-bit words go out 4 at a
+// time).  With jump seeding a piece is a batch's stream; with skip-ahead
 // seeding the pieces tile the run's one stream (piece p starts FE_XO_PIECE p
 // draws after the run's state).
-constexpr uint64_t FE_XO_PIECE = 200000;
-constexpr uint32_t FE_XO_PER_THREAD = 3125;
+constexpr uint32_t FE_XO_PER_THREAD = 3200;
+constexpr uint64_t FE_XO_PIECE = 64 * FE_XO_PER_THREAD;
+constexpr uint64_t FE_XO_WORDS = FE_XO_PIECE / 32; // accept-bit words per piece and test
+static_assert(FE_XO_PER_THREAD % 128 == 0, "whole groups of 4 bit words per lane");
 
 namespace {
 
@@ -413,16 +416,54 @@ struct FeStream {
     const uint32_t *s0_128 = nullptr; // xoshiro128+ (kind 3): the run's state (position 0)
 };
 
+// Accept-bit arrays for np xoshiro pieces in c->xo_acc: acc1, then acc3 at
+// c->xo_acc_words words (sized for the largest np seen, so that the split
+// stays put while the buffer is reused).
+int ensure_acc(Ctx *c, uint64_t np)
+{
+    const uint64_t words = std::max(np * FE_XO_WORDS, c->xo_acc_words);
+    int rc = ensure(c, c->xo_acc, 2 * words * sizeof(uint32_t));
+    if (rc)
+        return rc;
+    c->xo_acc_words = words;
+    return TP_OK;
+}
+
+// M^(FE_XO_PER_THREAD 2^j), j < 6, of both xoshiro generators: the expanding
+// lanes' skips (lane l skips l * FE_XO_PER_THREAD draws into its piece).
+int ensure_lane_tables(Ctx *c)
+{
+    if (!c->xo_lt.p) {
+        tp::Gf2Mat m;
+        tp::gf2_pow_from_table(tp::xoshiro_step_pow2(), FE_XO_PER_THREAD, m);
+        int rc = upload_mats(c, c->xo_lt, tp::gf2_pow2_table(m, 6));
+        if (rc)
+            return rc;
+    }
+    if (!c->xo128_lt.p) {
+        tp::Gf2Mat128 m;
+        tp::gf2_128_pow_from_table(tp::xoshiro128_step_pow2(), FE_XO_PER_THREAD, m);
+        const std::vector<tp::Gf2Mat128> t = tp::gf2_128_pow2_table(m, 6);
+        int rc = ensure(c, c->xo128_lt, t.size() * sizeof(tp::Gf2Mat128));
+        if (rc)
+            return rc;
+        HIPCHK(c, hipMemcpy(c->xo128_lt.p, t.data(), t.size() * sizeof(tp::Gf2Mat128), hipMemcpyHostToDevice));
+    }
+    return TP_OK;
+}
+
 // Expands xoshiro128+ pieces b_first .. b_first + np - 1 of piece_len f32
 // uniforms into c->xo128_draws (piece k from B^(b_first + k) s0, B as
 // ensure_xo128_piece_table made it).
 int xo128_expand(Ctx *c, const uint32_t s0[4], uint64_t b_first, uint64_t np)
 {
-    int rc = ensure_mt128(c);
+    int rc = ensure_lane_tables(c);
     if (!rc)
         rc = ensure(c, c->xo128_states, np * 4 * sizeof(uint32_t));
     if (!rc)
         rc = ensure(c, c->xo128_draws, np * FE_XO_PIECE * sizeof(float));
+    if (!rc)
+        rc = ensure_acc(c, np);
     if (rc)
         return rc;
     tpk::Xo128PieceArgs p;
@@ -430,14 +471,15 @@ int xo128_expand(Ctx *c, const uint32_t s0[4], uint64_t b_first, uint64_t np)
     std::memcpy(p.s0, s0, sizeof p.s0);
     p.bt = static_cast<const uint32_t *>(c->xo128_pt.p);
     p.kb = c->xo128_pt_kb;
-    p.mt = static_cast<const uint32_t *>(c->mt128.p);
-    p.km = KM128;
+    p.lt = static_cast<const uint32_t *>(c->xo128_lt.p);
     p.nb = np;
     p.b_first = b_first;
     p.states = static_cast<uint32_t *>(c->xo128_states.p);
     p.piece_len = FE_XO_PIECE;
     p.per_thread = FE_XO_PER_THREAD;
     p.xbuf = static_cast<float *>(c->xo128_draws.p);
+    p.acc1 = static_cast<uint32_t *>(c->xo_acc.p);
+    p.acc3 = p.acc1 + c->xo_acc_words;
     static_assert(FE_XO_PIECE < (1ull << KM128), "in-piece offsets within the step table");
     tpk::launch_xo128_expand(p, c->stream);
     HIPCHK(c, hipGetLastError());
@@ -490,11 +532,13 @@ int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_la
     Tables *t = nullptr;
     int rc = ensure_xo_piece_table(c, p_end, &t);
     if (!rc)
-        rc = ensure_mt(c);
+        rc = ensure_lane_tables(c);
     if (!rc)
         rc = ensure(c, c->xo_states, np * 4 * sizeof(uint64_t));
     if (!rc)
         rc = ensure(c, c->xo_draws, np * FE_XO_PIECE * sizeof(double));
+    if (!rc)
+        rc = ensure_acc(c, np);
     if (rc)
         return rc;
     tpk::XoPieceArgs p;
@@ -502,14 +546,15 @@ int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_la
     std::memcpy(p.s0, fs.s0, sizeof p.s0);
     p.bt = static_cast<const uint64_t *>(t->bt.p);
     p.kb = t->kb;
-    p.mt = static_cast<const uint64_t *>(c->mt.p);
-    p.km = c->km;
+    p.lt = static_cast<const uint64_t *>(c->xo_lt.p);
     p.nb = np;
     p.b_first = p_first;
     p.states = static_cast<uint64_t *>(c->xo_states.p);
     p.piece_len = FE_XO_PIECE;
     p.per_thread = FE_XO_PER_THREAD;
     p.xbuf = static_cast<double *>(c->xo_draws.p);
+    p.acc1 = static_cast<uint32_t *>(c->xo_acc.p);
+    p.acc3 = p.acc1 + c->xo_acc_words;
     tpk::launch_xo_expand(p, c->stream);
     HIPCHK(c, hipGetLastError());
     *qbase = p_first * FE_XO_PIECE;
@@ -558,8 +603,8 @@ int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, boo
     if (rc)
         return rc;
     w.buf = static_cast<const int32_t *>(c->draws.p);
-    w.xbuf = static_cast<const double *>(c->xo_draws.p);
-    w.x32buf = static_cast<const float *>(c->xo128_draws.p);
+    w.acc1 = static_cast<const uint32_t *>(c->xo_acc.p);
+    w.acc3 = w.acc1 + c->xo_acc_words;
     w.lists = static_cast<uint64_t *>(c->fe_lists.p);
     w.counts = static_cast<uint32_t *>(c->fe_meta.p);
     w.join_prev = reinterpret_cast<int32_t *>(w.counts + T);
@@ -952,9 +997,13 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
     Tables *t = nullptr;
     int rc = ensure_tables(c, TP_SEED_JUMP, batch_size, nb, grid_for(c, 4, 0, chunk), &t);
     if (!rc)
+        rc = ensure_lane_tables(c);
+    if (!rc)
         rc = ensure(c, c->xo_states, chunk * 4 * sizeof(uint64_t));
     if (!rc)
         rc = ensure(c, c->xo_draws, chunk * FE_XO_PIECE * sizeof(double));
+    if (!rc)
+        rc = ensure_acc(c, chunk);
     if (!rc)
         rc = ensure(c, c->evq, chunk * batch_size * sizeof(uint64_t));
     if (!rc)
@@ -969,19 +1018,20 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
     std::memcpy(p.s0, x->s, sizeof p.s0);
     p.bt = static_cast<const uint64_t *>(t->bt.p);
     p.kb = t->kb;
-    p.mt = static_cast<const uint64_t *>(c->mt.p);
-    p.km = c->km;
+    p.lt = static_cast<const uint64_t *>(c->xo_lt.p);
     p.states = static_cast<uint64_t *>(c->xo_states.p);
     p.piece_len = FE_XO_PIECE;
     p.per_thread = FE_XO_PER_THREAD;
     p.xbuf = static_cast<double *>(c->xo_draws.p);
-    static_assert(FE_XO_PIECE % FE_XO_PER_THREAD == 0, "whole threads per piece");
+    p.acc1 = static_cast<uint32_t *>(c->xo_acc.p);
+    p.acc3 = p.acc1 + c->xo_acc_words;
     tpk::FeBatchWalkArgs w;
     std::memset(&w, 0, sizeof w);
     rc = fe_batch_walk_setup(c, w, 2, chunk, batch_size);
     if (rc)
         return rc;
-    w.xbuf = p.xbuf;
+    w.acc1 = p.acc1;
+    w.acc3 = p.acc3;
     w.piece_stride = FE_XO_PIECE;
     w.num_events = num_events;
     w.batch_size = batch_size;
@@ -1088,7 +1138,8 @@ int enqueue_xo128_faster_evgen(Ctx *c, const tp_params *params, int seeding, con
         rc = xo128_expand(c, s0, b0, cnb);
         if (rc)
             return rc;
-        w.x32buf = static_cast<const float *>(c->xo128_draws.p);
+        w.acc1 = static_cast<const uint32_t *>(c->xo_acc.p);
+        w.acc3 = w.acc1 + c->xo_acc_words;
         w.nb = cnb;
         w.b_first = b0;
         tpk::launch_fe_walk_batches(w, c->stream);
@@ -1099,7 +1150,7 @@ int enqueue_xo128_faster_evgen(Ctx *c, const tp_params *params, int seeding, con
         }
         fa.nb = cnb;
         fa.b_first = b0;
-        fa.x32draws = w.x32buf;
+        fa.x32draws = static_cast<const float *>(c->xo128_draws.p);
         fa.qbase = 0;
         fa.evq = w.evq;
         fa.out = static_cast<tp_acc *>(per_batch_dev) + b0;
@@ -1412,7 +1463,7 @@ extern "C" int tp_destroy(void *ctx)
         for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
                           &c->mt128, &c->bt128, &c->st128, &c->f32ev, &c->fe_lists, &c->fe_meta, &c->fe_piece,
                           &c->fe_flag, &c->fe_seq, &c->xo_states, &c->xo_draws, &c->xo128_states,
-                          &c->xo128_draws, &c->xo128_pt, &c->adj_u})
+                          &c->xo128_draws, &c->xo128_pt, &c->xo_acc, &c->xo_lt, &c->xo128_lt, &c->adj_u})
             if (b->p)
                 (void)hipFree(b->p);
         for (auto &kv : c->tables) {

@@ -120,9 +120,12 @@ void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st);
 // the start of the master state's round; buf holds positions [qbase, qlimit).
 struct FeWalkArgs {
     const int32_t *buf;    // RANF values of positions qbase .. qlimit - 1 (kinds 0, 1)
-    const double *xbuf;    // xoshiro256+ uniforms of the same positions (kind 2)
-    const float *x32buf;   // xoshiro128+ uniforms of the same positions (kind 3)
     int32_t kind;          // 0: RANF, unit-disc test in f64; 1: RANF, in f32; 2: xoshiro256+; 3: xoshiro128+ (f32)
+    // kinds 2, 3: the walk reads accept bits, not uniforms.  Bit i of acc1
+    // (word i / 32, bit i % 32) is 1 when the point (2u - 1) of positions
+    // qbase + i and qbase + i + 1 passes the unit-disc test; acc3 likewise for
+    // qbase + i and qbase + i + 3 (the three points of each event's 6 values).
+    const uint32_t *acc1, *acc3;
     uint64_t qbase, qlimit;
     uint64_t q_start;
     uint64_t seg_origin, seg, overlap;
@@ -148,8 +151,8 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
 // (G % 55 = q % 55).  evq[k * batch_size + e] receives event e's start as G.
 // A walk that would leave its piece sets *overflow and fills the rest of its
 // batch with the piece's first position (every later read stays in bounds).
-// The xoshiro256+ pieces (kind 2) hold uniforms from position 0 of each
-// batch's own stream (p0 null).
+// The xoshiro pieces (kinds 2, 3) hold uniforms from position 0 of each
+// batch's own stream (p0 null), and the walk reads their accept bits.
 //
 // Each batch is walked by S segment walkers, as the one-stream walk is
 // (walker s of batch k starts at k * piece_stride + s * seg, walker 0 at the
@@ -158,10 +161,9 @@ void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_
 // events are covered is walked by one thread from its first position.
 struct FeBatchWalkArgs {
     const int32_t *buf;    // RANF pieces (kinds 0, 1)
-    const double *xbuf;    // xoshiro256+ pieces (kind 2)
-    const float *x32buf;   // xoshiro128+ pieces (kind 3)
     int32_t kind;          // 0: RANF, unit-disc test in f64; 1: in f32 (all-f32 mode); 2: xoshiro256+;
                            // 3: xoshiro128+ (all-f32 mode)
+    const uint32_t *acc1, *acc3; // kinds 2, 3: accept bits of the pieces' positions, as FeWalkArgs
     uint64_t piece_stride;
     const int32_t *p0;     // [nb] frame position of each piece's first value (RANF)
     uint64_t nb, b_first, num_events;
@@ -181,18 +183,20 @@ void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st);
 // Pieces of the xoshiro256+ stream: piece k starts from B^(b_first + k) s0
 // (B the 2^128 jump with jump seeding, each piece a batch's stream; or
 // M^piece_len, the pieces tiling one stream) and holds its first piece_len
-// uniforms at xbuf[k * piece_len ..].
+// uniforms at xbuf[k * piece_len ..], and the accept bits of those positions
+// at acc1 / acc3 (the last three positions' bits from the draws that follow
+// the piece: the next piece's first ones when the pieces tile one stream).
 struct XoPieceArgs {
     uint64_t s0[4];
     const uint64_t *bt;    // B^(2^k), k < kb
     int32_t kb;
-    const uint64_t *mt;    // M^(2^k), k < km
-    int32_t km;
+    const uint64_t *lt;    // M^(per_thread 2^j), j < 6: lane l's skip of l * per_thread draws
     uint64_t nb, b_first;
     uint64_t *states;      // [nb][4] scratch: each piece's first state
-    uint64_t piece_len;    // uniforms expanded per piece
-    uint32_t per_thread;   // uniforms per expanding thread (divides piece_len)
+    uint64_t piece_len;    // uniforms expanded per piece: 64 * per_thread
+    uint32_t per_thread;   // uniforms per lane, a multiple of 128
     double *xbuf;          // [nb][piece_len]
+    uint32_t *acc1, *acc3; // [nb][piece_len / 32] accept bits of the positions, as FeWalkArgs
 };
 void launch_xo_expand(const XoPieceArgs &a, hipStream_t st); // piece states and expansion
 
@@ -202,13 +206,13 @@ struct Xo128PieceArgs {
     uint32_t s0[4];
     const uint32_t *bt;    // B^(2^k), k < kb (128 columns x 4 words each)
     int32_t kb;
-    const uint32_t *mt;    // M^(2^k), k < km
-    int32_t km;
+    const uint32_t *lt;    // M^(per_thread 2^j), j < 6
     uint64_t nb, b_first;
     uint32_t *states;      // [nb][4] scratch: each piece's first state
-    uint64_t piece_len;
-    uint32_t per_thread;
+    uint64_t piece_len;    // 64 * per_thread
+    uint32_t per_thread;   // a multiple of 128
     float *xbuf;           // [nb][piece_len]
+    uint32_t *acc1, *acc3; // [nb][piece_len / 32] accept bits (the unit-disc test in f32)
 };
 void launch_xo128_expand(const Xo128PieceArgs &a, hipStream_t st);
 

@@ -593,52 +593,59 @@ template <class T = double> __device__ __forceinline__ bool fe_walk_event(const 
     return true;
 }
 
-// One event's consumption of the xoshiro stream: 9, 6, then 2 per re-rolled
-// point of the unit-disc test, no discards.  buf holds the uniforms of
-// positions qbase .. qlimit - 1.  False when the stored range ends inside the
-// event.
-__device__ __forceinline__ bool xo_walk_event(const double *buf, uint64_t qbase, uint64_t qlimit, uint64_t &q)
+// Accept bit of stored position r (bit r % 32 of word r / 32).
+__device__ __forceinline__ uint32_t acc_bit(const uint32_t *bits, uint64_t r) { return (bits[r >> 5] >> (r & 31)) & 1u; }
+
+// One event's consumption of a xoshiro stream (xoshiro256+ in f64 or
+// xoshiro128+ in f32): 9, 6, then 2 per re-rolled point of the unit-disc
+// test, no discards.  The test's outcomes come from the accept bits of the
+// stored positions qbase .. qlimit - 1: point i of the 6 values is
+// (u_p, u_p+3) at p = q + 9 + i (acc3), a re-roll is (u_p, u_p+1) (acc1).
+// The re-rolls of the failed points form one run of pairs from q + 15 on, so
+// with F failed points the event ends after the F-th accepted pair.  Away
+// from qlimit that is found from 64 positions of acc1 bits at once: the
+// accepted pairs are the set bits at even offsets, and clearing the lowest
+// F - 1 of them leaves the F-th as the lowest.  Near qlimit, or when the 32
+// pairs hold fewer than F accepted ones, the pairs are taken one at a time.
+// Every bit read pairs positions below qlimit.  False when the stored range
+// ends inside the event.
+__device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32_t *acc3, uint64_t qbase,
+                                              uint64_t qlimit, uint64_t &q)
 {
-    q += 9;
-    if (q + 6 > qlimit)
-        return false;
-    const double *r6 = buf + (q - qbase);
-    q += 6;
-    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
-    for (int i = 0; i < 3; ++i) {
-        const double x = 2. * r6[i] - 1., y = 2. * r6[3 + i] - 1.;
-        double r2 = (0. + x * x) + y * y;
-        while (r2 > 1. || r2 < min2) {
-            if (q + 2 > qlimit)
-                return false;
-            const double *v = buf + (q - qbase);
-            const double xa = 2. * v[0] - 1., ya = 2. * v[1] - 1.;
-            q += 2;
-            r2 = 0. + (xa * xa + ya * ya);
+    if (q + 111 <= qlimit) { // words up to position q + 15 + 95 lie inside the stored range
+        const uint64_t r3 = q + 9 - qbase, r1 = q + 15 - qbase;
+        const uint32_t *p3 = acc3 + (r3 >> 5), *p1 = acc1 + (r1 >> 5);
+        const uint32_t s3 = (uint32_t)(r3 & 31), s1 = (uint32_t)(r1 & 31);
+        const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> s3) & 7u;
+        const uint64_t lo = ((uint64_t)p1[1] << 32) | p1[0];
+        const uint64_t w1 = s1 ? (lo >> s1) | ((uint64_t)p1[2] << (64 - s1)) : lo;
+        const uint32_t fails = 3u - (uint32_t)__popc(ok3);
+        if (fails == 0) {
+            q += 15;
+            return true;
+        }
+        const uint64_t y0 = w1 & 0x5555555555555555ull, y1 = y0 & (y0 - 1), y2 = y1 & (y1 - 1);
+        const uint64_t y = fails == 1 ? y0 : fails == 2 ? y1 : y2;
+        if (y) {
+            q += 17 + (uint64_t)__builtin_ctzll(y);
+            return true;
         }
     }
-    return true;
-}
-
-// The same on the xoshiro128+ stream of the all-f32 mode: the unit-disc test
-// in f32, whose MIN_POSITIVE squared underflows to 0 (only r2 > 1 re-rolls).
-__device__ __forceinline__ bool xo128_walk_event(const float *buf, uint64_t qbase, uint64_t qlimit, uint64_t &q)
-{
     q += 9;
     if (q + 6 > qlimit)
         return false;
-    const float *r6 = buf + (q - qbase);
+    const uint64_t r = q - qbase;
     q += 6;
     for (int i = 0; i < 3; ++i) {
-        const float x = 2.f * r6[i] - 1.f, y = 2.f * r6[3 + i] - 1.f;
-        float r2 = (0.f + x * x) + y * y;
-        while (r2 > 1.f) {
+        if (acc_bit(acc3, r + i))
+            continue;
+        for (;;) {
             if (q + 2 > qlimit)
                 return false;
-            const float *v = buf + (q - qbase);
-            const float xa = 2.f * v[0] - 1.f, ya = 2.f * v[1] - 1.f;
+            const uint32_t ok = acc_bit(acc1, q - qbase);
             q += 2;
-            r2 = 0.f + (xa * xa + ya * ya);
+            if (ok)
+                break;
         }
     }
     return true;
@@ -648,10 +655,8 @@ __device__ __forceinline__ bool xo128_walk_event(const float *buf, uint64_t qbas
 // (0) or f32 (1), the xoshiro256+ stream (2) or the xoshiro128+ one (3).
 __device__ __forceinline__ bool fe_walk_any(const FeWalkArgs &a, uint64_t &q)
 {
-    if (a.kind == 2)
-        return xo_walk_event(a.xbuf, a.qbase, a.qlimit, q);
-    if (a.kind == 3)
-        return xo128_walk_event(a.x32buf, a.qbase, a.qlimit, q);
+    if (a.kind >= 2)
+        return xo_walk_event(a.acc1, a.acc3, a.qbase, a.qlimit, q);
     return a.kind == 1 ? fe_walk_event<float>(a, q) : fe_walk_event<double>(a, q);
 }
 
@@ -750,9 +755,9 @@ __device__ __forceinline__ FeWalkArgs fe_batch_view(const FeBatchWalkArgs &a, ui
 {
     FeWalkArgs w;
     w.buf = a.buf;
-    w.xbuf = a.xbuf;
-    w.x32buf = a.x32buf;
     w.kind = a.kind;
+    w.acc1 = a.acc1;
+    w.acc3 = a.acc3;
     w.qbase = 0;
     w.qlimit = (k + 1) * a.piece_stride;
     return w;
@@ -931,28 +936,100 @@ __global__ __launch_bounds__(64) void xo_piece_state_kernel(XoPieceArgs a)
         a.states[4 * k + w] = s[w];
 }
 
-// Each thread expands per_thread consecutive uniforms of one piece from the
-// piece's state skipped to its first draw.
-__global__ __launch_bounds__(64) void xo_expand_kernel(XoPieceArgs a)
+// Outcome of the unit-disc test on the point (x, y) = 2u - 1 of two
+// uniforms: r2 above 1 re-rolls, and in f64 so does r2 below MIN_POSITIVE^2
+// (f32's MIN_POSITIVE^2 is 0).  The walk sums (0 + x*x) + y*y for the six
+// values' points and 0 + (x*x + y*y) for a re-roll: without contraction both
+// are fl(fl(x*x) + fl(y*y)).
+__device__ __forceinline__ uint32_t disc_accept(double x, double y)
 {
-    const uint64_t per_piece = a.piece_len / a.per_thread;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nb * per_piece)
-        return;
-    const uint64_t k = t / per_piece, j = t % per_piece;
-    uint64_t s[4];
-#pragma unroll
-    for (int w = 0; w < 4; ++w)
-        s[w] = a.states[4 * k + w];
-    const uint64_t off = j * a.per_thread;
-    for (int i = 0; i < a.km; ++i)
-        if ((off >> i) & 1ull)
-            gf2_apply(a.mt + 1024 * i, s);
-    double *out = a.xbuf + k * a.piece_len + off;
-    for (uint32_t i = 0; i < a.per_thread; ++i)
-        out[i] = xo_uniform(s);
+    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    const double r2 = (0. + x * x) + y * y;
+    return (r2 > 1. || r2 < min2) ? 0u : 1u;
+}
+__device__ __forceinline__ uint32_t disc_accept(float x, float y)
+{
+    const float r2 = (0.f + x * x) + y * y;
+    return r2 > 1.f ? 0u : 1u;
 }
 
+struct Xo256Gen {
+    uint64_t s[4];
+    __device__ __forceinline__ double next() { return xo_uniform(s); }
+};
+struct Xo128Gen {
+    uint32_t s[4];
+    __device__ __forceinline__ float next() { return xo128_uniform(s); }
+};
+
+// One wavefront per piece.  Lane l expands positions [l * pt, (l + 1) * pt)
+// of the piece from its state skipped l * pt draws, 32 at a time, with the
+// draws of positions p + 1 .. p + 3 in hand when position p is stored, so
+// that p's accept bits (with p + 1 and with p + 3) are known; 3 draws past the
+// lane's range complete its last bits.  Each block of 32 values goes out
+// through LDS: a store then writes two runs of 32 consecutive positions (two
+// lanes' blocks) instead of 64 scattered values.  The bit words go out 4 at a
+// time (128 positions) per lane.
+template <class T, class Gen>
+__device__ __forceinline__ void xo_expand_piece(Gen g, T *piece, uint32_t *acc1, uint32_t *acc3, uint32_t pt)
+{
+    constexpr int B = 32, ROW = B + 1; // a padded LDS row per lane
+    __shared__ T st[64 * ROW];
+    __shared__ uint4 sw[2 * 64];       // the lane's bit words of the current 128 positions
+    const uint32_t lane = threadIdx.x;
+    T *row = st + lane * ROW;
+    uint32_t *m1 = reinterpret_cast<uint32_t *>(sw + lane), *m3 = reinterpret_cast<uint32_t *>(sw + 64 + lane);
+    uint4 *w1 = reinterpret_cast<uint4 *>(acc1 + (uint64_t)lane * (pt / 32));
+    uint4 *w3 = reinterpret_cast<uint4 *>(acc3 + (uint64_t)lane * (pt / 32));
+    T u0 = g.next(), u1 = g.next(), u2 = g.next(); // positions p, p + 1, p + 2
+    for (uint32_t grp = 0; grp < pt / (4 * B); ++grp) {
+#pragma unroll 1
+        for (int b = 0; b < 4; ++b) {
+            uint32_t b1 = 0, b3 = 0;
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                const T u3 = g.next();
+                const T x0 = (T)2. * u0 - (T)1.;
+                b1 |= disc_accept(x0, (T)2. * u1 - (T)1.) << i;
+                b3 |= disc_accept(x0, (T)2. * u3 - (T)1.) << i;
+                row[i] = u0;
+                u0 = u1;
+                u1 = u2;
+                u2 = u3;
+            }
+            m1[b] = b1;
+            m3[b] = b3;
+            __syncthreads();
+            const uint64_t pos = (uint64_t)grp * (4 * B) + (uint64_t)b * B;
+#pragma unroll 4
+            for (int r = 0; r < B; ++r) {
+                const uint32_t src = 2 * r + (lane >> 5), i = lane & 31;
+                piece[(uint64_t)src * pt + pos + i] = st[src * ROW + i];
+            }
+            __syncthreads();
+        }
+        w1[grp] = sw[lane];
+        w3[grp] = sw[64 + lane];
+    }
+}
+
+// Skip of the lane's state to its first draw: M^(lane * pt) as the product
+// of M^(pt 2^j) over the set bits j of the lane index, six matrices that every
+// lane reads alike.
+__global__ __launch_bounds__(64) void xo_expand_kernel(XoPieceArgs a)
+{
+    const uint64_t k = blockIdx.x;
+    Xo256Gen g;
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+        g.s[w] = a.states[4 * k + w];
+#pragma unroll 1
+    for (int j = 0; j < 6; ++j)
+        if ((threadIdx.x >> j) & 1u)
+            gf2_apply(a.lt + 1024 * j, g.s);
+    const uint64_t words = a.piece_len / 32;
+    xo_expand_piece<double>(g, a.xbuf + k * a.piece_len, a.acc1 + k * words, a.acc3 + k * words, a.per_thread);
+}
 // xoshiro128+ pieces, as the xoshiro256+ ones above.
 __global__ __launch_bounds__(64) void xo128_piece_state_kernel(Xo128PieceArgs a)
 {
@@ -971,36 +1048,29 @@ __global__ __launch_bounds__(64) void xo128_piece_state_kernel(Xo128PieceArgs a)
 
 __global__ __launch_bounds__(64) void xo128_expand_kernel(Xo128PieceArgs a)
 {
-    const uint64_t per_piece = a.piece_len / a.per_thread;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nb * per_piece)
-        return;
-    const uint64_t k = t / per_piece, j = t % per_piece;
-    uint32_t s[4];
+    const uint64_t k = blockIdx.x;
+    Xo128Gen g;
 #pragma unroll
     for (int w = 0; w < 4; ++w)
-        s[w] = a.states[4 * k + w];
-    const uint64_t off = j * a.per_thread;
-    for (int i = 0; i < a.km; ++i)
-        if ((off >> i) & 1ull)
-            gf2_apply128(a.mt + 512 * i, s);
-    float *out = a.xbuf + k * a.piece_len + off;
-    for (uint32_t i = 0; i < a.per_thread; ++i)
-        out[i] = xo128_uniform(s);
+        g.s[w] = a.states[4 * k + w];
+#pragma unroll 1
+    for (int j = 0; j < 6; ++j)
+        if ((threadIdx.x >> j) & 1u)
+            gf2_apply128(a.lt + 512 * j, g.s);
+    const uint64_t words = a.piece_len / 32;
+    xo_expand_piece<float>(g, a.xbuf + k * a.piece_len, a.acc1 + k * words, a.acc3 + k * words, a.per_thread);
 }
 
 void launch_xo128_expand(const Xo128PieceArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(xo128_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
-    const uint64_t threads = a.nb * (a.piece_len / a.per_thread);
-    hipLaunchKernelGGL(xo128_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(xo128_expand_kernel, dim3((unsigned)a.nb), dim3(64), 0, st, a);
 }
 
 void launch_xo_expand(const XoPieceArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(xo_piece_state_kernel, dim3((unsigned)((a.nb + 63) / 64)), dim3(64), 0, st, a);
-    const uint64_t threads = a.nb * (a.piece_len / a.per_thread);
-    hipLaunchKernelGGL(xo_expand_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(xo_expand_kernel, dim3((unsigned)a.nb), dim3(64), 0, st, a);
 }
 
 
