@@ -375,7 +375,7 @@ constexpr uint64_t FE_XO_EVENTS_PER_WALKER = 1024, FE_XO_OVERLAP = 17500;
 // time).  With jump seeding a piece is a batch's stream; with skip-ahead
 // seeding the pieces tile the run's one stream (piece p starts FE_XO_PIECE p
 // draws after the run's state).
-constexpr uint32_t FE_XO_PER_THREAD = 3200;
+constexpr uint32_t FE_XO_PER_THREAD = 2688;
 constexpr uint64_t FE_XO_PIECE = 64 * FE_XO_PER_THREAD;
 constexpr uint64_t FE_XO_WORDS = FE_XO_PIECE / 32; // accept-bit words per piece and test
 static_assert(FE_XO_PER_THREAD % 128 == 0, "whole groups of 4 bit words per lane");
@@ -982,10 +982,10 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
 // standard-random): batch b's stream starts at J^b s0 (J the 2^128 jump,
 // standard.rs:51-55) and random_array::<N> takes N consecutive draws with no
 // round to discard (standard.rs:33-35).  Each chunk's batches are expanded
-// into HBM as f64 uniforms, one piece per batch, walked one thread per batch,
-// and simulated from those starts.  A batch consumes about 166 000 draws
-// (16.6 per event); a piece holds 200 000.
-
+// into HBM as f64 uniforms with their accept bits, one piece per batch,
+// walked by segment walkers, and simulated from those starts.  A batch of
+// 10 000 events consumes 166 400 draws on average (16.64 per event, standard
+// deviation about 204 per batch); a piece holds 172 032.
 
 int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_state *x, uint64_t num_events,
                                  uint32_t batch_size, void *per_batch_dev)

@@ -651,6 +651,88 @@ __device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32
     return true;
 }
 
+// The xoshiro walkers' accept bits through LDS.  Bits read straight from
+// HBM cost each event five scattered loads, and a compute unit's walkers
+// touch more lines than its L1 holds.  So each walking lane keeps XO_WIN
+// words of acc1 and acc3 (XO_WIN * 32 positions) from the word of its next
+// event's first test on in rows of its own, and takes its events' tests from
+// there; the wavefront refills every lane's rows (16-byte loads) when some
+// lane's next event could leave its window.  No lane reads another's rows.
+constexpr int XO_WIN = 32, XO_ROW = XO_WIN + 4, XO_WALK_THREADS = 128;
+
+struct XoWin {
+    uint32_t *w1, *w3; // the lane's rows
+    uint64_t wb;       // first word held (counted from qbase)
+    uint64_t end;      // stream position where the held bits end
+};
+
+// Words from the one holding position q + 9 on, in whole groups of 4 below
+// qlimit.
+__device__ __forceinline__ void xo_win_fill(XoWin &v, const uint32_t *acc1, const uint32_t *acc3, uint64_t qbase,
+                                            uint64_t qlimit, uint64_t q)
+{
+    const uint64_t lim4 = ((qlimit - qbase) >> 5) & ~3ull;
+    const uint64_t wb = ((q + 9 - qbase) >> 5) & ~3ull;
+    uint64_t have = 0;
+#pragma unroll
+    for (int i = 0; i < XO_WIN / 4; ++i) {
+        const uint64_t w = wb + 4 * i;
+        if (w < lim4) {
+            *reinterpret_cast<uint4 *>(v.w1 + 4 * i) = *reinterpret_cast<const uint4 *>(acc1 + w);
+            *reinterpret_cast<uint4 *>(v.w3 + 4 * i) = *reinterpret_cast<const uint4 *>(acc3 + w);
+            have = 4 * i + 4;
+        }
+    }
+    v.wb = wb;
+    v.end = qbase + 32 * (wb + have);
+}
+
+// xo_walk_event with the tests from the lane's window when the event's bits
+// (as far as position q + 110) lie in it; otherwise from acc1 / acc3.
+__device__ __forceinline__ bool xo_walk_event_win(const XoWin &v, const uint32_t *acc1, const uint32_t *acc3,
+                                                  uint64_t qbase, uint64_t qlimit, uint64_t &q)
+{
+    if (q + 111 <= v.end) {
+        const uint64_t r3 = q + 9 - qbase, r1 = q + 15 - qbase;
+        const uint32_t *p3 = v.w3 + ((r3 >> 5) - v.wb), *p1 = v.w1 + ((r1 >> 5) - v.wb);
+        const uint32_t s3 = (uint32_t)(r3 & 31), s1 = (uint32_t)(r1 & 31);
+        const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> s3) & 7u;
+        const uint64_t lo = ((uint64_t)p1[1] << 32) | p1[0];
+        const uint64_t w1 = s1 ? (lo >> s1) | ((uint64_t)p1[2] << (64 - s1)) : lo;
+        const uint32_t fails = 3u - (uint32_t)__popc(ok3);
+        if (fails == 0) {
+            q += 15;
+            return true;
+        }
+        const uint64_t y0 = w1 & 0x5555555555555555ull, y1 = y0 & (y0 - 1), y2 = y1 & (y1 - 1);
+        const uint64_t y = fails == 1 ? y0 : fails == 2 ? y1 : y2;
+        if (y) {
+            q += 17 + (uint64_t)__builtin_ctzll(y);
+            return true;
+        }
+    }
+    return xo_walk_event(acc1, acc3, qbase, qlimit, q);
+}
+
+// Records starts from q while q < stop and the list has room (each event
+// consumes at least 15 positions, so the loop ends); returns the count.
+__device__ __forceinline__ uint32_t xo_walk_run(XoWin &v, const uint32_t *acc1, const uint32_t *acc3, uint64_t qbase,
+                                                uint64_t qlimit, uint64_t q, uint64_t stop, uint64_t *list,
+                                                uint32_t cap)
+{
+    v.wb = 0;
+    v.end = 0;
+    uint32_t n = 0;
+    while (q < stop && n < cap) {
+        list[n++] = q;
+        if (__any(q + 111 > v.end && q + 111 <= qlimit))
+            xo_win_fill(v, acc1, acc3, qbase, qlimit, q);
+        if (!xo_walk_event_win(v, acc1, acc3, qbase, qlimit, q))
+            break;
+    }
+    return n;
+}
+
 // One event of the walk of kind a.kind: RANF with the unit-disc test in f64
 // (0) or f32 (1), the xoshiro256+ stream (2) or the xoshiro128+ one (3).
 __device__ __forceinline__ bool fe_walk_any(const FeWalkArgs &a, uint64_t &q)
@@ -678,6 +760,21 @@ __global__ __launch_bounds__(256) void fe_walk_kernel(FeWalkArgs a)
             break;
     }
     a.counts[k] = n;
+}
+
+// The same on the xoshiro streams (kinds 2, 3), with the bits in LDS windows.
+__global__ __launch_bounds__(XO_WALK_THREADS) void fe_xo_walk_kernel(FeWalkArgs a)
+{
+    __shared__ uint32_t win[2 * XO_WALK_THREADS * XO_ROW];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nwalkers)
+        return;
+    XoWin v;
+    v.w1 = win + threadIdx.x * XO_ROW;
+    v.w3 = win + (XO_WALK_THREADS + threadIdx.x) * XO_ROW;
+    const uint64_t q = k == 0 ? a.q_start : a.seg_origin + (uint64_t)k * a.seg;
+    const uint64_t stop = a.seg_origin + (uint64_t)(k + 1) * a.seg + a.overlap;
+    a.counts[k] = xo_walk_run(v, a.acc1, a.acc3, a.qbase, a.qlimit, q, stop, a.lists + (size_t)k * a.cap, a.cap);
 }
 
 // Walker k's first start that walker k-1 also reached past segment k's start:
@@ -722,7 +819,11 @@ __global__ __launch_bounds__(256) void fe_join_kernel(FeWalkArgs a)
 void launch_fe_walk(const FeWalkArgs &a, hipStream_t st)
 {
     const unsigned blocks = (a.nwalkers + 255) / 256;
-    hipLaunchKernelGGL(fe_walk_kernel, dim3(blocks), dim3(256), 0, st, a);
+    if (a.kind >= 2)
+        hipLaunchKernelGGL(fe_xo_walk_kernel, dim3((a.nwalkers + XO_WALK_THREADS - 1) / XO_WALK_THREADS),
+                           dim3(XO_WALK_THREADS), 0, st, a);
+    else
+        hipLaunchKernelGGL(fe_walk_kernel, dim3(blocks), dim3(256), 0, st, a);
     hipLaunchKernelGGL(fe_join_kernel, dim3(blocks), dim3(256), 0, st, a);
 }
 
@@ -785,6 +886,24 @@ __global__ __launch_bounds__(256) void fe_seg_walk_kernel(FeBatchWalkArgs a)
             break;
     }
     a.counts[t] = n;
+}
+
+// The same on xoshiro pieces (kinds 2, 3), with the bits in LDS windows.
+__global__ __launch_bounds__(XO_WALK_THREADS) void fe_seg_xo_walk_kernel(FeBatchWalkArgs a)
+{
+    __shared__ uint32_t win[2 * XO_WALK_THREADS * XO_ROW];
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nb * a.S)
+        return;
+    const uint64_t k = t / a.S;
+    const uint32_t s = (uint32_t)(t % a.S);
+    const uint64_t base = k * a.piece_stride;
+    const uint64_t q = base + (uint64_t)s * a.seg;
+    const uint64_t stop = min(base + (uint64_t)(s + 1) * a.seg + a.overlap, base + a.piece_stride);
+    XoWin v;
+    v.w1 = win + threadIdx.x * XO_ROW;
+    v.w3 = win + (XO_WALK_THREADS + threadIdx.x) * XO_ROW;
+    a.counts[t] = xo_walk_run(v, a.acc1, a.acc3, 0, base + a.piece_stride, q, stop, a.lists + t * a.cap, a.cap);
 }
 
 // Walker s's first start that walker s - 1 of the same batch also reached
@@ -912,7 +1031,11 @@ void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st)
 {
     const uint64_t walkers = a.nb * a.S;
     const unsigned wb = (unsigned)((walkers + 255) / 256), bb = (unsigned)((a.nb + 63) / 64);
-    hipLaunchKernelGGL(fe_seg_walk_kernel, dim3(wb), dim3(256), 0, st, a);
+    if (a.kind >= 2)
+        hipLaunchKernelGGL(fe_seg_xo_walk_kernel, dim3((unsigned)((walkers + XO_WALK_THREADS - 1) / XO_WALK_THREADS)),
+                           dim3(XO_WALK_THREADS), 0, st, a);
+    else
+        hipLaunchKernelGGL(fe_seg_walk_kernel, dim3(wb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(fe_seg_join_kernel, dim3(wb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(fe_seg_assemble_kernel, dim3((unsigned)a.nb), dim3(64), 0, st, a);
     hipLaunchKernelGGL(fe_seq_walk_kernel, dim3(bb), dim3(64), 0, st, a);
@@ -963,17 +1086,19 @@ struct Xo128Gen {
 };
 
 // One wavefront per piece.  Lane l expands positions [l * pt, (l + 1) * pt)
-// of the piece from its state skipped l * pt draws, 32 at a time, with the
-// draws of positions p + 1 .. p + 3 in hand when position p is stored, so
-// that p's accept bits (with p + 1 and with p + 3) are known; 3 draws past the
-// lane's range complete its last bits.  Each block of 32 values goes out
-// through LDS: a store then writes two runs of 32 consecutive positions (two
-// lanes' blocks) instead of 64 scattered values.  The bit words go out 4 at a
-// time (128 positions) per lane.
+// of the piece from its state skipped l * pt draws, one 128-byte line of
+// values (16 f64 or 32 f32) at a time, with the draws of positions
+// p + 1 .. p + 3 in hand when position p is stored, so that p's accept bits
+// (with p + 1 and with p + 3) are known; 3 draws past the lane's range
+// complete its last bits.  Each block goes out through LDS: a store then
+// writes whole lines (4 lanes' blocks of f64, 2 of f32) instead of 64
+// scattered values.  The bit words go out 4 at a time (128 positions) per
+// lane.
 template <class T, class Gen>
 __device__ __forceinline__ void xo_expand_piece(Gen g, T *piece, uint32_t *acc1, uint32_t *acc3, uint32_t pt)
 {
-    constexpr int B = 32, ROW = B + 1; // a padded LDS row per lane
+    constexpr int B = 128 / sizeof(T), ROW = B + 1; // values per block (a 128-byte line), padded LDS rows
+    constexpr int NB = 128 / B;        // blocks per group of 4 bit words
     __shared__ T st[64 * ROW];
     __shared__ uint4 sw[2 * 64];       // the lane's bit words of the current 128 positions
     const uint32_t lane = threadIdx.x;
@@ -982,9 +1107,9 @@ __device__ __forceinline__ void xo_expand_piece(Gen g, T *piece, uint32_t *acc1,
     uint4 *w1 = reinterpret_cast<uint4 *>(acc1 + (uint64_t)lane * (pt / 32));
     uint4 *w3 = reinterpret_cast<uint4 *>(acc3 + (uint64_t)lane * (pt / 32));
     T u0 = g.next(), u1 = g.next(), u2 = g.next(); // positions p, p + 1, p + 2
-    for (uint32_t grp = 0; grp < pt / (4 * B); ++grp) {
+    for (uint32_t grp = 0; grp < pt / 128; ++grp) {
 #pragma unroll 1
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < NB; ++b) {
             uint32_t b1 = 0, b3 = 0;
 #pragma unroll
             for (int i = 0; i < B; ++i) {
@@ -997,13 +1122,15 @@ __device__ __forceinline__ void xo_expand_piece(Gen g, T *piece, uint32_t *acc1,
                 u1 = u2;
                 u2 = u3;
             }
-            m1[b] = b1;
-            m3[b] = b3;
+            const int wi = b * B / 32, sh = b * B % 32;
+            m1[wi] = sh ? m1[wi] | (b1 << sh) : b1;
+            m3[wi] = sh ? m3[wi] | (b3 << sh) : b3;
             __syncthreads();
-            const uint64_t pos = (uint64_t)grp * (4 * B) + (uint64_t)b * B;
+            // each store: 64 / B lanes' blocks, B consecutive positions each
+            const uint64_t pos = (uint64_t)grp * 128 + (uint64_t)b * B;
 #pragma unroll 4
             for (int r = 0; r < B; ++r) {
-                const uint32_t src = 2 * r + (lane >> 5), i = lane & 31;
+                const uint32_t src = r * (64 / B) + lane / B, i = lane % B;
                 piece[(uint64_t)src * pt + pos + i] = st[src * ROW + i];
             }
             __syncthreads();
@@ -1241,8 +1368,9 @@ void launch_merge(const void *per_batch, uint64_t nb, void *merged, hipStream_t 
 
 // ---- all-f32 mode --------------------------------------------------------------
 // One wavefront per batch (grid-stride).  Each lane takes a contiguous run of
-// the batch's events (one skip-ahead of the xoshiro128+ stream per run) and
-// writes each event's m[5] and cut decision; the batch sums are then formed
+// the batch's events (one skip-ahead of the xoshiro128+ stream per run; with
+// faster-evgen every 64th event instead) and writes each event's m[5] and
+// cut decision; the batch sums are then formed
 // in event order, as the reference's sequential loop does in f32
 // (resacc.rs:121-129), where a tree sum would round differently.
 // Events that pass the cuts are queued in LDS (momenta and event index) and
@@ -1279,7 +1407,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
         const uint32_t n = left < a.batch_size ? (uint32_t)left : a.batch_size;
         const uint32_t q = n / 64, r = n % 64;
         const uint32_t len = q + (lane < r ? 1u : 0u);
-        const uint32_t e0 = lane * q + (lane < r ? lane : r);
+        // a contiguous run of events per lane (one skip-ahead of the stream
+        // per run) for the sources that draw 12 values per event; with
+        // faster-evgen the events lane, lane + 64, ... (their draws lie side
+        // by side across the wavefront)
+        constexpr bool FE = SOURCE == 2 || SOURCE == 3;
+        const uint32_t e0 = FE ? lane : lane * q + (lane < r ? lane : r);
+        const uint32_t estep = FE ? 64u : 1u;
         uint32_t s[4] = {a.s0[0], a.s0[1], a.s0[2], a.s0[3]};
         if (SOURCE == 0) {
             // M^(12 e0) B^b s0; the next batch of the grid-stride loop is B^grid
@@ -1309,10 +1443,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
         const uint32_t iters = q + (r ? 1u : 0u);
         uint32_t qn = 0; // queued events (uniform)
         for (uint32_t i = 0; i < iters; ++i) {
-            const uint64_t e = lb * a.batch_size + e0 + i;
+            const uint64_t e = lb * a.batch_size + e0 + (uint64_t)i * estep;
             bool pass = false;
             float p[3][4];
-            if (i < len && (SOURCE == 2 || SOURCE == 3)) {
+            if (i < len && FE) {
                 if constexpr (SOURCE == 2) {
                     RanfWalkF32 w{a.draws, a.qbase, a.evq[e]};
                     pass = generate_faster_f32(a.P, w, p);
