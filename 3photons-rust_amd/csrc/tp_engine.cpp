@@ -379,14 +379,39 @@ constexpr uint32_t FE_XO_PER_THREAD = 2688;
 constexpr uint64_t FE_XO_PIECE = 64 * FE_XO_PER_THREAD;
 constexpr uint64_t FE_XO_WORDS = FE_XO_PIECE / 32; // accept-bit words per piece and test
 static_assert(FE_XO_PER_THREAD % 128 == 0, "whole groups of 4 bit words per lane");
+// Accept-bit words per RANF expansion segment (each segment's bits start on
+// a word of their own).
+constexpr uint64_t RANF_SEG_WORDS = 55ull * RANF_R / 32;
+static_assert(55ull * RANF_R % 32 == 0, "RANF segments of whole bit words");
 
 namespace {
 
-// RANF values of stream positions [q_first, q_last) into c->draws (position
-// q_first at index 0), from the master origin uploaded by the caller.
-int fe_expand(Ctx *c, uint64_t p0_master, int K, uint64_t q_first, uint64_t q_last)
+// Accept-bit arrays of `need` words each in c->xo_acc: acc1, then acc3 at
+// c->xo_acc_words words (sized for the largest need seen, so that the split
+// stays put while the buffer is reused).
+int ensure_acc_words(Ctx *c, uint64_t need)
 {
+    const uint64_t words = std::max(need, c->xo_acc_words);
+    int rc = ensure(c, c->xo_acc, 2 * words * sizeof(uint32_t));
+    if (rc)
+        return rc;
+    c->xo_acc_words = words;
+    return TP_OK;
+}
+
+// The same for np xoshiro pieces.
+int ensure_acc(Ctx *c, uint64_t np) { return ensure_acc_words(c, np * FE_XO_WORDS); }
+
+// RANF values of stream positions [q_first, q_last) into c->draws (position
+// q_first at index 0), from the master origin uploaded by the caller, and
+// their accept bits (the unit-disc test of stream kind 0 or 1) into c->xo_acc
+// from *bbase, the first position of the first expansion segment, on.
+int fe_expand(Ctx *c, uint64_t p0_master, int K, uint64_t q_first, uint64_t q_last, int kind, uint64_t *bbase)
+{
+    const uint64_t seg_first = q_first / 55 / RANF_R, nsegs = (q_last - 1) / 55 / RANF_R - seg_first + 1;
     int rc = ensure(c, c->draws, (q_last - q_first) * sizeof(int32_t));
+    if (!rc)
+        rc = ensure_acc_words(c, nsegs * RANF_SEG_WORDS);
     if (rc)
         return rc;
     tpk::RanfArgs r;
@@ -399,11 +424,15 @@ int fe_expand(Ctx *c, uint64_t p0_master, int K, uint64_t q_first, uint64_t q_la
     r.out = static_cast<int32_t *>(c->draws.p);
     r.d_begin = q_first - p0_master;
     r.d_end = q_last - p0_master;
-    r.seg_first = q_first / 55 / RANF_R;
-    r.nsegs = (q_last - 1) / 55 / RANF_R - r.seg_first + 1;
+    r.seg_first = seg_first;
+    r.nsegs = nsegs;
     r.piece_stride = 0;
+    r.bits = kind == 1 ? 2 : 1;
+    r.acc1 = static_cast<uint32_t *>(c->xo_acc.p);
+    r.acc3 = r.acc1 + c->xo_acc_words;
     tpk::launch_ranf_expand(r, 1, c->stream);
     HIPCHK(c, hipGetLastError());
+    *bbase = seg_first * 55 * RANF_R;
     return TP_OK;
 }
 
@@ -416,18 +445,6 @@ struct FeStream {
     const uint32_t *s0_128 = nullptr; // xoshiro128+ (kind 3): the run's state (position 0)
 };
 
-// Accept-bit arrays for np xoshiro pieces in c->xo_acc: acc1, then acc3 at
-// c->xo_acc_words words (sized for the largest np seen, so that the split
-// stays put while the buffer is reused).
-int ensure_acc(Ctx *c, uint64_t np)
-{
-    const uint64_t words = std::max(np * FE_XO_WORDS, c->xo_acc_words);
-    int rc = ensure(c, c->xo_acc, 2 * words * sizeof(uint32_t));
-    if (rc)
-        return rc;
-    c->xo_acc_words = words;
-    return TP_OK;
-}
 
 // M^(FE_XO_PER_THREAD 2^j), j < 6, of both xoshiro generators: the expanding
 // lanes' skips (lane l skips l * FE_XO_PER_THREAD draws into its piece).
@@ -509,14 +526,15 @@ int ensure_xo_piece_table(Ctx *c, uint64_t max_piece, Tables **out)
 }
 
 // The stream of positions [q_first, q_last) into HBM: RANF values into
-// c->draws from q_first on, or xoshiro256+ uniforms into c->xo_draws from the
-// first piece boundary at or before q_first.  *qbase is the position held at
-// index 0.
-int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_last, uint64_t *qbase)
+// c->draws from q_first on, or xoshiro uniforms into c->xo_draws /
+// c->xo128_draws from the first piece boundary at or before q_first, with
+// their accept bits in c->xo_acc.  *qbase is the position held at index 0,
+// *bbase the position of bit 0.
+int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_last, uint64_t *qbase, uint64_t *bbase)
 {
     if (fs.kind < 2) {
         *qbase = q_first;
-        return fe_expand(c, fs.p0_master, fs.K, q_first, q_last);
+        return fe_expand(c, fs.p0_master, fs.K, q_first, q_last, fs.kind, bbase);
     }
     const uint64_t p_first = q_first / FE_XO_PIECE, p_end = (q_last + FE_XO_PIECE - 1) / FE_XO_PIECE;
     const uint64_t np = p_end - p_first;
@@ -526,7 +544,7 @@ int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_la
             rc = xo128_expand(c, fs.s0_128, p_first, np);
         if (rc)
             return rc;
-        *qbase = p_first * FE_XO_PIECE;
+        *qbase = *bbase = p_first * FE_XO_PIECE;
         return TP_OK;
     }
     Tables *t = nullptr;
@@ -557,7 +575,7 @@ int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_la
     p.acc3 = p.acc1 + c->xo_acc_words;
     tpk::launch_xo_expand(p, c->stream);
     HIPCHK(c, hipGetLastError());
-    *qbase = p_first * FE_XO_PIECE;
+    *qbase = *bbase = p_first * FE_XO_PIECE;
     return TP_OK;
 }
 
@@ -591,7 +609,7 @@ int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, boo
     // every event takes at least 15 positions: no walk holds more starts
     w.cap = (uint32_t)((w.seg + w.overlap + 63) / 15 + 2);
     w.qlimit = w.seg_origin + (uint64_t)T * w.seg + w.overlap + 128;
-    int rc = fe_stream_expand(c, fs, q_start, w.qlimit, &w.qbase);
+    int rc = fe_stream_expand(c, fs, q_start, w.qlimit, &w.qbase, &w.bbase);
     if (!rc)
         rc = ensure(c, c->fe_lists, (size_t)T * w.cap * sizeof(uint64_t));
     if (!rc)
@@ -689,7 +707,7 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         bool ok = false;
-        uint64_t q_next = 0, qbase = 0;
+        uint64_t q_next = 0, qbase = 0, bbase = 0;
         if (device_walk) {
             int rc = fe_device_walk(c, fs, q_start, n, &ok, &qbase);
             if (rc)
@@ -702,7 +720,7 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
             host_walk(q_start, n, c->fe_starts.data(), &q_next);
             int rc = ensure(c, c->evq, n * sizeof(uint64_t));
             if (!rc)
-                rc = fe_stream_expand(c, fs, q_start, q_next, &qbase);
+                rc = fe_stream_expand(c, fs, q_start, q_next, &qbase, &bbase);
             if (rc)
                 return rc;
             HIPCHK(c, hipMemcpyAsync(c->evq.p, c->fe_starts.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice,
@@ -815,6 +833,7 @@ int enqueue_xo_faster_evgen(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_state *x
 // that in any batch of 10 000 events); a piece holds 220 000.
 constexpr uint64_t FE_JUMP_PIECE = 55 * 4000;
 static_assert(FE_JUMP_PIECE % 55 == 0, "pieces keep every position's place in its round");
+static_assert(FE_JUMP_PIECE % 32 == 0, "pieces of whole accept-bit words");
 
 // Segment walkers for the batches of a chunk (jump seeding): S walkers per
 // batch of the stream kind's length and overlap, their lists and join
@@ -871,6 +890,8 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
         rc = ensure(c, c->fe_flag, sizeof(uint32_t));
     if (!rc && f32)
         rc = ensure(c, c->f32ev, chunk * batch_size * 6 * sizeof(float));
+    if (!rc) // each piece's bits: its segments' words, the last one cut at the piece's end
+        rc = ensure_acc_words(c, (chunk + 1) * (PS / 32) + RANF_SEG_WORDS);
     if (rc)
         return rc;
     tpk::F32Args fa;
@@ -901,6 +922,10 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
         r.seg_first = 0;
         r.nsegs = (max_rounds + RANF_R - 1) / RANF_R + 1;
         r.piece_stride = PS;
+        r.bits = f32 ? 2 : 1;
+        r.bits_piece_words = PS / 32;
+        r.acc1 = static_cast<uint32_t *>(c->xo_acc.p) + first_piece * (PS / 32);
+        r.acc3 = r.acc1 + c->xo_acc_words;
         tpk::launch_ranf_expand(r, npieces, c->stream);
         HIPCHK(c, hipGetLastError());
         return TP_OK;
@@ -934,6 +959,8 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
         if (rc)
             return rc;
         w.buf = static_cast<const int32_t *>(c->draws.p);
+        w.acc1 = static_cast<const uint32_t *>(c->xo_acc.p);
+        w.acc3 = w.acc1 + c->xo_acc_words;
         w.piece_stride = PS;
         w.p0 = static_cast<const int32_t *>(c->p0.p);
         w.nb = cnb;

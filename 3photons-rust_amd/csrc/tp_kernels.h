@@ -63,6 +63,13 @@ struct RanfArgs {
     int32_t K;
     int32_t R;             // rounds per segment
     int32_t *out;
+    // faster-evgen: accept bits of every round the launch generates (bits 1:
+    // the unit-disc test in f64, 2: in f32; 0: none), as FeWalkArgs.  Piece
+    // k's segment s fills the words from k * bits_piece_words + (s - seg_first)
+    // * 55 R / 32 on (55 R a multiple of 32), bit j = the segment's j-th position.
+    int32_t bits;
+    uint64_t bits_piece_words;
+    uint32_t *acc1, *acc3;
 };
 
 void launch_fused(const FusedArgs &a,
@@ -121,11 +128,13 @@ void launch_f32_accumulate(const F32Args &a, int grid, hipStream_t st);
 struct FeWalkArgs {
     const int32_t *buf;    // RANF values of positions qbase .. qlimit - 1 (kinds 0, 1)
     int32_t kind;          // 0: RANF, unit-disc test in f64; 1: RANF, in f32; 2: xoshiro256+; 3: xoshiro128+ (f32)
-    // kinds 2, 3: the walk reads accept bits, not uniforms.  Bit i of acc1
-    // (word i / 32, bit i % 32) is 1 when the point (2u - 1) of positions
-    // qbase + i and qbase + i + 1 passes the unit-disc test; acc3 likewise for
-    // qbase + i and qbase + i + 3 (the three points of each event's 6 values).
+    // The walk reads accept bits, not values.  Bit i of acc1 (word i / 32,
+    // bit i % 32) is 1 when the point (2u - 1) of positions bbase + i and
+    // bbase + i + 1 passes the unit-disc test; acc3 likewise for bbase + i and
+    // bbase + i + 3 (the three points of each event's 6 values).  buf (RANF)
+    // still serves the one-thread walk of a batch the segments leave uncovered.
     const uint32_t *acc1, *acc3;
+    uint64_t bbase;        // position of bit 0 of acc1 / acc3
     uint64_t qbase, qlimit;
     uint64_t q_start;
     uint64_t seg_origin, seg, overlap;
@@ -163,7 +172,7 @@ struct FeBatchWalkArgs {
     const int32_t *buf;    // RANF pieces (kinds 0, 1)
     int32_t kind;          // 0: RANF, unit-disc test in f64; 1: in f32 (all-f32 mode); 2: xoshiro256+;
                            // 3: xoshiro128+ (all-f32 mode)
-    const uint32_t *acc1, *acc3; // kinds 2, 3: accept bits of the pieces' positions, as FeWalkArgs
+    const uint32_t *acc1, *acc3; // accept bits of the pieces' positions G, as FeWalkArgs (bbase 0)
     uint64_t piece_stride;
     const int32_t *p0;     // [nb] frame position of each piece's first value (RANF)
     uint64_t nb, b_first, num_events;

@@ -493,7 +493,98 @@ __device__ __forceinline__ int32_t ranf_jump(int32_t x, const uint32_t *__restri
     return lane < 55 ? (int32_t)acc : 0;
 }
 
-__global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
+// Outcome of the unit-disc test on the point (x, y) = 2u - 1 of two
+// uniforms: r2 above 1 re-rolls, and in f64 so does r2 below MIN_POSITIVE^2
+// (f32's MIN_POSITIVE^2 is 0).  The walk sums (0 + x*x) + y*y for the six
+// values' points and 0 + (x*x + y*y) for a re-roll: without contraction both
+// are fl(fl(x*x) + fl(y*y)).
+__device__ __forceinline__ uint32_t disc_accept(double x, double y)
+{
+    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    const double r2 = (0. + x * x) + y * y;
+    return (r2 > 1. || r2 < min2) ? 0u : 1u;
+}
+__device__ __forceinline__ uint32_t disc_accept(float x, float y)
+{
+    const float r2 = (0.f + x * x) + y * y;
+    return r2 > 1.f ? 0u : 1u;
+}
+
+// A RANF value as the walk's unit-disc test sees it: x = 2 (v * 1e-9) - 1 in
+// the run's float type (ranf.rs:99), and its square.
+template <class T> __device__ __forceinline__ T ranf_sq(int32_t v)
+{
+    const T x = fma((T)v * (T)1e-9, (T)2., (T)-1.); // 2u is exact: one rounding either way
+    return x * x;
+}
+
+// Accept bits of a segment's rounds.  Lane i holds the square of position
+// 55 r + 54 - i (each value's square is computed once, as the next round's,
+// and carried).  The squares of the round and the next go to LDS in position
+// order, so that each lane reads its partners +1 and +3 at its own place plus
+// 1 and 3 (barriers order the lanes' LDS writes and reads: the workgroup is
+// the one wavefront).  Each round's two ballots, bit-reversed into position
+// order, wait in LDS; every 32 rounds (1760 positions, 55 words) lanes 0..54
+// assemble one word each and store them together.
+template <class T> struct RanfBits {
+    uint64_t *m1, *m3;  // [32] LDS: the block's rounds' bits, bit j = position 55 r + j
+    T *s;               // [112] LDS: squares of this round and the next, in position order
+    uint32_t *d1, *d3;  // the block's first words
+    uint32_t k;         // rounds of the block so far
+    T sq;               // this round's squares
+    __device__ __forceinline__ void start(int32_t x) { sq = ranf_sq<T>(x); }
+    __device__ __forceinline__ void round(int32_t xn)
+    {
+        const int lane = (int)(threadIdx.x & 63u);
+        const T sqn = ranf_sq<T>(xn);
+        const bool live = lane < 55;
+        const int j = 54 - lane; // this lane's place in the round
+        __syncthreads(); // the last round's reads are done
+        if (live) {
+            s[j] = sq;
+            s[55 + j] = sqn;
+        }
+        __syncthreads(); // the other lanes' squares are in
+        // r2 = fl(fl(x*x) + fl(y*y)), the walk's sums (0 + x*x) + y*y and
+        // 0 + (x*x + y*y); a point re-rolls when r2 > 1 (the f64 test's
+        // r2 < MIN_POSITIVE^2 compares with 0, which a sum of squares never
+        // is below)
+        const T r1 = sq + s[live ? j + 1 : 0], r3 = sq + s[live ? j + 3 : 0];
+        constexpr uint64_t LIVE = (1ull << 55) - 1;
+        const uint64_t b1 = __builtin_bitreverse64(__ballot(r1 <= (T)1.) & LIVE) >> 9;
+        const uint64_t b3 = __builtin_bitreverse64(__ballot(r3 <= (T)1.) & LIVE) >> 9;
+        if (lane == 0) {
+            m1[k] = b1;
+            m3[k] = b3;
+        }
+        sq = sqn;
+        if (++k == 32)
+            flush();
+    }
+    // words j of the block with 32 j < 55 k (those past the segment's last
+    // round hold positions no walk reads)
+    __device__ __forceinline__ void flush()
+    {
+        const uint32_t lane = threadIdx.x & 63u;
+        __syncthreads(); // lane 0's masks are in
+        if (lane < 55 && 32 * lane < 55 * k) {
+            const uint32_t r0 = 32 * lane / 55, b0 = 32 * lane % 55;
+            uint64_t w1 = m1[r0] >> b0, w3 = m3[r0] >> b0;
+            if (55 - b0 < 32) {
+                w1 |= m1[r0 + 1] << (55 - b0);
+                w3 |= m3[r0 + 1] << (55 - b0);
+            }
+            d1[lane] = (uint32_t)w1;
+            d3[lane] = (uint32_t)w3;
+        }
+        d1 += 55;
+        d3 += 55;
+        k = 0;
+        __syncthreads(); // before the next block's masks overwrite these
+    }
+};
+
+template <int BITS> __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
 {
     __shared__ int32_t win[112];
     const int lane = threadIdx.x;
@@ -510,6 +601,27 @@ __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
             x = ranf_jump(x, a.polys + 55 * k, win);
     int32_t *out = a.out + piece * a.piece_stride;
     const RanfLanes l = ranf_lanes();
+    __shared__ uint64_t bm[2][BITS ? 32 : 1];
+    using BT = typename std::conditional<BITS == 2, float, double>::type;
+    __shared__ BT bsq[BITS ? 112 : 1];
+    RanfBits<BT> bits;
+    if (BITS) {
+        bits.s = bsq;
+        const uint64_t w0 = piece * a.bits_piece_words + (seg - a.seg_first) * (55ull * (uint64_t)a.R / 32);
+        bits.m1 = bm[0];
+        bits.m3 = bm[1];
+        bits.d1 = a.acc1 + w0;
+        bits.d3 = a.acc3 + w0;
+        bits.k = 0;
+        bits.start(x);
+    }
+    // the round's accept bits (from it and the next round), then on to the next
+    auto advance = [&](int32_t x) -> int32_t {
+        const int32_t xn = ranf_next_round(x, l);
+        if (BITS)
+            bits.round(xn);
+        return xn;
+    };
     // X[lane+1] of round r is emitted at stream position 55 r + 54 - lane.
     // Rounds [r_begin, r_end) reach into [q_begin, q_end); of those,
     // [f_lo, f_hi) lie wholly inside and are stored without range checks.
@@ -522,7 +634,7 @@ __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
         const uint64_t qq = 55 * r + qq0;
         if (lane < 55 && qq >= q_begin && qq < q_end)
             out[qq - q_begin] = x;
-        x = ranf_next_round(x, l);
+        x = advance(x);
     }
     int32_t *row = out + (55 * f_lo - q_begin); // uniform
     const int off = 54 - lane;
@@ -530,7 +642,7 @@ __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
     for (uint32_t k = 0; k < nfull; ++k) {
         if (lane < 55)
             row[off] = x;
-        x = ranf_next_round(x, l);
+        x = advance(x);
         row += 55;
     }
     r = f_hi;
@@ -538,14 +650,21 @@ __global__ __launch_bounds__(64) void ranf_expand_kernel(RanfArgs a)
         const uint64_t qq = 55 * r + qq0;
         if (lane < 55 && qq >= q_begin && qq < q_end)
             out[qq - q_begin] = x;
-        x = ranf_next_round(x, l);
+        x = advance(x);
     }
+    if (BITS && bits.k)
+        bits.flush();
 }
 
 void launch_ranf_expand(const RanfArgs &a, uint64_t npieces, hipStream_t st)
 {
     const uint64_t blocks = npieces * a.nsegs;
-    hipLaunchKernelGGL(ranf_expand_kernel, dim3((unsigned)blocks), dim3(64), 0, st, a);
+    if (a.bits == 1)
+        hipLaunchKernelGGL(ranf_expand_kernel<1>, dim3((unsigned)blocks), dim3(64), 0, st, a);
+    else if (a.bits == 2)
+        hipLaunchKernelGGL(ranf_expand_kernel<2>, dim3((unsigned)blocks), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(ranf_expand_kernel<0>, dim3((unsigned)blocks), dim3(64), 0, st, a);
 }
 
 // ---- faster-evgen stream walk --------------------------------------------------
@@ -596,36 +715,48 @@ template <class T = double> __device__ __forceinline__ bool fe_walk_event(const 
 // Accept bit of stored position r (bit r % 32 of word r / 32).
 __device__ __forceinline__ uint32_t acc_bit(const uint32_t *bits, uint64_t r) { return (bits[r >> 5] >> (r & 31)) & 1u; }
 
+// Of the accepted re-roll pairs marked in y (bit k: the pair starting k
+// positions on), the fails-th one (fails in 1..3): clearing the lowest
+// fails - 1 set bits leaves it as the lowest.  0 when y has fewer.
+__device__ __forceinline__ uint64_t nth_accept(uint64_t y, uint32_t fails)
+{
+    const uint64_t y1 = y & (y - 1), y2 = y1 & (y1 - 1);
+    return fails == 1 ? y : fails == 2 ? y1 : y2;
+}
+
+// 64 bits of a bit array from bit r on: the words at w[0], w[1], w[2] hold
+// bits from (r & ~31) on.
+__device__ __forceinline__ uint64_t bits64(const uint32_t *w, uint64_t r)
+{
+    const uint32_t s = (uint32_t)(r & 31);
+    const uint64_t lo = ((uint64_t)w[1] << 32) | w[0];
+    return s ? (lo >> s) | ((uint64_t)w[2] << (64 - s)) : lo;
+}
+
 // One event's consumption of a xoshiro stream (xoshiro256+ in f64 or
 // xoshiro128+ in f32): 9, 6, then 2 per re-rolled point of the unit-disc
-// test, no discards.  The test's outcomes come from the accept bits of the
-// stored positions qbase .. qlimit - 1: point i of the 6 values is
-// (u_p, u_p+3) at p = q + 9 + i (acc3), a re-roll is (u_p, u_p+1) (acc1).
-// The re-rolls of the failed points form one run of pairs from q + 15 on, so
-// with F failed points the event ends after the F-th accepted pair.  Away
-// from qlimit that is found from 64 positions of acc1 bits at once: the
-// accepted pairs are the set bits at even offsets, and clearing the lowest
-// F - 1 of them leaves the F-th as the lowest.  Near qlimit, or when the 32
-// pairs hold fewer than F accepted ones, the pairs are taken one at a time.
-// Every bit read pairs positions below qlimit.  False when the stored range
-// ends inside the event.
-__device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32_t *acc3, uint64_t qbase,
+// test, no discards.  The test's outcomes come from the accept bits (bit
+// index = position - bbase): point i of the 6 values is (u_p, u_p+3) at
+// p = q + 9 + i (acc3), a re-roll is (u_p, u_p+1) (acc1).  The re-rolls of
+// the failed points form one run of pairs from q + 15 on, so with F failed
+// points the event ends after the F-th accepted pair, found from 64 positions
+// of acc1 bits at once (the pairs are the even offsets).  Near qlimit, or when
+// the 32 pairs hold fewer than F accepted ones, the pairs are taken one at a
+// time.  Every bit read pairs positions below qlimit.  False when the stored
+// range ends inside the event.
+__device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32_t *acc3, uint64_t bbase,
                                               uint64_t qlimit, uint64_t &q)
 {
     if (q + 111 <= qlimit) { // words up to position q + 15 + 95 lie inside the stored range
-        const uint64_t r3 = q + 9 - qbase, r1 = q + 15 - qbase;
-        const uint32_t *p3 = acc3 + (r3 >> 5), *p1 = acc1 + (r1 >> 5);
-        const uint32_t s3 = (uint32_t)(r3 & 31), s1 = (uint32_t)(r1 & 31);
-        const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> s3) & 7u;
-        const uint64_t lo = ((uint64_t)p1[1] << 32) | p1[0];
-        const uint64_t w1 = s1 ? (lo >> s1) | ((uint64_t)p1[2] << (64 - s1)) : lo;
+        const uint64_t r3 = q + 9 - bbase, r1 = q + 15 - bbase;
+        const uint32_t *p3 = acc3 + (r3 >> 5);
+        const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> (r3 & 31)) & 7u;
         const uint32_t fails = 3u - (uint32_t)__popc(ok3);
         if (fails == 0) {
             q += 15;
             return true;
         }
-        const uint64_t y0 = w1 & 0x5555555555555555ull, y1 = y0 & (y0 - 1), y2 = y1 & (y1 - 1);
-        const uint64_t y = fails == 1 ? y0 : fails == 2 ? y1 : y2;
+        const uint64_t y = nth_accept(bits64(acc1 + (r1 >> 5), r1) & 0x5555555555555555ull, fails);
         if (y) {
             q += 17 + (uint64_t)__builtin_ctzll(y);
             return true;
@@ -634,7 +765,7 @@ __device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32
     q += 9;
     if (q + 6 > qlimit)
         return false;
-    const uint64_t r = q - qbase;
+    const uint64_t r = q - bbase;
     q += 6;
     for (int i = 0; i < 3; ++i) {
         if (acc_bit(acc3, r + i))
@@ -642,7 +773,7 @@ __device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32
         for (;;) {
             if (q + 2 > qlimit)
                 return false;
-            const uint32_t ok = acc_bit(acc1, q - qbase);
+            const uint32_t ok = acc_bit(acc1, q - bbase);
             q += 2;
             if (ok)
                 break;
@@ -651,31 +782,91 @@ __device__ __forceinline__ bool xo_walk_event(const uint32_t *acc1, const uint32
     return true;
 }
 
-// The xoshiro walkers' accept bits through LDS.  Bits read straight from
-// HBM cost each event five scattered loads, and a compute unit's walkers
-// touch more lines than its L1 holds.  So each walking lane keeps XO_WIN
-// words of acc1 and acc3 (XO_WIN * 32 positions) from the word of its next
-// event's first test on in rows of its own, and takes its events' tests from
-// there; the wavefront refills every lane's rows (16-byte loads) when some
-// lane's next event could leave its window.  No lane reads another's rows.
-constexpr int XO_WIN = 32, XO_ROW = XO_WIN + 4, XO_WALK_THREADS = 128;
+// The RANF stream (frame positions: place in the round a = q % 55): take(n)
+// first skips the round's leftovers when fewer than n remain (ranf.rs:89-92).
+__device__ __forceinline__ void ranf_skip(uint64_t &q, uint32_t &a, uint32_t n)
+{
+    const uint32_t rem = 55u - a;
+    if (rem < n) {
+        q += rem;
+        a = 0;
+    }
+}
+__device__ __forceinline__ void ranf_step(uint64_t &q, uint32_t &a, uint32_t n)
+{
+    q += n;
+    a += n;
+    if (a == 55u)
+        a = 0;
+}
 
-struct XoWin {
+// One event's consumption of the RANF stream on its accept bits, as
+// fe_walk_event: take(9), take(6) (its points are (p, p+3) for p the block's
+// first three positions, in any order: only the count of failures matters),
+// then a take(2) per re-roll until each failed point has an accepted pair.
+__device__ __forceinline__ bool ranf_walk_event(const uint32_t *acc1, const uint32_t *acc3, uint64_t bbase,
+                                                uint64_t qlimit, uint64_t &q, uint32_t &a)
+{
+    ranf_skip(q, a, 9);
+    if (q + 9 > qlimit)
+        return false;
+    ranf_step(q, a, 9);
+    ranf_skip(q, a, 6);
+    if (q + 6 > qlimit)
+        return false;
+    const uint64_t r = q - bbase;
+    ranf_step(q, a, 6);
+    uint32_t fails = 3u - (acc_bit(acc3, r) + acc_bit(acc3, r + 1) + acc_bit(acc3, r + 2));
+    while (fails) {
+        ranf_skip(q, a, 2);
+        if (q + 2 > qlimit)
+            return false;
+        fails -= acc_bit(acc1, q - bbase);
+        ranf_step(q, a, 2);
+    }
+    return true;
+}
+
+// The re-roll pairs open to a take(2) sequence that starts L = 55 - a
+// positions before its round ends (bit k: the pair starting k positions on):
+// even offsets whose pair fits before the round's end, then from each later
+// round's start (offset L + 55 m) the even offsets of that round (its last,
+// 55th, position is left over).
+__device__ __forceinline__ uint64_t ranf_pair_starts(uint32_t L)
+{
+    const uint64_t E = 0x5555555555555555ull;
+    auto low = [](uint32_t n) -> uint64_t { return n >= 64 ? ~0ull : (1ull << n) - 1ull; };
+    uint64_t c = (E & low(L - 1)) | ((E << L) & low(L + 53));
+    if (L + 55 < 64)
+        c |= (E << (L + 55)) & low(L + 108);
+    return c;
+}
+
+// The walkers' accept bits through LDS.  Bits read straight from HBM cost
+// each event several scattered loads, and a compute unit's walkers touch
+// more lines than its L1 holds.  So each walking lane keeps BW_WORDS words of
+// acc1 and acc3 (BW_WORDS * 32 positions) from the word of its next event's
+// first test on in rows of its own, and takes its events' tests from there;
+// the wavefront refills every lane's rows (16-byte loads) when some lane's
+// next event could leave its window.  No lane reads another's rows.
+constexpr int BW_WORDS = 32, BW_ROW = BW_WORDS + 4, BW_THREADS = 128;
+
+struct BitWin {
     uint32_t *w1, *w3; // the lane's rows
-    uint64_t wb;       // first word held (counted from qbase)
+    uint64_t wb;       // first word held (counted from bbase)
     uint64_t end;      // stream position where the held bits end
 };
 
 // Words from the one holding position q + 9 on, in whole groups of 4 below
 // qlimit.
-__device__ __forceinline__ void xo_win_fill(XoWin &v, const uint32_t *acc1, const uint32_t *acc3, uint64_t qbase,
-                                            uint64_t qlimit, uint64_t q)
+__device__ __forceinline__ void bit_win_fill(BitWin &v, const uint32_t *acc1, const uint32_t *acc3, uint64_t bbase,
+                                             uint64_t qlimit, uint64_t q)
 {
-    const uint64_t lim4 = ((qlimit - qbase) >> 5) & ~3ull;
-    const uint64_t wb = ((q + 9 - qbase) >> 5) & ~3ull;
+    const uint64_t lim4 = ((qlimit - bbase) >> 5) & ~3ull;
+    const uint64_t wb = ((q + 9 - bbase) >> 5) & ~3ull;
     uint64_t have = 0;
 #pragma unroll
-    for (int i = 0; i < XO_WIN / 4; ++i) {
+    for (int i = 0; i < BW_WORDS / 4; ++i) {
         const uint64_t w = wb + 4 * i;
         if (w < lim4) {
             *reinterpret_cast<uint4 *>(v.w1 + 4 * i) = *reinterpret_cast<const uint4 *>(acc1 + w);
@@ -684,50 +875,95 @@ __device__ __forceinline__ void xo_win_fill(XoWin &v, const uint32_t *acc1, cons
         }
     }
     v.wb = wb;
-    v.end = qbase + 32 * (wb + have);
+    v.end = bbase + 32 * (wb + have);
 }
 
 // xo_walk_event with the tests from the lane's window when the event's bits
-// (as far as position q + 110) lie in it; otherwise from acc1 / acc3.
-__device__ __forceinline__ bool xo_walk_event_win(const XoWin &v, const uint32_t *acc1, const uint32_t *acc3,
-                                                  uint64_t qbase, uint64_t qlimit, uint64_t &q)
-{
-    if (q + 111 <= v.end) {
-        const uint64_t r3 = q + 9 - qbase, r1 = q + 15 - qbase;
-        const uint32_t *p3 = v.w3 + ((r3 >> 5) - v.wb), *p1 = v.w1 + ((r1 >> 5) - v.wb);
-        const uint32_t s3 = (uint32_t)(r3 & 31), s1 = (uint32_t)(r1 & 31);
-        const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> s3) & 7u;
-        const uint64_t lo = ((uint64_t)p1[1] << 32) | p1[0];
-        const uint64_t w1 = s1 ? (lo >> s1) | ((uint64_t)p1[2] << (64 - s1)) : lo;
-        const uint32_t fails = 3u - (uint32_t)__popc(ok3);
-        if (fails == 0) {
-            q += 15;
-            return true;
+// (as far as position q + 110) lie in it.
+struct XoBitWalk {
+    static constexpr uint64_t REACH = 111;
+    const uint32_t *acc1, *acc3;
+    uint64_t bbase, qlimit;
+    __device__ __forceinline__ void start(uint64_t) {}
+    __device__ __forceinline__ bool event(const BitWin &v, uint64_t &q)
+    {
+        if (q + REACH <= v.end) {
+            const uint64_t r3 = q + 9 - bbase, r1 = q + 15 - bbase;
+            const uint32_t *p3 = v.w3 + ((r3 >> 5) - v.wb);
+            const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> (r3 & 31)) & 7u;
+            const uint32_t fails = 3u - (uint32_t)__popc(ok3);
+            if (fails == 0) {
+                q += 15;
+                return true;
+            }
+            const uint64_t y = nth_accept(bits64(v.w1 + ((r1 >> 5) - v.wb), r1) & 0x5555555555555555ull, fails);
+            if (y) {
+                q += 17 + (uint64_t)__builtin_ctzll(y);
+                return true;
+            }
         }
-        const uint64_t y0 = w1 & 0x5555555555555555ull, y1 = y0 & (y0 - 1), y2 = y1 & (y1 - 1);
-        const uint64_t y = fails == 1 ? y0 : fails == 2 ? y1 : y2;
-        if (y) {
-            q += 17 + (uint64_t)__builtin_ctzll(y);
-            return true;
-        }
+        return xo_walk_event(acc1, acc3, bbase, qlimit, q);
     }
-    return xo_walk_event(acc1, acc3, qbase, qlimit, q);
-}
+};
+
+// ranf_walk_event with the tests from the lane's window when the event's
+// bits (as far as position q + 123: up to 13 discarded leftovers, the 15
+// values, 64 positions of re-rolls and the rest of their last word) lie in
+// it: the 9 and 6 takes by position arithmetic, the re-rolls as the F-th
+// accepted pair among ranf_pair_starts.
+struct RanfBitWalk {
+    static constexpr uint64_t REACH = 128;
+    const uint32_t *acc1, *acc3;
+    uint64_t bbase, qlimit;
+    uint32_t a; // q % 55
+    __device__ __forceinline__ void start(uint64_t q) { a = (uint32_t)(q % 55u); }
+    __device__ __forceinline__ bool event(const BitWin &v, uint64_t &q)
+    {
+        if (q + REACH <= v.end) {
+            uint64_t q1 = q;
+            uint32_t a1 = a;
+            ranf_skip(q1, a1, 9);
+            ranf_step(q1, a1, 9);
+            ranf_skip(q1, a1, 6);
+            const uint64_t r3 = q1 - bbase;
+            ranf_step(q1, a1, 6);
+            const uint32_t *p3 = v.w3 + ((r3 >> 5) - v.wb);
+            const uint32_t ok3 = (uint32_t)((((uint64_t)p3[1] << 32) | p3[0]) >> (r3 & 31)) & 7u;
+            const uint32_t fails = 3u - (uint32_t)__popc(ok3);
+            if (fails == 0) {
+                q = q1;
+                a = a1;
+                return true;
+            }
+            const uint64_t r1 = q1 - bbase;
+            const uint64_t y = nth_accept(bits64(v.w1 + ((r1 >> 5) - v.wb), r1) & ranf_pair_starts(55u - a1), fails);
+            if (y) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(y) + 2; // past the accepted pair
+                q = q1 + k;
+                a = a1 + k;
+                a = a >= 110u ? a - 110u : a >= 55u ? a - 55u : a;
+                return true;
+            }
+        }
+        return ranf_walk_event(acc1, acc3, bbase, qlimit, q, a);
+    }
+};
 
 // Records starts from q while q < stop and the list has room (each event
 // consumes at least 15 positions, so the loop ends); returns the count.
-__device__ __forceinline__ uint32_t xo_walk_run(XoWin &v, const uint32_t *acc1, const uint32_t *acc3, uint64_t qbase,
-                                                uint64_t qlimit, uint64_t q, uint64_t stop, uint64_t *list,
-                                                uint32_t cap)
+template <class W>
+__device__ __forceinline__ uint32_t bit_walk_run(W w, BitWin &v, uint64_t q, uint64_t stop, uint64_t *list,
+                                                 uint32_t cap)
 {
     v.wb = 0;
     v.end = 0;
+    w.start(q);
     uint32_t n = 0;
     while (q < stop && n < cap) {
         list[n++] = q;
-        if (__any(q + 111 > v.end && q + 111 <= qlimit))
-            xo_win_fill(v, acc1, acc3, qbase, qlimit, q);
-        if (!xo_walk_event_win(v, acc1, acc3, qbase, qlimit, q))
+        if (__any(q + W::REACH > v.end && q + W::REACH <= w.qlimit))
+            bit_win_fill(v, w.acc1, w.acc3, w.bbase, w.qlimit, q);
+        if (!w.event(v, q))
             break;
     }
     return n;
@@ -738,43 +974,28 @@ __device__ __forceinline__ uint32_t xo_walk_run(XoWin &v, const uint32_t *acc1, 
 __device__ __forceinline__ bool fe_walk_any(const FeWalkArgs &a, uint64_t &q)
 {
     if (a.kind >= 2)
-        return xo_walk_event(a.acc1, a.acc3, a.qbase, a.qlimit, q);
+        return xo_walk_event(a.acc1, a.acc3, a.bbase, a.qlimit, q);
     return a.kind == 1 ? fe_walk_event<float>(a, q) : fe_walk_event<double>(a, q);
 }
 
 // One thread per walker.  Every step records a start and then consumes at
 // least 15 positions, and the loop stops at the segment end, the list
-// capacity or the stored range, so it always ends.
-__global__ __launch_bounds__(256) void fe_walk_kernel(FeWalkArgs a)
+// capacity or the stored range, so it always ends.  The tests come from the
+// accept bits through LDS windows (every kind).
+__global__ __launch_bounds__(BW_THREADS) void fe_bit_walk_kernel(FeWalkArgs a)
 {
+    __shared__ uint32_t win[2 * BW_THREADS * BW_ROW];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nwalkers)
         return;
-    uint64_t q = k == 0 ? a.q_start : a.seg_origin + (uint64_t)k * a.seg;
-    const uint64_t stop = a.seg_origin + (uint64_t)(k + 1) * a.seg + a.overlap;
-    uint64_t *list = a.lists + (size_t)k * a.cap;
-    uint32_t n = 0;
-    while (q < stop && n < a.cap) {
-        list[n++] = q;
-        if (!fe_walk_any(a, q))
-            break;
-    }
-    a.counts[k] = n;
-}
-
-// The same on the xoshiro streams (kinds 2, 3), with the bits in LDS windows.
-__global__ __launch_bounds__(XO_WALK_THREADS) void fe_xo_walk_kernel(FeWalkArgs a)
-{
-    __shared__ uint32_t win[2 * XO_WALK_THREADS * XO_ROW];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nwalkers)
-        return;
-    XoWin v;
-    v.w1 = win + threadIdx.x * XO_ROW;
-    v.w3 = win + (XO_WALK_THREADS + threadIdx.x) * XO_ROW;
+    BitWin v;
+    v.w1 = win + threadIdx.x * BW_ROW;
+    v.w3 = win + (BW_THREADS + threadIdx.x) * BW_ROW;
     const uint64_t q = k == 0 ? a.q_start : a.seg_origin + (uint64_t)k * a.seg;
     const uint64_t stop = a.seg_origin + (uint64_t)(k + 1) * a.seg + a.overlap;
-    a.counts[k] = xo_walk_run(v, a.acc1, a.acc3, a.qbase, a.qlimit, q, stop, a.lists + (size_t)k * a.cap, a.cap);
+    uint64_t *list = a.lists + (size_t)k * a.cap;
+    a.counts[k] = a.kind >= 2 ? bit_walk_run(XoBitWalk{a.acc1, a.acc3, a.bbase, a.qlimit}, v, q, stop, list, a.cap)
+                              : bit_walk_run(RanfBitWalk{a.acc1, a.acc3, a.bbase, a.qlimit, 0}, v, q, stop, list, a.cap);
 }
 
 // Walker k's first start that walker k-1 also reached past segment k's start:
@@ -819,11 +1040,8 @@ __global__ __launch_bounds__(256) void fe_join_kernel(FeWalkArgs a)
 void launch_fe_walk(const FeWalkArgs &a, hipStream_t st)
 {
     const unsigned blocks = (a.nwalkers + 255) / 256;
-    if (a.kind >= 2)
-        hipLaunchKernelGGL(fe_xo_walk_kernel, dim3((a.nwalkers + XO_WALK_THREADS - 1) / XO_WALK_THREADS),
-                           dim3(XO_WALK_THREADS), 0, st, a);
-    else
-        hipLaunchKernelGGL(fe_walk_kernel, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fe_bit_walk_kernel, dim3((a.nwalkers + BW_THREADS - 1) / BW_THREADS), dim3(BW_THREADS), 0, st,
+                       a);
     hipLaunchKernelGGL(fe_join_kernel, dim3(blocks), dim3(256), 0, st, a);
 }
 
@@ -859,6 +1077,7 @@ __device__ __forceinline__ FeWalkArgs fe_batch_view(const FeBatchWalkArgs &a, ui
     w.kind = a.kind;
     w.acc1 = a.acc1;
     w.acc3 = a.acc3;
+    w.bbase = 0;
     w.qbase = 0;
     w.qlimit = (k + 1) * a.piece_stride;
     return w;
@@ -867,43 +1086,24 @@ __device__ __forceinline__ FeWalkArgs fe_batch_view(const FeBatchWalkArgs &a, ui
 // Walker s of batch k (thread k * S + s).  Every step records a start and
 // then consumes at least 15 positions, and the loop stops at the segment end
 // plus the overlap, the piece's end or the list capacity, so it always ends.
-__global__ __launch_bounds__(256) void fe_seg_walk_kernel(FeBatchWalkArgs a)
+// The tests come from the accept bits through LDS windows (every kind).
+__global__ __launch_bounds__(BW_THREADS) void fe_seg_bit_walk_kernel(FeBatchWalkArgs a)
 {
+    __shared__ uint32_t win[2 * BW_THREADS * BW_ROW];
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nb * a.S)
         return;
     const uint64_t k = t / a.S;
     const uint32_t s = (uint32_t)(t % a.S);
-    const uint64_t base = k * a.piece_stride;
-    uint64_t q = s == 0 ? base + (a.p0 ? (uint64_t)a.p0[k] : 0) : base + (uint64_t)s * a.seg;
-    const uint64_t stop = min(base + (uint64_t)(s + 1) * a.seg + a.overlap, base + a.piece_stride);
-    const FeWalkArgs w = fe_batch_view(a, k);
+    const uint64_t base = k * a.piece_stride, end = base + a.piece_stride;
+    const uint64_t q = s == 0 ? base + (a.p0 ? (uint64_t)a.p0[k] : 0) : base + (uint64_t)s * a.seg;
+    const uint64_t stop = min(base + (uint64_t)(s + 1) * a.seg + a.overlap, end);
+    BitWin v;
+    v.w1 = win + threadIdx.x * BW_ROW;
+    v.w3 = win + (BW_THREADS + threadIdx.x) * BW_ROW;
     uint64_t *list = a.lists + t * a.cap;
-    uint32_t n = 0;
-    while (q < stop && n < a.cap) {
-        list[n++] = q;
-        if (!fe_walk_any(w, q))
-            break;
-    }
-    a.counts[t] = n;
-}
-
-// The same on xoshiro pieces (kinds 2, 3), with the bits in LDS windows.
-__global__ __launch_bounds__(XO_WALK_THREADS) void fe_seg_xo_walk_kernel(FeBatchWalkArgs a)
-{
-    __shared__ uint32_t win[2 * XO_WALK_THREADS * XO_ROW];
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nb * a.S)
-        return;
-    const uint64_t k = t / a.S;
-    const uint32_t s = (uint32_t)(t % a.S);
-    const uint64_t base = k * a.piece_stride;
-    const uint64_t q = base + (uint64_t)s * a.seg;
-    const uint64_t stop = min(base + (uint64_t)(s + 1) * a.seg + a.overlap, base + a.piece_stride);
-    XoWin v;
-    v.w1 = win + threadIdx.x * XO_ROW;
-    v.w3 = win + (XO_WALK_THREADS + threadIdx.x) * XO_ROW;
-    a.counts[t] = xo_walk_run(v, a.acc1, a.acc3, 0, base + a.piece_stride, q, stop, a.lists + t * a.cap, a.cap);
+    a.counts[t] = a.kind >= 2 ? bit_walk_run(XoBitWalk{a.acc1, a.acc3, 0, end}, v, q, stop, list, a.cap)
+                              : bit_walk_run(RanfBitWalk{a.acc1, a.acc3, 0, end, 0}, v, q, stop, list, a.cap);
 }
 
 // Walker s's first start that walker s - 1 of the same batch also reached
@@ -1031,11 +1231,8 @@ void launch_fe_walk_batches(const FeBatchWalkArgs &a, hipStream_t st)
 {
     const uint64_t walkers = a.nb * a.S;
     const unsigned wb = (unsigned)((walkers + 255) / 256), bb = (unsigned)((a.nb + 63) / 64);
-    if (a.kind >= 2)
-        hipLaunchKernelGGL(fe_seg_xo_walk_kernel, dim3((unsigned)((walkers + XO_WALK_THREADS - 1) / XO_WALK_THREADS)),
-                           dim3(XO_WALK_THREADS), 0, st, a);
-    else
-        hipLaunchKernelGGL(fe_seg_walk_kernel, dim3(wb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fe_seg_bit_walk_kernel, dim3((unsigned)((walkers + BW_THREADS - 1) / BW_THREADS)),
+                       dim3(BW_THREADS), 0, st, a);
     hipLaunchKernelGGL(fe_seg_join_kernel, dim3(wb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(fe_seg_assemble_kernel, dim3((unsigned)a.nb), dim3(64), 0, st, a);
     hipLaunchKernelGGL(fe_seq_walk_kernel, dim3(bb), dim3(64), 0, st, a);
@@ -1057,23 +1254,6 @@ __global__ __launch_bounds__(64) void xo_piece_state_kernel(XoPieceArgs a)
 #pragma unroll
     for (int w = 0; w < 4; ++w)
         a.states[4 * k + w] = s[w];
-}
-
-// Outcome of the unit-disc test on the point (x, y) = 2u - 1 of two
-// uniforms: r2 above 1 re-rolls, and in f64 so does r2 below MIN_POSITIVE^2
-// (f32's MIN_POSITIVE^2 is 0).  The walk sums (0 + x*x) + y*y for the six
-// values' points and 0 + (x*x + y*y) for a re-roll: without contraction both
-// are fl(fl(x*x) + fl(y*y)).
-__device__ __forceinline__ uint32_t disc_accept(double x, double y)
-{
-    const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
-    const double r2 = (0. + x * x) + y * y;
-    return (r2 > 1. || r2 < min2) ? 0u : 1u;
-}
-__device__ __forceinline__ uint32_t disc_accept(float x, float y)
-{
-    const float r2 = (0.f + x * x) + y * y;
-    return r2 > 1.f ? 0u : 1u;
 }
 
 struct Xo256Gen {
