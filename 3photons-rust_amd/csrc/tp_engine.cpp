@@ -581,10 +581,12 @@ int fe_stream_expand(Ctx *c, const FeStream &fs, uint64_t q_first, uint64_t q_la
 
 // Device walk of n + 1 event starts from the true start q_start into c->evq,
 // with the stream of the walked range expanded into HBM (c->draws or
-// c->xo_draws, position *qbase at index 0).  *ok is false when some walker
-// did not meet its predecessor or the joined walk is too short; the caller
-// then walks on the host.
-int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, bool *ok, uint64_t *qbase)
+// c->xo_draws, position *qbase at index 0) and its accept bits (from
+// position *bbase, valid below *blimit).  *ok is false when some walker did
+// not meet its predecessor or the joined walk is too short; the caller then
+// walks on the host.
+int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, bool *ok, uint64_t *qbase,
+                   uint64_t *bbase, uint64_t *blimit)
 {
     *ok = false;
     const bool xo = fs.kind >= 2; // the xoshiro streams: no rounds
@@ -672,6 +674,8 @@ int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, boo
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors are reused
     *qbase = w.qbase;
+    *bbase = w.bbase;
+    *blimit = w.qlimit;
     *ok = true;
     return TP_OK;
 }
@@ -707,9 +711,9 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         bool ok = false;
-        uint64_t q_next = 0, qbase = 0, bbase = 0;
+        uint64_t q_next = 0, qbase = 0, bbase = 0, blimit = 0;
         if (device_walk) {
-            int rc = fe_device_walk(c, fs, q_start, n, &ok, &qbase);
+            int rc = fe_device_walk(c, fs, q_start, n, &ok, &qbase, &bbase, &blimit);
             if (rc)
                 return rc;
             q_next = c->fe_qnext;
@@ -723,6 +727,7 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
                 rc = fe_stream_expand(c, fs, q_start, q_next, &qbase, &bbase);
             if (rc)
                 return rc;
+            blimit = q_next;
             HIPCHK(c, hipMemcpyAsync(c->evq.p, c->fe_starts.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice,
                                      c->stream));
         }
@@ -736,6 +741,10 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
             fa->draws = static_cast<const int32_t *>(c->draws.p);
             fa->x32draws = static_cast<const float *>(c->xo128_draws.p);
             fa->qbase = qbase;
+            fa->acc1 = static_cast<const uint32_t *>(c->xo_acc.p);
+            fa->bbase = bbase;
+            fa->blimit = blimit;
+            fa->bstride = 0;
             fa->evq = static_cast<const uint64_t *>(c->evq.p);
             fa->out = static_cast<tp_acc *>(per_batch_dev) + b0;
             tpk::launch_f32_events(*fa, f32_grid(c, fa->nb), c->stream);
@@ -748,6 +757,10 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
             a.draws = static_cast<const int32_t *>(c->draws.p);
             a.xdraws = static_cast<const double *>(c->xo_draws.p);
             a.qbase = qbase;
+            a.acc1 = static_cast<const uint32_t *>(c->xo_acc.p);
+            a.bbase = bbase;
+            a.blimit = blimit;
+            a.bstride = 0;
             a.evq = static_cast<const uint64_t *>(c->evq.p);
             a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
             tpk::launch_fused(a, source, grid_for(c, source, 0, a.nb), c->stream);
@@ -980,6 +993,9 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
             fa.b_first = b0;
             fa.draws = static_cast<const int32_t *>(c->draws.p);
             fa.qbase = 0;
+            fa.acc1 = w.acc1;
+            fa.bbase = 0;
+            fa.bstride = PS;
             fa.evq = static_cast<const uint64_t *>(c->evq.p);
             fa.out = static_cast<tp_acc *>(per_batch_dev) + b0;
             tpk::launch_f32_events(fa, f32_grid(c, cnb), c->stream);
@@ -992,6 +1008,9 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
         a.b_first = b0;
         a.draws = static_cast<const int32_t *>(c->draws.p);
         a.qbase = 0;
+        a.acc1 = w.acc1;
+        a.bbase = 0;
+        a.bstride = PS;
         a.evq = static_cast<const uint64_t *>(c->evq.p);
         a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
         tpk::launch_fused(a, 3, grid_for(c, 3, 0, cnb), c->stream);
@@ -1081,6 +1100,9 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
         a.b_first = b0;
         a.xdraws = p.xbuf;
         a.qbase = 0;
+        a.acc1 = w.acc1;
+        a.bbase = 0;
+        a.bstride = FE_XO_PIECE;
         a.evq = w.evq;
         a.out = static_cast<tp_acc *>(per_batch_dev) + b0;
         if (first) {
@@ -1179,6 +1201,9 @@ int enqueue_xo128_faster_evgen(Ctx *c, const tp_params *params, int seeding, con
         fa.b_first = b0;
         fa.x32draws = static_cast<const float *>(c->xo128_draws.p);
         fa.qbase = 0;
+        fa.acc1 = w.acc1;
+        fa.bbase = 0;
+        fa.bstride = FE_XO_PIECE;
         fa.evq = w.evq;
         fa.out = static_cast<tp_acc *>(per_batch_dev) + b0;
         tpk::launch_f32_events(fa, f32_grid(c, cnb), c->stream);

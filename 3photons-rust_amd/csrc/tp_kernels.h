@@ -45,6 +45,11 @@ struct FusedArgs {
     // faster-evgen from the xoshiro256+ stream (source 4): its uniforms, the
     // starts evq and qbase as above
     const double *xdraws;
+    // faster-evgen: the stream's accept bits (tpd::RerollBits), valid below
+    // blimit, or for pieces (bstride > 0) batch k's below (k + 1) * bstride;
+    // acc1 null: re-rolls one pair at a time
+    const uint32_t *acc1;
+    uint64_t bbase, blimit, bstride;
     // output: one accumulator per batch, index b - b_first
     void *out;             // tp_acc*
 };
@@ -112,6 +117,8 @@ struct F32Args {
     const uint64_t *evq;
     uint64_t qbase;
     const float *x32draws; // faster-evgen (source 3): xoshiro128+ uniforms from position qbase
+    const uint32_t *acc1;  // faster-evgen: accept bits, as FusedArgs
+    uint64_t bbase, blimit, bstride;
     float *ev;             // per event m[5] and keep (0/1), at batch_size*(b - b_first) + e
     void *out;             // tp_acc per batch, index b - b_first (f32 values)
 };

@@ -212,6 +212,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
         }
         const int32_t *src = (SOURCE == 1) ? a.draws + (size_t)lb * 12 * a.batch_size : nullptr;
         const size_t wbase = (size_t)lb * a.batch_size;
+        const RerollBits rb{a.acc1, a.bbase, a.bstride ? (lb + 1) * a.bstride : a.blimit};
         Acc<PREC> A;
         acc_zero(A);
         uint32_t qn = 0; // queued events (uniform)
@@ -220,10 +221,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
             double p[3][4];
             int d[3]; // sort slots of the three photons
             if (k < len && SOURCE == 3) {
-                RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
+                RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)], rb};
                 pass = generate_faster_f64(fresh_params(), w, p, d);
             } else if (k < len && SOURCE == 4) {
-                XoWalk w{a.xdraws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)]};
+                XoWalk w{a.xdraws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)], rb};
                 pass = generate_faster_f64(fresh_params(), w, p, d);
             } else if (k < len) {
                 double u[12];
@@ -825,21 +826,6 @@ __device__ __forceinline__ bool ranf_walk_event(const uint32_t *acc1, const uint
         ranf_step(q, a, 2);
     }
     return true;
-}
-
-// The re-roll pairs open to a take(2) sequence that starts L = 55 - a
-// positions before its round ends (bit k: the pair starting k positions on):
-// even offsets whose pair fits before the round's end, then from each later
-// round's start (offset L + 55 m) the even offsets of that round (its last,
-// 55th, position is left over).
-__device__ __forceinline__ uint64_t ranf_pair_starts(uint32_t L)
-{
-    const uint64_t E = 0x5555555555555555ull;
-    auto low = [](uint32_t n) -> uint64_t { return n >= 64 ? ~0ull : (1ull << n) - 1ull; };
-    uint64_t c = (E & low(L - 1)) | ((E << L) & low(L + 53));
-    if (L + 55 < 64)
-        c |= (E << (L + 55)) & low(L + 108);
-    return c;
 }
 
 // The walkers' accept bits through LDS.  Bits read straight from HBM cost
@@ -1627,11 +1613,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
             bool pass = false;
             float p[3][4];
             if (i < len && FE) {
+                const RerollBits rb{a.acc1, a.bbase, a.bstride ? (lb + 1) * a.bstride : a.blimit};
                 if constexpr (SOURCE == 2) {
-                    RanfWalkF32 w{a.draws, a.qbase, a.evq[e]};
+                    RanfWalkF32 w{a.draws, a.qbase, a.evq[e], rb};
                     pass = generate_faster_f32(a.P, w, p);
                 } else {
-                    Xo128WalkF32 w{a.x32draws, a.qbase, a.evq[e]};
+                    Xo128WalkF32 w{a.x32draws, a.qbase, a.evq[e], rb};
                     pass = generate_faster_f32(a.P, w, p);
                 }
                 if (!pass) {

@@ -482,6 +482,57 @@ __device__ __forceinline__ bool generate_f64(const Params &P, const double u[12]
     return keep_f64(P, p, d);
 }
 
+// ---- faster-evgen re-rolls from accept bits -------------------------------------
+// The stream's accept bits (bit i of acc1: the point (2u - 1) of positions
+// bbase + i and bbase + i + 1 passes the unit-disc test), valid below blimit.
+// acc1 null: none at hand.
+struct RerollBits {
+    const uint32_t *acc1;
+    uint64_t bbase, blimit;
+};
+
+// Pair starts open to a RANF take(2) sequence that starts L = 55 - a
+// positions before its round ends (bit k: a pair may start k positions on):
+// even offsets whose pair fits before the round's end, then from each later
+// round's start (offset L + 55 m) the even offsets of that round (its 55th
+// position is left over, ranf.rs:89-92).
+__device__ __forceinline__ uint64_t ranf_pair_starts(uint32_t L)
+{
+    const uint64_t E = 0x5555555555555555ull;
+    auto low = [](uint32_t n) -> uint64_t { return n >= 64 ? ~0ull : (1ull << n) - 1ull; };
+    uint64_t c = (E & low(L - 1)) | ((E << L) & low(L + 53));
+    if (L + 55 < 64)
+        c |= (E << (L + 55)) & low(L + 108);
+    return c;
+}
+
+// Offsets from the re-roll start q of the first accepted pairs among the
+// starts `cand` (k[0], k[1], k[2]: the first, second and third); false when
+// the bits near q are not at hand or the 64 positions hold fewer than F
+// accepted pairs.  Reads acc1 words up to position q + 95.
+__device__ __forceinline__ bool accepted_pairs(const RerollBits &b, uint64_t q, uint64_t cand, uint32_t F,
+                                               uint32_t k[3])
+{
+    if (!b.acc1 || q + 96 > b.blimit)
+        return false;
+    const uint64_t r = q - b.bbase;
+    const uint32_t *w = b.acc1 + (r >> 5);
+    const uint32_t s = (uint32_t)(r & 31);
+    const uint64_t lo = ((uint64_t)w[1] << 32) | w[0];
+    uint64_t y = (s ? (lo >> s) | ((uint64_t)w[2] << (64 - s)) : lo) & cand;
+    if ((uint32_t)__popcll(y) < F)
+        return false;
+    k[0] = (uint32_t)__builtin_ctzll(y);
+    y &= y - 1;
+    k[1] = y ? (uint32_t)__builtin_ctzll(y) : 0u;
+    y &= y - 1;
+    k[2] = y ? (uint32_t)__builtin_ctzll(y) : 0u;
+    return true;
+}
+
+// The pair of the j-th failed point (j = failed points before it).
+__device__ __forceinline__ uint32_t pair_of(const uint32_t k[3], uint32_t j) { return j == 0 ? k[0] : j == 1 ? k[1] : k[2]; }
+
 // ---- faster-evgen (evgen.rs:143-173, 221-249) --------------------------------
 // Reads the RANF stream expanded in HBM.  Stream position q holds the value
 // at index 55 - q % 55 of round q / 55; random_array::<N> first skips the
@@ -491,6 +542,7 @@ struct RanfWalk {
     const int32_t *buf; // stream values from position qbase on
     uint64_t qbase;
     uint64_t q;         // next position
+    RerollBits rb;
     __device__ __forceinline__ void take(int n, double *out)
     {
         const uint32_t rem = 55u - (uint32_t)(q % 55u);
@@ -501,6 +553,26 @@ struct RanfWalk {
             out[k] = (double)v[n - 1 - k] * 1e-9;
         q += (uint64_t)n;
     }
+    // The re-rolls of the failed points (bit i of fail: point i), each the
+    // next accepted take(2) pair, as the re-roll loop takes them; false
+    // (nothing taken) when the accept bits cannot tell.
+    __device__ __forceinline__ bool rerolls(uint32_t fail, double x[3], double y[3], double r2[3])
+    {
+        uint32_t k[3];
+        const uint32_t F = (uint32_t)__popc(fail);
+        if (!accepted_pairs(rb, q, ranf_pair_starts(55u - (uint32_t)(q % 55u)), F, k))
+            return false;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if ((fail >> i) & 1u) {
+                const int32_t *v = buf + (q - qbase) + pair_of(k, (uint32_t)__popc(fail & ((1u << i) - 1u)));
+                x[i] = 2. * ((double)v[1] * 1e-9) - 1.;
+                y[i] = 2. * ((double)v[0] * 1e-9) - 1.;
+                r2[i] = 0. + (x[i] * x[i] + y[i] * y[i]);
+            }
+        q += (uint64_t)pair_of(k, F - 1) + 2;
+        return true;
+    }
 };
 
 // The xoshiro256+ stream (standard.rs) expanded into HBM as its f64
@@ -510,12 +582,31 @@ struct XoWalk {
     const double *buf; // uniforms from position qbase on
     uint64_t qbase;
     uint64_t q;        // next position
+    RerollBits rb;
     __device__ __forceinline__ void take(int n, double *out)
     {
         const double *v = buf + (q - qbase);
         for (int k = 0; k < n; ++k)
             out[k] = v[k];
         q += (uint64_t)n;
+    }
+    // As RanfWalk::rerolls; every even offset is a pair start.
+    __device__ __forceinline__ bool rerolls(uint32_t fail, double x[3], double y[3], double r2[3])
+    {
+        uint32_t k[3];
+        const uint32_t F = (uint32_t)__popc(fail);
+        if (!accepted_pairs(rb, q, 0x5555555555555555ull, F, k))
+            return false;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if ((fail >> i) & 1u) {
+                const double *v = buf + (q - qbase) + pair_of(k, (uint32_t)__popc(fail & ((1u << i) - 1u)));
+                x[i] = 2. * v[0] - 1.;
+                y[i] = 2. * v[1] - 1.;
+                r2[i] = 0. + (x[i] * x[i] + y[i] * y[i]);
+            }
+        q += (uint64_t)pair_of(k, F - 1) + 2;
+        return true;
     }
 };
 
@@ -533,14 +624,20 @@ template <class Walk> __device__ __forceinline__ void raw_photons_faster_f64(Wal
         r2[i] = (0. + x[i] * x[i]) + y[i] * y[i];
     }
     const double min2 = 2.2250738585072014e-308 * 2.2250738585072014e-308;
+    uint32_t fail = 0;
+#pragma unroll
     for (int i = 0; i < 3; ++i)
-        while (r2[i] > 1. || r2[i] < min2) {
-            double r[2];
-            w.take(2, r);
-            x[i] = 2. * r[0] - 1.;
-            y[i] = 2. * r[1] - 1.;
-            r2[i] = 0. + (x[i] * x[i] + y[i] * y[i]);
-        }
+        fail |= (r2[i] > 1. || r2[i] < min2) ? 1u << i : 0u;
+    if (fail && !w.rerolls(fail, x, y, r2)) {
+        for (int i = 0; i < 3; ++i)
+            while (r2[i] > 1. || r2[i] < min2) {
+                double r[2];
+                w.take(2, r);
+                x[i] = 2. * r[0] - 1.;
+                y[i] = 2. * r[1] - 1.;
+                r2[i] = 0. + (x[i] * x[i] + y[i] * y[i]);
+            }
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const double nrm = dv(1., sq(r2[i]));
@@ -731,6 +828,7 @@ struct RanfWalkF32 {
     const int32_t *buf;
     uint64_t qbase;
     uint64_t q;
+    RerollBits rb;
     __device__ __forceinline__ void take(int n, float *out)
     {
         const uint32_t rem = 55u - (uint32_t)(q % 55u);
@@ -741,6 +839,24 @@ struct RanfWalkF32 {
             out[k] = (float)v[n - 1 - k] * 1e-9f;
         q += (uint64_t)n;
     }
+    // As RanfWalk::rerolls, in f32.
+    __device__ __forceinline__ bool rerolls(uint32_t fail, float x[3], float y[3], float r2[3])
+    {
+        uint32_t k[3];
+        const uint32_t F = (uint32_t)__popc(fail);
+        if (!accepted_pairs(rb, q, ranf_pair_starts(55u - (uint32_t)(q % 55u)), F, k))
+            return false;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if ((fail >> i) & 1u) {
+                const int32_t *v = buf + (q - qbase) + pair_of(k, (uint32_t)__popc(fail & ((1u << i) - 1u)));
+                x[i] = 2.f * ((float)v[1] * 1e-9f) - 1.f;
+                y[i] = 2.f * ((float)v[0] * 1e-9f) - 1.f;
+                r2[i] = 0.f + (x[i] * x[i] + y[i] * y[i]);
+            }
+        q += (uint64_t)pair_of(k, F - 1) + 2;
+        return true;
+    }
 };
 
 // The xoshiro128+ stream (standard.rs with the f32 feature) expanded into HBM
@@ -750,12 +866,31 @@ struct Xo128WalkF32 {
     const float *buf; // uniforms from position qbase on
     uint64_t qbase;
     uint64_t q;       // next position
+    RerollBits rb;
     __device__ __forceinline__ void take(int n, float *out)
     {
         const float *v = buf + (q - qbase);
         for (int k = 0; k < n; ++k)
             out[k] = v[k];
         q += (uint64_t)n;
+    }
+    // As XoWalk::rerolls, in f32.
+    __device__ __forceinline__ bool rerolls(uint32_t fail, float x[3], float y[3], float r2[3])
+    {
+        uint32_t k[3];
+        const uint32_t F = (uint32_t)__popc(fail);
+        if (!accepted_pairs(rb, q, 0x5555555555555555ull, F, k))
+            return false;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if ((fail >> i) & 1u) {
+                const float *v = buf + (q - qbase) + pair_of(k, (uint32_t)__popc(fail & ((1u << i) - 1u)));
+                x[i] = 2.f * v[0] - 1.f;
+                y[i] = 2.f * v[1] - 1.f;
+                r2[i] = 0.f + (x[i] * x[i] + y[i] * y[i]);
+            }
+        q += (uint64_t)pair_of(k, F - 1) + 2;
+        return true;
     }
 };
 
@@ -774,14 +909,20 @@ template <class Walk> __device__ __forceinline__ void raw_photons_faster_f32(Wal
     }
     const float minpos = 1.17549435e-38f;
     const float min2 = minpos * minpos;
+    uint32_t fail = 0;
+#pragma unroll
     for (int i = 0; i < 3; ++i)
-        while (r2[i] > 1.f || r2[i] < min2) {
-            float r[2];
-            w.take(2, r);
-            x[i] = 2.f * r[0] - 1.f;
-            y[i] = 2.f * r[1] - 1.f;
-            r2[i] = 0.f + (x[i] * x[i] + y[i] * y[i]);
-        }
+        fail |= (r2[i] > 1.f || r2[i] < min2) ? 1u << i : 0u;
+    if (fail && !w.rerolls(fail, x, y, r2)) {
+        for (int i = 0; i < 3; ++i)
+            while (r2[i] > 1.f || r2[i] < min2) {
+                float r[2];
+                w.take(2, r);
+                x[i] = 2.f * r[0] - 1.f;
+                y[i] = 2.f * r[1] - 1.f;
+                r2[i] = 0.f + (x[i] * x[i] + y[i] * y[i]);
+            }
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float nrm = 1.f / sqrtf(r2[i]);
