@@ -519,17 +519,32 @@ template <class T> __device__ __forceinline__ T ranf_sq(int32_t v)
     return x * x;
 }
 
+// Lane moves within the wavefront (DPP, no LDS): DPP_WAVE_ROR1 gives lane i
+// the value of lane i - 1 (lane 0: lane 63); DPP_ROW_SHR9 gives lane i of each
+// row of 16 the value of lane i - 9 of the row (lanes 61..63: lanes 52..54).
+constexpr int DPP_WAVE_ROR1 = 0x13C, DPP_ROW_SHR9 = 0x119;
+template <int CTRL> __device__ __forceinline__ double lanes_dpp(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int CTRL> __device__ __forceinline__ float lanes_dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
 // Accept bits of a segment's rounds.  Lane i holds the square of position
 // 55 r + 54 - i (each value's square is computed once, as the next round's,
-// and carried).  The squares of the round and the next go to LDS in position
-// order, so that each lane reads its partners +1 and +3 at its own place plus
-// 1 and 3 (barriers order the lanes' LDS writes and reads: the workgroup is
-// the one wavefront).  Each round's two ballots, bit-reversed into position
-// order, wait in LDS; every 32 rounds (1760 positions, 55 words) lanes 0..54
-// assemble one word each and store them together.
+// and carried); its partners +1 and +3 come from lanes i - 1 and i - 3, or
+// for the round's last positions from the next round's first ones, by lane
+// moves.  Each round's two ballots, bit-reversed into position order, wait in
+// LDS; every 32 rounds (1760 positions, 55 words) lanes 0..54 assemble one
+// word each and store them together (barriers order lane 0's writes and the
+// other lanes' reads: the workgroup is the one wavefront).
 template <class T> struct RanfBits {
     uint64_t *m1, *m3;  // [32] LDS: the block's rounds' bits, bit j = position 55 r + j
-    T *s;               // [112] LDS: squares of this round and the next, in position order
     uint32_t *d1, *d3;  // the block's first words
     uint32_t k;         // rounds of the block so far
     T sq;               // this round's squares
@@ -538,19 +553,22 @@ template <class T> struct RanfBits {
     {
         const int lane = (int)(threadIdx.x & 63u);
         const T sqn = ranf_sq<T>(xn);
-        const bool live = lane < 55;
-        const int j = 54 - lane; // this lane's place in the round
-        __syncthreads(); // the last round's reads are done
-        if (live) {
-            s[j] = sq;
-            s[55 + j] = sqn;
-        }
-        __syncthreads(); // the other lanes' squares are in
+        // t: this round's squares in lanes 0..54 and the next round's
+        // positions 2, 1, 0 (its lanes 52..54) in lanes 61..63; rotating t
+        // down the wavefront by 1 and by 3 lanes gives each lane its partners
+        // +1 and +3
+        // (the empty asm keeps the move out of the select's branch: every
+        // lane takes part, so the source lanes 52..54 are live)
+        T nx = lanes_dpp<DPP_ROW_SHR9>(sqn);
+        asm volatile("" : "+v"(nx));
+        const T t = lane < 55 ? sq : nx;
+        const T p1 = lanes_dpp<DPP_WAVE_ROR1>(t);
+        const T p3 = lanes_dpp<DPP_WAVE_ROR1>(lanes_dpp<DPP_WAVE_ROR1>(p1));
         // r2 = fl(fl(x*x) + fl(y*y)), the walk's sums (0 + x*x) + y*y and
         // 0 + (x*x + y*y); a point re-rolls when r2 > 1 (the f64 test's
         // r2 < MIN_POSITIVE^2 compares with 0, which a sum of squares never
         // is below)
-        const T r1 = sq + s[live ? j + 1 : 0], r3 = sq + s[live ? j + 3 : 0];
+        const T r1 = sq + p1, r3 = sq + p3;
         constexpr uint64_t LIVE = (1ull << 55) - 1;
         const uint64_t b1 = __builtin_bitreverse64(__ballot(r1 <= (T)1.) & LIVE) >> 9;
         const uint64_t b3 = __builtin_bitreverse64(__ballot(r3 <= (T)1.) & LIVE) >> 9;
@@ -604,10 +622,8 @@ template <int BITS> __global__ __launch_bounds__(64) void ranf_expand_kernel(Ran
     const RanfLanes l = ranf_lanes();
     __shared__ uint64_t bm[2][BITS ? 32 : 1];
     using BT = typename std::conditional<BITS == 2, float, double>::type;
-    __shared__ BT bsq[BITS ? 112 : 1];
     RanfBits<BT> bits;
     if (BITS) {
-        bits.s = bsq;
         const uint64_t w0 = piece * a.bits_piece_words + (seg - a.seg_first) * (55ull * (uint64_t)a.R / 32);
         bits.m1 = bm[0];
         bits.m3 = bm[1];
