@@ -48,7 +48,7 @@ struct Ctx {
     int cus = 1;                    // compute units of the device
     int blocks_per_cu[5][2] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}}; // resident fused workgroups per CU [source][precision]
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr}; // [4]: the device walk's status copy
     std::string err;
     DevBuf mt;       // M^(2^k)
     int km = 0;
@@ -72,6 +72,8 @@ struct Ctx {
     // joined pieces (begin, length, output offset), and host staging of them
     DevBuf fe_lists, fe_meta, fe_piece;
     DevBuf fe_flag;  // jump seeding: set by a batch walk that left its piece
+    DevBuf fe_stat;  // the device walk's stitch status: failed, start n
+    uint64_t *fe_stat_host = nullptr; // its pinned host copy (hipHostMalloc)
     DevBuf fe_seq;   // jump seeding: batches left to the one-thread walk
     DevBuf xo_states, xo_draws; // faster-evgen on xoshiro256+: piece states and uniforms
     DevBuf xo128_states, xo128_draws, xo128_pt; // the same on xoshiro128+ (f32), and the piece matrix powers
@@ -82,10 +84,7 @@ struct Ctx {
     int xo128_pt_kb = 0;
     DevBuf adj_u;               // tp_adjoint: the stream's points, expanded
     uint32_t fe_flag_host = 0;
-    std::vector<uint32_t> fe_counts, fe_beg, fe_len;
-    std::vector<int32_t> fe_jprev, fe_jnext;
-    std::vector<uint64_t> fe_off;
-    uint64_t fe_qnext = 0;
+    uint64_t fe_qnext = 0;  // the device walk's start n: where the next chunk begins
     uint64_t fe_xo_end = 0; // faster-evgen on xoshiro256+ with skip-ahead seeding: draws the last run consumed
     uint64_t fe_host_chunks = 0; // faster-evgen: chunks of the last run walked on the host
 };

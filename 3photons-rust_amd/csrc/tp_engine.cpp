@@ -358,6 +358,7 @@ constexpr uint64_t FE_DEVICE_WALK_MIN = 1u << 20;
 constexpr uint64_t FE_WALK_BATCHES = 6144; // batches per chunk: two strides of the event-loop grid
 constexpr double FE_POSITIONS_PER_EVENT = 18.5; // stream positions allowed for per event (mean 17.9)
 constexpr uint64_t FE_EVENTS_PER_WALKER = 256, FE_MAX_WALKERS = 1u << 18;
+static_assert(FE_MAX_WALKERS <= 256ull * tpk::FE_STITCH_THREADS, "the device stitch's block sums fit one workgroup");
 // Positions a walker goes past its segment's end: 24 rounds, ~74 events; a
 // walker meets the true walk within 49 events in the runs checked.
 constexpr uint64_t FE_OVERLAP = 24 * 55;
@@ -620,6 +621,12 @@ int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, boo
         rc = ensure(c, c->fe_piece, (size_t)T * (2 * sizeof(uint32_t) + sizeof(uint64_t)));
     if (!rc)
         rc = ensure(c, c->evq, (n + 1) * sizeof(uint64_t));
+    if (!rc)
+        rc = ensure(c, c->fe_stat, tpk::FE_STITCH_STATUS * sizeof(uint64_t));
+    if (!rc && !c->fe_stat_host && hipHostMalloc(reinterpret_cast<void **>(&c->fe_stat_host), 2 * sizeof(uint64_t)) != hipSuccess) {
+        c->fe_stat_host = nullptr;
+        rc = fail(c, TP_ERR_NOMEM, "hipHostMalloc of the walk status failed");
+    }
     if (rc)
         return rc;
     w.buf = static_cast<const int32_t *>(c->draws.p);
@@ -631,48 +638,24 @@ int fe_device_walk(Ctx *c, const FeStream &fs, uint64_t q_start, uint64_t n, boo
     w.join_next = w.join_prev + T;
     tpk::launch_fe_walk(w, c->stream);
     HIPCHK(c, hipGetLastError());
-    c->fe_counts.resize(T);
-    c->fe_jprev.resize(T);
-    c->fe_jnext.resize(T);
-    HIPCHK(c, hipMemcpyAsync(c->fe_counts.data(), w.counts, T * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->fe_jprev.data(), w.join_prev, T * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->fe_jnext.data(), w.join_next, T * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    // The true walk: walker 0's list from its start; walker k's from the index
-    // where walker k-1 met it, which must come after walker k-1 itself joined.
-    c->fe_beg.assign(T, 0);
-    c->fe_len.assign(T, 0);
-    c->fe_off.assign(T, 0);
-    uint32_t begin = 0;
-    for (uint32_t k = 1; k < T; ++k) {
-        if (c->fe_jprev[k] < 0 || (uint32_t)c->fe_jprev[k] < begin)
-            return TP_OK;
-        c->fe_beg[k - 1] = begin;
-        c->fe_len[k - 1] = (uint32_t)c->fe_jprev[k] - begin;
-        begin = (uint32_t)c->fe_jnext[k];
-    }
-    if (c->fe_counts[T - 1] < begin)
-        return TP_OK;
-    c->fe_beg[T - 1] = begin;
-    c->fe_len[T - 1] = c->fe_counts[T - 1] - begin;
-    uint64_t have = 0;
-    for (uint32_t k = 0; k < T; ++k) { // keep exactly n + 1 starts
-        c->fe_off[k] = have;
-        c->fe_len[k] = (uint32_t)std::min<uint64_t>(c->fe_len[k], n + 1 - have);
-        have += c->fe_len[k];
-    }
-    if (have < n + 1)
-        return TP_OK;
+    // The true walk, chained on the device: walker 0's list from its start,
+    // walker k's from the index where walker k-1 met it, which must come after
+    // walker k-1 itself joined; the first n + 1 starts go to evq.  The host
+    // reads the status (and start n, where the next chunk begins) while the
+    // copies run.
     uint32_t *dbeg = static_cast<uint32_t *>(c->fe_piece.p), *dlen = dbeg + T;
     uint64_t *doff = reinterpret_cast<uint64_t *>(dlen + T);
-    HIPCHK(c, hipMemcpyAsync(dbeg, c->fe_beg.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(dlen, c->fe_len.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(doff, c->fe_off.data(), T * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    uint64_t *dstat = static_cast<uint64_t *>(c->fe_stat.p);
+    tpk::launch_fe_stitch(w, n + 1, dbeg, dlen, doff, dstat, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->fe_stat_host, dstat, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     tpk::launch_fe_scatter(w, dbeg, dlen, doff, static_cast<uint64_t *>(c->evq.p), c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(&c->fe_qnext, static_cast<uint64_t *>(c->evq.p) + n, sizeof(uint64_t),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream)); // the staging vectors are reused
+    HIPCHK(c, hipEventSynchronize(c->ev[4]));
+    if (c->fe_stat_host[0]) // a failed walk's scatter copies only pieces that lie in their lists
+        return TP_OK;
+    c->fe_qnext = c->fe_stat_host[1];
     *qbase = w.qbase;
     *bbase = w.bbase;
     *blimit = w.qlimit;
@@ -1515,7 +1498,7 @@ extern "C" int tp_destroy(void *ctx)
         for (DevBuf *b : {&c->mt, &c->polys, &c->draws, &c->origin, &c->p0, &c->per_batch, &c->merged, &c->evq,
                           &c->mt128, &c->bt128, &c->st128, &c->f32ev, &c->fe_lists, &c->fe_meta, &c->fe_piece,
                           &c->fe_flag, &c->fe_seq, &c->xo_states, &c->xo_draws, &c->xo128_states,
-                          &c->xo128_draws, &c->xo128_pt, &c->xo_acc, &c->xo_lt, &c->xo128_lt, &c->adj_u})
+                          &c->xo128_draws, &c->xo128_pt, &c->xo_acc, &c->xo_lt, &c->xo128_lt, &c->fe_stat, &c->adj_u})
             if (b->p)
                 (void)hipFree(b->p);
         for (auto &kv : c->tables) {
@@ -1527,6 +1510,8 @@ extern "C" int tp_destroy(void *ctx)
         for (auto &e : c->ev)
             if (e)
                 (void)hipEventDestroy(e);
+        if (c->fe_stat_host)
+            (void)hipHostFree(c->fe_stat_host);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return TP_OK;

@@ -155,6 +155,16 @@ struct FeWalkArgs {
 void launch_fe_walk(const FeWalkArgs &a, hipStream_t st);
 // Copies the pieces of the joined walk into out: piece k is lists[k][begin[k] ..
 // begin[k] + len[k]) at out[off[k] ..].
+// The chaining of the joined walk on the device: piece k of the first `need`
+// starts is lists[k][begin[k] .. begin[k] + len[k]) at out[off[k] ..];
+// status[0] = 1 when some walker does not meet its successor after its own
+// join, or the pieces hold fewer than `need` starts; else status[1] = start
+// need - 1.  status holds FE_STITCH_STATUS words (the rest: per-block sums and
+// flags); at most 256 * FE_STITCH_THREADS walkers.
+constexpr int FE_STITCH_THREADS = 1024;
+constexpr int FE_STITCH_STATUS = 2 + 2 * FE_STITCH_THREADS;
+void launch_fe_stitch(const FeWalkArgs &a, uint64_t need, uint32_t *begin, uint32_t *len, uint64_t *off,
+                      uint64_t *status, hipStream_t st);
 void launch_fe_scatter(const FeWalkArgs &a, const uint32_t *begin, const uint32_t *len, const uint64_t *off,
                        uint64_t *out, hipStream_t st);
 

@@ -1047,6 +1047,109 @@ void launch_fe_walk(const FeWalkArgs &a, hipStream_t st)
     hipLaunchKernelGGL(fe_join_kernel, dim3(blocks), dim3(256), 0, st, a);
 }
 
+// Walker k's piece of the true walk is its list from where walker k - 1 met
+// it (walker 0: from its start) to where it meets walker k + 1 (the last
+// walker: to its end); it must meet walker k + 1 after that point.  The
+// pieces are cut to the first `need` starts: off = min(prefix, need).  Three
+// passes over blocks of FE_STITCH_BLOCK walkers (at most FE_STITCH_THREADS
+// blocks): each piece and its block's sum and failure; the scan of the block
+// sums (one workgroup) and the status; each block's own scan, the cut and
+// start need - 1.  status[2 ..] holds the blocks' sums (then offsets) and
+// status[2 + FE_STITCH_THREADS ..] their failures.
+constexpr int FE_STITCH_BLOCK = 256;
+
+__global__ __launch_bounds__(FE_STITCH_BLOCK) void fe_stitch_pieces_kernel(FeWalkArgs a, uint32_t *beg, uint32_t *len,
+                                                                            uint64_t *status)
+{
+    __shared__ uint64_t s[FE_STITCH_BLOCK];
+    __shared__ uint32_t anybad;
+    const uint32_t T = a.nwalkers, k = blockIdx.x * FE_STITCH_BLOCK + threadIdx.x;
+    if (threadIdx.x == 0)
+        anybad = 0;
+    uint32_t l = 0;
+    bool ok = true;
+    if (k < T) {
+        const uint32_t b = k == 0 ? 0u : (uint32_t)a.join_next[k];
+        const int64_t e = k + 1 < T ? (int64_t)a.join_prev[k + 1] : (int64_t)a.counts[T - 1];
+        ok = e >= 0 && (uint64_t)e >= (uint64_t)b;
+        l = ok ? (uint32_t)(e - (int64_t)b) : 0u;
+        beg[k] = b;
+        len[k] = l;
+    }
+    s[threadIdx.x] = l;
+    __syncthreads();
+    if (!ok)
+        atomicOr(&anybad, 1u);
+    for (uint32_t d = FE_STITCH_BLOCK / 2; d > 0; d >>= 1) {
+        if (threadIdx.x < d)
+            s[threadIdx.x] += s[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        status[2 + blockIdx.x] = s[0];
+        status[2 + FE_STITCH_THREADS + blockIdx.x] = anybad;
+    }
+}
+
+__global__ __launch_bounds__(FE_STITCH_THREADS) void fe_stitch_scan_kernel(uint32_t nblocks, uint64_t need,
+                                                                           uint64_t *status)
+{
+    __shared__ uint64_t part[FE_STITCH_THREADS];
+    __shared__ uint32_t anybad;
+    const uint32_t t = threadIdx.x;
+    if (t == 0)
+        anybad = 0;
+    const uint64_t v = t < nblocks ? status[2 + t] : 0;
+    part[t] = v;
+    __syncthreads();
+    if (t < nblocks && status[2 + FE_STITCH_THREADS + t])
+        atomicOr(&anybad, 1u);
+    for (uint32_t d = 1; d < FE_STITCH_THREADS; d <<= 1) { // inclusive scan
+        const uint64_t u = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += u;
+        __syncthreads();
+    }
+    if (t < nblocks)
+        status[2 + t] = part[t] - v; // the block's first walker's prefix
+    if (t == 0)
+        status[0] = (anybad || part[FE_STITCH_THREADS - 1] < need) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(FE_STITCH_BLOCK) void fe_stitch_cut_kernel(FeWalkArgs a, uint64_t need, const uint32_t *beg,
+                                                                         uint32_t *len, uint64_t *off, uint64_t *status)
+{
+    __shared__ uint64_t s[FE_STITCH_BLOCK];
+    const uint32_t T = a.nwalkers, t = threadIdx.x, k = blockIdx.x * FE_STITCH_BLOCK + t;
+    const uint32_t l = k < T ? len[k] : 0u;
+    s[t] = l;
+    __syncthreads();
+    for (uint32_t d = 1; d < FE_STITCH_BLOCK; d <<= 1) { // inclusive scan
+        const uint64_t u = t >= d ? s[t - d] : 0;
+        __syncthreads();
+        s[t] += u;
+        __syncthreads();
+    }
+    if (k >= T)
+        return;
+    const uint64_t o = min(status[2 + blockIdx.x] + s[t] - l, need);
+    const uint32_t cut = (uint32_t)min<uint64_t>(l, need - o);
+    if (o < need && need - 1 < o + cut) // this piece holds start need - 1
+        status[1] = a.lists[(size_t)k * a.cap + beg[k] + (need - 1 - o)];
+    off[k] = o;
+    len[k] = cut;
+}
+
+void launch_fe_stitch(const FeWalkArgs &a, uint64_t need, uint32_t *begin, uint32_t *len, uint64_t *off,
+                      uint64_t *status, hipStream_t st)
+{
+    const uint32_t nblocks = (a.nwalkers + FE_STITCH_BLOCK - 1) / FE_STITCH_BLOCK; // <= FE_STITCH_THREADS
+    hipLaunchKernelGGL(fe_stitch_pieces_kernel, dim3(nblocks), dim3(FE_STITCH_BLOCK), 0, st, a, begin, len, status);
+    hipLaunchKernelGGL(fe_stitch_scan_kernel, dim3(1), dim3(FE_STITCH_THREADS), 0, st, nblocks, need, status);
+    hipLaunchKernelGGL(fe_stitch_cut_kernel, dim3(nblocks), dim3(FE_STITCH_BLOCK), 0, st, a, need, begin, len, off,
+                       status);
+}
+
 // One block per piece.
 __global__ __launch_bounds__(256) void fe_scatter_kernel(FeWalkArgs a, const uint32_t *begin, const uint32_t *len,
                                                          const uint64_t *off, uint64_t *out)
