@@ -216,15 +216,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) void
         Acc<PREC> A;
         acc_zero(A);
         uint32_t qn = 0; // queued events (uniform)
+        // faster-evgen: the lane's next event start, loaded an event ahead
+        constexpr bool FE = SOURCE == 3 || SOURCE == 4;
+        uint64_t qnext = FE && len > 0 ? a.evq[wbase + lane] : 0;
         for (uint32_t k = 0; k < iters; ++k) {
             bool pass = false;
             double p[3][4];
             int d[3]; // sort slots of the three photons
+            const uint64_t qk = qnext;
+            if (FE && k + 1 < len)
+                qnext = a.evq[wbase + (size_t)((k + 1) * 64 + lane)];
             if (k < len && SOURCE == 3) {
-                RanfWalk w{a.draws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)], rb};
+                RanfWalk w{a.draws, a.qbase, qk, rb};
                 pass = generate_faster_f64(fresh_params(), w, p, d);
             } else if (k < len && SOURCE == 4) {
-                XoWalk w{a.xdraws, a.qbase, a.evq[wbase + (size_t)(k * 64 + lane)], rb};
+                XoWalk w{a.xdraws, a.qbase, qk, rb};
                 pass = generate_faster_f64(fresh_params(), w, p, d);
             } else if (k < len) {
                 double u[12];
@@ -1727,17 +1733,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
         }
         const uint32_t iters = q + (r ? 1u : 0u);
         uint32_t qn = 0; // queued events (uniform)
+        // faster-evgen: the lane's next event start, loaded an event ahead
+        uint64_t qnext = FE && len > 0 ? a.evq[lb * a.batch_size + e0] : 0;
         for (uint32_t i = 0; i < iters; ++i) {
             const uint64_t e = lb * a.batch_size + e0 + (uint64_t)i * estep;
             bool pass = false;
             float p[3][4];
+            const uint64_t qk = qnext;
+            if (FE && i + 1 < len)
+                qnext = a.evq[e + estep];
             if (i < len && FE) {
                 const RerollBits rb{a.acc1, a.bbase, a.bstride ? (lb + 1) * a.bstride : a.blimit};
                 if constexpr (SOURCE == 2) {
-                    RanfWalkF32 w{a.draws, a.qbase, a.evq[e], rb};
+                    RanfWalkF32 w{a.draws, a.qbase, qk, rb};
                     pass = generate_faster_f32(a.P, w, p);
                 } else {
-                    Xo128WalkF32 w{a.x32draws, a.qbase, a.evq[e], rb};
+                    Xo128WalkF32 w{a.x32draws, a.qbase, qk, rb};
                     pass = generate_faster_f32(a.P, w, p);
                 }
                 if (!pass) {
