@@ -646,6 +646,8 @@ template <int BITS> __global__ __launch_bounds__(64) void ranf_expand_kernel(Ran
         return xn;
     };
     // X[lane+1] of round r is emitted at stream position 55 r + 54 - lane.
+    // (A round's store is 220 bytes, not whole lines: the L2 merges them, so
+    // they are ordinary stores; streaming stores took twice as long.)
     // Rounds [r_begin, r_end) reach into [q_begin, q_end); of those,
     // [f_lo, f_hi) lie wholly inside and are stored without range checks.
     const uint64_t r_end = min(r_begin + (uint64_t)a.R, (q_end + 54) / 55);
@@ -1420,9 +1422,9 @@ __device__ __forceinline__ void xo_expand_piece(Gen g, T *piece, uint32_t *acc1,
             // each store: 64 / B lanes' blocks, B consecutive positions each
             const uint64_t pos = (uint64_t)grp * 128 + (uint64_t)b * B;
 #pragma unroll 4
-            for (int r = 0; r < B; ++r) {
+            for (int r = 0; r < B; ++r) { // streaming stores of whole lines: a later kernel reads them
                 const uint32_t src = r * (64 / B) + lane / B, i = lane % B;
-                piece[(uint64_t)src * pt + pos + i] = st[src * ROW + i];
+                __builtin_nontemporal_store(st[src * ROW + i], &piece[(uint64_t)src * pt + pos + i]);
             }
             __syncthreads();
         }
