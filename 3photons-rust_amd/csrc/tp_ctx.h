@@ -87,6 +87,7 @@ struct Ctx {
     uint64_t fe_qnext = 0;  // the device walk's start n: where the next chunk begins
     uint64_t fe_xo_end = 0; // faster-evgen on xoshiro256+ with skip-ahead seeding: draws the last run consumed
     uint64_t fe_host_chunks = 0; // faster-evgen: chunks of the last run walked on the host
+    uint64_t fe_chunks = 0;      // faster-evgen: chunks the last run was split into
 };
 
 constexpr int KM = 32;              // intra-call offsets < 2^32 draws

@@ -355,7 +355,7 @@ int check_common(Ctx *c, const tp_params *params, int rng_kind, const void *stat
 // joined where they meet); shorter runs, and any chunk whose device walkers
 // do not all join, take the host walk.
 constexpr uint64_t FE_DEVICE_WALK_MIN = 1u << 20;
-constexpr uint64_t FE_WALK_BATCHES = 6144; // batches per chunk: two strides of the event-loop grid
+constexpr uint64_t FE_WALK_BATCHES = 12288; // batches per chunk: four strides of the event-loop grid
 constexpr double FE_POSITIONS_PER_EVENT = 18.5; // stream positions allowed for per event (mean 17.9)
 constexpr uint64_t FE_EVENTS_PER_WALKER = 256, FE_MAX_WALKERS = 1u << 18;
 static_assert(FE_MAX_WALKERS <= 256ull * tpk::FE_STITCH_THREADS, "the device stitch's block sums fit one workgroup");
@@ -688,6 +688,7 @@ int enqueue_fe_chunks(Ctx *c, const FeStream &fs, const FeHostWalk &host_walk, t
     const int source = fs.kind == 2 ? 4 : 3;
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        ++c->fe_chunks;
         const uint64_t b1 = std::min(nb, b0 + chunk);
         const uint64_t e0 = b0 * batch_size, e1 = std::min<uint64_t>(b1 * batch_size, num_events);
         const uint64_t n = e1 - e0;
@@ -928,6 +929,7 @@ int enqueue_ranf_faster_evgen_jump(Ctx *c, const tp_params *params, tpk::FusedAr
     };
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        ++c->fe_chunks;
         const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -1068,6 +1070,7 @@ int enqueue_xo_faster_evgen_jump(Ctx *c, tpk::FusedArgs &a, const tp_xoshiro_sta
     w.overflow = static_cast<uint32_t *>(c->fe_flag.p);
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        ++c->fe_chunks;
         const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -1164,6 +1167,7 @@ int enqueue_xo128_faster_evgen(Ctx *c, const tp_params *params, int seeding, con
     w.overflow = static_cast<uint32_t *>(c->fe_flag.p);
     bool first = true;
     for (uint64_t b0 = 0; b0 < nb; b0 += chunk) {
+        ++c->fe_chunks;
         const uint64_t cnb = std::min(nb, b0 + chunk) - b0;
         if (first)
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -1802,6 +1806,8 @@ extern "C" int tp_adjoint(void *ctx, const tp_params *params, int rng_kind, cons
             c->last.events = n;
             c->last.batches = 0;
             c->last.host_walk_chunks = 0;
+        c->last.chunks = 0;
+            c->last.chunks = 0;
         }
         return rc;
     });
@@ -1883,6 +1889,7 @@ extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, i
         // Event 0 and 1 brackets the first kernel; 2 follows the event loop.
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         c->fe_host_chunks = 0;
+        c->fe_chunks = 0;
         rc = enqueue_run(c, params, rng_kind, seeding, rng_state0, num_events, TP_BATCH_SIZE, per_batch_dev);
         if (rc)
             return rc;
@@ -1894,6 +1901,7 @@ extern "C" int tp_run_device(void *ctx, const tp_params *params, int rng_kind, i
         c->last.events = num_events;
         c->last.batches = nb;
         c->last.host_walk_chunks = c->fe_host_chunks;
+        c->last.chunks = c->fe_chunks;
         c->timed = true;
         if (stats)
             *stats = c->last;
@@ -1963,6 +1971,7 @@ extern "C" int tp_run_device_noalloc(void *ctx, const tp_params *params, int rng
         c->last.events = num_events;
         c->last.batches = nb;
         c->last.host_walk_chunks = 0;
+        c->last.chunks = 0;
         c->timed = true;
         if (stats)
             *stats = c->last;

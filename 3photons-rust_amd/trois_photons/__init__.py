@@ -69,7 +69,7 @@ class Xoshiro128State(C.Structure):
 
 class TpStats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("kernel_ms", C.c_double), ("events", C.c_uint64),
-                ("batches", C.c_uint64), ("host_walk_chunks", C.c_uint64)]
+                ("batches", C.c_uint64), ("host_walk_chunks", C.c_uint64), ("chunks", C.c_uint64)]
 
 
 class TpConfig(C.Structure):

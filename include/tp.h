@@ -126,6 +126,8 @@ typedef struct {
     uint64_t host_walk_chunks; /* faster-evgen with skip-ahead seeding: chunks of batches whose event
                                   starts came from the host walk (runs below 2^20 events, the
                                   closure, or device walkers that did not all meet) */
+    uint64_t chunks;           /* faster-evgen: chunks of batches the run was split into (each one's
+                                  stream expanded, walked and simulated in turn); 0 otherwise */
 } tp_stats;
 
 /* Context: binds device `device` (HIP ordinal), creates its stream. */

@@ -9,6 +9,13 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "3photons-rust_amd")
+
+# faster-evgen runs go through the engine in chunks of 12 288 batches of
+# 10 000 events (tp_stats.chunks reports how many): 1.24e8 events are two
+# chunks, 12 288 + 112 batches.  Their first two batches, the two at the
+# chunk edge and the last.
+FE_TWO_CHUNKS = 124_000_000
+FE_EDGE_BATCHES = (0, 1, 12287, 12288, 12399)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 for p in (PKG, os.path.join(ROOT, "tests")):
     if p not in sys.path:

@@ -3,13 +3,14 @@ as (n as f32) * 1e-9 and the xoshiro128+ stream, the per-event path in f32
 with the reference's operations, batch sums in event order, f32 fold and
 finalize, against the f32 oracle batch by batch and against the reference's
 shipped f32 outputs at N = 1e7."""
+import json
 import math
 
 import numpy as np
 import pytest
 
 import trois_photons as tp
-from conftest import golden_path
+from conftest import FE_EDGE_BATCHES, FE_TWO_CHUNKS, golden_path
 from numdiff import printed_match
 from resdata import golden_numbers, printed_values
 
@@ -30,9 +31,11 @@ def cfg(valeurs):
     return tp.Configuration.load(valeurs)
 
 
-def close_f32(a, b, n_events):
+def close_f32(a, b, n_events, limit=2e-5):
     """Selected counts exact; sums to the f32 rounding that one-ulp
-    differences of sinf/cosf/logf (ocml against glibc) leave."""
+    differences of sinf/cosf/logf (ocml against glibc) leave (limit: see
+    F32_CONDITIONING_LIMIT for the batches that hold an ill-conditioned
+    event)."""
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     assert a[0] == b[0], (a[0], b[0], n_events)
     assert np.array_equal(np.isnan(a), np.isnan(b)), (a, b, n_events)  # NaN batches: see test_f32_spinor_nan_*
@@ -42,8 +45,43 @@ def close_f32(a, b, n_events):
         rel[4] = abs(a[5] - b[5]) / max(abs(b[10]) ** 0.5 * max(b[0], 1) ** 0.5, 1e-300)
     rel = np.where(np.isnan(b[1:]), 0.0, rel)
     worst = float(rel.max()) if len(rel) else 0.0
-    assert rel.max() <= 2e-5, (n_events, rel)
+    assert rel.max() <= limit, (n_events, rel)
     return worst
+
+
+def conditioning_state():
+    """The RANF state before the f32 faster-evgen event that f32 arithmetic
+    gets percents away from f64 (tests/golden/f32_conditioning_event.json,
+    pinned against both oracles by test_oracle_f32_conditioning_event), and
+    the batch that holds it."""
+    with open(golden_path("f32_conditioning_event.json")) as f:
+        d = json.load(f)
+    st = tp.rng_state(tp.RNG_RANF)
+    for i, x in enumerate(d["ranf_numbers"]):
+        st.numbers[i] = x
+    st.index = d["ranf_index"]
+    return st, d["batch"]
+
+
+# A batch holding that event differs from the f32 oracle by up to one event's
+# share of its sums (the event's own sums differ by percents): 9.3e-5 measured.
+# Such events come about once in 1e8 on each stream.
+F32_CONDITIONING_LIMIT = 5e-4
+
+
+def close_f32_batches(per, oper):
+    """Every batch of a run of whole batches against the f32 oracle's: counts
+    exact in all; sums within 2e-5 in all but one batch in 4000, which stay
+    within F32_CONDITIONING_LIMIT.  Returns those batches."""
+    loose = []
+    for b in range(len(per)):
+        row = per[b]
+        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
+        if close_f32(a, oper[b].as_array(), 10000, F32_CONDITIONING_LIMIT) > 2e-5:
+            loose.append(b)
+    print(f"  batches beyond 2e-5 (ill-conditioned events): {loose}")
+    assert len(loose) <= max(1, len(per) // 4000), loose
+    return loose
 
 
 @pytest.mark.parametrize("rng", ["ranf", "xoshiro"])
@@ -239,13 +277,14 @@ def test_f32_faster_evgen_jump_per_batch_matches_oracle(engine, cfg, oracle32, v
 
 
 def test_f32_faster_evgen_jump_across_chunks(engine, cfg, oracle32, valeurs):
-    n_events = 62_000_000
+    n_events = FE_TWO_CHUNKS
     print(f"f32 faster-evgen jump N={n_events}", flush=True)
     params = tp.params_from_config(cfg.with_events(n_events), precision=tp.PREC_F32, faster_evgen=True)
-    per, _, _ = engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, tp.rng_state(tp.RNG_RANF), n_events)
+    per, _, stats = engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, tp.rng_state(tp.RNG_RANF), n_events)
+    assert stats.chunks == 2
     ocfg = oracle32.config(valeurs, n_events)
     opts = oracle32.opts("ranf", faster_evgen=True)
-    for b in (0, 6143, 6144, 6199):
+    for b in FE_EDGE_BATCHES:
         g = oracle32.rng("ranf")
         for _ in range(b):
             oracle32.jump(g)
@@ -274,20 +313,46 @@ def test_f32_faster_evgen_per_batch_matches_oracle(engine, cfg, oracle32, valeur
 
 
 def test_f32_faster_evgen_device_walk_across_chunks(engine, cfg, oracle32, valeurs):
-    """N = 6.2e7: the device walk with the unit-disc test in f32, two chunks;
+    """N = 1.24e8: the device walk with the unit-disc test in f32, two chunks;
     every batch against the f32 oracle's run."""
-    n_events = 62_000_000
+    n_events = FE_TWO_CHUNKS
     print(f"f32 faster-evgen device walk N={n_events}", flush=True)
     params = tp.params_from_config(cfg.with_events(n_events), precision=tp.PREC_F32, faster_evgen=True)
     per, merged, stats = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_RANF), n_events)
     assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    assert stats.chunks == 2
     print("oracle run", flush=True)
     om, oper = oracle32.run(oracle32.config(valeurs, n_events), oracle32.opts("ranf", threads=16, faster_evgen=True),
                             per_batch=True)
-    for b in range(len(per)):
-        row = per[b]
-        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
-        close_f32(a, oper[b].as_array(), 10000)
+    _, cond_batch = conditioning_state()
+    assert close_f32_batches(per, oper) == [cond_batch]
+    close_f32(merged.as_array(), om.as_array(), n_events)
+
+
+def test_f32_faster_evgen_conditioning_event(engine, cfg, oracle, oracle32, valeurs):
+    """The one event of the 1.24e8-event f32 faster-evgen RANF run whose sums
+    the GPU's f32 and the f32 oracle put percents apart: from the same state,
+    the GPU's one-event closure selects it as both oracles do, consumes the
+    same draws, and lands no further from the f64 oracle than four times the
+    f32 oracle does."""
+    st, _ = conditioning_state()
+    params = tp.params_from_config(cfg, precision=tp.PREC_F32, faster_evgen=True)
+    a = engine.simulate_batch(params, tp.RNG_RANF, st, 1).as_array()
+    errs = []
+    for lib in (oracle32, oracle):
+        g = lib.rng("ranf")
+        st0, _ = conditioning_state()
+        for i in range(56):
+            g.numbers[i] = st0.numbers[i]
+        g.index = st0.index
+        o = lib.simulate_batch(lib.config(valeurs), lib.opts("ranf", faster_evgen=True), g, 1).as_array()
+        assert a[0] == o[0] == 1 and st.index == g.index == 2
+        errs.append(o)
+    o32, o64 = errs
+    gpu_err = float(np.max(np.abs(a[1:] - o64[1:]) / np.abs(o64[1:])))
+    oracle_err = float(np.max(np.abs(o32[1:] - o64[1:]) / np.abs(o64[1:])))
+    print(f"  relative distance to f64: GPU f32 {gpu_err:.3f}, f32 oracle {oracle_err:.3f}")
+    assert gpu_err <= 4 * oracle_err, (gpu_err, oracle_err)
 
 
 def test_f32_faster_evgen_simulate_batch_advances_like_the_closure(engine, cfg, oracle32, valeurs):
@@ -327,23 +392,22 @@ def test_f32_faster_evgen_xoshiro128_per_batch_matches_oracle(engine, cfg, oracl
 
 @pytest.mark.parametrize("jump", [False, True])
 def test_f32_faster_evgen_xoshiro128_across_chunks(engine, cfg, oracle32, valeurs, jump):
-    """N = 6.2e7 (two chunks): with skip-ahead seeding the device walk of the
+    """N = 1.24e8 (two chunks): with skip-ahead seeding the device walk of the
     one stream, with jump seeding the per-batch walks; every batch against the
     f32 oracle's run."""
-    n_events = 62_000_000
+    n_events = FE_TWO_CHUNKS
     print(f"f32 faster-evgen xoshiro128 N={n_events} jump={jump}", flush=True)
     params = tp.params_from_config(cfg.with_events(n_events), precision=tp.PREC_F32, faster_evgen=True)
     seeding = tp.SEED_JUMP if jump else tp.SEED_SKIPAHEAD
     per, merged, stats = engine.run(params, tp.RNG_XOSHIRO128, seeding, tp.rng_state(tp.RNG_XOSHIRO128), n_events)
     assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    assert stats.chunks == 2
     print("oracle run", flush=True)
     om, oper = oracle32.run(oracle32.config(valeurs, n_events),
                             oracle32.opts("xoshiro", faster_threading=jump, threads=16, faster_evgen=True),
                             per_batch=True)
-    for b in range(len(per)):
-        row = per[b]
-        a = np.array([row["selected_events"], *row["spm2"], *row["vars"], row["sigma"], row["variance"]])
-        close_f32(a, oper[b].as_array(), 10000)
+    close_f32_batches(per, oper)
+    close_f32(merged.as_array(), om.as_array(), n_events)
 
 
 def test_f32_faster_evgen_xoshiro128_simulate_batch_advances_like_the_closure(engine, cfg, oracle32, valeurs):

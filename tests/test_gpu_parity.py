@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import trois_photons as tp
-from conftest import PKG, golden_path
+from conftest import FE_EDGE_BATCHES, FE_TWO_CHUNKS, PKG, golden_path
 from numdiff import compare_texts, tokens
 from test_host import to_tp_acc
 
@@ -170,24 +170,25 @@ def test_faster_evgen_per_batch_matches_oracle(engine, cfg, oracle, valeurs, n_e
 
 
 def test_faster_evgen_device_walk_across_chunks(engine, cfg, oracle, valeurs, walklib):
-    """N = 6.2e7 (6200 batches): the event starts come from the device walk,
-    in two chunks (6144 + 56 batches), the second from the first one's last
-    start.  Batches at the chunk edge and the last one against the oracle
-    started where the host walk (checked against the sequential walk in
-    test_walk.py) puts those batches."""
+    """N = 1.24e8 (12 400 batches): the event starts come from the device
+    walk, in two chunks (12 288 + 112 batches), the second from the first
+    one's last start.  Batches at the chunk edge and the last one against the
+    oracle started where the host walk (checked against the sequential walk
+    in test_walk.py) puts those batches."""
     import ctypes as C
-    n_events = 62_000_000
+    n_events = FE_TWO_CHUNKS
     print(f"faster-evgen device walk N={n_events}", flush=True)
     c = cfg.with_events(n_events)
     params = tp.params_from_config(c, faster_evgen=True)
     st0 = tp.rng_state(tp.RNG_RANF)
     per, _, stats = engine.run(params, tp.RNG_RANF, tp.SEED_SKIPAHEAD, st0, n_events)
     assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    assert stats.chunks == 2
     ocfg = oracle.config(valeurs, n_events)
     opts = oracle.opts("ranf", faster_evgen=True)
     numbers = (C.c_int32 * 56)(*list(st0.numbers))
     q0 = 55 - st0.index
-    for b in (0, 1, 6143, 6144, 6199):
+    for b in FE_EDGE_BATCHES:
         q = walklib.tp_walk_end(numbers, st0.index, 10000 * b, 8, 0)
         g = oracle.rng("ranf")
         oracle.skip(g, q - q0)
@@ -230,19 +231,20 @@ def test_faster_evgen_jump_per_batch_matches_oracle(engine, cfg, oracle, valeurs
 
 
 def test_faster_evgen_jump_across_chunks_and_from_mid_round(engine, cfg, oracle, valeurs):
-    """N = 6.2e7 (6 200 batches: chunks of 6 144) from a state 7 values into
-    its round: batch 0 runs on that state, batch b on seeded_new(seed +
+    """N = 1.24e8 (12 400 batches: chunks of 12 288) from a state 7 values
+    into its round: batch 0 runs on that state, batch b on seeded_new(seed +
     123456 b); batches at the chunk edge and the last one against the
     oracle started from those states."""
-    n_events = 62_000_000
+    n_events = FE_TWO_CHUNKS
     print(f"faster-evgen jump N={n_events} from mid-round", flush=True)
     params = tp.params_from_config(cfg.with_events(n_events), faster_evgen=True)
     st = tp.rng_state(tp.RNG_RANF)
     tp.rng_skip(tp.RNG_RANF, st, 7)
-    per, _, _ = engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, st, n_events)
+    per, _, stats = engine.run(params, tp.RNG_RANF, tp.SEED_JUMP, st, n_events)
+    assert stats.chunks == 2
     ocfg = oracle.config(valeurs, n_events)
     opts = oracle.opts("ranf", faster_evgen=True)
-    for b in (0, 1, 6143, 6144, 6199):
+    for b in FE_EDGE_BATCHES:
         g = oracle.rng("ranf")
         if b == 0:
             oracle.skip(g, 7)
@@ -274,17 +276,18 @@ def test_faster_evgen_xoshiro_jump_per_batch_matches_oracle(engine, cfg, oracle,
 
 
 def test_faster_evgen_xoshiro_jump_across_chunks(engine, cfg, oracle, valeurs):
-    """N = 6.2e7 (two chunks of batches): the batches at the chunk edge and the
-    last one against the oracle started from the state jumped b times."""
-    n_events = 62_000_000
+    """N = 1.24e8 (two chunks of batches): the batches at the chunk edge and
+    the last one against the oracle started from the state jumped b times."""
+    n_events = FE_TWO_CHUNKS
     print(f"faster-evgen xoshiro jump N={n_events}", flush=True)
     params = tp.params_from_config(cfg.with_events(n_events), faster_evgen=True)
-    per, _, _ = engine.run(params, tp.RNG_XOSHIRO, tp.SEED_JUMP, tp.rng_state(tp.RNG_XOSHIRO), n_events)
+    per, _, stats = engine.run(params, tp.RNG_XOSHIRO, tp.SEED_JUMP, tp.rng_state(tp.RNG_XOSHIRO), n_events)
+    assert stats.chunks == 2
     ocfg = oracle.config(valeurs, n_events)
     opts = oracle.opts("xoshiro", faster_evgen=True)
     g = oracle.rng("xoshiro")
     done = 0
-    for b in (0, 1, 6143, 6144, 6199):
+    for b in FE_EDGE_BATCHES:
         for _ in range(b - done):
             oracle.jump(g)
         done = b
@@ -316,15 +319,16 @@ def test_faster_evgen_xoshiro_per_batch_matches_oracle(engine, cfg, oracle, vale
 
 
 def test_faster_evgen_xoshiro_device_walk_across_chunks(engine, cfg, oracle, valeurs):
-    """N = 6.2e7 (6200 batches): the event starts come from the device walk
-    (segment walkers of 1024 events on the xoshiro256+ stream, joined where
-    they meet), in two chunks, the second from the first one's end.  Every
-    batch against the oracle's run."""
-    n_events = 62_000_000
+    """N = 1.24e8 (12 400 batches): the event starts come from the device
+    walk (segment walkers of 1024 events on the xoshiro256+ stream, joined
+    where they meet), in two chunks, the second from the first one's end.
+    Every batch against the oracle's run."""
+    n_events = FE_TWO_CHUNKS
     print(f"faster-evgen xoshiro device walk N={n_events}", flush=True)
     params = tp.params_from_config(cfg.with_events(n_events), faster_evgen=True)
     per, merged, stats = engine.run(params, tp.RNG_XOSHIRO, tp.SEED_SKIPAHEAD, tp.rng_state(tp.RNG_XOSHIRO), n_events)
     assert stats.host_walk_chunks == 0, "the device walkers did not all meet"
+    assert stats.chunks == 2
     print("oracle run", flush=True)
     om, oper = oracle.run(oracle.config(valeurs, n_events), oracle.opts("xoshiro", threads=16, faster_evgen=True),
                           per_batch=True)
