@@ -302,19 +302,22 @@ __device__ __forceinline__ bool event_adjoint_sweeps(const AdjParams<T> &P, cons
     const int A_pmm = 0, A_ppm = 1;
     const int aty[6] = {A_pmm, A_pmm, A_ppm, A_pmm, A_ppm, A_ppm};
     const int ak[6][3] = {{4, 2, 3}, {3, 2, 4}, {3, 4, 2}, {2, 3, 4}, {2, 4, 3}, {2, 3, 4}};
+    // ns01 and nt01 are real (the beams run along x): the amplitudes without
+    // couplings scale by their real parts
+    const AmpK<T> K{P.ns01, P.nt01, P.g_a, P.g_bp, P.g_bm, P.ns01.re, P.nt01.re, P.ns01.re, P.nt01.re, P.ns01.re, P.nt01.re};
     C<T> a[6], bp[6];
 #pragma unroll
     for (int h = 0; h < 6; ++h) {
         const int k1 = ak[h][0], k2 = ak[h][1], k3 = ak[h][2];
         if (aty[h] == A_pmm) {
-            a[h] = a_pmm(S, P.nt01, k1, k2, k3);
-            bp[h] = bp_pmm(S, P.ns01, k1, k2, k3);
+            a[h] = a_pmm<T, true>(S, K, k1, k2, k3);
+            bp[h] = bp_pmm<T, true>(S, K, k1, k2, k3);
         } else {
-            a[h] = a_ppm(S, P.ns01, k1, k2, k3);
-            bp[h] = bp_ppm(S, P.nt01, k1, k2, k3);
+            a[h] = a_ppm<T, true>(S, K, k1, k2, k3);
+            bp[h] = bp_ppm<T, true>(S, K, k1, k2, k3);
         }
     }
-    const C<T> bmm = bm_mmm(S, P.nt01), bpp = bm_ppp(S, P.ns01);
+    const C<T> bmm = bm_mmm<T, true>(S, K), bpp = bm_ppp<T, true>(S, K);
     // helicity sums (matelems.rs:65-86)
     {
         T sA = 0, sP = 0, sR = 0, sI = 0;

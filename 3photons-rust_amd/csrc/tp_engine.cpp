@@ -53,6 +53,14 @@ tpd::Params make_params(const tp_params *p)
     P.ns01im = N * s01im;
     P.nt01re = N * -s01re; // t = -conj(s)
     P.nt01im = N * s01im;
+    // The beams run along x: f0im = f1im = 0, so s01im = 0 and ns01, nt01 are
+    // real; the amplitudes scale by these products of them and the couplings.
+    P.ka_s = P.ns01re * P.g_a;
+    P.ka_t = P.nt01re * P.g_a;
+    P.kp_s = P.ns01re * P.g_bp;
+    P.kp_t = P.nt01re * P.g_bp;
+    P.km_s = P.ns01re * P.g_bm;
+    P.km_t = P.nt01re * P.g_bm;
     P.sort = p->sort_photons ? 1 : 0;
     return P;
 }

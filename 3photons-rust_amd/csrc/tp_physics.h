@@ -39,6 +39,9 @@ struct Params {
     double xx1, f1re, f1im;       // positron spinor
     double ns01re, ns01im;        // -RAC8 * s(e-, e+)
     double nt01re, nt01im;        // -RAC8 * t(e-, e+)
+    // ns01re and nt01re times g_a, g_bp and g_bm (the f64 and mixed
+    // amplitudes' real scale factors; the imaginary parts are +-0)
+    double ka_s, ka_t, kp_s, kp_t, km_s, km_t;
     int sort;
     int pad;
 };
@@ -257,7 +260,24 @@ template <class T> struct Spin {
     C<T> s[5][5]; // only off-diagonal entries are set
 };
 
-template <class T>
+// a b - c d: with FAST one fma (the f64 event loop), otherwise the
+// reference's two products and a difference.
+template <bool FAST, class T> __device__ __forceinline__ T mul_sub(T a, T b, T c, T d)
+{
+    if constexpr (FAST)
+        return fma(a, b, -(c * d));
+    else
+        return a * b - c * d;
+}
+template <bool FAST, class T> __device__ __forceinline__ T mul_add(T a, T b, T c, T d)
+{
+    if constexpr (FAST)
+        return fma(a, b, c * d);
+    else
+        return a * b + c * d;
+}
+
+template <class T, bool FAST = false>
 __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T f1re, T f1im, const T p[3][4],
                                                 Spin<T> &S)
 {
@@ -275,8 +295,8 @@ __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T 
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        C<T> s0k = mk<T>(f0re * xx[k] - fx[k].re * xx0, f0im * xx[k] - fx[k].im * xx0);
-        C<T> s1k = mk<T>(f1re * xx[k] - fx[k].re * xx1, f1im * xx[k] - fx[k].im * xx1);
+        C<T> s0k = mk<T>(mul_sub<FAST>(f0re, xx[k], fx[k].re, xx0), mul_sub<FAST>(f0im, xx[k], fx[k].im, xx0));
+        C<T> s1k = mk<T>(mul_sub<FAST>(f1re, xx[k], fx[k].re, xx1), mul_sub<FAST>(f1im, xx[k], fx[k].im, xx1));
         S.s[0][2 + k] = s0k;
         S.s[2 + k][0] = neg(s0k);
         S.s[1][2 + k] = s1k;
@@ -286,16 +306,28 @@ __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T 
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int l = k + 1; l < 3; ++l) {
-            C<T> skl = mk<T>(fx[k].re * xx[l] - fx[l].re * xx[k], fx[k].im * xx[l] - fx[l].im * xx[k]);
+            C<T> skl = mk<T>(mul_sub<FAST>(fx[k].re, xx[l], fx[l].re, xx[k]),
+                             mul_sub<FAST>(fx[k].im, xx[l], fx[l].im, xx[k]));
             S.s[2 + k][2 + l] = skl;
             S.s[2 + l][2 + k] = neg(skl);
         }
 }
 
-// Complex products of the amplitude algebra.  With FAST (f32 only) they
-// contract multiply-adds into fma and divide by a hardware reciprocal (the
-// mixed-precision path and the f32 adjoint); otherwise they are the
-// reference's plain operations (the f64 and all-f32 parity paths).
+// Complex products of the amplitude algebra.  With FAST they contract
+// multiply-adds into fma: in f64 (the f64 event loops) the result moves by an
+// ulp or so and the division stays the correctly rounded one; in f32 (the
+// mixed-precision path and the f32 adjoint) they also divide by a hardware
+// reciprocal.  Without FAST they are the reference's plain operations (the
+// all-f32 parity path).
+__device__ __forceinline__ C<double> cmul_fast(C<double> a, C<double> b)
+{
+    return C<double>{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ C<double> cdiv_fast(C<double> a, C<double> b)
+{
+    const double n = fma(b.re, b.re, b.im * b.im);
+    return C<double>{tpm::div_n(fma(a.re, b.re, a.im * b.im), n), tpm::div_n(fma(a.im, b.re, -(a.re * b.im)), n)};
+}
 __device__ __forceinline__ C<float> cmul_fast(C<float> a, C<float> b)
 {
     return C<float>{fmaf(a.re, b.re, -(a.im * b.im)), fmaf(a.re, b.im, a.im * b.re)};
@@ -307,78 +339,105 @@ __device__ __forceinline__ C<float> cdiv_fast(C<float> a, C<float> b)
 }
 template <bool FAST, class T> __device__ __forceinline__ C<T> xmul(C<T> a, C<T> b)
 {
-    if constexpr (FAST && std::is_same<T, float>::value)
+    if constexpr (FAST)
         return cmul_fast(a, b);
     else
         return cmul(a, b);
 }
 template <bool FAST, class T> __device__ __forceinline__ C<T> xdiv(C<T> a, C<T> b)
 {
-    if constexpr (FAST && std::is_same<T, float>::value)
+    if constexpr (FAST)
         return cdiv_fast(a, b);
     else
         return cdiv(a, b);
 }
+template <bool FAST, class T> __device__ __forceinline__ T xnorm(C<T> z) { return mul_add<FAST>(z.re, z.re, z.im, z.im); }
 template <bool FAST, class T> __device__ __forceinline__ C<T> xsq(C<T> z) { return xmul<FAST>(z, z); }
 
-// -RAC8*x*y/z is ((-RAC8*x)*y)/z; the (-RAC8)*s(e-,e+) and (-RAC8)*t(e-,e+)
-// factors are per-run constants (ns01, nt01).
-template <class T, bool FAST = true>
-__device__ __forceinline__ C<T> a_ppm(const Spin<T> &S, C<T> ns01, int k1, int k2, int k3)
+// Amplitude constants.  Plain (the all-f32 parity path): -RAC8*x*y/z is
+// ((-RAC8*x)*y)/z with the per-run factors (-RAC8) s(e-,e+) and (-RAC8)
+// t(e-,e+) (ns01, nt01), and the couplings multiply each amplitude after it.
+// FAST: the beams run along x (evgen.rs:66-69), so s(e-,e+) and t(e-,e+) are
+// real (imaginary parts +-0) for every configuration; each amplitude is its
+// spinor-product expression scaled by one real constant, (-RAC8) s or t times
+// its coupling, formed on the host (k_*; a few ulp from the reference's order
+// of operations).
+template <class T> struct AmpK {
+    C<T> ns01, nt01;
+    T g_a, g_bp, g_bm;
+    T ka_s, ka_t, kp_s, kp_t, km_s, km_t; // FAST: ns01 g_a, nt01 g_a, ns01 g_bp, nt01 g_bp, ns01 g_bm, nt01 g_bm
+};
+template <class T> __device__ __forceinline__ C<T> rscale(C<T> z, T k) { return C<T>{z.re * k, z.im * k}; }
+// The factor c (plain: complex, FAST: real k) times z.
+template <bool FAST, class T> __device__ __forceinline__ C<T> kmul(C<T> c, T k, C<T> z)
 {
-    C<T> num = xmul<FAST>(ns01, xsq<FAST>(S.s[0][k3]));
+    if constexpr (FAST)
+        return rscale(z, k);
+    else
+        return cmul(c, z);
+}
+
+template <class T, bool FAST>
+__device__ __forceinline__ C<T> a_ppm(const Spin<T> &S, const AmpK<T> &K, int k1, int k2, int k3)
+{
+    C<T> num = kmul<FAST>(K.ns01, K.ka_s, xsq<FAST>(S.s[0][k3]));
     C<T> den = xmul<FAST>(xmul<FAST>(xmul<FAST>(S.s[0][k1], S.s[0][k2]), S.s[1][k1]), S.s[1][k2]);
     return xdiv<FAST>(num, den);
 }
-template <class T, bool FAST = true>
-__device__ __forceinline__ C<T> a_pmm(const Spin<T> &S, C<T> nt01, int k1, int k2, int k3)
+template <class T, bool FAST>
+__device__ __forceinline__ C<T> a_pmm(const Spin<T> &S, const AmpK<T> &K, int k1, int k2, int k3)
 {
-    C<T> num = xmul<FAST>(nt01, xsq<FAST>(tof(S.s[1][k1])));
+    C<T> num = kmul<FAST>(K.nt01, K.ka_t, xsq<FAST>(tof(S.s[1][k1])));
     C<T> den = xmul<FAST>(xmul<FAST>(xmul<FAST>(tof(S.s[1][k2]), tof(S.s[1][k3])), tof(S.s[0][k2])), tof(S.s[0][k3]));
     return xdiv<FAST>(num, den);
 }
-template <class T, bool FAST = true>
-__device__ __forceinline__ C<T> bp_ppm(const Spin<T> &S, C<T> nt01, int k1, int k2, int k3)
+template <class T, bool FAST>
+__device__ __forceinline__ C<T> bp_ppm(const Spin<T> &S, const AmpK<T> &K, int k1, int k2, int k3)
 {
-    return xmul<FAST>(nt01, xsq<FAST>(xmul<FAST>(tof(S.s[k1][k2]), S.s[k3][0])));
+    return kmul<FAST>(K.nt01, K.kp_t, xsq<FAST>(xmul<FAST>(tof(S.s[k1][k2]), S.s[k3][0])));
 }
-template <class T, bool FAST = true>
-__device__ __forceinline__ C<T> bp_pmm(const Spin<T> &S, C<T> ns01, int k1, int k2, int k3)
+template <class T, bool FAST>
+__device__ __forceinline__ C<T> bp_pmm(const Spin<T> &S, const AmpK<T> &K, int k1, int k2, int k3)
 {
-    return xmul<FAST>(ns01, xsq<FAST>(xmul<FAST>(tof(S.s[k1][1]), S.s[k2][k3])));
+    return kmul<FAST>(K.ns01, K.kp_s, xsq<FAST>(xmul<FAST>(tof(S.s[k1][1]), S.s[k2][k3])));
 }
-template <class T, bool FAST = true> __device__ __forceinline__ C<T> bm_ppp(const Spin<T> &S, C<T> ns01)
+template <class T, bool FAST> __device__ __forceinline__ C<T> bm_ppp(const Spin<T> &S, const AmpK<T> &K)
 {
     const int k1 = 2, k2 = 3, k3 = 4;
     C<T> s = cadd(cadd(xsq<FAST>(xmul<FAST>(tof(S.s[k1][k2]), tof(S.s[k3][1]))), xsq<FAST>(xmul<FAST>(tof(S.s[k1][k3]), tof(S.s[k2][1])))),
                   xsq<FAST>(xmul<FAST>(tof(S.s[k2][k3]), tof(S.s[k1][1]))));
-    return xmul<FAST>(ns01, s);
+    return kmul<FAST>(K.ns01, K.km_s, s);
 }
-template <class T, bool FAST = true> __device__ __forceinline__ C<T> bm_mmm(const Spin<T> &S, C<T> nt01)
+template <class T, bool FAST> __device__ __forceinline__ C<T> bm_mmm(const Spin<T> &S, const AmpK<T> &K)
 {
     const int k1 = 2, k2 = 3, k3 = 4;
     C<T> s = cadd(cadd(xsq<FAST>(xmul<FAST>(S.s[k1][0], S.s[k2][k3])), xsq<FAST>(xmul<FAST>(S.s[k2][0], S.s[k1][k3]))),
                   xsq<FAST>(xmul<FAST>(S.s[k3][0], S.s[k1][k2])));
-    return xmul<FAST>(nt01, s);
+    return kmul<FAST>(K.nt01, K.km_t, s);
 }
 
 // Helicity sums of the five matrix-element contributions (matelems.rs:53-86):
 // m[0] = sum|A g_a|^2, m[1] = sum|B+ g_b+|^2, m[2] = sum|B- g_b-|^2,
 // m[3] + i m[4] = sum (2 A g_a) conj(B+ g_b+).  Helicities MMP..PPM carry A and
-// B+ (spinor.rs:72-98), MMM and PPP carry B- only (spinor.rs:100-113).
-template <class T, bool FAST = true>
-__device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> nt01, T g_a, T g_bp, T g_bm,
-                                              T m[5])
+// B+ (spinor.rs:72-98), MMM and PPP carry B- only (spinor.rs:100-113).  The
+// interference terms are summed without their factor 2, which multiplies the
+// sums at the end: doubling is exact in binary, so every partial sum is the
+// reference's halved and the results are the same bits.
+template <class T, bool FAST>
+__device__ __forceinline__ void helicity_sums(const Spin<T> &S, const AmpK<T> &K, T m[5])
 {
     // (A, B+) for h = MMP, MPM, MPP, PMM, PMP, PPM, each folded into the sums
     // as soon as it is formed (same order of additions as a loop over h).
     T sA = 0, sP = 0, sR = 0, sI = 0;
     auto fold = [&](int h, C<T> a, C<T> b) {
-        C<T> A = mk<T>(a.re * g_a, a.im * g_a);
-        C<T> Pp = mk<T>(b.re * g_bp, b.im * g_bp);
-        C<T> X = xmul<FAST>(mk<T>((T)2. * A.re, (T)2. * A.im), mk<T>(Pp.re, -Pp.im));
-        T nA = A.re * A.re + A.im * A.im;
-        T nP = Pp.re * Pp.re + Pp.im * Pp.im;
+        C<T> A = a, Pp = b;
+        if constexpr (!FAST) {
+            A = rscale(a, K.g_a);
+            Pp = rscale(b, K.g_bp);
+        }
+        C<T> X = xmul<FAST>(A, mk<T>(Pp.re, -Pp.im));
+        T nA = xnorm<FAST>(A);
+        T nP = xnorm<FAST>(Pp);
         if (h == 0) {
             sA = nA;
             sP = nP;
@@ -391,21 +450,23 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, C<T> ns01, C<T> 
             sI = sI + X.im;
         }
     };
-    fold(0, a_pmm<T, FAST>(S, nt01, 4, 2, 3), bp_pmm<T, FAST>(S, ns01, 4, 2, 3));
-    fold(1, a_pmm<T, FAST>(S, nt01, 3, 2, 4), bp_pmm<T, FAST>(S, ns01, 3, 2, 4));
-    fold(2, a_ppm<T, FAST>(S, ns01, 3, 4, 2), bp_ppm<T, FAST>(S, nt01, 3, 4, 2));
-    fold(3, a_pmm<T, FAST>(S, nt01, 2, 3, 4), bp_pmm<T, FAST>(S, ns01, 2, 3, 4));
-    fold(4, a_ppm<T, FAST>(S, ns01, 2, 4, 3), bp_ppm<T, FAST>(S, nt01, 2, 4, 3));
-    fold(5, a_ppm<T, FAST>(S, ns01, 2, 3, 4), bp_ppm<T, FAST>(S, nt01, 2, 3, 4));
-    C<T> bmm = bm_mmm<T, FAST>(S, nt01);
-    C<T> bpp = bm_ppp<T, FAST>(S, ns01);
-    C<T> Bm = mk<T>(bmm.re * g_bm, bmm.im * g_bm);
-    C<T> Bp = mk<T>(bpp.re * g_bm, bpp.im * g_bm);
+    fold(0, a_pmm<T, FAST>(S, K, 4, 2, 3), bp_pmm<T, FAST>(S, K, 4, 2, 3));
+    fold(1, a_pmm<T, FAST>(S, K, 3, 2, 4), bp_pmm<T, FAST>(S, K, 3, 2, 4));
+    fold(2, a_ppm<T, FAST>(S, K, 3, 4, 2), bp_ppm<T, FAST>(S, K, 3, 4, 2));
+    fold(3, a_pmm<T, FAST>(S, K, 2, 3, 4), bp_pmm<T, FAST>(S, K, 2, 3, 4));
+    fold(4, a_ppm<T, FAST>(S, K, 2, 4, 3), bp_ppm<T, FAST>(S, K, 2, 4, 3));
+    fold(5, a_ppm<T, FAST>(S, K, 2, 3, 4), bp_ppm<T, FAST>(S, K, 2, 3, 4));
+    C<T> Bm = bm_mmm<T, FAST>(S, K);
+    C<T> Bp = bm_ppp<T, FAST>(S, K);
+    if constexpr (!FAST) {
+        Bm = rscale(Bm, K.g_bm);
+        Bp = rscale(Bp, K.g_bm);
+    }
     m[0] = sA;
     m[1] = sP;
-    m[2] = (Bm.re * Bm.re + Bm.im * Bm.im) + (Bp.re * Bp.re + Bp.im * Bp.im);
-    m[3] = sR;
-    m[4] = sI;
+    m[2] = xnorm<FAST>(Bm) + xnorm<FAST>(Bp);
+    m[3] = (T)2. * sR;
+    m[4] = (T)2. * sI;
 }
 
 // ---- mixed precision: f32 spinor/amplitude algebra ------------------------
@@ -666,8 +727,10 @@ template <class Walk> __device__ __forceinline__ bool generate_faster_f64(const 
 __device__ __forceinline__ void matrix_elements_f64(const Params &P, const double p[3][4], double m[5])
 {
     Spin<double> S;
-    spinor_products<double>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
-    helicity_sums<double>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+    spinor_products<double, true>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
+    const AmpK<double> K{mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm,
+                         P.ka_s, P.ka_t, P.kp_s, P.kp_t, P.km_s, P.km_t};
+    helicity_sums<double, true>(S, K, m);
 }
 
 // Mixed precision keeps RAMBO (massless q, conformal map), the sort and the
@@ -689,8 +752,10 @@ __device__ __forceinline__ void matrix_elements_mixed(const Params &P, const dou
     Spin<float> S;
     spinor_products_f32(P, pf, ez, S);
     float mf[5];
-    helicity_sums<float>(S, mk((float)P.ns01re, (float)P.ns01im), mk((float)P.nt01re, (float)P.nt01im),
-                         (float)P.g_a, (float)P.g_bp, (float)P.g_bm, mf);
+    const AmpK<float> K{mk((float)P.ns01re, (float)P.ns01im), mk((float)P.nt01re, (float)P.nt01im),
+                        (float)P.g_a, (float)P.g_bp, (float)P.g_bm, (float)P.ka_s, (float)P.ka_t,
+                        (float)P.kp_s, (float)P.kp_t, (float)P.km_s, (float)P.km_t};
+    helicity_sums<float, true>(S, K, mf);
 #pragma unroll
     for (int c = 0; c < 5; ++c)
         m[c] = (double)mf[c];
@@ -818,7 +883,8 @@ __device__ __forceinline__ void matrix_elements_f32(const FParams &P, const floa
 {
     Spin<float> S;
     spinor_products<float>(P.xx0, P.f0re, P.f0im, P.xx1, P.f1re, P.f1im, p, S);
-    helicity_sums<float, false>(S, mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, m);
+    const AmpK<float> K{mk(P.ns01re, P.ns01im), mk(P.nt01re, P.nt01im), P.g_a, P.g_bp, P.g_bm, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    helicity_sums<float, false>(S, K, m);
 }
 
 // ---- faster-evgen in f32 (evgen.rs:143-173, 221-249 with Float = f32) -------
