@@ -84,6 +84,17 @@ TPM_FN double div_n(double a, double b)
     const double r = __builtin_fma(-b, q, a);
     return __builtin_fma(r, y, q);
 }
+// 1 / b for the same b: the refined reciprocal of div_n, within 1 ulp
+// (not always the nearest double).  a * rcp_n(b) is within 1.5 ulp of a / b:
+// for quotients that may move by an ulp or so (the amplitude algebra).
+TPM_FN double rcp_n(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.);
+    return __builtin_fma(y, e, y);
+}
 TPM_FN double sqrt_n(double x)
 {
     const double y = __builtin_amdgcn_rsq(x);
@@ -100,6 +111,7 @@ TPM_FN double sqrt_n(double x)
 }
 #else
 TPM_FN double div_n(double a, double b) { return a / b; }
+TPM_FN double rcp_n(double b) { return 1. / b; }
 TPM_FN double sqrt_n(double x) { return std::sqrt(x); }
 #endif
 

@@ -288,8 +288,14 @@ __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T 
         xx[k] = sq(p[k][3] + p[k][2]);
         // f32::MIN_POSITIVE / f64::MIN_POSITIVE
         const T minpos = sizeof(T) == 8 ? (T)2.2250738585072014e-308 : (T)1.17549435e-38f;
-        if (xx[k] > minpos)
-            fx[k] = mk<T>(dv(p[k][0], xx[k]), dv(p[k][1], xx[k]));
+        if (xx[k] > minpos) {
+            if constexpr (FAST) {
+                const T y = tpm::rcp_n(xx[k]);
+                fx[k] = mk<T>(p[k][0] * y, p[k][1] * y);
+            } else {
+                fx[k] = mk<T>(dv(p[k][0], xx[k]), dv(p[k][1], xx[k]));
+            }
+        }
         else
             fx[k] = mk<T>(sq((T)2. * p[k][3]), (T)0.);
     }
@@ -314,10 +320,10 @@ __device__ __forceinline__ void spinor_products(T xx0, T f0re, T f0im, T xx1, T 
 }
 
 // Complex products of the amplitude algebra.  With FAST they contract
-// multiply-adds into fma: in f64 (the f64 event loops) the result moves by an
-// ulp or so and the division stays the correctly rounded one; in f32 (the
-// mixed-precision path and the f32 adjoint) they also divide by a hardware
-// reciprocal.  Without FAST they are the reference's plain operations (the
+// multiply-adds into fma and divide by multiplying with one reciprocal of
+// |b|^2 (in f64 the refined one, in f32 the hardware's), so the result moves
+// by an ulp or so (the f64 event loops, the mixed-precision path, the
+// adjoints).  Without FAST they are the reference's plain operations (the
 // all-f32 parity path).
 __device__ __forceinline__ C<double> cmul_fast(C<double> a, C<double> b)
 {
@@ -325,8 +331,8 @@ __device__ __forceinline__ C<double> cmul_fast(C<double> a, C<double> b)
 }
 __device__ __forceinline__ C<double> cdiv_fast(C<double> a, C<double> b)
 {
-    const double n = fma(b.re, b.re, b.im * b.im);
-    return C<double>{tpm::div_n(fma(a.re, b.re, a.im * b.im), n), tpm::div_n(fma(a.im, b.re, -(a.re * b.im)), n)};
+    const double y = tpm::rcp_n(fma(b.re, b.re, b.im * b.im));
+    return C<double>{fma(a.re, b.re, a.im * b.im) * y, fma(a.im, b.re, -(a.re * b.im)) * y};
 }
 __device__ __forceinline__ C<float> cmul_fast(C<float> a, C<float> b)
 {
@@ -416,6 +422,31 @@ template <class T, bool FAST> __device__ __forceinline__ C<T> bm_mmm(const Spin<
     return kmul<FAST>(K.nt01, K.km_t, s);
 }
 
+// FAST: the A amplitudes' denominators.  a_ppm(k1, k2, k3) divides by
+// D{k1,k2} = s(0,k1) s(0,k2) s(1,k1) s(1,k2) and a_pmm(k1, k2, k3) by
+// t(1,k2) t(1,k3) t(0,k2) t(0,k3) = conj D{k2,k3} (t = -conj s, the four signs
+// cancel), so the six amplitudes share three products D and their 1/|D|^2:
+// num / D = num conj(D) / |D|^2 and num / conj(D) = num D / |D|^2.
+template <class T> struct ADen {
+    C<T> d;
+    T inv; // 1 / |d|^2
+};
+__device__ __forceinline__ double rcp_amp(double x) { return tpm::rcp_n(x); }
+__device__ __forceinline__ float rcp_amp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <class T> __device__ __forceinline__ ADen<T> a_den(const Spin<T> &S, int a, int b)
+{
+    const C<T> d = xmul<true>(xmul<true>(xmul<true>(S.s[0][a], S.s[0][b]), S.s[1][a]), S.s[1][b]);
+    return ADen<T>{d, rcp_amp(xnorm<true>(d))};
+}
+template <class T> __device__ __forceinline__ C<T> a_ppm_d(const Spin<T> &S, const AmpK<T> &K, const ADen<T> &D, int k3)
+{
+    return rscale(xmul<true>(xsq<true>(S.s[0][k3]), mk<T>(D.d.re, -D.d.im)), K.ka_s * D.inv);
+}
+template <class T> __device__ __forceinline__ C<T> a_pmm_d(const Spin<T> &S, const AmpK<T> &K, const ADen<T> &D, int k1)
+{
+    return rscale(xmul<true>(xsq<true>(tof(S.s[1][k1])), D.d), K.ka_t * D.inv);
+}
+
 // Helicity sums of the five matrix-element contributions (matelems.rs:53-86):
 // m[0] = sum|A g_a|^2, m[1] = sum|B+ g_b+|^2, m[2] = sum|B- g_b-|^2,
 // m[3] + i m[4] = sum (2 A g_a) conj(B+ g_b+).  Helicities MMP..PPM carry A and
@@ -450,12 +481,22 @@ __device__ __forceinline__ void helicity_sums(const Spin<T> &S, const AmpK<T> &K
             sI = sI + X.im;
         }
     };
-    fold(0, a_pmm<T, FAST>(S, K, 4, 2, 3), bp_pmm<T, FAST>(S, K, 4, 2, 3));
-    fold(1, a_pmm<T, FAST>(S, K, 3, 2, 4), bp_pmm<T, FAST>(S, K, 3, 2, 4));
-    fold(2, a_ppm<T, FAST>(S, K, 3, 4, 2), bp_ppm<T, FAST>(S, K, 3, 4, 2));
-    fold(3, a_pmm<T, FAST>(S, K, 2, 3, 4), bp_pmm<T, FAST>(S, K, 2, 3, 4));
-    fold(4, a_ppm<T, FAST>(S, K, 2, 4, 3), bp_ppm<T, FAST>(S, K, 2, 4, 3));
-    fold(5, a_ppm<T, FAST>(S, K, 2, 3, 4), bp_ppm<T, FAST>(S, K, 2, 3, 4));
+    if constexpr (FAST) {
+        const ADen<T> d23 = a_den(S, 2, 3), d24 = a_den(S, 2, 4), d34 = a_den(S, 3, 4);
+        fold(0, a_pmm_d(S, K, d23, 4), bp_pmm<T, FAST>(S, K, 4, 2, 3));
+        fold(1, a_pmm_d(S, K, d24, 3), bp_pmm<T, FAST>(S, K, 3, 2, 4));
+        fold(2, a_ppm_d(S, K, d34, 2), bp_ppm<T, FAST>(S, K, 3, 4, 2));
+        fold(3, a_pmm_d(S, K, d34, 2), bp_pmm<T, FAST>(S, K, 2, 3, 4));
+        fold(4, a_ppm_d(S, K, d24, 3), bp_ppm<T, FAST>(S, K, 2, 4, 3));
+        fold(5, a_ppm_d(S, K, d23, 4), bp_ppm<T, FAST>(S, K, 2, 3, 4));
+    } else {
+        fold(0, a_pmm<T, FAST>(S, K, 4, 2, 3), bp_pmm<T, FAST>(S, K, 4, 2, 3));
+        fold(1, a_pmm<T, FAST>(S, K, 3, 2, 4), bp_pmm<T, FAST>(S, K, 3, 2, 4));
+        fold(2, a_ppm<T, FAST>(S, K, 3, 4, 2), bp_ppm<T, FAST>(S, K, 3, 4, 2));
+        fold(3, a_pmm<T, FAST>(S, K, 2, 3, 4), bp_pmm<T, FAST>(S, K, 2, 3, 4));
+        fold(4, a_ppm<T, FAST>(S, K, 2, 4, 3), bp_ppm<T, FAST>(S, K, 2, 4, 3));
+        fold(5, a_ppm<T, FAST>(S, K, 2, 3, 4), bp_ppm<T, FAST>(S, K, 2, 3, 4));
+    }
     C<T> Bm = bm_mmm<T, FAST>(S, K);
     C<T> Bp = bm_ppp<T, FAST>(S, K);
     if constexpr (!FAST) {

@@ -77,12 +77,14 @@ def test_events_match_oracle(engine, cfg, oracle, valeurs, rng, sort):
     print(f"m rel: max {rel.max():.2e}, p99 {np.percentile(rel, 99) / eps:.1f} eps, p99.9 "
           f"{np.percentile(rel, 99.9) / eps:.1f} eps, {rel_same:.1f} eps on the {same_p.mean():.4f} of events with "
           f"bit-identical momenta; mixed {relx.max():.2e}; exact fraction {np.mean(mk == omk):.4f}")
-    # The amplitude algebra fuses multiply-adds and folds the couplings into
-    # its real constants, so each |M|^2 lands a few ulp from the reference's
-    # order of operations: measured p99 4.3 eps, p99.9 7.3 eps.  The tail is
-    # the momenta's few-ulp differences, propagated (measured max 3.3e-13).
+    # The amplitude algebra fuses multiply-adds, folds the couplings into its
+    # real constants and divides through one reciprocal per shared
+    # denominator, so each
+    # |M|^2 lands a few ulp from the reference's order of operations: measured
+    # p99 5.8 eps, p99.9 8.8 eps.  The tail is the momenta's few-ulp
+    # differences, propagated (measured max 3.3e-13).
     assert np.percentile(rel, 99) <= 8 * eps and np.percentile(rel, 99.9) <= 16 * eps
-    assert rel_same <= 16  # measured 9.1: the fused amplitude algebra alone
+    assert rel_same <= 24  # measured 14.5: the amplitude algebra's own rounding
     assert rel.max() < 5e-13 and relx.max() < 5e-13
     assert np.all(m[~k] == 0)
 
