@@ -3,7 +3,7 @@
 // v ~ U(0,1)^12 (from the xoshiro stream) -> z = logit(v) -> six masked affine
 // coupling layers -> u = sigmoid(z).  Coupling layer l keeps the features
 // in cond_mask[l] and maps the others as z_j <- z_j exp(s_j) + t_j with
-// (s, t) = MLP_l(z * cond_mask[l]), MLP = 12 -> 64 -> 64 -> 24, SiLU, and
+// (s, t) = MLP_l(z * cond_mask[l]), MLP = 12 -> 64 -> 64 -> 24, hard-swish, and
 // s = FLOW_SCALE * tanh(raw).  The density of u is
 //   log q(u) = sum_j log(v_j (1 - v_j)) - sum_l sum_j s_lj - sum_j log(u_j (1 - u_j)).
 // The three matrix products of every MLP run on bf16 MFMA with f32

@@ -57,29 +57,14 @@ __device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x)
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sigm(float x) { return frcp(1.f + fexp(-x)); }
 
-// The bf16 copy of the parameters (what the kernels multiply with) is scaled
-// so that the hidden layers work in base 2: W1, b1 and b2 times -log2(e), W3
-// times -ln 2, W2 and b3 as they are.  A hidden pre-activation then comes out
-// as c' = -log2(e) c, and SiLU(c) = c sigmoid(c) becomes
-// c' / (1 + 2^c') = -log2(e) SiLU(c): one exp2 and one reciprocal with no
-// scaling multiply.  W2 sees -log2(e) SiLU and yields -log2(e) times the next
-// pre-activation (so b2 carries the factor), and W3's -ln 2 takes it out
-// again: the network is the same function, up to the bf16 rounding of the
-// scaled values.  The master f32 parameters are the unscaled ones, and their
-// gradient is the scaled one's times the same factor.
-__host__ __device__ constexpr float param_scale(int i)
+// Hard-swish y = c min(max(c / 6 + 1/2, 0), 1) and its derivative (0 below
+// -3, c / 3 + 1/2 up to 3, 1 above; PyTorch's hardswish): three full-rate
+// operations where a sigmoid-based activation takes an exp2 and a reciprocal
+// at a quarter of the rate (the forward-only kernels drop d).
+__device__ __forceinline__ void hswish_d(float c, float &y, float &d)
 {
-    const int o = i % LP;
-    return o < OFF_W2 ? -LOG2E : o < OFF_B2 ? 1.f : o < OFF_W3 ? -LOG2E : o < OFF_B3 ? -LN2 : 1.f;
-}
-// SiLU in base 2 and its derivative: y = c' / (1 + 2^c') and
-// dy/dc' = s (1 - ln 2 c' (1 - s)), s = 1 / (1 + 2^c') (the forward-only
-// kernels drop d).
-__device__ __forceinline__ void silu_d(float c, float &y, float &d)
-{
-    const float s = frcp(1.f + __builtin_amdgcn_exp2f(c));
-    y = c * s;
-    d = s * (1.f - LN2 * c * (1.f - s));
+    y = c * fminf(fmaxf(fmaf(c, 1.f / 6.f, 0.5f), 0.f), 1.f);
+    d = c < -3.f ? 0.f : (c > 3.f ? 1.f : fmaf(c, 1.f / 3.f, 0.5f));
 }
 __device__ __forceinline__ float ftanh(float x) { return 1.f - 2.f * frcp(fexp(2.f * x) + 1.f); }
 __device__ __forceinline__ float softplus(float x) { return fmaxf(x, 0.f) + flog(1.f + fexp(-fabsf(x))); }
@@ -166,7 +151,7 @@ template <class Lds> __device__ __forceinline__ f4 bias4(const Lds &s, int r0)
 
 struct MlpFwd {
     s4 a;          // bf16 input (features 4g..4g+3)
-    f4 h1d[4];     // SiLU'(pre-activation), 4 row blocks (backward only)
+    f4 h1d[4];     // hard-swish'(pre-activation), 4 row blocks (backward only)
     s4 h1[4];
     f4 h2d[4];
     s4 h2[4];
@@ -184,7 +169,7 @@ template <class Lds> __device__ __forceinline__ void mlp_forward(const Lds &s, f
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             float y, dy;
-            silu_d(c[i], y, dy);
+            hswish_d(c[i], y, dy);
             h[i] = y;
             d[i] = dy;
         }
@@ -201,7 +186,7 @@ template <class Lds> __device__ __forceinline__ void mlp_forward(const Lds &s, f
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             float y, dy;
-            silu_d(c[i], y, dy);
+            hswish_d(c[i], y, dy);
             h[i] = y;
             d[i] = dy;
         }
@@ -699,7 +684,7 @@ __global__ __launch_bounds__(256) void flow_grad_reduce_kernel(const float *part
         for (int j = 0; j < 16; ++j)
             sum += v[j];
     }
-    grad[i] = sum * (inv_n * param_scale(i));
+    grad[i] = sum * inv_n;
 }
 
 __global__ void flow_adam_kernel(float *w, uint16_t *wbf, float *m, float *v, const float *g, int step, float lr,
@@ -716,14 +701,14 @@ __global__ void flow_adam_kernel(float *w, uint16_t *wbf, float *m, float *v, co
     const float mh = mi / (1.f - powf(b1, (float)step)), vh = vi / (1.f - powf(b2, (float)step));
     const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
     w[i] = wi;
-    wbf[i] = f2bf(param_scale(i) * wi);
+    wbf[i] = f2bf(wi);
 }
 
 __global__ void flow_to_bf16_kernel(const float *w, uint16_t *wbf)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < NPARAM)
-        wbf[i] = f2bf(param_scale(i) * w[i]);
+        wbf[i] = f2bf(w[i]);
 }
 
 // ---- launchers ---------------------------------------------------------------

@@ -224,7 +224,7 @@ int tp_sync(void *ctx, tp_stats *stats);
  * No reference counterpart: the reference samples RAMBO's 12 uniforms flat
  * (evgen.rs:174-207).  These entry points add a RealNVP normalising flow on
  * the 12-dimensional unit cube (6 affine coupling layers, each with a
- * 12 -> 64 -> 64 -> 24 SiLU MLP on bf16 MFMA with f32 accumulation), trained
+ * 12 -> 64 -> 64 -> 24 hard-swish MLP on bf16 MFMA with f32 accumulation), trained
  * on the GPU by reverse KL through the adjoint of the event integrand, and a
  * weighted event loop that samples the mixture alpha q + (1 - alpha) U and
  * feeds the same per-batch accumulators (tp_acc) with every |M|^2 multiplied

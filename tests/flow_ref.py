@@ -4,17 +4,15 @@ Restates the flow computed by the bf16-MFMA kernels in
 3photons-rust_amd/csrc/tp_flow.hip from its definition (csrc/tp_flow.h):
 z0 = logit(v); six masked affine couplings z <- z exp(s) + t with
 (s_raw, t) = MLP_l(z * cond_l), s = 1.5 tanh(s_raw), MLP 12(16) -> 64 -> 64 ->
-32 with SiLU; u = sigmoid(z_6);
+32 with hard-swish (PyTorch's hardswish); u = sigmoid(z_6);
 log q(u) = sum log v(1 - v) - sum s - sum log u(1 - u).
 
 With ``emulate=True`` every operand of a matrix product (weights, biases and
 activations) is rounded to bf16 as the kernels round them, with f32
 arithmetic elsewhere; the rounding passes gradients straight through, so
 autograd gives the gradient the kernels' backward pass approximates.  The
-kernels' hidden layers work in base 2: they multiply bf16 copies of W1, b1
-and b2 scaled by -log2(e) and of W3 scaled by -ln 2, and evaluate
-c' / (1 + 2^c') = -log2(e) SiLU(c) -- the same network in exact arithmetic,
-so only the emulation follows that form.
+kernels evaluate hard-swish as c min(max(c / 6 + 1/2, 0), 1), which the
+emulation follows.
 """
 from __future__ import annotations
 
@@ -26,8 +24,6 @@ LP = 7328
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 1024, 1088, 5184, 5248, 7296
 MASKS = (0x03F, 0xFC0, 0x555, 0xAAA, 0x1C7, 0xE38)
 SCALE = 1.5
-NEG_LOG2E = torch.tensor(-1.4426950408889634, dtype=torch.float32)  # the kernels' f32 constants
-NEG_LN2 = torch.tensor(-0.6931471805599453, dtype=torch.float32)
 
 
 def bf(x: torch.Tensor) -> torch.Tensor:
@@ -48,16 +44,15 @@ def cond_mask(l: int) -> torch.Tensor:
 def mlp(a16: torch.Tensor, prm, emulate: bool):
     W1, b1, W2, b2, W3, b3 = prm
     if not emulate:
-        h1 = torch.nn.functional.silu(a16 @ W1.T + b1)
-        h2 = torch.nn.functional.silu(h1 @ W2.T + b2)
+        h1 = torch.nn.functional.hardswish(a16 @ W1.T + b1)
+        h2 = torch.nn.functional.hardswish(h1 @ W2.T + b2)
         o = h2 @ W3.T + b3
         return o[:, :D], o[:, 16:16 + D]
-    # base-2 hidden layers: c' = -log2(e) c, and c' / (1 + 2^c') = c' sigmoid(-ln 2 c')
-    c1 = bf(a16) @ bf(NEG_LOG2E * W1).T + bf(NEG_LOG2E * b1)
-    h1 = c1 * torch.sigmoid(NEG_LN2 * c1)
-    c2 = bf(h1) @ bf(W2).T + bf(NEG_LOG2E * b2)
-    h2 = c2 * torch.sigmoid(NEG_LN2 * c2)
-    o = bf(h2) @ bf(NEG_LN2 * W3).T + bf(b3)
+    c1 = bf(a16) @ bf(W1).T + bf(b1)
+    h1 = c1 * torch.clamp(c1 * (1. / 6.) + 0.5, 0., 1.)
+    c2 = bf(h1) @ bf(W2).T + bf(b2)
+    h2 = c2 * torch.clamp(c2 * (1. / 6.) + 0.5, 0., 1.)
+    o = bf(h2) @ bf(W3).T + bf(b3)
     return o[:, :D], o[:, 16:16 + D]
 
 
