@@ -176,7 +176,7 @@ extern "C" int tp_flow_default_opts(tp_flow_opts *o)
     return tp_guard(nullptr, [&]() -> int {
         if (!o)
             return TP_ERR_ARG;
-        o->alpha = 0.9;
+        o->alpha = 0.7;
         o->batch = 65536;
         o->steps = 2000;
         o->lr = 3e-3f;

@@ -242,7 +242,7 @@ int tp_sync(void *ctx, tp_stats *stats);
 #define TP_FLOW_NPARAM 43968
 
 typedef struct {
-    double alpha;       /* flow share of the sampling mixture (0.9) */
+    double alpha;       /* flow share of the sampling mixture (0.7) */
     uint64_t batch;     /* samples per training step (65536; multiple of 16) */
     int32_t steps;      /* training steps (2000) */
     float lr;           /* Adam step size (3e-3) */
